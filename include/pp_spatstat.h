@@ -1,0 +1,142 @@
+/*
+ * pp_spatstat.h -- C ABI of the MI355X (gfx950) RDF / kNN spatial-statistics path.
+ *
+ * Drop-in replacement for the compute step of the reference's spatial
+ * statistics: the KD-tree range-ball queries and the OpenMP center loop of
+ *   horderdist::compute_generic_spatstat3()      src/PARAPROBE_SolverHdl.cpp:3685-4296
+ * fed by
+ *   solver::init_spatialindexing()               src/PARAPROBE_SolverHdl.cpp:687-731
+ * and called from
+ *   solver::characterize_spatstat()              src/PARAPROBE_SolverHdl.cpp:745-776.
+ * The reference function takes no arguments and reads globals
+ * (Settings::SpatStat*, spatstat_tasks, the per-thread KD-trees); here every
+ * input is explicit, every host buffer is owned by the caller, the library owns
+ * device buffers behind opaque handles, and no exception crosses the ABI.
+ *
+ * Histogram layout (every family): one int64 histogram is
+ *   [cnts_lowest, cnts[0 .. nbins-1], cnts_highest]       (nbins + 2 entries)
+ * matching histogram::cnts_lowest / cnts / cnts_highest
+ * (src/PARAPROBE_Histogram.h:168-170; the reference keeps doubles, which are
+ * exact for these integer counts).  Family arrays are task-major in task parse
+ * order; MKNN is [task][kth][nbins + 2] with kth indexing the k list sorted in
+ * descending order (S.cpp:3759, 4086); CNTNB is [task][cntnb_cap] indexed by
+ * the neighbour count (discrhistogram, src/PARAPROBE_DiscreteHistogram.h:83-97).
+ */
+#ifndef PP_SPATSTAT_H
+#define PP_SPATSTAT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum pp_status {
+    PP_OK = 0,
+    PP_ERR_INVALID_ARG = 1,        /* bad settings or arguments (cf. Settings::checkUserInput) */
+    PP_ERR_OOM_DEVICE = 2,         /* device allocation failed (cf. bad_alloc catches) */
+    PP_ERR_HIP = 3,                /* HIP runtime error */
+    PP_ERR_ADMISSION_REJECTED = 4, /* serving: the job can never fit the HBM budget */
+    PP_ERR_NO_DEVICE = 5,          /* no gfx950 device */
+    PP_ERR_UNSUPPORTED = 6,        /* e.g. NPCORR, or bins finer than float resolution */
+    PP_ERR_OVERFLOW = 7,           /* a CNTNB count exceeded cntnb_cap */
+    PP_ERR_NOT_READY = 8           /* serving: ticket still in flight */
+} pp_status;
+
+/* Family bits: digit d of the reference's AnalysisSpatDistrType maps to
+ * bit (d - 1) (src/PARAPROBE_Settings.cpp:347-393). */
+#define PP_FAM_RDF (1u << 0)
+#define PP_FAM_1NN (1u << 1)
+#define PP_FAM_RIPK (1u << 2)
+#define PP_FAM_MKNN (1u << 3)
+#define PP_FAM_NPCORR (1u << 4) /* not provided: returns PP_ERR_UNSUPPORTED */
+#define PP_FAM_CNTNB (1u << 5)
+
+typedef struct pp_ctx pp_ctx;
+typedef struct pp_cloud pp_cloud;
+typedef struct pp_server pp_server;
+
+/* One spatial-statistics job: the settings compute_generic_spatstat3 reads
+ * (Settings::SpatStatRadiusMin/Incr/Max, SpatStatDo*, DescrStatMKNNCode) and
+ * the parsed tasks (TypeSpecDescrStat trgcandidates/envcandidates,
+ * src/PARAPROBE_PeriodicTable.h:100-127) as 256-bit type masks. */
+typedef struct pp_job {
+    int32_t ntasks;
+    const uint8_t *trg_mask;     /* ntasks x 32 bytes; bit t: type id t is a central type */
+    const uint8_t *env_mask;     /* ntasks x 32 bytes; bit t: type id t is a neighbour type */
+    float rmin, rincr, rmax;     /* SpatStatRadiusMin / Incr / Max (float settings) */
+    uint32_t families;           /* PP_FAM_* */
+    int32_t nk;                  /* MKNN: number of k values */
+    const int64_t *kminus1_desc; /* MKNN: k - 1 values in descending order (S.cpp:3739-3760) */
+} pp_job;
+
+/* Caller-allocated outputs.  Any family pointer may be NULL when that family
+ * is not requested. */
+typedef struct pp_hist_out {
+    int64_t *rdf;    /* [ntasks][nbins + 2] */
+    int64_t *nn1;    /* [ntasks][nbins + 2] */
+    int64_t *ripk;   /* [ntasks][nbins + 2] */
+    int64_t *mknn;   /* [ntasks][nk][nbins + 2] */
+    int64_t *cntnb;  /* [ntasks][cntnb_cap] */
+    int64_t cntnb_cap;
+    /* out: */
+    int64_t nbins;
+    int64_t cntnb_len;          /* largest count seen + 1 (discrhistogram size) */
+    int64_t centers_considered; /* MyIonsCurrRegionConsider summed over regions */
+    int64_t ions_discarded;     /* MyIonsCurrRegionDiscard summed over regions */
+    int64_t pairs_inball;       /* sum over considered centers of |neighbours within R|, all types */
+    int64_t pairs_candidate;    /* candidate pair-distance evaluations made by the GPU */
+    double ms_index;            /* cell-list build (0 when resident) */
+    double ms_prep;             /* center flags, compaction, tiles */
+    double ms_query;            /* query kernel */
+    double ms_total;            /* whole call, host wall clock */
+} pp_hist_out;
+
+/* Context bound to one device; one per host thread (calls on a context are
+ * serialised by the caller). */
+pp_status pp_spatstat_create(int32_t device, pp_ctx **out);
+pp_status pp_spatstat_destroy(pp_ctx *ctx);
+const char *pp_last_error_string(const pp_ctx *ctx);
+
+/* Histogram geometry of histogram(rmin, rincr, rmax)
+ * (src/PARAPROBE_Histogram.h:54-89), for sizing outputs. */
+pp_status pp_hist_geometry(float rmin, float rincr, float rmax, int64_t *nbins);
+
+/* Uploads a point cloud (ions: x,y,z AoS float, uint8 type id, squared
+ * distance to the tip surface -- the reference's ion2surf -- as float).
+ * cell_width > 0 builds a resident cell list of that width now (serving /
+ * batching: every later job with any Rmax reuses it); cell_width == 0 defers
+ * the cell list to each job (width chosen from that job's Rmax).
+ * Replaces decompositor::loadpartitioning + threadmemory::build_local_kdtree
+ * (S.cpp:1819-1934, 1593-1663). */
+pp_status pp_cloud_upload(pp_ctx *ctx, const float *xyz, const uint8_t *type,
+                          const float *ion2surf_sqr, uint64_t n, float cell_width,
+                          pp_cloud **out);
+/* Same, from device pointers already resident in HBM (no host copy). */
+pp_status pp_cloud_from_device(pp_ctx *ctx, const float *d_xyz, const uint8_t *d_type,
+                               const float *d_ion2surf_sqr, uint64_t n, float cell_width,
+                               pp_cloud **out);
+pp_status pp_cloud_free(pp_ctx *ctx, pp_cloud *cloud);
+/* Device bytes a cloud holds. */
+pp_status pp_cloud_footprint(const pp_cloud *cloud, uint64_t *bytes);
+
+/* Runs one job synchronously: the compute step of compute_generic_spatstat3
+ * (S.cpp:3702-4176).  Histograms are written to *out. */
+pp_status pp_spatstat_run(pp_ctx *ctx, const pp_cloud *cloud, const pp_job *job, pp_hist_out *out);
+
+/* Runs njobs jobs (each on its own cloud) in one persistent launch of the
+ * query kernel; outs[i] receives job i.  Results equal njobs separate
+ * pp_spatstat_run calls. */
+pp_status pp_batch_run(pp_ctx *ctx, int32_t njobs, const pp_cloud *const *clouds,
+                       const pp_job *jobs, pp_hist_out *outs);
+
+/* Query-kernel tuning: centers per tile (8, 16 or 32; 0 = automatic) and
+ * the cell width as a fraction of Rmax (cells per Rmax: 1..4; 0 = automatic). */
+pp_status pp_set_tuning(pp_ctx *ctx, int32_t centers_per_tile, int32_t cells_per_rmax);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PP_SPATSTAT_H */

@@ -1,0 +1,334 @@
+"""ctypes binding of the C ABI in ``include/pp_spatstat.h``.
+
+This is the Python-side caller of the gfx950 library (the C++ host mirror in
+``paraprobe_amd/host`` is the other).  It never falls back to anything: if
+``libpp_spatstat.so`` is missing or no gfx950 device is present, it raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "libpp_spatstat.so")
+
+PP_OK = 0
+STATUS_NAMES = {
+    0: "PP_OK",
+    1: "PP_ERR_INVALID_ARG",
+    2: "PP_ERR_OOM_DEVICE",
+    3: "PP_ERR_HIP",
+    4: "PP_ERR_ADMISSION_REJECTED",
+    5: "PP_ERR_NO_DEVICE",
+    6: "PP_ERR_UNSUPPORTED",
+    7: "PP_ERR_OVERFLOW",
+    8: "PP_ERR_NOT_READY",
+}
+
+FAM_RDF = 1 << 0
+FAM_1NN = 1 << 1
+FAM_RIPK = 1 << 2
+FAM_MKNN = 1 << 3
+FAM_NPCORR = 1 << 4
+FAM_CNTNB = 1 << 5
+
+# every symbol declared in include/pp_spatstat.h
+EXPORTED = [
+    "pp_spatstat_create",
+    "pp_spatstat_destroy",
+    "pp_last_error_string",
+    "pp_hist_geometry",
+    "pp_cloud_upload",
+    "pp_cloud_from_device",
+    "pp_cloud_free",
+    "pp_cloud_footprint",
+    "pp_spatstat_run",
+    "pp_batch_run",
+    "pp_set_tuning",
+]
+
+
+class PPError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {msg}")
+        self.status = status
+
+
+class _Job(C.Structure):
+    _fields_ = [
+        ("ntasks", C.c_int32),
+        ("trg_mask", C.POINTER(C.c_uint8)),
+        ("env_mask", C.POINTER(C.c_uint8)),
+        ("rmin", C.c_float),
+        ("rincr", C.c_float),
+        ("rmax", C.c_float),
+        ("families", C.c_uint32),
+        ("nk", C.c_int32),
+        ("kminus1_desc", C.POINTER(C.c_int64)),
+    ]
+
+
+class _Out(C.Structure):
+    _fields_ = [
+        ("rdf", C.POINTER(C.c_int64)),
+        ("nn1", C.POINTER(C.c_int64)),
+        ("ripk", C.POINTER(C.c_int64)),
+        ("mknn", C.POINTER(C.c_int64)),
+        ("cntnb", C.POINTER(C.c_int64)),
+        ("cntnb_cap", C.c_int64),
+        ("nbins", C.c_int64),
+        ("cntnb_len", C.c_int64),
+        ("centers_considered", C.c_int64),
+        ("ions_discarded", C.c_int64),
+        ("pairs_inball", C.c_int64),
+        ("pairs_candidate", C.c_int64),
+        ("ms_index", C.c_double),
+        ("ms_prep", C.c_double),
+        ("ms_query", C.c_double),
+        ("ms_total", C.c_double),
+    ]
+
+
+_lib = None
+
+
+def load_library() -> C.CDLL:
+    """Loads libpp_spatstat.so (raises when it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`"
+        )
+    lib = C.CDLL(LIB_PATH)
+    P = C.c_void_p
+    lib.pp_spatstat_create.argtypes = [C.c_int32, C.POINTER(P)]
+    lib.pp_spatstat_destroy.argtypes = [P]
+    lib.pp_last_error_string.argtypes = [P]
+    lib.pp_last_error_string.restype = C.c_char_p
+    lib.pp_hist_geometry.argtypes = [C.c_float, C.c_float, C.c_float, C.POINTER(C.c_int64)]
+    lib.pp_cloud_upload.argtypes = [P, P, P, P, C.c_uint64, C.c_float, C.POINTER(P)]
+    lib.pp_cloud_from_device.argtypes = [P, P, P, P, C.c_uint64, C.c_float, C.POINTER(P)]
+    lib.pp_cloud_free.argtypes = [P, P]
+    lib.pp_cloud_footprint.argtypes = [P, C.POINTER(C.c_uint64)]
+    lib.pp_spatstat_run.argtypes = [P, P, C.POINTER(_Job), C.POINTER(_Out)]
+    lib.pp_batch_run.argtypes = [P, C.c_int32, C.POINTER(P), C.POINTER(_Job), C.POINTER(_Out)]
+    lib.pp_set_tuning.argtypes = [P, C.c_int32, C.c_int32]
+    for name in EXPORTED:
+        if name != "pp_last_error_string":
+            getattr(lib, name).restype = C.c_int
+    _lib = lib
+    return lib
+
+
+def hist_geometry(rmin: float, rincr: float, rmax: float) -> int:
+    lib = load_library()
+    nb = C.c_int64()
+    st = lib.pp_hist_geometry(rmin, rincr, rmax, C.byref(nb))
+    if st != PP_OK:
+        raise PPError(st, "invalid histogram geometry")
+    return nb.value
+
+
+def type_mask(types) -> np.ndarray:
+    """32-byte bit mask of type ids."""
+    m = np.zeros(32, dtype=np.uint8)
+    for t in types:
+        t = int(t)
+        if not 0 <= t < 256:
+            raise ValueError(f"type id {t} out of range")
+        m[t >> 3] |= np.uint8(1 << (t & 7))
+    return m
+
+
+@dataclass
+class Job:
+    """One spatial-statistics job: tasks as (central types, neighbour types)."""
+
+    tasks: list  # list of (iterable of trg type ids, iterable of env type ids)
+    rmin: float
+    rincr: float
+    rmax: float
+    families: int
+    kvalues: list = field(default_factory=list)  # k values (1-based), any order
+
+    def kminus1_desc(self) -> np.ndarray:
+        return np.array(sorted((int(k) - 1 for k in self.kvalues), reverse=True), dtype=np.int64)
+
+    def masks(self):
+        trg = np.zeros((max(len(self.tasks), 1), 32), dtype=np.uint8)
+        env = np.zeros((max(len(self.tasks), 1), 32), dtype=np.uint8)
+        for i, (t, e) in enumerate(self.tasks):
+            trg[i] = type_mask(t)
+            env[i] = type_mask(e)
+        return trg, env
+
+
+@dataclass
+class Result:
+    nbins: int
+    rdf: np.ndarray | None
+    nn1: np.ndarray | None
+    ripk: np.ndarray | None
+    mknn: np.ndarray | None
+    cntnb: np.ndarray | None
+    cntnb_len: int
+    centers_considered: int
+    ions_discarded: int
+    pairs_inball: int
+    pairs_candidate: int
+    ms_index: float
+    ms_prep: float
+    ms_query: float
+    ms_total: float
+
+    def arrays(self) -> dict:
+        return {k: getattr(self, k) for k in ("rdf", "nn1", "ripk", "mknn", "cntnb")
+                if getattr(self, k) is not None}
+
+
+class _Prepared:
+    """Keeps the numpy buffers a ctypes job/out pair points to alive."""
+
+    def __init__(self, job: Job, cntnb_cap: int):
+        self.job = job
+        self.trg, self.env = job.masks()
+        self.kd = job.kminus1_desc()
+        nb = hist_geometry(job.rmin, job.rincr, job.rmax)
+        self.nbins = nb
+        C_ = nb + 2
+        T = len(job.tasks)
+        fam = job.families
+        self.arr = {}
+        self.cj = _Job()
+        self.cj.ntasks = T
+        self.cj.trg_mask = self.trg.ctypes.data_as(C.POINTER(C.c_uint8))
+        self.cj.env_mask = self.env.ctypes.data_as(C.POINTER(C.c_uint8))
+        self.cj.rmin, self.cj.rincr, self.cj.rmax = job.rmin, job.rincr, job.rmax
+        self.cj.families = fam
+        self.cj.nk = len(self.kd)
+        self.cj.kminus1_desc = self.kd.ctypes.data_as(C.POINTER(C.c_int64))
+        self.co = _Out()
+        for name, bit, shape in (
+            ("rdf", FAM_RDF, (T, C_)),
+            ("nn1", FAM_1NN, (T, C_)),
+            ("ripk", FAM_RIPK, (T, C_)),
+            ("mknn", FAM_MKNN, (T, len(self.kd), C_)),
+            ("cntnb", FAM_CNTNB, (T, cntnb_cap)),
+        ):
+            if fam & bit:
+                a = np.zeros(shape, dtype=np.int64)
+                self.arr[name] = a
+                setattr(self.co, name, a.ctypes.data_as(C.POINTER(C.c_int64)))
+        self.co.cntnb_cap = cntnb_cap if fam & FAM_CNTNB else 0
+
+    def result(self) -> Result:
+        o = self.co
+        return Result(
+            nbins=o.nbins,
+            rdf=self.arr.get("rdf"),
+            nn1=self.arr.get("nn1"),
+            ripk=self.arr.get("ripk"),
+            mknn=self.arr.get("mknn"),
+            cntnb=self.arr.get("cntnb"),
+            cntnb_len=o.cntnb_len,
+            centers_considered=o.centers_considered,
+            ions_discarded=o.ions_discarded,
+            pairs_inball=o.pairs_inball,
+            pairs_candidate=o.pairs_candidate,
+            ms_index=o.ms_index,
+            ms_prep=o.ms_prep,
+            ms_query=o.ms_query,
+            ms_total=o.ms_total,
+        )
+
+
+class Cloud:
+    def __init__(self, ctx: "SpatStat", handle: C.c_void_p, n: int):
+        self._ctx = ctx
+        self.handle = handle
+        self.n = n
+
+    def footprint(self) -> int:
+        b = C.c_uint64()
+        self._ctx._check(self._ctx.lib.pp_cloud_footprint(self.handle, C.byref(b)))
+        return b.value
+
+    def free(self):
+        if self.handle:
+            self._ctx.lib.pp_cloud_free(self._ctx.ctx, self.handle)
+            self.handle = None
+
+
+class SpatStat:
+    """A context on one gfx950 device."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load_library()
+        ctx = C.c_void_p()
+        st = self.lib.pp_spatstat_create(device, C.byref(ctx))
+        if st != PP_OK:
+            raise PPError(st, f"cannot create a context on device {device}")
+        self.ctx = ctx
+
+    def _check(self, st: int):
+        if st != PP_OK:
+            msg = self.lib.pp_last_error_string(self.ctx)
+            raise PPError(st, msg.decode() if msg else "")
+
+    def close(self):
+        if self.ctx:
+            self.lib.pp_spatstat_destroy(self.ctx)
+            self.ctx = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def set_tuning(self, centers_per_tile: int = 0, cells_per_rmax: int = 0):
+        self._check(self.lib.pp_set_tuning(self.ctx, centers_per_tile, cells_per_rmax))
+
+    def upload(self, xyz: np.ndarray, types: np.ndarray, ion2surf_sqr: np.ndarray,
+               cell_width: float = 0.0) -> Cloud:
+        xyz = np.ascontiguousarray(xyz, dtype=np.float32).reshape(-1, 3)
+        types = np.ascontiguousarray(types, dtype=np.uint8)
+        s2 = np.ascontiguousarray(ion2surf_sqr, dtype=np.float32)
+        n = xyz.shape[0]
+        if types.shape[0] != n or s2.shape[0] != n:
+            raise ValueError("xyz, types and ion2surf_sqr lengths differ")
+        h = C.c_void_p()
+        self._check(self.lib.pp_cloud_upload(self.ctx, xyz.ctypes.data, types.ctypes.data,
+                                             s2.ctypes.data, n, cell_width, C.byref(h)))
+        return Cloud(self, h, n)
+
+    def upload_device(self, d_xyz: int, d_types: int, d_s2: int, n: int,
+                      cell_width: float = 0.0) -> Cloud:
+        """Cloud from device pointers (e.g. torch tensors' data_ptr())."""
+        h = C.c_void_p()
+        self._check(self.lib.pp_cloud_from_device(self.ctx, C.c_void_p(d_xyz), C.c_void_p(d_types),
+                                                  C.c_void_p(d_s2), n, cell_width, C.byref(h)))
+        return Cloud(self, h, n)
+
+    def run(self, cloud: Cloud, job: Job, cntnb_cap: int = 65536) -> Result:
+        p = _Prepared(job, cntnb_cap)
+        self._check(self.lib.pp_spatstat_run(self.ctx, cloud.handle, C.byref(p.cj), C.byref(p.co)))
+        return p.result()
+
+    def run_batch(self, clouds: list, jobs: list, cntnb_cap: int = 65536) -> list:
+        n = len(jobs)
+        preps = [_Prepared(j, cntnb_cap) for j in jobs]
+        cj = (_Job * n)(*[p.cj for p in preps])
+        co = (_Out * n)(*[p.co for p in preps])
+        ch = (C.c_void_p * n)(*[c.handle for c in clouds])
+        self._check(self.lib.pp_batch_run(self.ctx, n, ch, cj, co))
+        out = []
+        for i, p in enumerate(preps):
+            p.co = co[i]
+            out.append(p.result())
+        return out

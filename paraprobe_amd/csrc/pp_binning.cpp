@@ -1,0 +1,107 @@
+// pp_binning.cpp -- exact histogram geometry and the squared-distance edge table.
+#include "pp_binning.h"
+
+#include <cmath>
+#include <cstring>
+
+namespace pp {
+
+static constexpr double kEpsilon = 1.0e-6;  // src/PARAPROBE_Numerics.h:45
+
+static inline float f_of_bits(uint32_t u)
+{
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+
+HistGeometry::HistGeometry(float rmin, float rincr, float rmax)
+{
+    const double s = rmin, w = rincr, e = rmax;
+    if (w > kEpsilon) {
+        const size_t nb = (size_t)((e - s) / w);
+        if (nb <= 0xFFFFFFFFull) {
+            start = s;
+            width = w;
+            end = e;
+            nbins = (uint32_t)nb;
+            tmp = 1.0 / (e - s) * (double)nb;
+            valid = true;
+        }
+    }
+}
+
+uint32_t HistGeometry::category(float x) const
+{
+    const double xx = x;
+    if (xx >= start && xx <= end) {
+        const float rb = (float)((xx - start) * tmp);
+        const unsigned int b = (unsigned int)std::floor(rb);
+        return b < nbins ? 1u + b : nbins + 1u;
+    }
+    return xx < start ? 0u : nbins + 1u;
+}
+
+uint32_t HistGeometry::category_d2(float d2) const { return category(std::sqrt(d2)); }
+
+bool EdgeTable::build(const HistGeometry &g, std::string *why)
+{
+    if (!g.valid || g.nbins == 0) {
+        if (why)
+            *why = "invalid histogram geometry (width <= 1e-6 or no bins)";
+        return false;
+    }
+    const uint32_t C = g.ncat();
+    edges.assign(C, 0u);
+    const uint32_t top = 0x7F800000u;  // +inf: every finite non-negative d2 is below
+    for (uint32_t c = 1; c < C; ++c) {
+        // smallest u in [0, top] with category(u) >= c (category(top) is highest)
+        uint32_t lo = 0, hi = top;
+        if (g.category_d2(f_of_bits(lo)) >= c) {
+            edges[c] = 0;
+            continue;
+        }
+        while (hi - lo > 1) {
+            const uint32_t mid = lo + (hi - lo) / 2;
+            if (g.category_d2(f_of_bits(mid)) >= c)
+                hi = mid;
+            else
+                lo = mid;
+        }
+        edges[c] = hi;
+    }
+    // Edge c sits at distance start + (c - 1) / tmp in the ideal geometry, so
+    // the index space coordinate y(d) = (d - start) * tmp + 1 puts it at c.
+    // The device rounds an approximate y to the nearest integer q and answers
+    // q - 1 + (bits(d2) >= edges[q]); that is exact whenever every d2 lies
+    // within 0.5 of an index between its two bracketing edges, i.e. when
+    // every exact edge is within (0.5 - device error) of its ideal index.
+    max_offset = 0;
+    for (uint32_t c = 1; c < C; ++c) {
+        if (c > 1 && edges[c] <= edges[c - 1]) {
+            if (why)
+                *why = "empty histogram category; bins narrower than float resolution";
+            return false;
+        }
+        if (c == 1 && g.start <= 0.0)
+            continue;  // edge 1 at d = 0: every d2 is at or above it, the clamp suffices
+        const uint32_t probes[2] = {edges[c], edges[c] > 0 ? edges[c] - 1 : 0};
+        for (uint32_t u : probes) {
+            const double d = std::sqrt((double)f_of_bits(u));
+            const double y = (d - g.start) * g.tmp + 1.0;
+            const double off = std::fabs(y - (double)c);
+            if (off > max_offset)
+                max_offset = off;
+        }
+    }
+    if (max_offset > 0.25) {
+        if (why)
+            *why = "bin edges too far from their ideal positions for the nearest-edge guess";
+        return false;
+    }
+    idx_scale = (float)g.tmp;
+    idx_off = (float)(1.5 - g.start * g.tmp);
+    return true;
+}
+
+} // namespace pp

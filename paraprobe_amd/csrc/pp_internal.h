@@ -1,0 +1,83 @@
+// pp_internal.h -- structures shared by the host runtime and the gfx950 kernels
+// of the RDF / kNN spatial-statistics path.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pp {
+
+constexpr int kWave = 64;          // CDNA wavefront width
+constexpr int kWaves = 4;          // wavefronts per query workgroup
+constexpr int kBlock = kWave * kWaves;
+constexpr int kMaxRows = 512;      // stencil rows (dy, dz) per query
+constexpr int kMaxEnv = 32;        // distinct neighbour (env) types per job
+constexpr int kMaxTrg = 32;        // distinct center (target) types per job
+constexpr int kMaxTasks = 64;      // type-pair tasks per job
+constexpr int kMaxKmax = 4096;     // largest MKNN order
+constexpr int kXcds = 8;           // MI355X accelerator complex dies
+
+// Stencil row: cells (cx-dxm .. cx+dxm, cy+dy, cz+dz) are contiguous in the
+// cell-sorted ion array because the cell key is x-fastest.
+struct Row {
+    int16_t dy, dz, dxm, pad;
+};
+
+// One query tile: up to T considered centers of one cell of one job.
+struct Tile {
+    uint32_t key;    // linear cell key
+    uint32_t first;  // first entry in the job's center list
+    uint16_t count;  // centers in this tile
+    uint16_t job;    // job index within the launch
+    uint32_t pad;
+};
+
+enum : uint32_t {
+    FAM_RDF = 1u << 0,
+    FAM_1NN = 1u << 1,
+    FAM_RIPK = 1u << 2,
+    FAM_MKNN = 1u << 3,
+    FAM_NPCORR = 1u << 4,
+    FAM_CNTNB = 1u << 5,
+};
+
+// Everything a query kernel needs to know about one job; lives in device
+// memory as an array indexed by Tile::job.
+struct JobDev {
+    // cloud: cell-sorted ions (x, y, z, type in the bits of w) and cell offsets
+    const float4 *pos;
+    const uint32_t *cellstart;
+    int nx, ny, nz;
+    int nrows;
+    const Row *rows;
+    const uint32_t *centers;  // considered centers, sorted-array indices
+    float R2;                 // fl32(R * R)
+    // binning: category = q - 1 + (bits(d2) >= edges[q]), q from the nearest
+    // ideal edge; edges[] holds the exact smallest d2 bit pattern per category
+    int C;                    // nbins + 2 categories (lowest, bins, highest)
+    const uint32_t *edges;    // C entries, edges[0] unused
+    float idx_scale, idx_off;
+    int cat0;                 // category of d = 0 (self pair)
+    int cat_sentinel;         // category of float(R + 1e-6)
+    // types
+    int E;                    // env slots
+    int ntrg;                 // target slots
+    const int32_t *envoff;    // [256] env slot * C, or -1
+    const int32_t *trgslot;   // [256] target slot, or -1
+    // tasks
+    int ntasks;
+    const uint32_t *task_trg; // [ntasks] bit set over target slots
+    const uint32_t *task_env; // [ntasks] bit set over env slots
+    uint32_t families;
+    int kmax, nk;
+    const int32_t *kidx_of_k; // [kmax + 1] index into the descending k list, or -1
+    // outputs (device)
+    unsigned long long *P;    // [ntrg][E][C] pair counts per center type / env type / category
+    uint32_t *K;              // [ntasks][nk][C] MKNN
+    uint32_t *N1;             // [ntasks][C] 1NN
+    uint32_t *CN;             // [ntasks][cn_cap] neighbour-count histogram
+    int cn_cap;
+    int percenter;            // 1 when MKNN, 1NN or CNTNB need per-center histograms
+    unsigned long long *stats;// [0] in-ball pairs (all types), [1] candidate evaluations
+};
+
+} // namespace pp

@@ -1,0 +1,909 @@
+// pp_kernels.hip -- gfx950 kernels of the RDF / kNN spatial-statistics path.
+//
+// Replaces the reference's KD-tree range-ball queries
+// (src/PARAPROBE_KDTree.cpp:423-655) and the OpenMP center loop of
+// horderdist::compute_generic_spatstat3 (src/PARAPROBE_SolverHdl.cpp:3770-4176)
+// with a uniform cell list and one persistent query kernel:
+//   * cell list: bounding box, x-fastest linear cell keys computed in FP64,
+//     a one-digit counting sort (count with atomics, exclusive scan, scatter)
+//     into cell-sorted float4 (x, y, z, type) plus the sorted ion2surf array;
+//   * per job: considered-center flags, stream compaction, tiles of up to T
+//     centers of one cell;
+//   * query: each workgroup pulls tiles from a global queue; the four
+//     wavefronts split the tile's stencil rows into 64-candidate chunks, each
+//     lane holds one candidate and loops over the T centers held in
+//     registers, evaluating the reference's FP32 squared distance
+//     ((dx*dx + dy*dy) + dz*dz, no contraction) and binning in-ball pairs
+//     into LDS histograms with integer atomics.  With MKNN / 1NN / CNTNB the
+//     histograms are per center (16-bit pairs when a bound allows), and after
+//     the tile one wavefront per center scans them to find the category of
+//     every k-th neighbour.
+#include "pp_launch.h"
+
+#pragma clang fp contract(off)
+
+namespace pp {
+
+// ---------------------------------------------------------------------------
+// small helpers
+
+__host__ __device__ inline uint32_t float_to_ord(float f)
+{
+    uint32_t u;
+    __builtin_memcpy(&u, &f, 4);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+float ord_to_float(uint32_t u)
+{
+    uint32_t v = (u & 0x80000000u) ? (u & 0x7FFFFFFFu) : ~u;
+    float f;
+    __builtin_memcpy(&f, &v, 4);
+    return f;
+}
+
+static inline unsigned grid_for(uint64_t n, unsigned block)
+{
+    uint64_t g = (n + block - 1) / block;
+    if (g > 0x7FFFFFFFull)
+        g = 0x7FFFFFFFull;
+    return (unsigned)(g ? g : 1);
+}
+
+__device__ inline uint32_t wave_incl_scan(uint32_t v, int lane)
+{
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        uint32_t t = __shfl_up(v, d, kWave);
+        if (lane >= d)
+            v += t;
+    }
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// exclusive scan (reduce-then-scan, 2048 items per workgroup)
+
+constexpr int kScanBlock = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kScanBlock * kScanItems;
+
+// in and out may alias (the block totals are scanned in place): each thread
+// reads its items before the barrier and writes them after it
+__global__ __launch_bounds__(kScanBlock) void k_scan_tiles(const uint32_t *in, uint32_t *out,
+                                                           uint32_t *sums, uint64_t n)
+{
+    __shared__ uint32_t s_w[kScanBlock / kWave];
+    const uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanItems;
+    uint32_t v[kScanItems];
+    uint32_t tot = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        v[k] = (base + k < n) ? in[base + k] : 0u;
+        tot += v[k];
+    }
+    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+    uint32_t incl = wave_incl_scan(tot, lane);
+    if (lane == kWave - 1)
+        s_w[wid] = incl;
+    __syncthreads();
+    uint32_t woff = 0;
+    for (int w = 0; w < wid; ++w)
+        woff += s_w[w];
+    uint32_t run = woff + incl - tot;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        if (base + k < n)
+            out[base + k] = run;
+        run += v[k];
+    }
+    if (threadIdx.x == kScanBlock - 1)
+        sums[blockIdx.x] = run;
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_scan_add(uint32_t *__restrict__ out,
+                                                         const uint32_t *__restrict__ offs,
+                                                         uint64_t n)
+{
+    const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
+    const uint32_t o = offs[blockIdx.x];
+    for (int k = threadIdx.x; k < kScanTile; k += kScanBlock)
+        if (base + k < n)
+            out[base + k] += o;
+}
+
+__global__ void k_store_total(uint32_t *out, uint64_t n, const uint32_t *sums, uint32_t nblk)
+{
+    // out[n] = exclusive(out)[n-1] + in[n-1] == offs-added sum of the last tile
+    if (threadIdx.x == 0 && blockIdx.x == 0)
+        out[n] = sums[nblk];
+}
+
+size_t scan_workspace_words(uint64_t n)
+{
+    size_t words = 0;
+    uint64_t m = n;
+    do {
+        uint64_t nb = (m + kScanTile - 1) / kScanTile;
+        if (nb == 0)
+            nb = 1;
+        words += nb + 1 + 4;
+        m = nb;
+    } while (m > 1);
+    return words + 16;
+}
+
+// sums buffer of level L holds nblk + 1 entries; its exclusive scan (level
+// L + 1) gives every tile's offset and, at index nblk, the grand total.
+static hipError_t scan_level(const uint32_t *in, uint32_t *out, uint64_t n, uint32_t *ws,
+                             hipStream_t s)
+{
+    const uint64_t nb64 = (n + kScanTile - 1) / kScanTile;
+    const uint32_t nblk = (uint32_t)(nb64 ? nb64 : 1);
+    uint32_t *sums = ws;           // nblk + 1
+    uint32_t *rest = ws + nblk + 4;
+    hipLaunchKernelGGL(k_scan_tiles, dim3(nblk), dim3(kScanBlock), 0, s, in, out, sums, n);
+    if (nblk == 1) {
+        // single tile: the tile total is the grand total
+        hipLaunchKernelGGL(k_store_total, dim3(1), dim3(1), 0, s, out, n, sums, 0u);
+        return hipGetLastError();
+    }
+    // scan the tile totals in place (with a total at index nblk)
+    hipError_t e = scan_level(sums, sums, nblk, rest, s);
+    if (e != hipSuccess)
+        return e;
+    hipLaunchKernelGGL(k_scan_add, dim3(nblk), dim3(kScanBlock), 0, s, out, sums, n);
+    hipLaunchKernelGGL(k_store_total, dim3(1), dim3(1), 0, s, out, n, sums, nblk);
+    return hipGetLastError();
+}
+
+hipError_t scan_exclusive(const uint32_t *in, uint32_t *out, uint64_t n, uint32_t *ws,
+                          hipStream_t s)
+{
+    if (n == 0) {
+        return hipMemsetAsync(out, 0, sizeof(uint32_t), s);
+    }
+    return scan_level(in, out, n, ws, s);
+}
+
+// ---------------------------------------------------------------------------
+// bounding box
+
+__global__ __launch_bounds__(256) void k_bbox(const float *__restrict__ xyz, uint64_t n,
+                                              uint32_t *__restrict__ out6)
+{
+    uint32_t mn[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, mx[3] = {0u, 0u, 0u};
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            uint32_t o = float_to_ord(xyz[3 * i + a]);
+            mn[a] = o < mn[a] ? o : mn[a];
+            mx[a] = o > mx[a] ? o : mx[a];
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        for (int d = kWave / 2; d > 0; d >>= 1) {
+            uint32_t t0 = __shfl_xor(mn[a], d, kWave), t1 = __shfl_xor(mx[a], d, kWave);
+            mn[a] = t0 < mn[a] ? t0 : mn[a];
+            mx[a] = t1 > mx[a] ? t1 : mx[a];
+        }
+    }
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        for (int a = 0; a < 3; ++a) {
+            atomicMin(&out6[a], mn[a]);
+            atomicMax(&out6[3 + a], mx[a]);
+        }
+    }
+}
+
+hipError_t launch_bbox(const float *xyz, uint64_t n, uint32_t *out6, hipStream_t s)
+{
+    uint32_t init[6] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u, 0u};
+    hipError_t e = hipMemcpyAsync(out6, init, sizeof(init), hipMemcpyHostToDevice, s);
+    if (e != hipSuccess)
+        return e;
+    unsigned g = grid_for(n, 256);
+    if (g > 4096)
+        g = 4096;
+    hipLaunchKernelGGL(k_bbox, dim3(g), dim3(256), 0, s, xyz, n, out6);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// cell list
+
+__device__ inline int cell_coord(float v, double lo, double inv, int dim)
+{
+    // FP64: the computed cell differs from the exact floor((v - lo) / w) only
+    // for points within ~1e-15 relative of a cell face, far inside the
+    // stencil margin.  The result is monotone in v, clamped to the grid.
+    double t = ((double)v - lo) * inv;
+    int c = (int)floor(t);
+    c = c < 0 ? 0 : c;
+    return c >= dim ? dim - 1 : c;
+}
+
+__global__ __launch_bounds__(256) void k_cellkeys(const float *__restrict__ xyz, uint64_t n,
+                                                  GridGeom g, uint32_t *__restrict__ count,
+                                                  uint32_t *__restrict__ key,
+                                                  uint32_t *__restrict__ rank)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const int cx = cell_coord(xyz[3 * i], g.lo[0], g.inv_w, g.nx);
+    const int cy = cell_coord(xyz[3 * i + 1], g.lo[1], g.inv_w, g.ny);
+    const int cz = cell_coord(xyz[3 * i + 2], g.lo[2], g.inv_w, g.nz);
+    const uint32_t k = ((uint32_t)cz * (uint32_t)g.ny + (uint32_t)cy) * (uint32_t)g.nx + (uint32_t)cx;
+    key[i] = k;
+    rank[i] = atomicAdd(&count[k], 1u);
+}
+
+hipError_t launch_cellkeys(const float *xyz, uint64_t n, GridGeom g, uint32_t *count,
+                           uint32_t *key, uint32_t *rank, hipStream_t s)
+{
+    if (n == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_cellkeys, dim3(grid_for(n, 256)), dim3(256), 0, s, xyz, n, g, count,
+                       key, rank);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void k_scatter(const float *__restrict__ xyz,
+                                                 const uint8_t *__restrict__ type,
+                                                 const float *__restrict__ s2, uint64_t n,
+                                                 const uint32_t *__restrict__ key,
+                                                 const uint32_t *__restrict__ rank,
+                                                 const uint32_t *__restrict__ cellstart,
+                                                 float4 *__restrict__ pos,
+                                                 float *__restrict__ s2sorted)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const uint32_t d = cellstart[key[i]] + rank[i];
+    pos[d] = make_float4(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2],
+                         __uint_as_float((uint32_t)type[i]));
+    s2sorted[d] = s2[i];
+}
+
+hipError_t launch_scatter(const float *xyz, const uint8_t *type, const float *s2, uint64_t n,
+                          const uint32_t *key, const uint32_t *rank, const uint32_t *cellstart,
+                          float4 *pos, float *s2sorted, hipStream_t s)
+{
+    if (n == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_scatter, dim3(grid_for(n, 256)), dim3(256), 0, s, xyz, type, s2, n, key,
+                       rank, cellstart, pos, s2sorted);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// per-job centers and tiles
+
+__global__ __launch_bounds__(256) void k_flags(const float4 *__restrict__ pos,
+                                               const float *__restrict__ s2sorted, uint64_t n,
+                                               const int32_t *__restrict__ trgslot, float R2,
+                                               uint32_t *__restrict__ flags)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const uint32_t t = __float_as_uint(pos[i].w) & 0xFFu;
+    // considered iff the type is a target of some task and ion2surf >= R^2
+    // (src/PARAPROBE_SolverHdl.cpp:3846-3860)
+    flags[i] = (trgslot[t] >= 0 && s2sorted[i] >= R2) ? 1u : 0u;
+}
+
+hipError_t launch_flags(const float4 *pos, const float *s2sorted, uint64_t n,
+                        const int32_t *trgslot, float R2, uint32_t *flags, hipStream_t s)
+{
+    if (n == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_flags, dim3(grid_for(n, 256)), dim3(256), 0, s, pos, s2sorted, n, trgslot,
+                       R2, flags);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void k_compact(const uint32_t *__restrict__ flags,
+                                                 const uint32_t *__restrict__ cpos, uint64_t n,
+                                                 uint32_t *__restrict__ centers)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && flags[i])
+        centers[cpos[i]] = (uint32_t)i;
+}
+
+hipError_t launch_compact(const uint32_t *flags, const uint32_t *cpos, uint64_t n,
+                          uint32_t *centers, hipStream_t s)
+{
+    if (n == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_compact, dim3(grid_for(n, 256)), dim3(256), 0, s, flags, cpos, n,
+                       centers);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void k_tilecount(const uint32_t *__restrict__ cellstart,
+                                                   const uint32_t *__restrict__ cpos,
+                                                   uint64_t ncells, int T,
+                                                   uint32_t *__restrict__ ntile)
+{
+    const uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncells)
+        return;
+    const uint32_t lo = cpos[cellstart[c]], hi = cpos[cellstart[c + 1]];
+    ntile[c] = (hi - lo + (uint32_t)T - 1u) / (uint32_t)T;
+}
+
+hipError_t launch_tilecount(const uint32_t *cellstart, const uint32_t *cpos, uint64_t ncells,
+                            int T, uint32_t *ntile, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_tilecount, dim3(grid_for(ncells, 256)), dim3(256), 0, s, cellstart, cpos,
+                       ncells, T, ntile);
+    return hipGetLastError();
+}
+
+__device__ inline void row_range(const Row &r, int cx, int cy, int cz, const GridGeom &g,
+                                 const uint32_t *__restrict__ cellstart, uint32_t &rs, uint32_t &re)
+{
+    const int y = cy + r.dy, z = cz + r.dz;
+    if (y < 0 || y >= g.ny || z < 0 || z >= g.nz) {
+        rs = re = 0;
+        return;
+    }
+    const int x0 = cx - r.dxm < 0 ? 0 : cx - r.dxm;
+    const int x1 = cx + r.dxm >= g.nx ? g.nx - 1 : cx + r.dxm;
+    const uint32_t k0 = ((uint32_t)z * (uint32_t)g.ny + (uint32_t)y) * (uint32_t)g.nx + (uint32_t)x0;
+    rs = cellstart[k0];
+    re = cellstart[k0 + (uint32_t)(x1 - x0) + 1u];
+}
+
+__global__ __launch_bounds__(256) void k_tilewrite(const uint32_t *__restrict__ cellstart,
+                                                   const uint32_t *__restrict__ cpos,
+                                                   const uint32_t *__restrict__ toff,
+                                                   uint64_t ncells, int T, int job, GridGeom g,
+                                                   const Row *__restrict__ rows, int nrows,
+                                                   Tile *__restrict__ tiles,
+                                                   uint32_t *__restrict__ maxcand)
+{
+    const uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncells)
+        return;
+    const uint32_t lo = cpos[cellstart[c]], hi = cpos[cellstart[c + 1]];
+    if (hi == lo)
+        return;
+    const uint32_t cnt = hi - lo;
+    const uint32_t t0 = toff[c];
+    for (uint32_t m = 0; m * (uint32_t)T < cnt; ++m) {
+        Tile t;
+        t.key = (uint32_t)c;
+        t.first = lo + m * (uint32_t)T;
+        const uint32_t left = cnt - m * (uint32_t)T;
+        t.count = (uint16_t)(left < (uint32_t)T ? left : (uint32_t)T);
+        t.job = (uint16_t)job;
+        t.pad = 0;
+        tiles[t0 + m] = t;
+    }
+    // candidates of this cell's stencil bound every center's neighbour count
+    const int cx = (int)(c % (uint64_t)g.nx);
+    const int cy = (int)((c / (uint64_t)g.nx) % (uint64_t)g.ny);
+    const int cz = (int)(c / ((uint64_t)g.nx * (uint64_t)g.ny));
+    uint32_t tot = 0;
+    for (int r = 0; r < nrows; ++r) {
+        uint32_t rs, re;
+        row_range(rows[r], cx, cy, cz, g, cellstart, rs, re);
+        tot += re - rs;
+    }
+    atomicMax(maxcand, tot);
+}
+
+hipError_t launch_tilewrite(const uint32_t *cellstart, const uint32_t *cpos, const uint32_t *toff,
+                            uint64_t ncells, int T, int job, GridGeom g, const Row *rows, int nrows,
+                            Tile *tiles, uint32_t *maxcand, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_tilewrite, dim3(grid_for(ncells, 256)), dim3(256), 0, s, cellstart, cpos,
+                       toff, ncells, T, job, g, rows, nrows, tiles, maxcand);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// query kernel
+
+struct Lay {
+    int edges, envoff, trgslot, kidx, rows, rowchunk, cen, meta, P, H, total;
+};
+
+__host__ __device__ inline int al4(int x) { return (x + 3) & ~3; }
+
+__host__ __device__ inline Lay make_layout(int T, int c16, int pc, int maxC, int maxE, int maxTrg,
+                                           int maxKmax, int maxRows)
+{
+    Lay L;
+    int o = 0;
+    L.edges = o;
+    o += al4(maxC);
+    L.envoff = o;
+    o += 256;
+    L.trgslot = o;
+    o += 256;
+    L.kidx = o;
+    o += pc ? al4(maxKmax + 2) : 0;
+    L.rows = o;
+    o += al4(2 * maxRows);
+    L.rowchunk = o;
+    o += al4(maxRows + 1);
+    L.cen = o;
+    o += 4 * T;
+    L.meta = o;
+    o += al4(2 * T);
+    L.P = o;
+    o += al4(maxTrg * maxE * maxC);
+    L.H = o;
+    o += pc ? al4((c16 ? (T + 1) / 2 : T) * maxE * maxC) : 0;
+    L.total = o;
+    return L;
+}
+
+size_t query_lds_bytes(int T, int c16, int percenter, int maxC, int maxE, int maxTrg, int maxKmax,
+                       int maxRows)
+{
+    return (size_t)make_layout(T, c16, percenter, maxC, maxE, maxTrg, maxKmax, maxRows).total * 4u;
+}
+
+struct QArgs {
+    const JobDev *jobs;
+    const uint32_t *tile_off;
+    const Tile *const *tiles;
+    uint32_t *queue;
+    uint32_t total_tiles;
+    int njobs;
+    int maxC, maxE, maxTrg, maxKmax, maxRows;
+};
+
+__device__ inline int find_job(const uint32_t *__restrict__ off, int njobs, uint32_t g)
+{
+    int lo = 0, hi = njobs;  // off[lo] <= g < off[hi]
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (off[mid] <= g)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+__device__ inline uint32_t rfl(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
+typedef float fvec4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) const fvec4 gfvec4;
+
+// 16-byte load through the global address space (global_load_dwordx4
+// rather than a flat load)
+__device__ inline float4 load_g4(const gfvec4 *p, uint32_t i)
+{
+    const fvec4 v = p[i];
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+
+template <int T, bool C16, bool PC>
+__global__ __launch_bounds__(kBlock) void k_query(QArgs a)
+{
+    extern __shared__ uint32_t lds[];
+    __shared__ uint32_t s_g, s_ctot;
+    const Lay L = make_layout(T, C16, PC, a.maxC, a.maxE, a.maxTrg, a.maxKmax, a.maxRows);
+    uint32_t *s_edges = lds + L.edges;
+    int32_t *s_envoff = (int32_t *)(lds + L.envoff);
+    int32_t *s_trgslot = (int32_t *)(lds + L.trgslot);
+    int32_t *s_kidx = (int32_t *)(lds + L.kidx);
+    uint32_t *s_rows = lds + L.rows;
+    uint32_t *s_rowchunk = lds + L.rowchunk;
+    float4 *s_cen = (float4 *)(lds + L.cen);
+    uint32_t *s_meta = lds + L.meta;  // [2c] tc | es << 8, [2c+1] unused
+    uint32_t *s_P = lds + L.P;
+    uint32_t *s_H = lds + L.H;
+
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
+    int cur = -1;
+    unsigned long long inball = 0;   // wave-uniform
+    unsigned long long cand = 0;     // block-uniform
+    unsigned long long pending = 0;  // pairs accumulated in s_P since its last flush
+
+    for (;;) {
+        __syncthreads();
+        if (tid == 0)
+            s_g = atomicAdd(a.queue, 1u);
+        __syncthreads();
+        const uint32_t g = rfl(s_g);
+        const bool done = g >= a.total_tiles;
+        const int job = done ? -1 : (int)rfl((uint32_t)find_job(a.tile_off, a.njobs, g));
+
+        if (job != cur || (!done && pending > 0x7FFFFFFFull)) {
+            if (cur >= 0) {
+                // flush the accumulated pair counts and statistics of the job
+                const JobDev &P0 = a.jobs[cur];
+                const int n = P0.ntrg * P0.E * P0.C;
+                if (P0.families & (FAM_RDF | FAM_RIPK)) {
+                    for (int i = tid; i < n; i += kBlock) {
+                        const uint32_t v = s_P[i];
+                        if (v)
+                            atomicAdd(&P0.P[i], (unsigned long long)v);
+                    }
+                }
+                if (lane == 0 && inball)
+                    atomicAdd(&P0.stats[0], inball);
+                if (tid == 0 && cand)
+                    atomicAdd(&P0.stats[1], cand);
+                inball = 0;
+                cand = 0;
+                pending = 0;
+                __syncthreads();
+            }
+            if (done)
+                break;
+            const JobDev &J = a.jobs[job];
+            if (job != cur) {
+                for (int i = tid; i < J.C; i += kBlock)
+                    s_edges[i] = J.edges[i];
+                for (int i = tid; i < 256; i += kBlock) {
+                    s_envoff[i] = J.envoff[i];
+                    s_trgslot[i] = J.trgslot[i];
+                }
+                if (PC)
+                    for (int i = tid; i <= J.kmax; i += kBlock)
+                        s_kidx[i] = J.kidx_of_k ? J.kidx_of_k[i] : -1;
+            }
+            const int n = J.ntrg * J.E * J.C;
+            for (int i = tid; i < n; i += kBlock)
+                s_P[i] = 0u;
+            cur = job;
+            __syncthreads();
+        }
+        if (done)
+            break;
+
+        const JobDev &J = a.jobs[job];
+        const Tile t = a.tiles[job][g - a.tile_off[job]];
+        const int cnt = (int)rfl(t.count);
+        const uint32_t key = rfl(t.key);
+        const int C = J.C, E = J.E, EC = E * C;
+        const int nrows = J.nrows;
+        const int cx = (int)(key % (uint32_t)J.nx);
+        const int cy = (int)((key / (uint32_t)J.nx) % (uint32_t)J.ny);
+        const int cz = (int)(key / ((uint32_t)J.nx * (uint32_t)J.ny));
+
+        if (tid < cnt) {
+            const uint32_t i = J.centers[t.first + tid];
+            const float4 p = J.pos[i];
+            s_cen[tid] = p;
+            const uint32_t ty = __float_as_uint(p.w) & 0xFFu;
+            const int eo = s_envoff[ty];
+            const uint32_t es = eo >= 0 ? (uint32_t)(eo / C) : 0xFFu;
+            s_meta[2 * tid] = ((uint32_t)s_trgslot[ty] & 0xFFu) | (es << 8);
+        }
+        for (int r = tid; r < nrows; r += kBlock) {
+            const Row rw = J.rows[r];
+            uint32_t rs = 0, re = 0;
+            const int y = cy + rw.dy, z = cz + rw.dz;
+            if (y >= 0 && y < J.ny && z >= 0 && z < J.nz) {
+                const int x0 = cx - rw.dxm < 0 ? 0 : cx - rw.dxm;
+                const int x1 = cx + rw.dxm >= J.nx ? J.nx - 1 : cx + rw.dxm;
+                const uint32_t k0 =
+                    ((uint32_t)z * (uint32_t)J.ny + (uint32_t)y) * (uint32_t)J.nx + (uint32_t)x0;
+                rs = J.cellstart[k0];
+                re = J.cellstart[k0 + (uint32_t)(x1 - x0) + 1u];
+            }
+            s_rows[2 * r] = rs;
+            s_rows[2 * r + 1] = re;
+        }
+        if (PC) {
+            const int nh = (C16 ? (cnt + 1) / 2 : cnt) * EC;
+            for (int i = tid; i < nh; i += kBlock)
+                s_H[i] = 0u;
+        }
+        __syncthreads();
+        if (wid == 0) {
+            // chunk prefix over rows (64 candidates per chunk)
+            uint32_t carry = 0, ctot = 0;
+            for (int r0 = 0; r0 < nrows; r0 += kWave) {
+                const int r = r0 + lane;
+                uint32_t len = 0;
+                if (r < nrows)
+                    len = s_rows[2 * r + 1] - s_rows[2 * r];
+                const uint32_t ch = (len + kWave - 1) / kWave;
+                const uint32_t inc = wave_incl_scan(ch, lane);
+                const uint32_t linc = wave_incl_scan(len, lane);
+                if (r < nrows)
+                    s_rowchunk[r] = carry + inc - ch;
+                carry += __shfl(inc, kWave - 1, kWave);
+                ctot += __shfl(linc, kWave - 1, kWave);
+            }
+            if (lane == 0) {
+                s_rowchunk[nrows] = carry;
+                s_ctot = ctot;
+            }
+        }
+        __syncthreads();
+        {
+            const unsigned long long pairs =
+                (unsigned long long)rfl(s_ctot) * (unsigned long long)cnt;
+            pending += pairs;
+            if (tid == 0)
+                cand += pairs;
+        }
+
+        // ---------------- main candidate loop ----------------
+        {
+            float cxr[T], cyr[T], czr[T];
+            int hb[T];
+#pragma unroll
+            for (int c = 0; c < T; ++c) {
+                const float4 p = s_cen[c < cnt ? c : 0];
+                cxr[c] = p.x;
+                cyr[c] = p.y;
+                czr[c] = p.z;
+                if (PC)
+                    hb[c] = (C16 ? (c >> 1) : c) * EC;
+                else
+                    hb[c] = (int)(s_meta[2 * (c < cnt ? c : 0)] & 0xFFu) * EC;
+            }
+            const float R2 = J.R2;
+            const float isc = J.idx_scale, ioff = J.idx_off;
+            const float Cm1f = (float)(C - 1);
+            const gfvec4 *__restrict__ pos = (const gfvec4 *)J.pos;
+            uint32_t *s_acc = PC ? s_H : s_P;
+            const int nch = (int)rfl(s_rowchunk[nrows]);
+            unsigned long long my_in = 0;
+            // software pipeline: the loads of chunk q + kWaves are in flight
+            // while chunk q is evaluated against the T centers
+            int r = 0;
+            auto locate = [&](int q, uint32_t &jj, uint32_t &rs) -> bool {
+                while ((int)rfl(s_rowchunk[r + 1]) <= q)
+                    ++r;
+                rs = rfl(s_rows[2 * r]);
+                const uint32_t re = rfl(s_rows[2 * r + 1]);
+                jj = rs + (uint32_t)(q - (int)rfl(s_rowchunk[r])) * kWave + lane;
+                return jj < re;
+            };
+            float4 pn = make_float4(0.f, 0.f, 0.f, 0.f);
+            bool okn = false;
+            if (wid < nch) {
+                uint32_t jj, rs;
+                okn = locate(wid, jj, rs);
+                pn = load_g4(pos, okn ? jj : rs);
+            }
+            for (int q = wid; q < nch; q += kWaves) {
+                float4 p = pn;
+                // keep every use of this chunk's data below this point, so the
+                // wait for its load is not hoisted into the prefetch of the next
+                asm volatile("" : "+v"(p.x), "+v"(p.y), "+v"(p.z), "+v"(p.w));
+                const bool ok = okn;
+                const int qn = q + kWaves;
+                if (qn < nch) {
+                    uint32_t jj, rs;
+                    okn = locate(qn, jj, rs);
+                    pn = load_g4(pos, okn ? jj : rs);
+                }
+                const float px = ok ? p.x : __builtin_nanf("");
+                const int woff = ok ? s_envoff[__float_as_uint(p.w) & 0xFFu] : -1;
+                const bool envok = woff >= 0;
+#pragma unroll
+                for (int c = 0; c < T; ++c) {
+                    if (c < cnt) {
+                        const float dx = cxr[c] - px;
+                        const float dy = cyr[c] - p.y;
+                        const float dz = czr[c] - p.z;
+                        const float d2 = dx * dx + dy * dy + dz * dz;
+                        const bool in = d2 <= R2;
+                        my_in += (unsigned long long)__builtin_popcountll(
+                            __builtin_amdgcn_ballot_w64(in));
+                        if (in && envok) {
+                            // nearest ideal edge from an approximate sqrt, then
+                            // one exact compare against the edge's d2 bits
+                            const float sq = __builtin_amdgcn_sqrtf(d2);
+                            const float fq =
+                                __builtin_amdgcn_fmed3f(__builtin_fmaf(sq, isc, ioff), 1.0f, Cm1f);
+                            const int qi = (int)fq;
+                            const int cat = qi - (__float_as_uint(d2) < s_edges[qi] ? 1 : 0);
+                            const uint32_t inc = (C16 && (c & 1)) ? 65536u : 1u;
+                            __hip_atomic_fetch_add(&s_acc[hb[c] + woff + cat], inc,
+                                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                        }
+                    }
+                }
+            }
+            inball += my_in;
+        }
+        __syncthreads();
+
+        if (!PC) {
+            // remove the self pair (d = 0) each center added to its own type
+            if (tid < cnt) {
+                const uint32_t m = s_meta[2 * tid];
+                const uint32_t es = (m >> 8) & 0xFFu;
+                if (es != 0xFFu)
+                    __hip_atomic_fetch_add(&s_P[(m & 0xFFu) * EC + es * C + J.cat0], 0xFFFFFFFFu,
+                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
+            continue;
+        }
+
+        // ---------------- per-center extraction ----------------
+        const int cat0 = J.cat0;
+        if (J.families & (FAM_RDF | FAM_RIPK)) {
+            // pair counts per (center type, env type, category); each thread
+            // owns distinct (env, category) entries, so no atomics are needed
+            for (int idx = tid; idx < EC; idx += kBlock) {
+                const int e = idx / C;
+                const int cat = idx - e * C;
+                for (int c = 0; c < cnt; ++c) {
+                    const uint32_t w = s_H[(C16 ? (c >> 1) : c) * EC + idx];
+                    uint32_t v = C16 ? ((c & 1) ? (w >> 16) : (w & 0xFFFFu)) : w;
+                    const uint32_t m = s_meta[2 * c];
+                    if ((int)((m >> 8) & 0xFFu) == e && cat == cat0)
+                        v -= 1u;
+                    s_P[(m & 0xFFu) * EC + idx] += v;
+                }
+            }
+        }
+        const bool want_k = (J.families & FAM_MKNN) && J.nk > 0;
+        const bool want_1 = (J.families & FAM_1NN) != 0;
+        const bool want_cn = (J.families & FAM_CNTNB) != 0;
+        if (want_k || want_1 || want_cn) {
+            const int kmax = want_k ? J.kmax : 0;
+            for (int c = wid; c < cnt; c += kWaves) {
+                const uint32_t m = s_meta[2 * c];
+                const uint32_t tc = m & 0xFFu;
+                const int es = (int)((m >> 8) & 0xFFu);
+                const int hoff = (C16 ? (c >> 1) : c) * EC;
+                const int sh = (C16 && (c & 1)) ? 16 : 0;
+                for (int tk = 0; tk < J.ntasks; ++tk) {
+                    if (!((J.task_trg[tk] >> tc) & 1u))
+                        continue;
+                    const uint32_t envset = J.task_env[tk];
+                    if (want_k || want_1) {
+                        uint32_t carry = 0;
+                        const uint32_t need = want_k ? (uint32_t)kmax : 1u;
+                        for (int cb = 0; cb < C; cb += kWave) {
+                            const int cat = cb + lane;
+                            uint32_t v = 0;
+                            if (cat < C) {
+                                for (int e = 0; e < E; ++e) {
+                                    if (!((envset >> e) & 1u))
+                                        continue;
+                                    uint32_t w = s_H[hoff + e * C + cat];
+                                    w = C16 ? ((w >> sh) & 0xFFFFu) : w;
+                                    if (e == es && cat == cat0)
+                                        w -= 1u;
+                                    v += w;
+                                }
+                            }
+                            const uint32_t incl = wave_incl_scan(v, lane);
+                            const uint32_t cum = carry + incl, prev = cum - v;
+                            if (want_1 && prev < 1u && cum >= 1u)
+                                atomicAdd(&J.N1[tk * C + cat], 1u);
+                            if (want_k) {
+                                const uint32_t hi = cum < (uint32_t)kmax ? cum : (uint32_t)kmax;
+                                for (uint32_t k = prev + 1u; k <= hi; ++k) {
+                                    const int kid = s_kidx[k];
+                                    if (kid >= 0)
+                                        atomicAdd(&J.K[((size_t)tk * J.nk + kid) * C + cat], 1u);
+                                }
+                            }
+                            carry += __shfl(incl, kWave - 1, kWave);
+                            if (carry >= need)
+                                break;
+                        }
+                        // fewer than k env neighbours: R + EPSILON, i.e. the
+                        // sentinel category (S.cpp:4093-4095, 4035-4036)
+                        if (want_1 && carry == 0u && lane == 0)
+                            atomicAdd(&J.N1[tk * C + J.cat_sentinel], 1u);
+                        if (want_k) {
+                            for (int k = (int)carry + 1 + lane; k <= kmax; k += kWave) {
+                                const int kid = s_kidx[k];
+                                if (kid >= 0)
+                                    atomicAdd(&J.K[((size_t)tk * J.nk + kid) * C + J.cat_sentinel],
+                                              1u);
+                            }
+                        }
+                    }
+                    if (want_cn) {
+                        for (int e = 0; e < E; ++e) {
+                            if (!((envset >> e) & 1u))
+                                continue;
+                            uint32_t tot = 0;
+                            for (int cat = lane; cat < C; cat += kWave) {
+                                uint32_t w = s_H[hoff + e * C + cat];
+                                w = C16 ? ((w >> sh) & 0xFFFFu) : w;
+                                if (e == es && cat == cat0)
+                                    w -= 1u;
+                                tot += w;
+                            }
+                            for (int d = kWave / 2; d > 0; d >>= 1)
+                                tot += __shfl_xor(tot, d, kWave);
+                            if (lane == 0) {
+                                if ((int)tot < J.cn_cap)
+                                    atomicAdd(&J.CN[(size_t)tk * J.cn_cap + tot], 1u);
+                                else
+                                    atomicMax((unsigned int *)&J.stats[2], 1u);
+                            }
+                        }
+                    }
+                }
+            }
+        }
+    }
+}
+
+template <int T, bool C16, bool PC>
+static hipError_t launch_query_t(const QueryLaunch &q, hipStream_t s)
+{
+    const size_t lds = query_lds_bytes(T, C16, PC, q.maxC, q.maxE, q.maxTrg, q.maxKmax, q.maxRows);
+    QArgs a;
+    a.jobs = q.jobs;
+    a.tile_off = q.tile_off;
+    a.tiles = q.tiles;
+    a.queue = q.queue;
+    a.total_tiles = q.total_tiles;
+    a.njobs = q.njobs;
+    a.maxC = q.maxC;
+    a.maxE = q.maxE;
+    a.maxTrg = q.maxTrg;
+    a.maxKmax = q.maxKmax;
+    a.maxRows = q.maxRows;
+    hipError_t e = hipFuncSetAttribute((const void *)k_query<T, C16, PC>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess)
+        return e;
+    hipLaunchKernelGGL((k_query<T, C16, PC>), dim3(q.grid), dim3(kBlock), lds, s, a);
+    return hipGetLastError();
+}
+
+template <int T, bool C16, bool PC>
+static hipError_t occ_t(size_t lds, int *nb)
+{
+    hipError_t e = hipFuncSetAttribute((const void *)k_query<T, C16, PC>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess)
+        return e;
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(nb, k_query<T, C16, PC>, kBlock, lds);
+}
+
+#define PP_DISPATCH(FN, ...)                                                 \
+    do {                                                                     \
+        if (!percenter) {                                                    \
+            if (T == 8) return FN<8, false, false>(__VA_ARGS__);             \
+            if (T == 16) return FN<16, false, false>(__VA_ARGS__);           \
+            if (T == 32) return FN<32, false, false>(__VA_ARGS__);           \
+        } else if (c16) {                                                    \
+            if (T == 8) return FN<8, true, true>(__VA_ARGS__);               \
+            if (T == 16) return FN<16, true, true>(__VA_ARGS__);             \
+            if (T == 32) return FN<32, true, true>(__VA_ARGS__);             \
+        } else {                                                             \
+            if (T == 8) return FN<8, false, true>(__VA_ARGS__);              \
+            if (T == 16) return FN<16, false, true>(__VA_ARGS__);            \
+            if (T == 32) return FN<32, false, true>(__VA_ARGS__);            \
+        }                                                                    \
+        return hipErrorInvalidValue;                                         \
+    } while (0)
+
+hipError_t query_max_blocks_per_cu(int T, int c16, int percenter, size_t lds, int *nb)
+{
+    PP_DISPATCH(occ_t, lds, nb);
+}
+
+hipError_t launch_query(const QueryLaunch &q, hipStream_t s)
+{
+    const int T = q.T, c16 = q.c16, percenter = q.percenter;
+    hipError_t e = hipMemsetAsync(q.queue, 0, sizeof(uint32_t) * (kXcds + 1), s);
+    if (e != hipSuccess)
+        return e;
+    if (q.total_tiles == 0)
+        return hipSuccess;
+    PP_DISPATCH(launch_query_t, q, s);
+}
+
+} // namespace pp

@@ -1,0 +1,66 @@
+// pp_launch.h -- host-side launchers for the gfx950 kernels of the spatial
+// statistics path (implemented in pp_kernels.hip).
+#pragma once
+#include "pp_internal.h"
+
+namespace pp {
+
+struct GridGeom {
+    double lo[3];
+    double inv_w;
+    double w;
+    int nx, ny, nz;
+    uint64_t ncells() const { return (uint64_t)nx * ny * nz; }
+};
+
+// Exclusive scan of n uint32 values; out has n + 1 entries (out[n] = total).
+// ws must hold scan_workspace_words(n) uint32.
+size_t scan_workspace_words(uint64_t n);
+hipError_t scan_exclusive(const uint32_t *in, uint32_t *out, uint64_t n, uint32_t *ws,
+                          hipStream_t s);
+
+// Ordered-uint min/max of an AoS xyz array: out[0..2] min, out[3..5] max
+// (order-preserving encodings of float; decode with ord_to_float).
+hipError_t launch_bbox(const float *xyz, uint64_t n, uint32_t *out6, hipStream_t s);
+float ord_to_float(uint32_t u);
+
+// Cell-list build: key/rank per ion, then scatter into cell-sorted float4
+// (x, y, z, type) and the sorted ion2surf array.
+hipError_t launch_cellkeys(const float *xyz, uint64_t n, GridGeom g, uint32_t *count,
+                           uint32_t *key, uint32_t *rank, hipStream_t s);
+hipError_t launch_scatter(const float *xyz, const uint8_t *type, const float *s2, uint64_t n,
+                          const uint32_t *key, const uint32_t *rank, const uint32_t *cellstart,
+                          float4 *pos, float *s2sorted, hipStream_t s);
+
+// Per job: flag considered centers, compact them, count and write tiles.
+hipError_t launch_flags(const float4 *pos, const float *s2sorted, uint64_t n,
+                        const int32_t *trgslot, float R2, uint32_t *flags, hipStream_t s);
+hipError_t launch_compact(const uint32_t *flags, const uint32_t *cpos, uint64_t n,
+                          uint32_t *centers, hipStream_t s);
+hipError_t launch_tilecount(const uint32_t *cellstart, const uint32_t *cpos, uint64_t ncells,
+                            int T, uint32_t *ntile, hipStream_t s);
+hipError_t launch_tilewrite(const uint32_t *cellstart, const uint32_t *cpos,
+                            const uint32_t *toff, uint64_t ncells, int T, int job,
+                            GridGeom g, const Row *rows, int nrows, Tile *tiles,
+                            uint32_t *maxcand, hipStream_t s);
+
+// Query launch over all jobs of a launch.
+struct QueryLaunch {
+    const JobDev *jobs;        // device array
+    int njobs;
+    const uint32_t *tile_off;  // device [njobs + 1] prefix of per-job tile counts
+    const Tile *const *tiles;  // device [njobs] per-job tile arrays
+    uint32_t total_tiles;
+    uint32_t *queue;           // device counters (kXcds + 1), zeroed by the launcher
+    int T;                     // centers per tile
+    int c16;                   // 16-bit per-center counters
+    int percenter;             // per-center histograms (MKNN / 1NN / CNTNB)
+    int maxC, maxE, maxTrg, maxKmax, maxRows;
+    int grid;                  // workgroups (persistent)
+};
+size_t query_lds_bytes(int T, int c16, int percenter, int maxC, int maxE, int maxTrg,
+                       int maxKmax, int maxRows);
+hipError_t query_max_blocks_per_cu(int T, int c16, int percenter, size_t lds, int *nb);
+hipError_t launch_query(const QueryLaunch &q, hipStream_t s);
+
+} // namespace pp

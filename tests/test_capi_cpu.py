@@ -1,0 +1,54 @@
+"""C-ABI library checks that need no GPU: it loads, it exports every symbol
+declared in include/*.h, and its host-side entry points behave."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+from conftest import ROOT, gpu_available
+
+import paraprobe_amd.capi as capi
+
+
+def _declared_functions():
+    names = []
+    inc = os.path.join(ROOT, "include")
+    for f in sorted(os.listdir(inc)):
+        if not f.endswith(".h"):
+            continue
+        text = open(os.path.join(inc, f)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names += re.findall(r"^\s*(?:const\s+)?[A-Za-z_][\w\s\*]*?\b(pp[h]?_\w+)\s*\(", text, flags=re.M)
+    return sorted(set(names))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = capi.load_library()
+    declared = [n for n in _declared_functions() if n.startswith("pp_")]
+    assert declared, "no declarations found"
+    for name in declared:
+        assert hasattr(lib, name), name
+    assert set(capi.EXPORTED) == set(declared)
+
+
+def test_hist_geometry_matches_reference_quirks():
+    assert capi.hist_geometry(0.0, 0.1, 2.0) == 19
+    assert capi.hist_geometry(0.0, 0.01, 5.0) == 500
+    assert capi.hist_geometry(0.0, 0.005, 3.0) == 600
+    with pytest.raises(capi.PPError):
+        capi.hist_geometry(0.0, 1e-7, 2.0)
+
+
+@pytest.mark.skipif(gpu_available(), reason="checks the no-device path")
+def test_create_without_device_fails_cleanly():
+    lib = capi.load_library()
+    h = C.c_void_p()
+    st = lib.pp_spatstat_create(0, C.byref(h))
+    assert st == 5  # PP_ERR_NO_DEVICE
+    assert not h.value
+
+
+def test_type_mask():
+    m = capi.type_mask([0, 1, 9, 255])
+    assert m[0] == 0b11 and m[1] == 0b10 and m[31] == 0x80

@@ -1,0 +1,223 @@
+"""GPU parity: the gfx950 path through the C ABI against the CPU oracle.
+
+Integer histograms must be bit-identical (the contract is the reference's FP32
+distance expression with fp-contract off and histogram::add binning).
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from paraprobe_amd import FAM_1NN, FAM_CNTNB, FAM_MKNN, FAM_RDF, FAM_RIPK, Job
+from paraprobe_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+ALL = FAM_RDF | FAM_1NN | FAM_RIPK | FAM_MKNN | FAM_CNTNB
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def _cube(n, seed, L=6.0, p=(0.6, 0.3, 0.1)):
+    rng = np.random.default_rng(seed)
+    xyz = rng.uniform(0, L, size=(n, 3)).astype(np.float32)
+    ty = rng.choice([1, 2, 3], size=n, p=list(p)).astype(np.uint8)
+    s2 = rng.uniform(0, 3, size=n).astype(np.float32)
+    return xyz, ty, s2
+
+
+def _compare(ctx, xyz, ty, s2, job: Job, cloud=None, cntnb_cap=65536):
+    if cloud is None:
+        cloud = ctx.upload(xyz, ty, s2)
+        own = True
+    else:
+        own = False
+    try:
+        g = ctx.run(cloud, job, cntnb_cap=cntnb_cap)
+    finally:
+        if own:
+            cloud.free()
+    o, st = O.spatstat(xyz, ty, s2, job.tasks, job.rmin, job.rincr, job.rmax, job.families,
+                       kvalues=job.kvalues, cntnb_cap=cntnb_cap)
+    for name, ref in o.items():
+        got = getattr(g, name)
+        assert got is not None, name
+        if not np.array_equal(got, ref):
+            bad = np.argwhere(got != ref)[:8]
+            raise AssertionError(f"{name} differs at {bad.tolist()}: gpu "
+                                 f"{got[tuple(bad.T)].tolist()} oracle {ref[tuple(bad.T)].tolist()}")
+    assert g.centers_considered == st["considered"]
+    assert g.ions_discarded == st["discarded"]
+    assert g.pairs_inball == st["inball"]
+    return g, o, st
+
+
+TASKS_MIX = [((1,), (1,)), ((2,), (1, 3)), ((1, 2), (2,)), ((3,), (1, 2, 3))]
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_small_all_families(ctx, seed):
+    _say(f"test_small_all_families seed={seed}")
+    xyz, ty, s2 = _cube(3000, seed)
+    job = Job(TASKS_MIX, 0.0, 0.05, 1.2, ALL, kvalues=[1, 2, 5, 17])
+    _compare(ctx, xyz, ty, s2, job)
+
+
+@pytest.mark.parametrize("T,r", [(8, 1), (16, 2), (32, 3), (16, 4), (8, 2), (32, 1)])
+def test_tuning_does_not_change_results(ctx, T, r):
+    _say(f"test_tuning T={T} cells_per_rmax={r}")
+    xyz, ty, s2 = _cube(4000, 11)
+    ctx.set_tuning(T, r)
+    try:
+        job = Job(TASKS_MIX, 0.0, 0.02, 1.0, ALL, kvalues=[1, 3, 9, 30])
+        _compare(ctx, xyz, ty, s2, job)
+        job2 = Job(TASKS_MIX, 0.0, 0.02, 1.0, FAM_RDF)
+        _compare(ctx, xyz, ty, s2, job2)
+    finally:
+        ctx.set_tuning(0, 0)
+
+
+def test_rmin_positive_and_truncated_bins(ctx):
+    _say("test_rmin_positive_and_truncated_bins")
+    xyz, ty, s2 = _cube(3000, 3)
+    # 2.0/0.1 -> 19 bins of 2/19; start 0.3 gives a populated lowest category
+    _compare(ctx, xyz, ty, s2, Job(TASKS_MIX, 0.0, 0.1, 2.0, ALL, kvalues=[1, 4]))
+    _compare(ctx, xyz, ty, s2, Job(TASKS_MIX, 0.3, 0.01, 1.5, ALL, kvalues=[2, 6]))
+
+
+def test_duplicates_and_sentinels(ctx):
+    _say("test_duplicates_and_sentinels")
+    rng = np.random.default_rng(4)
+    base = rng.uniform(0, 3, size=(200, 3)).astype(np.float32)
+    xyz = np.concatenate([base, base, base[:50]])  # coincident points: d = 0 pairs
+    ty = rng.choice([1, 2], size=xyz.shape[0]).astype(np.uint8)
+    s2 = np.full(xyz.shape[0], 100.0, np.float32)
+    job = Job([((1,), (1,)), ((2,), (1, 2))], 0.0, 0.01, 0.6, ALL, kvalues=[1, 2, 3, 50, 200])
+    _compare(ctx, xyz, ty, s2, job)
+
+
+def test_constructed_fp32_cases(ctx):
+    _say("test_constructed_fp32_cases")
+    t = 2.0 ** -12
+    # summation order and no-FMA cases (see the oracle pins)
+    for nb, R in (((1.0, t, t), 1.0), ((1.0 + t, t, 0.0), 1.0 + t)):
+        xyz = np.array([(0, 0, 0), nb], np.float32)
+        ty = np.array([1, 1], np.uint8)
+        s2 = np.array([1e6, 0.0], np.float32)
+        g, o, st = _compare(ctx, xyz, ty, s2, Job([((1,), (1,))], 0.0, 0.5, R, FAM_RDF))
+        assert st["inball"] == 1 and g.pairs_inball == 1
+
+
+def test_fcc_lattice_closed_form(ctx):
+    _say("test_fcc_lattice_closed_form")
+    a = 0.404
+    xyz = synth.lattice(True, a, 9, 9, 9)
+    L = a * 9
+    d = np.minimum(xyz, (L - a / 2) - xyz).min(axis=1).astype(np.float64)
+    s2 = (np.maximum(d, 0) ** 2).astype(np.float32)
+    ty = np.ones(xyz.shape[0], np.uint8)
+    g, o, st = _compare(ctx, xyz, ty, s2, Job([((1,), (1,))], 0.0, 0.01, 0.65, ALL,
+                                               kvalues=list(range(1, 90))))
+    assert g.pairs_inball == 78 * g.centers_considered
+
+
+def test_edge_cases(ctx):
+    _say("test_edge_cases")
+    # empty cloud
+    e = np.zeros((0, 3), np.float32)
+    g, _, _ = _compare(ctx, e, np.zeros(0, np.uint8), np.zeros(0, np.float32),
+                       Job([((1,), (1,))], 0.0, 0.1, 1.0, ALL, kvalues=[1]))
+    assert g.centers_considered == 0
+    # one ion
+    one = np.array([[1, 2, 3]], np.float32)
+    _compare(ctx, one, np.array([1], np.uint8), np.array([10], np.float32),
+             Job([((1,), (1,))], 0.0, 0.1, 1.0, ALL, kvalues=[1, 2]))
+    # every center excluded
+    xyz, ty, s2 = _cube(2000, 5)
+    g, _, _ = _compare(ctx, xyz, ty, np.zeros_like(s2), Job(TASKS_MIX, 0.0, 0.05, 1.0, ALL,
+                                                             kvalues=[1]))
+    assert g.centers_considered == 0
+    # a type with no ions at all, and a flat (2-D) cloud
+    xyz2 = xyz.copy()
+    xyz2[:, 2] = 1.0
+    _compare(ctx, xyz2, ty, s2, Job([((7,), (1,)), ((1,), (7,)), ((2,), (1, 2, 3))], 0.0, 0.05,
+                                    0.8, ALL, kvalues=[1, 2]))
+
+
+def test_invalid_jobs_raise(ctx):
+    from paraprobe_amd import PPError
+
+    xyz, ty, s2 = _cube(100, 6)
+    cl = ctx.upload(xyz, ty, s2)
+    try:
+        with pytest.raises(PPError):
+            ctx.run(cl, Job([((1,), (1,))], 0.0, 1e-7, 1.0, FAM_RDF))
+        with pytest.raises(PPError):
+            ctx.run(cl, Job([((1,), (1,))], 1.0, 0.1, 0.5, FAM_RDF))
+        with pytest.raises(PPError):
+            ctx.run(cl, Job([((1,), (1,))], 0.0, 0.1, 1.0, 1 << 4))  # NPCORR
+        with pytest.raises(PPError):
+            ctx.run(cl, Job([((1,), (1,))], 0.0, 0.1, 1.0, 0))
+    finally:
+        cl.free()
+
+
+def test_cylinder_1m_rdf_and_knn(ctx):
+    """The CPU-runnable workload: 1M ions in a cylinder, B-B RDF (R=2, 0.01)
+    and B-B kNN k=1..10 (bin 0.005)."""
+    _say("test_cylinder_1m generate")
+    tip = synth.cylinder_1m()
+    cl = ctx.upload(tip.xyz, tip.types, tip.ion2surf_sqr)
+    try:
+        _say("test_cylinder_1m rdf")
+        g, o, st = _compare(ctx, tip.xyz, tip.types, tip.ion2surf_sqr,
+                            Job([((2,), (2,))], 0.0, 0.01, 2.0, FAM_RDF), cloud=cl)
+        assert 50_000 < g.centers_considered < 62_000
+        _say("test_cylinder_1m knn")
+        _compare(ctx, tip.xyz, tip.types, tip.ion2surf_sqr,
+                 Job([((2,), (2,))], 0.0, 0.005, 2.0, FAM_MKNN, kvalues=list(range(1, 11))),
+                 cloud=cl)
+    finally:
+        cl.free()
+
+
+def test_batch_equals_single_runs(ctx):
+    _say("test_batch_equals_single_runs")
+    clouds, jobs, raw = [], [], []
+    specs = [(2500, 21, Job(TASKS_MIX, 0.0, 0.05, 1.2, ALL, kvalues=[1, 2, 5])),
+             (1800, 22, Job([((2,), (3,))], 0.0, 0.01, 0.7, FAM_RDF)),
+             (3000, 23, Job([((1,), (1, 2))], 0.1, 0.02, 1.5, FAM_MKNN | FAM_1NN,
+                            kvalues=[1, 10, 50]))]
+    for n, seed, job in specs:
+        xyz, ty, s2 = _cube(n, seed)
+        raw.append((xyz, ty, s2))
+        clouds.append(ctx.upload(xyz, ty, s2))
+        jobs.append(job)
+    try:
+        outs = ctx.run_batch(clouds, jobs)
+        for (xyz, ty, s2), c, j, gb in zip(raw, clouds, jobs, outs):
+            gs = ctx.run(c, j)
+            for k, v in gs.arrays().items():
+                np.testing.assert_array_equal(getattr(gb, k), v)
+            o, st = O.spatstat(xyz, ty, s2, j.tasks, j.rmin, j.rincr, j.rmax, j.families,
+                               kvalues=j.kvalues)
+            for k, v in o.items():
+                np.testing.assert_array_equal(getattr(gb, k), v)
+    finally:
+        for c in clouds:
+            c.free()
+
+
+def test_resident_cell_list_any_rmax(ctx):
+    """A cloud uploaded with a fixed cell width serves jobs of any Rmax."""
+    _say("test_resident_cell_list_any_rmax")
+    xyz, ty, s2 = _cube(5000, 31, L=8.0)
+    cl = ctx.upload(xyz, ty, s2, cell_width=0.45)
+    try:
+        for R in (0.3, 0.9, 1.7):
+            _compare(ctx, xyz, ty, s2, Job(TASKS_MIX, 0.0, 0.01, R, ALL, kvalues=[1, 4]), cloud=cl)
+    finally:
+        cl.free()
