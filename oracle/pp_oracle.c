@@ -170,6 +170,9 @@ static int ogrid_build(ogrid *g, int64_t n, const float *xyz, float R)
             if (v > hi[a])
                 hi[a] = v;
         }
+    if (n == 0)
+        for (int a = 0; a < 3; ++a)
+            g->lo[a] = hi[a] = 0.0;
     double w = (double)R * 1.001;
     if (w <= 0.0)
         w = 1.0;

@@ -518,7 +518,9 @@ void compose_outputs(const JobRun &r, const std::vector<unsigned long long> &P,
     o->nbins = p.hg.nbins;
     o->centers_considered = r.h_counts[0];
     o->ions_discarded = (int64_t)r.cloud->n - (int64_t)r.h_counts[0];
-    o->pairs_inball = (int64_t)stats[0];
+    // the kernel's ballot count includes each considered center's self pair
+    // (d = 0, always in its own cell); the reference excludes it by index
+    o->pairs_inball = (int64_t)stats[0] - (int64_t)r.h_counts[0];
     o->pairs_candidate = (int64_t)stats[1];
     *overflow = stats[2] != 0;
     if (p.families & (PP_FAM_RDF | PP_FAM_RIPK)) {
