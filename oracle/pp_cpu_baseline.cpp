@@ -1,0 +1,467 @@
+// pp_cpu_baseline.cpp -- the reference's CPU algorithm for the RDF / kNN path,
+// restated for timing on the host cores.
+//
+// TEST / BENCH INFRASTRUCTURE ONLY: the cpu_baseline leg of bench.py and the
+// tests load it; the product library never links or calls it.
+//
+// The structure follows the reference step by step, because the point of
+// this file is to time that structure:
+//   * decompositor::loadpartitioning (src/PARAPROBE_SolverHdl.cpp:1819-1934):
+//     sort all z, split at the (t / nthreads) quantiles of
+//     mathHdl::quantiles_nosort (src/PARAPROBE_Math.cpp:46-77), one z-slab per
+//     thread, ions in the half-open [zmi, zmx) (S.cpp:1330-1468);
+//   * threadmemory::build_local_kdtree + kd_tree::build + pack_p3dm1_dist
+//     (S.cpp:1593-1663, src/PARAPROBE_KDTree.cpp:34-193): median splits with
+//     nth_element cycling x, y, z, leaves of fewer than 256 + 1 points, ions
+//     and ion2surf permuted into leaf order;
+//   * kd_tree::range_rball_noclear_nosort / _external
+//     (src/PARAPROBE_KDTree.cpp:423-655): box pruning with the float
+//     cuboid::outside_proximity, self excluded by index in the own slab;
+//   * horderdist::compute_generic_spatstat3 (S.cpp:3770-4176): the team
+//     sweeps the regions one after the other with `omp for schedule(dynamic,1)
+//     nowait` and a barrier per region; per considered center the neighbours
+//     are materialised as (sqrt(d2), type), probing the other slabs when the
+//     ball crosses the slab ((z - R) > zmi && (z + R) < zmx); the type
+//     reorganiser (counting sort by type); RDF / RIPK add every env distance;
+//     1NN the minimum or R + EPSILON; CNTNB one count per env type; MKNN
+//     nth_element for every k in descending order or R + EPSILON;
+//     thread-local double histograms reduced under `omp critical`.
+// Output layout matches the C ABI: [lowest, bins..., highest] as int64.
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+#include <omp.h>
+
+namespace {
+
+constexpr double kEpsilon = 1.0e-6;
+constexpr size_t kLeaf = 256;
+constexpr int kStack = 32;
+
+struct P3 {
+    float x, y, z;
+    uint32_t m;
+};
+struct Nbor {
+    float d;
+    uint32_t m;
+};
+struct Box {
+    float mn[3], mx[3];
+    // squared float distance of p to the box, 0 inside (cuboid::outside_proximity)
+    float outside(const P3 &p) const
+    {
+        const float c[3] = {p.x, p.y, p.z};
+        float r = 0.f;
+        for (int a = 0; a < 3; ++a) {
+            if (c[a] < mn[a]) {
+                const float t = mn[a] - c[a];
+                r += t * t;
+            } else if (c[a] > mx[a]) {
+                const float t = c[a] - mx[a];
+                r += t * t;
+            }
+        }
+        return r;
+    }
+};
+struct Node {
+    size_t i0 = (size_t)-1, i1 = (size_t)-1, split = (size_t)-1;
+    float pos = 3.402823466e38f;
+    bool leaf() const { return split == (size_t)-1; }
+};
+
+struct Hist {
+    double start = 0, width = 0, end = 0, tmp = 0;
+    uint32_t nbins = 0;
+    std::vector<double> cnts;
+    double lowest = 0, highest = 0;
+    Hist() = default;
+    Hist(float s, float w, float e)
+    {
+        const double ds = s, dw = w, de = e;
+        if (dw > kEpsilon) {
+            const size_t nb = (size_t)((de - ds) / dw);
+            start = ds;
+            width = dw;
+            end = de;
+            nbins = (uint32_t)nb;
+            tmp = 1.0 / (de - ds) * (double)nb;
+            cnts.assign(nbins, 0.0);
+        }
+    }
+    void add(float x)
+    {
+        const double xx = x;
+        if (xx >= start && xx <= end) {
+            const float rb = (float)((xx - start) * tmp);
+            const unsigned b = (unsigned)std::floor(rb);
+            if (b < nbins)
+                cnts[b] += 1.0;
+            else
+                highest += 1.0;
+        } else {
+            if (xx < start)
+                lowest += 1.0;
+            if (xx > end)
+                highest += 1.0;
+        }
+    }
+    void merge(const Hist &o)
+    {
+        lowest += o.lowest;
+        highest += o.highest;
+        for (uint32_t b = 0; b < nbins; ++b)
+            cnts[b] += o.cnts[b];
+    }
+};
+
+struct KdTree {
+    std::vector<Node> nodes;
+    float mn[3] = {3.4e38f, 3.4e38f, 3.4e38f}, mx[3] = {-3.4e38f, -3.4e38f, -3.4e38f};
+
+    void build(const std::vector<P3> &pts, std::vector<size_t> &perm)
+    {
+        const size_t count = pts.size();
+        perm.resize(count);
+        for (size_t i = 0; i < count; ++i)
+            perm[i] = i;
+        if (count == 0)
+            return;
+        struct Task {
+            size_t first, last, node, dim;
+        };
+        Task st[kStack];
+        int top = 0;
+        nodes.push_back(Node());
+        st[0] = {0, count - 1, 0, 0};
+        auto coord = [&](size_t i, size_t d) {
+            return d == 0 ? pts[i].x : (d == 1 ? pts[i].y : pts[i].z);
+        };
+        while (top >= 0) {
+            const Task t = st[top];
+            if (t.last - t.first < kLeaf) {
+                Node &n = nodes[t.node];
+                n.i0 = t.first;
+                n.i1 = t.last;
+                for (size_t k = t.first; k <= t.last; ++k) {
+                    const P3 &p = pts[perm[k]];
+                    const float c[3] = {p.x, p.y, p.z};
+                    for (int a = 0; a < 3; ++a) {
+                        mn[a] = std::min(mn[a], c[a]);
+                        mx[a] = std::max(mx[a], c[a]);
+                    }
+                }
+                --top;
+                continue;
+            }
+            const size_t k = (t.first + t.last) / 2;
+            const size_t d = t.dim;
+            std::nth_element(perm.begin() + t.first, perm.begin() + k, perm.begin() + t.last + 1,
+                             [&](size_t a, size_t b) { return coord(a, d) < coord(b, d); });
+            const size_t i0 = nodes.size(), i1 = i0 + 1;
+            nodes[t.node].pos = coord(perm[k], d);
+            nodes[t.node].split = k;
+            nodes[t.node].i0 = i0;
+            nodes[t.node].i1 = i1;
+            const size_t nd = (d + 1) % 3;
+            st[top] = {t.first, k, i0, nd};
+            nodes.push_back(Node());
+            ++top;
+            st[top] = {k + 1, t.last, i1, nd};
+            nodes.push_back(Node());
+        }
+    }
+
+    // range query: self excluded by index when self != SIZE_MAX
+    void range(const P3 &target, size_t self, const std::vector<P3> &sorted, float r2,
+               std::vector<Nbor> &out) const
+    {
+        if (nodes.empty())
+            return;
+        struct TT {
+            Box bx;
+            size_t node, dim;
+            float d;
+        };
+        TT st[kStack];
+        Box root;
+        for (int a = 0; a < 3; ++a) {
+            root.mn[a] = mn[a];
+            root.mx[a] = mx[a];
+        }
+        st[0] = {root, 0, 0, root.outside(target)};
+        int top = 0;
+        const float tc[3] = {target.x, target.y, target.z};
+        while (top >= 0) {
+            const TT t = st[top];
+            if (t.d > r2) {
+                --top;
+                continue;
+            }
+            const Node &n = nodes[t.node];
+            if (n.leaf()) {
+                for (size_t i = n.i0; i <= n.i1; ++i) {
+                    const P3 &q = sorted[i];
+                    const float dx = target.x - q.x, dy = target.y - q.y, dz = target.z - q.z;
+                    const float d = dx * dx + dy * dy + dz * dz;
+                    if (d > r2)
+                        continue;
+                    if (i != self)
+                        out.push_back(Nbor{std::sqrt(d), q.m});
+                }
+                --top;
+                continue;
+            }
+            const int a = (int)t.dim;
+            Box lo = t.bx, hi = t.bx;
+            lo.mx[a] = n.pos;
+            hi.mn[a] = n.pos;
+            TT nearT, farT;
+            if (tc[a] <= n.pos) {
+                nearT.bx = lo;
+                nearT.node = n.i0;
+                farT.bx = hi;
+                farT.node = n.i1;
+            } else {
+                nearT.bx = hi;
+                nearT.node = n.i1;
+                farT.bx = lo;
+                farT.node = n.i0;
+            }
+            const size_t nd = (t.dim + 1) % 3;
+            nearT.d = 0.f;
+            nearT.dim = nd;
+            farT.d = farT.bx.outside(target);
+            farT.dim = nd;
+            st[top] = farT;
+            ++top;
+            st[top] = nearT;
+        }
+    }
+};
+
+struct Region {
+    float zmi, zmx;
+    std::vector<P3> ions;  // leaf order
+    std::vector<float> s2; // leaf order
+    KdTree tree;
+};
+
+inline bool has_type(const uint8_t *mask32, uint32_t t)
+{
+    return t < 256 && ((mask32[t >> 3] >> (t & 7)) & 1);
+}
+
+float lerpf(float v0, float v1, float t) { return (1.f - t) * v0 + t * v1; }
+
+} // namespace
+
+extern "C" {
+
+// Returns 0 on success.  timings[0] = index build seconds (slabs + KD-trees),
+// timings[1] = compute seconds (the DescrStatsThreadparallelCompute span).
+int pp_cpu_spatstat(int64_t n, const float *xyz, const uint8_t *type, const float *ion2surf,
+                    int32_t ntasks, const uint8_t *trg_mask, const uint8_t *env_mask, float rmin,
+                    float rincr, float rmax, uint32_t families, int32_t nk,
+                    const int64_t *kminus1_desc, int32_t nthreads, int64_t *rdf, int64_t *nn1,
+                    int64_t *ripk, int64_t *mknn, int64_t *cntnb, int64_t cntnb_cap,
+                    int64_t *stats, double *timings)
+{
+    if (nthreads <= 0)
+        nthreads = omp_get_max_threads();
+    const bool doRDF = families & 1u, do1NN = families & 2u, doRIPK = families & 4u,
+               doMKNN = (families & 8u) && nk > 0, doCNT = families & 32u;
+    Hist proto(rmin, rincr, rmax);
+    if (proto.nbins == 0)
+        return 1;
+    const int64_t C = (int64_t)proto.nbins + 2;
+    const float R = rmax, RSQR = R * R;
+    const double t0 = omp_get_wtime();
+
+    // ---- loadpartitioning: quantile z-slabs
+    std::vector<float> zs((size_t)n);
+    for (int64_t i = 0; i < n; ++i)
+        zs[i] = xyz[3 * i + 2];
+    std::sort(zs.begin(), zs.end());
+    std::vector<Region> reg(nthreads);
+    for (int t = 0; t < nthreads; ++t) {
+        auto quant = [&](int tid) {
+            const float q = (float)tid / (float)nthreads;
+            const float poi = lerpf(-0.5f, (float)n - 0.5f, q);
+            const size_t left = (size_t)std::max<int64_t>((int64_t)std::floor(poi), 0);
+            const size_t right = (size_t)std::min<int64_t>((int64_t)std::ceil(poi), n - 1);
+            return lerpf(zs[left], zs[right], poi - (float)left);
+        };
+        reg[t].zmi = (t == 0) ? (n ? zs.front() - 1.0e-4f : 0.f) : quant(t);
+        reg[t].zmx = (t == nthreads - 1) ? (n ? zs.back() + 1.0e-4f : 0.f) : quant(t + 1);
+    }
+    // ---- read_localions + build_local_kdtree, one region per thread
+#pragma omp parallel num_threads(nthreads)
+    {
+        const int t = omp_get_thread_num();
+        Region &r = reg[t];
+        std::vector<P3> loc;
+        std::vector<float> ls2;
+        for (int64_t i = 0; i < n; ++i) {
+            const float z = xyz[3 * i + 2];
+            if (z >= r.zmi && z < r.zmx) {
+                loc.push_back(P3{xyz[3 * i], xyz[3 * i + 1], z, (uint32_t)type[i]});
+                ls2.push_back(ion2surf[i]);
+            }
+        }
+        std::vector<size_t> perm;
+        r.tree.build(loc, perm);
+        r.ions.reserve(loc.size());
+        r.s2.reserve(loc.size());
+        for (size_t k = 0; k < perm.size(); ++k) {
+            r.ions.push_back(loc[perm[k]]);
+            r.s2.push_back(ls2[perm[k]]);
+        }
+    }
+    const double t1 = omp_get_wtime();
+
+    // ---- compute_generic_spatstat3
+    uint8_t anytrg[32] = {0};
+    for (int t = 0; t < ntasks; ++t)
+        for (int b = 0; b < 32; ++b)
+            anytrg[b] |= trg_mask[32 * t + b];
+    const uint32_t maxtype = 256;
+    std::vector<Hist> g_rdf(ntasks, proto), g_1nn(ntasks, proto), g_mknn;
+    if (doMKNN)
+        g_mknn.assign((size_t)ntasks * nk, proto);
+    std::vector<std::vector<double>> g_cnt(ntasks);
+    int64_t considered = 0, discarded = 0, inball = 0;
+#pragma omp parallel num_threads(nthreads) reduction(+ : considered, discarded, inball)
+    {
+        std::vector<Hist> m_rdf(ntasks, proto), m_1nn(ntasks, proto), m_mknn;
+        if (doMKNN)
+            m_mknn.assign((size_t)ntasks * nk, proto);
+        std::vector<std::vector<double>> m_cnt(ntasks);
+        for (int thr = 0; thr < nthreads; ++thr) {
+            const Region &cur = reg[thr];
+#pragma omp for schedule(dynamic, 1) nowait
+            for (size_t i = 0; i < cur.ions.size(); ++i) {
+                const P3 me = cur.ions[i];
+                if (!(cur.s2[i] >= RSQR && has_type(anytrg, me.m))) {
+                    ++discarded;
+                    continue;
+                }
+                ++considered;
+                std::vector<Nbor> nb;
+                cur.tree.range(me, i, cur.ions, RSQR, nb);
+                if (!((me.z - R) > cur.zmi && (me.z + R) < cur.zmx)) {
+                    for (int o = thr + 1; o < nthreads; ++o)
+                        reg[o].tree.range(me, (size_t)-1, reg[o].ions, RSQR, nb);
+                    for (int o = thr - 1; o > -1; --o)
+                        reg[o].tree.range(me, (size_t)-1, reg[o].ions, RSQR, nb);
+                }
+                inball += (int64_t)nb.size();
+                // type reorganiser: counting sort of the neighbours by type
+                std::vector<size_t> cnt(maxtype, 0), s(maxtype, 0);
+                for (const Nbor &q : nb)
+                    cnt[q.m]++;
+                for (uint32_t a = 1; a < maxtype; ++a)
+                    s[a] = s[a - 1] + cnt[a - 1];
+                std::vector<float> by(nb.size(), 3.402823466e38f);
+                std::vector<size_t> fill(s);
+                for (const Nbor &q : nb)
+                    by[fill[q.m]++] = q.d;
+                for (int t = 0; t < ntasks; ++t) {
+                    if (!has_type(trg_mask + 32 * t, me.m))
+                        continue;
+                    const uint8_t *env = env_mask + 32 * t;
+                    if (doRDF || doRIPK)
+                        for (uint32_t e = 0; e < maxtype; ++e)
+                            if (has_type(env, e))
+                                for (size_t q = s[e]; q < s[e] + cnt[e]; ++q)
+                                    m_rdf[t].add(by[q]);
+                    if (do1NN) {
+                        float closest = 3.402823466e38f;
+                        bool found = false;
+                        for (uint32_t e = 0; e < maxtype; ++e)
+                            if (has_type(env, e))
+                                for (size_t q = s[e]; q < s[e] + cnt[e]; ++q)
+                                    if (!(by[q] > closest)) {
+                                        closest = by[q];
+                                        found = true;
+                                    }
+                        m_1nn[t].add(found ? closest : (float)((double)R + kEpsilon));
+                    }
+                    if (doCNT)
+                        for (uint32_t e = 0; e < maxtype; ++e)
+                            if (has_type(env, e)) {
+                                auto &v = m_cnt[t];
+                                if (cnt[e] >= v.size())
+                                    v.resize(cnt[e] + 1, 0.0);
+                                v[cnt[e]] += 1.0;
+                            }
+                    if (doMKNN) {
+                        std::vector<float> tmp;
+                        for (uint32_t e = 0; e < maxtype; ++e)
+                            if (has_type(env, e))
+                                for (size_t q = s[e]; q < s[e] + cnt[e]; ++q)
+                                    tmp.push_back(by[q]);
+                        for (int kth = 0; kth < nk; ++kth) {
+                            const size_t k = (size_t)kminus1_desc[kth];
+                            Hist &h = m_mknn[(size_t)t * nk + kth];
+                            if (tmp.size() > k) {
+                                std::nth_element(tmp.begin(), tmp.begin() + k, tmp.end());
+                                h.add(tmp[k]);
+                            } else {
+                                h.add((float)((double)R + kEpsilon));
+                            }
+                        }
+                    }
+                }
+            }
+#pragma omp barrier
+        }
+#pragma omp critical
+        {
+            for (int t = 0; t < ntasks; ++t) {
+                g_rdf[t].merge(m_rdf[t]);
+                g_1nn[t].merge(m_1nn[t]);
+                auto &g = g_cnt[t];
+                if (m_cnt[t].size() > g.size())
+                    g.resize(m_cnt[t].size(), 0.0);
+                for (size_t b = 0; b < m_cnt[t].size(); ++b)
+                    g[b] += m_cnt[t][b];
+            }
+            for (size_t q = 0; q < m_mknn.size(); ++q)
+                g_mknn[q].merge(m_mknn[q]);
+        }
+    }
+    const double t2 = omp_get_wtime();
+    auto emit = [&](const Hist &h, int64_t *dst) {
+        dst[0] = (int64_t)h.lowest;
+        for (uint32_t b = 0; b < h.nbins; ++b)
+            dst[1 + b] = (int64_t)h.cnts[b];
+        dst[h.nbins + 1] = (int64_t)h.highest;
+    };
+    for (int t = 0; t < ntasks; ++t) {
+        if (doRDF && rdf)
+            emit(g_rdf[t], rdf + t * C);
+        if (doRIPK && ripk)
+            emit(g_rdf[t], ripk + t * C);
+        if (do1NN && nn1)
+            emit(g_1nn[t], nn1 + t * C);
+        if (doMKNN && mknn)
+            for (int kth = 0; kth < nk; ++kth)
+                emit(g_mknn[(size_t)t * nk + kth], mknn + ((int64_t)t * nk + kth) * C);
+        if (doCNT && cntnb)
+            for (int64_t b = 0; b < cntnb_cap; ++b)
+                cntnb[t * cntnb_cap + b] =
+                    b < (int64_t)g_cnt[t].size() ? (int64_t)g_cnt[t][b] : 0;
+    }
+    stats[0] = considered;
+    stats[1] = discarded;
+    stats[2] = inball;
+    timings[0] = t1 - t0;
+    timings[1] = t2 - t1;
+    return 0;
+}
+
+} // extern "C"
