@@ -14,9 +14,9 @@ neighbour list), divided by the step time.
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import os
-import platform
 import subprocess
 import sys
 import time
@@ -94,13 +94,14 @@ def main():
           flush=True)
     job = headline_job()
 
-    try:
-        import torch
+    # The library calls are synchronous; the device-wide synchronisation
+    # around the timed region goes through the same HIP runtime the library
+    # uses (torch is not imported: its wheel carries a second HIP runtime).
+    hip = ctypes.CDLL("libamdhip64.so.7")
 
-        sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
-    except Exception:
-        torch = None
-        sync = lambda: None  # noqa: E731
+    def sync():
+        if hip.hipDeviceSynchronize() != 0:
+            raise SystemExit("bench: hipDeviceSynchronize failed")
 
     with SpatStat(0) as ctx:
         cloud = ctx.upload(tip.xyz, tip.types, tip.ion2surf_sqr)
@@ -131,11 +132,18 @@ def main():
         gwin = ctx.run(cw, job)
         cw.free()
         cloud.free()
-        if args.calib and torch is not None:
-            x = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
-            y = x.clone()
-            torch.cuda.synchronize()
-            del x, y
+        if args.calib:
+            # 1 GiB device-to-device copy: its FETCH_SIZE / WRITE_SIZE per
+            # dispatch scale the counters of the query kernel to bytes
+            a, b = ctypes.c_void_p(), ctypes.c_void_p()
+            nb = 1 << 30
+            if hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nb)) == 0 and \
+                    hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nb)) == 0:
+                hip.hipMemset(a, 1, ctypes.c_size_t(nb))
+                hip.hipMemcpy(b, a, ctypes.c_size_t(nb), 3)  # hipMemcpyDeviceToDevice
+                sync()
+                hip.hipFree(a)
+                hip.hipFree(b)
 
     ms_step = dt * 1e3 / args.steps
     inball = last.pairs_inball
@@ -173,7 +181,7 @@ def main():
             "unit": "Tops/s (FP32 VALU element-ops, 9 per candidate pair)",
             "frac": achieved / (VALU_PEAK_OPS / 1e12),
             "traffic": None,
-            "kernel": "k_query<16,true,true>",
+            "kernel": last.kernel,
             "kernel_ms": q_ms,
             "candidate_evals": cand,
         },

@@ -91,6 +91,10 @@ typedef struct pp_hist_out {
     double ms_prep;             /* center flags, compaction, tiles */
     double ms_query;            /* query kernel */
     double ms_total;            /* whole call, host wall clock */
+    int32_t tile_centers;       /* query-kernel configuration used: centers per tile, */
+    int32_t cells_per_rmax;     /*   cell width as a fraction of Rmax (0: resident list), */
+    int32_t counters16;         /*   16-bit per-center counters, */
+    int32_t percenter;          /*   per-center histograms */
 } pp_hist_out;
 
 /* Context bound to one device; one per host thread (calls on a context are

@@ -89,6 +89,10 @@ class _Out(C.Structure):
         ("ms_prep", C.c_double),
         ("ms_query", C.c_double),
         ("ms_total", C.c_double),
+        ("tile_centers", C.c_int32),
+        ("cells_per_rmax", C.c_int32),
+        ("counters16", C.c_int32),
+        ("percenter", C.c_int32),
     ]
 
 
@@ -185,6 +189,7 @@ class Result:
     ms_prep: float
     ms_query: float
     ms_total: float
+    kernel: str = ""
 
     def arrays(self) -> dict:
         return {k: getattr(self, k) for k in ("rdf", "nn1", "ripk", "mknn", "cntnb")
@@ -244,6 +249,9 @@ class _Prepared:
             ms_prep=o.ms_prep,
             ms_query=o.ms_query,
             ms_total=o.ms_total,
+            kernel="k_query<%d,%s,%s> cells/Rmax=%d" % (
+                o.tile_centers, "true" if o.counters16 else "false",
+                "true" if o.percenter else "false", o.cells_per_rmax),
         )
 
 

@@ -637,12 +637,14 @@ __global__ __launch_bounds__(kBlock) void k_query(QArgs a)
 
         // ---------------- main candidate loop ----------------
         {
+            // the T centers in registers; absent centers of a partial tile
+            // get NaN coordinates, so their squared distances compare false
             float cxr[T], cyr[T], czr[T];
             int hb[T];
 #pragma unroll
             for (int c = 0; c < T; ++c) {
                 const float4 p = s_cen[c < cnt ? c : 0];
-                cxr[c] = p.x;
+                cxr[c] = c < cnt ? p.x : __builtin_nanf("");
                 cyr[c] = p.y;
                 czr[c] = p.z;
                 if (PC)
@@ -690,27 +692,65 @@ __global__ __launch_bounds__(kBlock) void k_query(QArgs a)
                 const float px = ok ? p.x : __builtin_nanf("");
                 const int woff = ok ? s_envoff[__float_as_uint(p.w) & 0xFFu] : -1;
                 const bool envok = woff >= 0;
-#pragma unroll
-                for (int c = 0; c < T; ++c) {
-                    if (c < cnt) {
+                if (PC) {
+                    // 1) all T squared distances (the reference's FP32 expression)
+                    float d2[T];
+    #pragma unroll
+                    for (int c = 0; c < T; ++c) {
                         const float dx = cxr[c] - px;
                         const float dy = cyr[c] - p.y;
                         const float dz = czr[c] - p.z;
-                        const float d2 = dx * dx + dy * dy + dz * dz;
-                        const bool in = d2 <= R2;
+                        d2[c] = dx * dx + dy * dy + dz * dz;
+                    }
+                    // 2) nearest ideal edge per pair and its exact d2 threshold; the
+                    //    index is clamped to [1, C-1] (NaN -> 0), so every lane
+                    //    reads inside the table and the LDS reads overlap
+                    int qi[T];
+                    uint32_t ed[T];
+    #pragma unroll
+                    for (int c = 0; c < T; ++c) {
+                        const float sq = __builtin_amdgcn_sqrtf(d2[c]);
+                        qi[c] = (int)__builtin_amdgcn_fmed3f(__builtin_fmaf(sq, isc, ioff), 1.0f, Cm1f);
+                    }
+    #pragma unroll
+                    for (int c = 0; c < T; ++c)
+                        ed[c] = s_edges[qi[c]];
+                    // 3) in-ball pairs: count all types, bin env types
+    #pragma unroll
+                    for (int c = 0; c < T; ++c) {
+                        const bool in = d2[c] <= R2;
                         my_in += (unsigned long long)__builtin_popcountll(
                             __builtin_amdgcn_ballot_w64(in));
                         if (in && envok) {
-                            // nearest ideal edge from an approximate sqrt, then
-                            // one exact compare against the edge's d2 bits
-                            const float sq = __builtin_amdgcn_sqrtf(d2);
-                            const float fq =
-                                __builtin_amdgcn_fmed3f(__builtin_fmaf(sq, isc, ioff), 1.0f, Cm1f);
-                            const int qi = (int)fq;
-                            const int cat = qi - (__float_as_uint(d2) < s_edges[qi] ? 1 : 0);
+                            const int cat = qi[c] - (__float_as_uint(d2[c]) < ed[c] ? 1 : 0);
                             const uint32_t inc = (C16 && (c & 1)) ? 65536u : 1u;
-                            __hip_atomic_fetch_add(&s_acc[hb[c] + woff + cat], inc,
-                                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                            __hip_atomic_fetch_add(&s_acc[hb[c] + woff + cat], inc, __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+                        }
+                    }
+                } else {
+                    // direct mode (pair counts per center type): one center
+                    // at a time keeps the register count, and so the number
+                    // of resident wavefronts, higher
+#pragma unroll
+                    for (int c = 0; c < T; ++c) {
+                        if (c < cnt) {
+                            const float dx = cxr[c] - px;
+                            const float dy = cyr[c] - p.y;
+                            const float dz = czr[c] - p.z;
+                            const float d2 = dx * dx + dy * dy + dz * dz;
+                            const bool in = d2 <= R2;
+                            my_in += (unsigned long long)__builtin_popcountll(
+                                __builtin_amdgcn_ballot_w64(in));
+                            if (in && envok) {
+                                const float sq = __builtin_amdgcn_sqrtf(d2);
+                                const int q1 = (int)__builtin_amdgcn_fmed3f(
+                                    __builtin_fmaf(sq, isc, ioff), 1.0f, Cm1f);
+                                const int cat = q1 - (__float_as_uint(d2) < s_edges[q1] ? 1 : 0);
+                                __hip_atomic_fetch_add(&s_acc[hb[c] + woff + cat], 1u,
+                                                       __ATOMIC_RELAXED,
+                                                       __HIP_MEMORY_SCOPE_WORKGROUP);
+                            }
                         }
                     }
                 }

@@ -13,10 +13,19 @@ def pytest_configure(config):
 
 
 def gpu_available() -> bool:
+    """True when the library can open a context on device 0 (torch is not
+    imported: its wheel carries a second HIP runtime)."""
     try:
-        import torch
+        import ctypes
 
-        return torch.cuda.is_available()
+        from paraprobe_amd import capi
+
+        lib = capi.load_library()
+        h = ctypes.c_void_p()
+        if lib.pp_spatstat_create(0, ctypes.byref(h)) != 0:
+            return False
+        lib.pp_spatstat_destroy(h)
+        return True
     except Exception:
         return False
 
