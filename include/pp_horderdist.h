@@ -1,0 +1,143 @@
+/*
+ * pp_horderdist.h -- C++ host mirror of the reference's spatial-statistics task
+ * interface, and C entry points to it.
+ *
+ * The reference drives the RDF / kNN path through
+ *   solver::characterize_spatstat                  src/PARAPROBE_SolverHdl.cpp:745-776
+ *   horderdist::initialize_descrstat_tasks         src/PARAPROBE_SolverHdl.cpp:3019-3028
+ *   horderdist::compute_generic_spatstat3          src/PARAPROBE_SolverHdl.cpp:3685-4296
+ *   horderdist::report_apriori_descrstat2 / 3      src/PARAPROBE_SolverHdl.cpp:4299-4453
+ * with tasks parsed by parse_tasks / TypeSpecDescrStat::define_iontask4
+ * (src/PARAPROBE_PeriodicTable.cpp:498-690) and settings read from the XML keys
+ * of src/PARAPROBE_Settings.cpp:347-393, 473-503.  pph::Horderdist keeps those
+ * method names and that behaviour; its compute step calls libpp_spatstat.so
+ * through include/pp_spatstat.h.
+ */
+#ifndef PP_HORDERDIST_H
+#define PP_HORDERDIST_H
+
+#include <stdint.h>
+
+#include "pp_spatstat.h"
+
+#ifdef __cplusplus
+#include <map>
+#include <string>
+#include <vector>
+
+namespace pph {
+
+/* Element symbols (the reference's NuclidTable keys, iterated in map order)
+ * and the ion-type registry IonTypes (name -> type id, "Unknown" -> 0). */
+struct PeriodicTable {
+    std::map<std::string, int> nuclids;
+    std::map<std::string, unsigned> ion_types;
+    PeriodicTable();
+    void add_iontype(const std::string &name, unsigned id);
+    unsigned max_type_id() const;
+};
+
+/* TypeSpecDescrStat (src/PARAPROBE_PeriodicTable.h:100-127). */
+struct TypeSpecDescrStat {
+    std::map<std::string, unsigned> trgcandidates;
+    std::map<std::string, unsigned> envcandidates;
+    bool define_iontask4(const std::string &command, const PeriodicTable &pse);
+};
+
+/* parse_tasks (src/PARAPROBE_PeriodicTable.cpp:653-690): "T1,T2-E1;..." */
+void parse_tasks(const std::string &command, std::vector<TypeSpecDescrStat> &these,
+                 const PeriodicTable &pse);
+
+/* The MKNN code of compute_generic_spatstat3 (S.cpp:3739-3767): "1;10;100"
+ * -> k-1 values sorted descending.  Returns false when a value is below 1
+ * (the reference aborts the whole computation then). */
+bool parse_mknn_code(const std::string &code, std::vector<int64_t> &kminus1_desc);
+
+/* The spatial-statistics subset of Settings (src/PARAPROBE_Settings.cpp). */
+struct SpatstatSettings {
+    unsigned simid = 0;
+    bool do_rdf = false, do_1nn = false, do_ripk = false, do_mknn = false, do_npcorr = false,
+         do_cntnb = false, add_label_rnd = false;
+    std::string tasks_code;
+    float rmin = 0.f, rincr = 0.1f, rmax = 2.f;
+    size_t knn_order = 0;
+    std::string mknn_code;
+    /* Reads <Parameters> children from an XML text (the keys the reference reads). */
+    bool read_xml_text(const std::string &xml, std::string *why);
+    /* Settings::checkUserInput for these keys (S.cpp:871-885). */
+    bool check(std::string *why) const;
+    uint32_t families() const;
+};
+
+/* RDF normalisation metadata the reference takes from its volume binner
+ * (binner->metadata, S.cpp:4325-4336; size_t and apt_real there,
+ * src/PARAPROBE_Datatypes.h:667-669). */
+struct VolumeMeta {
+    uint64_t nions_inside = 0;
+    float volume_inside = 0.f;
+};
+
+class Horderdist {
+public:
+    Horderdist(pp_ctx *ctx, const SpatstatSettings &s, const PeriodicTable &pse);
+    void initialize_descrstat_tasks();
+    /* Runs the path on the GPU and writes the reference's CSV files into
+     * outdir; returns the library status (no exception crosses). */
+    pp_status compute_generic_spatstat3(const pp_cloud *cloud, const VolumeMeta &meta,
+                                        const std::string &outdir, bool randomized = false);
+    const std::vector<TypeSpecDescrStat> &tasks() const { return spatstat_tasks; }
+    pp_hist_out last;                       /* statistics of the last run */
+    std::vector<int64_t> rdf, nn1, ripk, mknn, cntnb;
+
+private:
+    pp_ctx *ctx_;
+    SpatstatSettings s_;
+    const PeriodicTable &pse_;
+    std::vector<TypeSpecDescrStat> spatstat_tasks;
+};
+
+/* CSV writers (S.cpp:4299-4453).  kind: 1 RDF, 2 1NN, 3 RIPK, 4 MKNN.
+ * hist holds [lowest, bins[nbins], highest]. */
+bool report_apriori_descrstat2(const std::string &path, int kind, const std::string &metric,
+                               float rmin, float rincr, float rmax, const int64_t *hist,
+                               const VolumeMeta &meta);
+bool report_apriori_descrstat3(const std::string &path, const int64_t *cnts, int64_t len);
+std::string report_filename(unsigned simid, const std::string &metric, const std::string &trg,
+                            const std::string &env, bool randomized);
+
+} // namespace pph
+
+extern "C" {
+#endif
+
+/* C entry points (for callers without C++ and for the tests). */
+
+/* Parses a task code against a registry of ion-type names/ids.  Writes up to
+ * max_tasks masks (32 bytes each) and the concatenated trg / env names
+ * (names_stride bytes each).  Returns the number of valid tasks. */
+int32_t pph_parse_tasks(const char *code, int32_t ntypes, const char *const *type_names,
+                        const uint32_t *type_ids, int32_t max_tasks, uint8_t *trg_masks,
+                        uint8_t *env_masks, char *trg_names, char *env_names,
+                        int32_t names_stride);
+/* Parses an MKNN code; returns the number of values (k-1, descending) or -1. */
+int32_t pph_parse_mknn_code(const char *code, int64_t *kminus1_desc, int32_t cap);
+/* Reads the spatstat keys from XML text; out[0] = families, out[1] = knn order,
+ * radii in radii[3]; copies the task and MKNN codes.  Returns 0 on success. */
+int32_t pph_read_settings(const char *xml, uint32_t *families, float *radii, char *tasks_code,
+                          int32_t tasks_cap, char *mknn_code, int32_t mknn_cap);
+/* Writes one report file; returns 0 on success. */
+int32_t pph_write_descrstat2(const char *path, int32_t kind, const char *metric, float rmin,
+                             float rincr, float rmax, const int64_t *hist, uint64_t nions_inside,
+                             float volume_inside);
+int32_t pph_write_descrstat3(const char *path, const int64_t *cnts, int64_t len);
+/* End to end: settings XML text + type registry + a cloud already uploaded ->
+ * GPU computation -> the reference's CSV files in outdir.  Returns pp_status. */
+int32_t pph_run_spatstat(pp_ctx *ctx, const pp_cloud *cloud, const char *xml, uint32_t simid,
+                         int32_t ntypes, const char *const *type_names, const uint32_t *type_ids,
+                         uint64_t nions_inside, float volume_inside, const char *outdir);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PP_HORDERDIST_H */

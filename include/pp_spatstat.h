@@ -122,6 +122,9 @@ pp_status pp_cloud_from_device(pp_ctx *ctx, const float *d_xyz, const uint8_t *d
                                const float *d_ion2surf_sqr, uint64_t n, float cell_width,
                                pp_cloud **out);
 pp_status pp_cloud_free(pp_ctx *ctx, pp_cloud *cloud);
+/* Adds one more resident cell list of the given width to a resident cloud;
+ * each job then uses the resident width that suits its Rmax best. */
+pp_status pp_cloud_add_cell_list(pp_ctx *ctx, pp_cloud *cloud, float cell_width);
 /* Device bytes a cloud holds. */
 pp_status pp_cloud_footprint(const pp_cloud *cloud, uint64_t *bytes);
 
@@ -135,9 +138,38 @@ pp_status pp_spatstat_run(pp_ctx *ctx, const pp_cloud *cloud, const pp_job *job,
 pp_status pp_batch_run(pp_ctx *ctx, int32_t njobs, const pp_cloud *const *clouds,
                        const pp_job *jobs, pp_hist_out *outs);
 
-/* Query-kernel tuning: centers per tile (8, 16 or 32; 0 = automatic) and
- * the cell width as a fraction of Rmax (cells per Rmax: 1..4; 0 = automatic). */
+/* Query-kernel tuning: centers per tile (8, 16 or 32) and the cell width as a
+ * fraction of Rmax (cells per Rmax: 1..4).  0 = built-in choice (density
+ * rule); -1 for both = autotune: the first job of each (density x Rmax^3,
+ * families) bucket times the candidate shapes and the winner is kept for the
+ * bucket. */
 pp_status pp_set_tuning(pp_ctx *ctx, int32_t centers_per_tile, int32_t cells_per_rmax);
+
+/* ---- serving: a request loop over resident clouds (continuous batching) ----
+ * Requests are queued by pp_serve_submit and admitted by their device-memory
+ * footprint against hbm_budget_bytes; each of `slots` workers takes every
+ * admitted request waiting at that moment (up to max_batch_jobs) and runs them
+ * as one batch on its own HIP stream, so batches formed while others run
+ * overlap on the device.  Clouds must be resident (uploaded with a cell width).
+ * The caller keeps each request's pp_job arrays and pp_hist_out buffers alive
+ * until its ticket is reported done. */
+typedef struct pp_serve_opts {
+    int32_t slots;              /* concurrent batches (1..8) */
+    int32_t max_batch_jobs;     /* requests per batch (1..65535) */
+    uint64_t hbm_budget_bytes;  /* admission budget for per-request device memory; 0: half the free HBM */
+} pp_serve_opts;
+
+pp_status pp_serve_create(pp_ctx *ctx, const pp_serve_opts *opts, pp_server **out);
+/* Enqueues one request; *ticket identifies it.  PP_ERR_ADMISSION_REJECTED when
+ * its footprint exceeds the whole budget (it could never be admitted). */
+pp_status pp_serve_submit(pp_server *srv, const pp_cloud *cloud, const pp_job *job,
+                          pp_hist_out *out, uint64_t *ticket);
+/* Returns up to cap completed tickets (with their status) not returned before. */
+pp_status pp_serve_poll(pp_server *srv, uint64_t *tickets, int32_t *statuses, int32_t cap,
+                        int32_t *n);
+/* Blocks until every submitted request has completed. */
+pp_status pp_serve_drain(pp_server *srv);
+pp_status pp_serve_destroy(pp_server *srv);
 
 #ifdef __cplusplus
 }

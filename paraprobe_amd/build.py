@@ -88,8 +88,8 @@ def build_horderdist() -> str:
     if not srcs:
         return ""
     out_tmp = LIB_HORDERDIST + ".tmp"
-    _run([CXX, "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", "-I", INCLUDE,
-          *srcs, "-o", out_tmp, "-ldl"])
+    _run([CXX, "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", "-Wall", "-I", INCLUDE,
+          *srcs, "-o", out_tmp, "-L", PKG, "-lpp_spatstat", "-Wl,-rpath,$ORIGIN"])
     os.replace(out_tmp, LIB_HORDERDIST)
     return LIB_HORDERDIST
 

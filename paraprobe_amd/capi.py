@@ -48,7 +48,18 @@ EXPORTED = [
     "pp_spatstat_run",
     "pp_batch_run",
     "pp_set_tuning",
+    "pp_cloud_add_cell_list",
+    "pp_serve_create",
+    "pp_serve_submit",
+    "pp_serve_poll",
+    "pp_serve_drain",
+    "pp_serve_destroy",
 ]
+
+
+class _ServeOpts(C.Structure):
+    _fields_ = [("slots", C.c_int32), ("max_batch_jobs", C.c_int32),
+                ("hbm_budget_bytes", C.c_uint64)]
 
 
 class PPError(RuntimeError):
@@ -122,6 +133,12 @@ def load_library() -> C.CDLL:
     lib.pp_spatstat_run.argtypes = [P, P, C.POINTER(_Job), C.POINTER(_Out)]
     lib.pp_batch_run.argtypes = [P, C.c_int32, C.POINTER(P), C.POINTER(_Job), C.POINTER(_Out)]
     lib.pp_set_tuning.argtypes = [P, C.c_int32, C.c_int32]
+    lib.pp_cloud_add_cell_list.argtypes = [P, P, C.c_float]
+    lib.pp_serve_create.argtypes = [P, C.POINTER(_ServeOpts), C.POINTER(P)]
+    lib.pp_serve_submit.argtypes = [P, P, C.POINTER(_Job), C.POINTER(_Out), C.POINTER(C.c_uint64)]
+    lib.pp_serve_poll.argtypes = [P, P, P, C.c_int32, C.POINTER(C.c_int32)]
+    lib.pp_serve_drain.argtypes = [P]
+    lib.pp_serve_destroy.argtypes = [P]
     for name in EXPORTED:
         if name != "pp_last_error_string":
             getattr(lib, name).restype = C.c_int
@@ -261,6 +278,9 @@ class Cloud:
         self.handle = handle
         self.n = n
 
+    def add_cell_list(self, width: float):
+        self._ctx._check(self._ctx.lib.pp_cloud_add_cell_list(self._ctx.ctx, self.handle, width))
+
     def footprint(self) -> int:
         b = C.c_uint64()
         self._ctx._check(self._ctx.lib.pp_cloud_footprint(self.handle, C.byref(b)))
@@ -340,3 +360,55 @@ class SpatStat:
             p.co = co[i]
             out.append(p.result())
         return out
+
+
+class Server:
+    """Continuous-batching request loop over resident clouds (pp_serve_*)."""
+
+    def __init__(self, ctx: SpatStat, slots: int = 4, max_batch_jobs: int = 64,
+                 hbm_budget_bytes: int = 0):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        o = _ServeOpts(slots, max_batch_jobs, hbm_budget_bytes)
+        h = C.c_void_p()
+        ctx._check(self.lib.pp_serve_create(ctx.ctx, C.byref(o), C.byref(h)))
+        self.h = h
+        self._live = {}
+
+    def submit(self, cloud: Cloud, job: Job, cntnb_cap: int = 65536) -> int:
+        p = _Prepared(job, cntnb_cap)
+        t = C.c_uint64()
+        st = self.lib.pp_serve_submit(self.h, cloud.handle, C.byref(p.cj), C.byref(p.co),
+                                      C.byref(t))
+        if st != PP_OK:
+            msg = self.lib.pp_last_error_string(self.ctx.ctx)
+            raise PPError(st, msg.decode() if msg else "")
+        self._live[t.value] = p
+        return t.value
+
+    def poll(self, cap: int = 4096):
+        """-> list of (ticket, status, Result)."""
+        tk = np.zeros(cap, np.uint64)
+        stv = np.zeros(cap, np.int32)
+        n = C.c_int32()
+        self.ctx._check(self.lib.pp_serve_poll(self.h, tk.ctypes.data, stv.ctypes.data, cap,
+                                               C.byref(n)))
+        out = []
+        for i in range(n.value):
+            p = self._live.pop(int(tk[i]))
+            out.append((int(tk[i]), int(stv[i]), p.result()))
+        return out
+
+    def drain(self):
+        self.ctx._check(self.lib.pp_serve_drain(self.h))
+
+    def close(self):
+        if self.h:
+            self.lib.pp_serve_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

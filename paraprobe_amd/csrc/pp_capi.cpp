@@ -4,14 +4,23 @@
 // (the settings and tasks of horderdist::compute_generic_spatstat3,
 // src/PARAPROBE_SolverHdl.cpp:3685-4296) into device tables, runs the cell
 // list, center, tile and query kernels, and composes the reference's
-// histograms from the device counters.
+// histograms from the device counters.  Work runs on "slots" (a HIP stream
+// with its own scratch memory): the context's main slot for the synchronous
+// entry points, one slot per worker of a serving loop.
 #include "../../include/pp_spatstat.h"
 
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <set>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "pp_binning.h"
@@ -44,7 +53,7 @@ struct Carver {
 
 struct CellList {
     GridGeom g{};
-    float w = 0;
+    float w = 0;                    // requested width (the grid may widen it)
     uint32_t *cellstart = nullptr;  // ncells + 1
     float4 *pos = nullptr;          // n
     float *s2 = nullptr;            // n
@@ -52,19 +61,25 @@ struct CellList {
     size_t bytes = 0;
 };
 
+struct Slot {
+    hipStream_t stream = nullptr;
+    DevBuf scratch;  // cell-list temporaries, job arenas
+    DevBuf jobbuf;   // JobDev array + tile pointer table + tile offsets + queue
+    DevBuf cnbuf;    // CNTNB counters
+    hipEvent_t ev[6] = {};
+};
+
 } // namespace
 
 struct pp_ctx {
     int device = 0;
-    hipStream_t stream = nullptr;
-    std::string err;
     int cu_count = 0;
     size_t lds_per_block = 0;
+    std::mutex mu;  // err, tuning state
+    std::string err;
     int tune_T = 0, tune_r = 0;
-    DevBuf scratch;   // per-call scratch (cell-list temporaries, job arenas)
-    DevBuf jobbuf;    // JobDev array + tile pointer table + tile offsets + queue
-    DevBuf cnbuf;     // CNTNB counters
-    hipEvent_t ev[6] = {};
+    std::map<long, std::pair<int, int>> tuned;  // autotune bucket -> (T, cells per Rmax)
+    Slot main;
 };
 
 struct pp_cloud {
@@ -76,16 +91,28 @@ struct pp_cloud {
     void *raw = nullptr;
     size_t raw_bytes = 0;
     float lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
-    bool resident = false;    // cell list fixed at upload
-    CellList cl;              // resident or cached per-job cell list
+    bool resident = false;       // cell lists fixed at upload
+    std::vector<CellList> lists; // resident widths, or the cached per-job list
+    std::mutex mu;               // a non-resident cloud serves one call at a time
+    double density() const
+    {
+        // ions per nm^3 inside the tip: the bounding box holds the tip with
+        // roughly two thirds filled (cylinders pi/4, frusta ~0.6)
+        double v = 1.0;
+        for (int a = 0; a < 3; ++a)
+            v *= std::max(1e-3, (double)hi[a] - (double)lo[a]);
+        return (double)n / v * 1.5;
+    }
 };
 
 namespace {
 
 pp_status fail(pp_ctx *c, pp_status st, const std::string &msg)
 {
-    if (c)
+    if (c) {
+        std::lock_guard<std::mutex> lk(c->mu);
         c->err = msg;
+    }
     return st;
 }
 
@@ -97,12 +124,12 @@ pp_status fail(pp_ctx *c, pp_status st, const std::string &msg)
                         std::string(#expr) + ": " + hipGetErrorString(_e));                  \
     } while (0)
 
-pp_status ensure(pp_ctx *c, DevBuf &b, size_t bytes)
+pp_status ensure(pp_ctx *c, Slot &s, DevBuf &b, size_t bytes)
 {
     if (b.cap >= bytes && b.p)
         return PP_OK;
     if (b.p) {
-        PP_HIP(c, hipStreamSynchronize(c->stream));
+        PP_HIP(c, hipStreamSynchronize(s.stream));
         PP_HIP(c, hipFree(b.p));
         b.p = nullptr;
         b.cap = 0;
@@ -111,6 +138,35 @@ pp_status ensure(pp_ctx *c, DevBuf &b, size_t bytes)
     PP_HIP(c, hipMalloc(&b.p, want));
     b.cap = want;
     return PP_OK;
+}
+
+pp_status slot_init(pp_ctx *c, Slot &s)
+{
+    PP_HIP(c, hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
+    for (auto &e : s.ev)
+        PP_HIP(c, hipEventCreate(&e));
+    return PP_OK;
+}
+
+void slot_free(Slot &s)
+{
+    if (s.stream)
+        (void)hipStreamSynchronize(s.stream);
+    for (DevBuf *b : {&s.scratch, &s.jobbuf, &s.cnbuf})
+        if (b->p) {
+            (void)hipFree(b->p);
+            b->p = nullptr;
+            b->cap = 0;
+        }
+    for (auto &e : s.ev)
+        if (e) {
+            (void)hipEventDestroy(e);
+            e = nullptr;
+        }
+    if (s.stream) {
+        (void)hipStreamDestroy(s.stream);
+        s.stream = nullptr;
+    }
 }
 
 inline bool has_type(const uint8_t *mask32, int t) { return (mask32[t >> 3] >> (t & 7)) & 1; }
@@ -135,6 +191,7 @@ struct JobPlan {
     std::vector<int32_t> kth_to_u;  // kdesc entry -> unique index
     int kmax = 0, nku = 0;
     int cat0 = 0, cat_sentinel = 0;
+    int r = 0;          // cells per Rmax of this job's cell list
 };
 
 pp_status plan_job(pp_ctx *c, const pp_job *j, JobPlan &p)
@@ -264,11 +321,11 @@ std::vector<Row> make_rows(double w, double R)
     return rows;
 }
 
-pp_status build_celllist(pp_ctx *c, pp_cloud *cl, float w, CellList &L)
+
+pp_status build_celllist(pp_ctx *c, Slot &s, pp_cloud *cl, float w, CellList &L)
 {
     const uint64_t n = cl->n;
     GridGeom g{};
-    g.w = w;
     // grow the cell when the grid would be huge compared with the cloud
     double ww = w;
     for (;;) {
@@ -291,14 +348,13 @@ pp_status build_celllist(pp_ctx *c, pp_cloud *cl, float w, CellList &L)
     for (int a = 0; a < 3; ++a)
         g.lo[a] = cl->lo[a];
     const uint64_t nc = g.ncells();
-    // persistent part
     Carver m0(nullptr);
     m0.take<uint32_t>(nc + 1);
     m0.take<float4>(n);
     m0.take<float>(n);
     const size_t bytes = m0.off + 256;
     if (L.mem && L.bytes < bytes) {
-        PP_HIP(c, hipStreamSynchronize(c->stream));
+        PP_HIP(c, hipStreamSynchronize(s.stream));
         PP_HIP(c, hipFree(L.mem));
         L.mem = nullptr;
         L.bytes = 0;
@@ -312,26 +368,25 @@ pp_status build_celllist(pp_ctx *c, pp_cloud *cl, float w, CellList &L)
     L.pos = m.take<float4>(n);
     L.s2 = m.take<float>(n);
     L.g = g;
-    L.w = (float)ww;
-    // temporaries
+    L.w = w;
     Carver t0(nullptr);
     t0.take<uint32_t>(nc);
     t0.take<uint32_t>(n);
     t0.take<uint32_t>(n);
     t0.take<uint32_t>(scan_workspace_words(nc));
-    pp_status st = ensure(c, c->scratch, t0.off + 256);
+    pp_status st = ensure(c, s, s.scratch, t0.off + 256);
     if (st != PP_OK)
         return st;
-    Carver t(c->scratch.p);
+    Carver t(s.scratch.p);
     uint32_t *count = t.take<uint32_t>(nc);
     uint32_t *key = t.take<uint32_t>(n);
     uint32_t *rank = t.take<uint32_t>(n);
     uint32_t *ws = t.take<uint32_t>(scan_workspace_words(nc));
-    PP_HIP(c, hipMemsetAsync(count, 0, nc * sizeof(uint32_t), c->stream));
-    PP_HIP(c, launch_cellkeys(cl->xyz, n, g, count, key, rank, c->stream));
-    PP_HIP(c, scan_exclusive(count, L.cellstart, nc, ws, c->stream));
+    PP_HIP(c, hipMemsetAsync(count, 0, nc * sizeof(uint32_t), s.stream));
+    PP_HIP(c, launch_cellkeys(cl->xyz, n, g, count, key, rank, s.stream));
+    PP_HIP(c, scan_exclusive(count, L.cellstart, nc, ws, s.stream));
     PP_HIP(c, launch_scatter(cl->xyz, cl->type, cl->s2, n, key, rank, L.cellstart, L.pos, L.s2,
-                             c->stream));
+                             s.stream));
     return PP_OK;
 }
 
@@ -342,7 +397,7 @@ struct JobRun {
     const CellList *cl = nullptr;
     std::vector<Row> rows;
     size_t ncells = 0;
-    // device pointers (carved from the scratch arena)
+    // device pointers (carved from the slot's scratch arena)
     uint32_t *flags = nullptr, *cpos = nullptr, *centers = nullptr, *ntile = nullptr,
              *toff = nullptr, *ws = nullptr, *tables = nullptr, *maxcand = nullptr;
     Tile *tiles = nullptr;
@@ -354,6 +409,7 @@ struct JobRun {
            o_rows = 0, tables_words = 0;
     // read back after prep
     uint32_t h_counts[4] = {0, 0, 0, 0};  // considered, tiles, maxcand, pad
+    std::vector<uint32_t> h_tables;       // host staging of the device tables
     JobDev dev{};
 };
 
@@ -368,8 +424,7 @@ void layout_job(JobRun &r, Carver &m)
     r.ntile = m.take<uint32_t>(nc);
     r.toff = m.take<uint32_t>(nc + 1);
     r.ws = m.take<uint32_t>(scan_workspace_words(std::max<uint64_t>(n, nc)));
-    // tiles: at most one per considered center
-    r.tiles = m.take<Tile>(n + 1);
+    r.tiles = m.take<Tile>(n + 1);  // at most one tile per considered center
     size_t o = 0;
     r.o_edges = o;
     o += p.C;
@@ -395,15 +450,30 @@ void layout_job(JobRun &r, Carver &m)
     // CN is carved later: its size follows from the stencil bound
 }
 
-pp_status prep_tiles(pp_ctx *c, JobRun &r, int T);
+pp_status prep_tiles(pp_ctx *c, Slot &sl, JobRun &r, int T)
+{
+    const uint64_t n = r.cloud->n;
+    const uint64_t nc = r.ncells;
+    hipStream_t s = sl.stream;
+    PP_HIP(c, hipMemsetAsync(r.maxcand, 0, 16, s));
+    PP_HIP(c, launch_tilecount(r.cl->cellstart, r.cpos, nc, T, r.ntile, s));
+    PP_HIP(c, scan_exclusive(r.ntile, r.toff, nc, r.ws, s));
+    PP_HIP(c, launch_tilewrite(r.cl->cellstart, r.cpos, r.toff, nc, T, 0, r.cl->g,
+                               (const Row *)(r.tables + r.o_rows), (int)r.rows.size(), r.tiles,
+                               r.maxcand, s));
+    PP_HIP(c, hipMemcpyAsync(&r.h_counts[0], r.cpos + n, 4, hipMemcpyDeviceToHost, s));
+    PP_HIP(c, hipMemcpyAsync(&r.h_counts[1], r.toff + nc, 4, hipMemcpyDeviceToHost, s));
+    PP_HIP(c, hipMemcpyAsync(&r.h_counts[2], r.maxcand, 4, hipMemcpyDeviceToHost, s));
+    return PP_OK;
+}
 
-pp_status prep_job(pp_ctx *c, JobRun &r, int T)
+pp_status prep_job(pp_ctx *c, Slot &sl, JobRun &r, int T)
 {
     const JobPlan &p = r.plan;
     const uint64_t n = r.cloud->n;
-    hipStream_t s = c->stream;
-    // tables
-    std::vector<uint32_t> h(r.tables_words, 0u);
+    hipStream_t s = sl.stream;
+    std::vector<uint32_t> &h = r.h_tables;  // lives until the job's results are back
+    h.assign(r.tables_words, 0u);
     std::memcpy(&h[r.o_edges], p.et.edges.data(), p.C * 4);
     std::memcpy(&h[r.o_envoff], p.envoff, 256 * 4);
     std::memcpy(&h[r.o_trgslot], p.trgslot, 256 * 4);
@@ -420,29 +490,11 @@ pp_status prep_job(pp_ctx *c, JobRun &r, int T)
     PP_HIP(c, hipMemsetAsync(r.stats, 0, sizeof(unsigned long long) * 4, s));
     PP_HIP(c, hipMemsetAsync(r.K, 0, 4 * (size_t)std::max(p.ntasks * p.nku * p.C, 1), s));
     PP_HIP(c, hipMemsetAsync(r.N1, 0, 4 * (size_t)std::max(p.ntasks * p.C, 1), s));
-    // centers
     const int32_t *d_trgslot = (const int32_t *)(r.tables + r.o_trgslot);
     PP_HIP(c, launch_flags(r.cl->pos, r.cl->s2, n, d_trgslot, p.R2, r.flags, s));
     PP_HIP(c, scan_exclusive(r.flags, r.cpos, n, r.ws, s));
     PP_HIP(c, launch_compact(r.flags, r.cpos, n, r.centers, s));
-    return prep_tiles(c, r, T);
-}
-
-pp_status prep_tiles(pp_ctx *c, JobRun &r, int T)
-{
-    const uint64_t n = r.cloud->n;
-    const uint64_t nc = r.ncells;
-    hipStream_t s = c->stream;
-    PP_HIP(c, hipMemsetAsync(r.maxcand, 0, 16, s));
-    PP_HIP(c, launch_tilecount(r.cl->cellstart, r.cpos, nc, T, r.ntile, s));
-    PP_HIP(c, scan_exclusive(r.ntile, r.toff, nc, r.ws, s));
-    PP_HIP(c, launch_tilewrite(r.cl->cellstart, r.cpos, r.toff, nc, T, 0, r.cl->g,
-                               (const Row *)(r.tables + r.o_rows), (int)r.rows.size(), r.tiles,
-                               r.maxcand, s));
-    PP_HIP(c, hipMemcpyAsync(&r.h_counts[0], r.cpos + n, 4, hipMemcpyDeviceToHost, s));
-    PP_HIP(c, hipMemcpyAsync(&r.h_counts[1], r.toff + nc, 4, hipMemcpyDeviceToHost, s));
-    PP_HIP(c, hipMemcpyAsync(&r.h_counts[2], r.maxcand, 4, hipMemcpyDeviceToHost, s));
-    return PP_OK;
+    return prep_tiles(c, sl, r, T);
 }
 
 void fill_jobdev(JobRun &r)
@@ -482,33 +534,6 @@ void fill_jobdev(JobRun &r)
     d.cn_cap = r.cn_cap;
     d.percenter = p.percenter ? 1 : 0;
     d.stats = r.stats;
-}
-
-int choose_cells_per_rmax(pp_ctx *c)
-{
-    // finer cells cut the candidates per in-ball pair (6.3 at Rmax, 3.7 at
-    // Rmax/2, 2.3 at Rmax/4) for a few more, shorter stencil rows; Rmax/4
-    // measured fastest on the 20M-ion tip
-    return c->tune_r > 0 ? c->tune_r : 4;
-}
-
-pp_status ensure_celllist(pp_ctx *c, pp_cloud *cl, const JobPlan &p, const CellList *&out)
-{
-    if (cl->resident) {
-        out = &cl->cl;
-        return PP_OK;
-    }
-    const int r = choose_cells_per_rmax(c);
-    const float w = (float)((double)p.R / r * (1.0 + std::ldexp(1.0, -10)));
-    if (!(cl->cl.mem && cl->cl.w == w)) {
-        // a cell list at a different width is rebuilt (grid growth may widen it)
-        pp_status st = build_celllist(c, cl, w, cl->cl);
-        if (st != PP_OK)
-            return st;
-        cl->cl.w = w;
-    }
-    out = &cl->cl;
-    return PP_OK;
 }
 
 void compose_outputs(const JobRun &r, const std::vector<unsigned long long> &P,
@@ -578,36 +603,160 @@ void compose_outputs(const JobRun &r, const std::vector<unsigned long long> &P,
     }
 }
 
-int choose_T(pp_ctx *c, const JobPlan &p)
+
+// ---------------------------------------------------------------------------
+// kernel shape: centers per tile and cells per Rmax
+
+// Density rule: cells of Rmax / r with r as large as possible (fewer
+// candidates per in-ball pair: 6.3 at r = 1, 3.7 at 2, 2.3 at 4) while a cell
+// still holds ~16 ions, so that stencil rows fill 64-lane chunks.
+int rule_r(double density, double R)
 {
-    if (c->tune_T > 0)
-        return c->tune_T;
-    (void)p;
-    return 8;  // measured fastest for both modes (fewer VGPRs, more resident waves)
+    const int r = (int)std::floor(R * std::cbrt(std::max(density, 1e-9) / 16.0));
+    return std::min(4, std::max(1, r));
 }
 
-pp_status run_jobs(pp_ctx *c, int njobs, const pp_cloud *const *clouds, const pp_job *jobs,
-                   pp_hist_out *outs)
+long tune_bucket(double density, const JobPlan &p)
+{
+    const double nball = density * 4.18879 * (double)p.R * p.R * p.R;
+    const long b = (long)std::lround(2.0 * std::log2(std::max(nball, 1.0)));
+    return b * 4 + (p.percenter ? 1 : 0) + ((p.families & (PP_FAM_RDF | PP_FAM_RIPK)) ? 2 : 0);
+}
+
+// Chooses the cell list a job runs on: a resident cloud offers fixed widths
+// (pick the one closest to Rmax / r in log scale), a non-resident one builds
+// (or reuses) the width Rmax / r.
+pp_status pick_celllist(pp_ctx *c, Slot &s, pp_cloud *cl, const JobPlan &p,
+                        const CellList *&out)
+{
+    const double want = (double)p.R / p.r * (1.0 + std::ldexp(1.0, -10));
+    if (cl->resident) {
+        double best = 1e300;
+        for (const CellList &L : cl->lists) {
+            const double d = std::fabs(std::log(L.g.w / want));
+            if (d < best) {
+                best = d;
+                out = &L;
+            }
+        }
+        return out ? PP_OK : fail(c, PP_ERR_INVALID_ARG, "resident cloud without a cell list");
+    }
+    if (cl->lists.empty())
+        cl->lists.emplace_back();
+    CellList &L = cl->lists[0];
+    if (!(L.mem && L.w == (float)want)) {
+        pp_status st = build_celllist(c, s, cl, (float)want, L);
+        if (st != PP_OK)
+            return st;
+    }
+    out = &L;
+    return PP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// running jobs
+
+struct Shape {
+    int T = 0;  // 0: choose
+    int r = 0;  // 0: choose
+};
+
+pp_status run_jobs(pp_ctx *c, Slot &sl, int njobs, const pp_cloud *const *clouds,
+                   const pp_job *jobs, pp_hist_out *outs, Shape force = Shape());
+
+// Times the candidate shapes on this job and keeps the fastest for its bucket.
+pp_status autotune(pp_ctx *c, Slot &sl, const pp_cloud *cl, const pp_job *job, const JobPlan &p,
+                   long bucket)
+{
+    pp_hist_out scratch;
+    double best = 1e300;
+    std::pair<int, int> win(8, rule_r(cl->density(), p.R));
+    for (int T : {8, 16})
+        for (int r = 1; r <= 4; ++r) {
+            std::memset(&scratch, 0, sizeof(scratch));
+            Shape f;
+            f.T = T;
+            f.r = r;
+            pp_status st = run_jobs(c, sl, 1, &cl, job, &scratch, f);
+            if (st != PP_OK)
+                continue;
+            if (scratch.ms_query < best) {
+                best = scratch.ms_query;
+                win = {T, r};
+            }
+        }
+    std::lock_guard<std::mutex> lk(c->mu);
+    c->tuned[bucket] = win;
+    return PP_OK;
+}
+
+pp_status run_jobs(pp_ctx *c, Slot &sl, int njobs, const pp_cloud *const *clouds,
+                   const pp_job *jobs, pp_hist_out *outs, Shape force)
 {
     auto t_start = std::chrono::steady_clock::now();
+    if (!outs)
+        return fail(c, PP_ERR_INVALID_ARG, "outputs are NULL");
     std::vector<JobRun> runs(njobs);
+    int tuneT, tuneR;
+    {
+        std::lock_guard<std::mutex> lk(c->mu);
+        tuneT = c->tune_T;
+        tuneR = c->tune_r;
+    }
     int T = 0;
     for (int i = 0; i < njobs; ++i) {
         if (!clouds[i])
             return fail(c, PP_ERR_INVALID_ARG, "cloud is NULL");
-        if (!outs)
-            return fail(c, PP_ERR_INVALID_ARG, "outputs are NULL");
         pp_status st = plan_job(c, &jobs[i], runs[i].plan);
         if (st != PP_OK)
             return st;
         runs[i].cloud = clouds[i];
-        T = std::max(T, choose_T(c, runs[i].plan));
+        JobPlan &p = runs[i].plan;
+        int jT = 8;
+        p.r = rule_r(clouds[i]->density(), p.R);
+        if (force.T > 0 || force.r > 0) {
+            if (force.T > 0)
+                jT = force.T;
+            if (force.r > 0)
+                p.r = force.r;
+        } else if (tuneT < 0) {
+            const long b = tune_bucket(clouds[i]->density(), p);
+            bool have;
+            {
+                std::lock_guard<std::mutex> lk(c->mu);
+                have = c->tuned.count(b) != 0;
+            }
+            if (!have) {
+                st = autotune(c, sl, clouds[i], &jobs[i], p, b);
+                if (st != PP_OK)
+                    return st;
+            }
+            std::lock_guard<std::mutex> lk(c->mu);
+            jT = c->tuned[b].first;
+            p.r = c->tuned[b].second;
+        } else {
+            if (tuneT > 0)
+                jT = tuneT;
+            if (tuneR > 0)
+                p.r = tuneR;
+        }
+        T = std::max(T, jT);
     }
     PP_HIP(c, hipSetDevice(c->device));
-    PP_HIP(c, hipEventRecord(c->ev[0], c->stream));
-    // cell lists (per-job ones are rebuilt only when the width changes)
+    // a non-resident cloud's cached cell list is rebuilt per width: hold it
+    std::vector<std::unique_lock<std::mutex>> locks;
+    {
+        std::set<pp_cloud *> own;
+        for (int i = 0; i < njobs; ++i)
+            if (!clouds[i]->resident)
+                own.insert(const_cast<pp_cloud *>(clouds[i]));
+        for (pp_cloud *cl : own)
+            locks.emplace_back(cl->mu);
+    }
+    PP_HIP(c, hipEventRecord(sl.ev[0], sl.stream));
     for (int i = 0; i < njobs; ++i) {
-        pp_status st = ensure_celllist(c, const_cast<pp_cloud *>(clouds[i]), runs[i].plan, runs[i].cl);
+        pp_status st = pick_celllist(c, sl, const_cast<pp_cloud *>(clouds[i]), runs[i].plan,
+                                     runs[i].cl);
         if (st != PP_OK)
             return st;
         runs[i].rows = make_rows(runs[i].cl->g.w, runs[i].plan.R);
@@ -615,25 +764,24 @@ pp_status run_jobs(pp_ctx *c, int njobs, const pp_cloud *const *clouds, const pp
             return fail(c, PP_ERR_UNSUPPORTED, "stencil too large (cell width far below Rmax)");
         runs[i].ncells = runs[i].cl->g.ncells();
     }
-    PP_HIP(c, hipEventRecord(c->ev[1], c->stream));
-    // job arenas (after the cell lists: build_celllist may have used scratch)
+    PP_HIP(c, hipEventRecord(sl.ev[1], sl.stream));
+    // job arenas (after the cell lists: a build may have used the scratch)
     Carver m0(nullptr);
     for (int i = 0; i < njobs; ++i)
         layout_job(runs[i], m0);
-    pp_status st = ensure(c, c->scratch, m0.off + 256);
+    pp_status st = ensure(c, sl, sl.scratch, m0.off + 256);
     if (st != PP_OK)
         return st;
-    Carver m(c->scratch.p);
+    Carver m(sl.scratch.p);
     for (int i = 0; i < njobs; ++i)
         layout_job(runs[i], m);
     for (int i = 0; i < njobs; ++i) {
-        st = prep_job(c, runs[i], T);
+        st = prep_job(c, sl, runs[i], T);
         if (st != PP_OK)
             return st;
     }
-    PP_HIP(c, hipStreamSynchronize(c->stream));
-    PP_HIP(c, hipEventRecord(c->ev[2], c->stream));
-    // launch configuration
+    PP_HIP(c, hipStreamSynchronize(sl.stream));
+    PP_HIP(c, hipEventRecord(sl.ev[2], sl.stream));
     int maxC = 1, maxE = 1, maxTrg = 1, maxK = 0, maxRows = 1;
     bool pc = false;
     uint32_t maxcand = 0, total = 0;
@@ -651,17 +799,17 @@ pp_status run_jobs(pp_ctx *c, int njobs, const pp_cloud *const *clouds, const pp
         total += runs[i].h_counts[1];
     }
     tile_off[njobs] = total;
-    int c16 = maxcand < 65535u ? 1 : 0;
+    const int c16 = maxcand < 65535u ? 1 : 0;
     size_t lds = query_lds_bytes(T, c16, pc, maxC, maxE, maxTrg, maxK, maxRows);
     while (lds > c->lds_per_block && T > 8) {
         // smaller tiles: rebuild the tile lists
         T /= 2;
         for (int i = 0; i < njobs; ++i) {
-            st = prep_tiles(c, runs[i], T);
+            st = prep_tiles(c, sl, runs[i], T);
             if (st != PP_OK)
                 return st;
         }
-        PP_HIP(c, hipStreamSynchronize(c->stream));
+        PP_HIP(c, hipStreamSynchronize(sl.stream));
         total = 0;
         for (int i = 0; i < njobs; ++i) {
             tile_off[i] = total;
@@ -672,37 +820,36 @@ pp_status run_jobs(pp_ctx *c, int njobs, const pp_cloud *const *clouds, const pp
     }
     if (lds > c->lds_per_block)
         return fail(c, PP_ERR_UNSUPPORTED, "histograms too large for LDS (bins x types x tile)");
-    // CNTNB counters: no neighbour count exceeds the largest stencil candidate count
     {
+        // CNTNB counters: no neighbour count exceeds the largest stencil candidate count
         size_t words = 0;
         for (int i = 0; i < njobs; ++i) {
             JobRun &r = runs[i];
             r.cn_cap = (r.plan.families & PP_FAM_CNTNB) ? (int)r.h_counts[2] + 1 : 1;
             words += ((size_t)std::max(r.plan.ntasks, 1) * (size_t)r.cn_cap + 63) & ~(size_t)63;
         }
-        st = ensure(c, c->cnbuf, words * 4 + 256);
+        st = ensure(c, sl, sl.cnbuf, words * 4 + 256);
         if (st != PP_OK)
             return st;
         size_t o = 0;
         for (int i = 0; i < njobs; ++i) {
             JobRun &r = runs[i];
-            r.CN = (uint32_t *)c->cnbuf.p + o;
+            r.CN = (uint32_t *)sl.cnbuf.p + o;
             const size_t w = (size_t)std::max(r.plan.ntasks, 1) * (size_t)r.cn_cap;
             o += (w + 63) & ~(size_t)63;
-            PP_HIP(c, hipMemsetAsync(r.CN, 0, w * 4, c->stream));
+            PP_HIP(c, hipMemsetAsync(r.CN, 0, w * 4, sl.stream));
             fill_jobdev(r);
         }
     }
-    // device job table: JobDev[njobs] | Tile*[njobs] | tile_off[njobs+1] | queue
     Carver jb0(nullptr);
     jb0.take<JobDev>(njobs);
     jb0.take<const Tile *>(njobs);
     jb0.take<uint32_t>(njobs + 1);
     jb0.take<uint32_t>(kXcds + 1);
-    st = ensure(c, c->jobbuf, jb0.off + 256);
+    st = ensure(c, sl, sl.jobbuf, jb0.off + 256);
     if (st != PP_OK)
         return st;
-    Carver jb(c->jobbuf.p);
+    Carver jb(sl.jobbuf.p);
     JobDev *d_jobs = jb.take<JobDev>(njobs);
     const Tile **d_tiles = jb.take<const Tile *>(njobs);
     uint32_t *d_off = jb.take<uint32_t>(njobs + 1);
@@ -713,9 +860,12 @@ pp_status run_jobs(pp_ctx *c, int njobs, const pp_cloud *const *clouds, const pp
         hj[i] = runs[i].dev;
         ht[i] = runs[i].tiles;
     }
-    PP_HIP(c, hipMemcpyAsync(d_jobs, hj.data(), sizeof(JobDev) * njobs, hipMemcpyHostToDevice, c->stream));
-    PP_HIP(c, hipMemcpyAsync(d_tiles, ht.data(), sizeof(Tile *) * njobs, hipMemcpyHostToDevice, c->stream));
-    PP_HIP(c, hipMemcpyAsync(d_off, tile_off.data(), 4 * (njobs + 1), hipMemcpyHostToDevice, c->stream));
+    PP_HIP(c, hipMemcpyAsync(d_jobs, hj.data(), sizeof(JobDev) * njobs, hipMemcpyHostToDevice,
+                             sl.stream));
+    PP_HIP(c, hipMemcpyAsync(d_tiles, ht.data(), sizeof(Tile *) * njobs, hipMemcpyHostToDevice,
+                             sl.stream));
+    PP_HIP(c, hipMemcpyAsync(d_off, tile_off.data(), 4 * (njobs + 1), hipMemcpyHostToDevice,
+                             sl.stream));
     int nb = 0;
     PP_HIP(c, query_max_blocks_per_cu(T, c16, pc, lds, &nb));
     if (nb < 1)
@@ -737,51 +887,89 @@ pp_status run_jobs(pp_ctx *c, int njobs, const pp_cloud *const *clouds, const pp
     q.maxRows = maxRows;
     const uint32_t want = (uint32_t)c->cu_count * (uint32_t)nb;
     q.grid = (int)std::max<uint32_t>(1u, std::min<uint32_t>(want, std::max<uint32_t>(total, 1u)));
-    PP_HIP(c, hipEventRecord(c->ev[3], c->stream));
-    PP_HIP(c, launch_query(q, c->stream));
-    PP_HIP(c, hipEventRecord(c->ev[4], c->stream));
+    PP_HIP(c, hipEventRecord(sl.ev[3], sl.stream));
+    PP_HIP(c, launch_query(q, sl.stream));
+    PP_HIP(c, hipEventRecord(sl.ev[4], sl.stream));
     // read back
+    struct Back {
+        std::vector<unsigned long long> P;
+        std::vector<uint32_t> K, N1, CN;
+        unsigned long long stats[4];
+    };
+    std::vector<Back> back(njobs);
     for (int i = 0; i < njobs; ++i) {
         JobRun &r = runs[i];
         const JobPlan &p = r.plan;
-        std::vector<unsigned long long> P((size_t)std::max(p.ntrg * p.E * p.C, 1));
-        std::vector<uint32_t> K((size_t)std::max(p.ntasks * p.nku * p.C, 1));
-        std::vector<uint32_t> N1((size_t)std::max(p.ntasks * p.C, 1));
-        std::vector<uint32_t> CN;
-        unsigned long long stats[4];
-        PP_HIP(c, hipMemcpyAsync(P.data(), r.P, P.size() * 8, hipMemcpyDeviceToHost, c->stream));
-        PP_HIP(c, hipMemcpyAsync(stats, r.stats, sizeof(stats), hipMemcpyDeviceToHost, c->stream));
+        Back &b = back[i];
+        b.P.assign((size_t)std::max(p.ntrg * p.E * p.C, 1), 0ull);
+        b.K.assign((size_t)std::max(p.ntasks * p.nku * p.C, 1), 0u);
+        b.N1.assign((size_t)std::max(p.ntasks * p.C, 1), 0u);
+        PP_HIP(c, hipMemcpyAsync(b.P.data(), r.P, b.P.size() * 8, hipMemcpyDeviceToHost, sl.stream));
+        PP_HIP(c, hipMemcpyAsync(b.stats, r.stats, sizeof(b.stats), hipMemcpyDeviceToHost,
+                                 sl.stream));
         if (p.families & PP_FAM_MKNN)
-            PP_HIP(c, hipMemcpyAsync(K.data(), r.K, K.size() * 4, hipMemcpyDeviceToHost, c->stream));
+            PP_HIP(c, hipMemcpyAsync(b.K.data(), r.K, b.K.size() * 4, hipMemcpyDeviceToHost,
+                                     sl.stream));
         if (p.families & PP_FAM_1NN)
-            PP_HIP(c, hipMemcpyAsync(N1.data(), r.N1, N1.size() * 4, hipMemcpyDeviceToHost, c->stream));
+            PP_HIP(c, hipMemcpyAsync(b.N1.data(), r.N1, b.N1.size() * 4, hipMemcpyDeviceToHost,
+                                     sl.stream));
         if (p.families & PP_FAM_CNTNB) {
-            // counts never exceed the largest stencil candidate count
-            CN.assign((size_t)p.ntasks * r.cn_cap, 0u);
-            PP_HIP(c, hipMemcpyAsync(CN.data(), r.CN, CN.size() * 4, hipMemcpyDeviceToHost,
-                                     c->stream));
+            b.CN.assign((size_t)p.ntasks * r.cn_cap, 0u);
+            PP_HIP(c, hipMemcpyAsync(b.CN.data(), r.CN, b.CN.size() * 4, hipMemcpyDeviceToHost,
+                                     sl.stream));
         }
-        PP_HIP(c, hipStreamSynchronize(c->stream));
+    }
+    PP_HIP(c, hipStreamSynchronize(sl.stream));
+    float ms_idx = 0, ms_prep = 0, ms_q = 0;
+    PP_HIP(c, hipEventElapsedTime(&ms_idx, sl.ev[0], sl.ev[1]));
+    PP_HIP(c, hipEventElapsedTime(&ms_prep, sl.ev[1], sl.ev[2]));
+    PP_HIP(c, hipEventElapsedTime(&ms_q, sl.ev[3], sl.ev[4]));
+    const double ms_total =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start)
+            .count();
+    bool any_overflow = false;
+    for (int i = 0; i < njobs; ++i) {
         bool overflow = false;
-        compose_outputs(r, P, K, N1, CN, stats, &outs[i], &overflow);
-        float ms_idx = 0, ms_prep = 0, ms_q = 0;
-        PP_HIP(c, hipEventElapsedTime(&ms_idx, c->ev[0], c->ev[1]));
-        PP_HIP(c, hipEventElapsedTime(&ms_prep, c->ev[1], c->ev[2]));
-        PP_HIP(c, hipEventElapsedTime(&ms_q, c->ev[3], c->ev[4]));
+        compose_outputs(runs[i], back[i].P, back[i].K, back[i].N1, back[i].CN, back[i].stats,
+                        &outs[i], &overflow);
         outs[i].ms_index = ms_idx;
         outs[i].ms_prep = ms_prep;
         outs[i].ms_query = ms_q;
+        outs[i].ms_total = ms_total;
         outs[i].tile_centers = T;
-        outs[i].cells_per_rmax = r.cloud->resident ? 0 : choose_cells_per_rmax(c);
+        outs[i].cells_per_rmax = runs[i].cloud->resident ? 0 : runs[i].plan.r;
         outs[i].counters16 = c16;
         outs[i].percenter = pc ? 1 : 0;
-        outs[i].ms_total = std::chrono::duration<double, std::milli>(
-                               std::chrono::steady_clock::now() - t_start)
-                               .count();
-        if (overflow)
-            return fail(c, PP_ERR_OVERFLOW, "a CNTNB neighbour count exceeded cntnb_cap");
+        any_overflow |= overflow;
     }
+    if (any_overflow)
+        return fail(c, PP_ERR_OVERFLOW, "a CNTNB neighbour count exceeded cntnb_cap");
     return PP_OK;
+}
+
+// Device bytes of one job's arena (admission control of the serving loop).
+size_t job_footprint(const pp_cloud *cl, const JobPlan &p)
+{
+    JobRun r;
+    r.plan = p;
+    r.cloud = cl;
+    size_t nc = 1;
+    double w = (double)p.R / std::max(p.r, 1);
+    if (cl->resident && !cl->lists.empty())
+        for (const CellList &L : cl->lists)
+            nc = std::max<size_t>(nc, L.g.ncells()), w = L.g.w;
+    else {
+        double v = 1.0;
+        for (int a = 0; a < 3; ++a)
+            v *= std::floor(((double)cl->hi[a] - (double)cl->lo[a]) / w) + 1.0;
+        nc = (size_t)v;
+    }
+    r.ncells = nc;
+    r.rows = make_rows(w, p.R);
+    Carver m(nullptr);
+    layout_job(r, m);
+    size_t cn = (p.families & PP_FAM_CNTNB) ? (size_t)p.ntasks * 65536 * 4 : 0;
+    return m.off + cn + 4096;
 }
 
 pp_status upload_common(pp_ctx *c, const float *xyz, const uint8_t *type, const float *s2,
@@ -796,7 +984,8 @@ pp_status upload_common(pp_ctx *c, const float *xyz, const uint8_t *type, const 
     if (cell_width < 0 || std::isnan(cell_width))
         return fail(c, PP_ERR_INVALID_ARG, "cell_width must be >= 0");
     PP_HIP(c, hipSetDevice(c->device));
-    pp_cloud *cl = new pp_cloud();
+    Slot &sl = c->main;
+    std::unique_ptr<pp_cloud> cl(new pp_cloud());
     cl->ctx = c;
     cl->n = n;
     Carver m0(nullptr);
@@ -804,37 +993,32 @@ pp_status upload_common(pp_ctx *c, const float *xyz, const uint8_t *type, const 
     m0.take<uint8_t>(n);
     m0.take<float>(n);
     cl->raw_bytes = m0.off + 256;
-    hipError_t e = hipMalloc(&cl->raw, cl->raw_bytes);
-    if (e != hipSuccess) {
-        delete cl;
-        return fail(c, e == hipErrorOutOfMemory ? PP_ERR_OOM_DEVICE : PP_ERR_HIP,
-                    std::string("hipMalloc: ") + hipGetErrorString(e));
-    }
+    PP_HIP(c, hipMalloc(&cl->raw, cl->raw_bytes));
+    auto cleanup = [&](pp_status st) {
+        (void)hipStreamSynchronize(sl.stream);
+        if (cl->raw)
+            (void)hipFree(cl->raw);
+        for (CellList &L : cl->lists)
+            if (L.mem)
+                (void)hipFree(L.mem);
+        return st;
+    };
     Carver m(cl->raw);
     cl->xyz = m.take<float>(3 * n);
     cl->type = m.take<uint8_t>(n);
     cl->s2 = m.take<float>(n);
-    auto cleanup = [&](pp_status st) {
-        if (cl->raw)
-            (void)hipFree(cl->raw);
-        if (cl->cl.mem)
-            (void)hipFree(cl->cl.mem);
-        delete cl;
-        return st;
-    };
     if (n > 0) {
-        if (hipMemcpyAsync(cl->xyz, xyz, 12 * n, kind, c->stream) != hipSuccess ||
-            hipMemcpyAsync(cl->type, type, n, kind, c->stream) != hipSuccess ||
-            hipMemcpyAsync(cl->s2, s2, 4 * n, kind, c->stream) != hipSuccess)
+        if (hipMemcpyAsync(cl->xyz, xyz, 12 * n, kind, sl.stream) != hipSuccess ||
+            hipMemcpyAsync(cl->type, type, n, kind, sl.stream) != hipSuccess ||
+            hipMemcpyAsync(cl->s2, s2, 4 * n, kind, sl.stream) != hipSuccess)
             return cleanup(fail(c, PP_ERR_HIP, "input copy failed"));
-        uint32_t *bb = nullptr;
-        if (ensure(c, c->scratch, 64) != PP_OK)
+        if (ensure(c, sl, sl.scratch, 64) != PP_OK)
             return cleanup(PP_ERR_OOM_DEVICE);
-        bb = (uint32_t *)c->scratch.p;
+        uint32_t *bb = (uint32_t *)sl.scratch.p;
         uint32_t h[6];
-        if (launch_bbox(cl->xyz, n, bb, c->stream) != hipSuccess ||
-            hipMemcpyAsync(h, bb, sizeof(h), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
-            hipStreamSynchronize(c->stream) != hipSuccess)
+        if (launch_bbox(cl->xyz, n, bb, sl.stream) != hipSuccess ||
+            hipMemcpyAsync(h, bb, sizeof(h), hipMemcpyDeviceToHost, sl.stream) != hipSuccess ||
+            hipStreamSynchronize(sl.stream) != hipSuccess)
             return cleanup(fail(c, PP_ERR_HIP, "bounding box failed"));
         for (int a = 0; a < 3; ++a) {
             cl->lo[a] = ord_to_float(h[a]);
@@ -844,18 +1028,98 @@ pp_status upload_common(pp_ctx *c, const float *xyz, const uint8_t *type, const 
         }
     }
     if (cell_width > 0) {
-        pp_status st = build_celllist(c, cl, cell_width, cl->cl);
+        cl->lists.emplace_back();
+        pp_status st = build_celllist(c, sl, cl.get(), cell_width, cl->lists.back());
         if (st != PP_OK)
             return cleanup(st);
         cl->resident = true;
-        if (hipStreamSynchronize(c->stream) != hipSuccess)
+        if (hipStreamSynchronize(sl.stream) != hipSuccess)
             return cleanup(fail(c, PP_ERR_HIP, "cell list build failed"));
     }
-    *out = cl;
+    *out = cl.release();
     return PP_OK;
 }
 
 } // namespace
+
+// ---------------------------------------------------------------------------
+// serving loop
+
+namespace {
+
+struct Request {
+    uint64_t ticket = 0;
+    const pp_cloud *cloud = nullptr;
+    pp_job job{};
+    std::vector<uint8_t> trg, env;
+    std::vector<int64_t> kd;
+    pp_hist_out *out = nullptr;
+    size_t footprint = 0;
+};
+
+} // namespace
+
+struct pp_server {
+    pp_ctx *ctx = nullptr;
+    pp_serve_opts opts{};
+    size_t budget = 0;
+    std::vector<std::unique_ptr<Slot>> slots;
+    std::vector<std::thread> workers;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<Request> pending;
+    size_t used = 0;
+    bool stop = false;
+    uint64_t next_ticket = 1, submitted = 0, completed = 0;
+    std::vector<std::pair<uint64_t, int32_t>> done;
+
+    void worker(int i)
+    {
+        (void)hipSetDevice(ctx->device);
+        Slot &sl = *slots[i];
+        for (;;) {
+            std::vector<Request> batch;
+            size_t fp = 0;
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                cv_work.wait(lk, [&] {
+                    return (stop && pending.empty()) ||
+                           (!pending.empty() && used + pending.front().footprint <= budget);
+                });
+                if (pending.empty())
+                    return;  // stop requested and nothing left
+                // every admitted request waiting now joins this batch (FIFO,
+                // stopping at the first one the budget cannot hold yet)
+                while (!pending.empty() && (int)batch.size() < opts.max_batch_jobs &&
+                       used + pending.front().footprint <= budget) {
+                    used += pending.front().footprint;
+                    fp += pending.front().footprint;
+                    batch.push_back(std::move(pending.front()));
+                    pending.pop_front();
+                }
+            }
+            std::vector<const pp_cloud *> cls;
+            std::vector<pp_job> jobs;
+            std::vector<pp_hist_out> outs;
+            for (Request &r : batch) {
+                cls.push_back(r.cloud);
+                jobs.push_back(r.job);
+                outs.push_back(*r.out);
+            }
+            const pp_status st = run_jobs(ctx, sl, (int)batch.size(), cls.data(), jobs.data(),
+                                          outs.data());
+            for (size_t k = 0; k < batch.size(); ++k)
+                *batch[k].out = outs[k];
+            std::lock_guard<std::mutex> lk(mu);
+            used -= fp;
+            completed += batch.size();
+            for (Request &r : batch)
+                done.emplace_back(r.ticket, (int32_t)st);
+            cv_done.notify_all();
+            cv_work.notify_all();
+        }
+    }
+};
 
 extern "C" {
 
@@ -876,20 +1140,15 @@ pp_status pp_spatstat_create(int32_t device, pp_ctx **out)
         return PP_ERR_NO_DEVICE;
     if (hipSetDevice(device) != hipSuccess)
         return PP_ERR_HIP;
-    pp_ctx *c = new pp_ctx();
+    std::unique_ptr<pp_ctx> c(new pp_ctx());
     c->device = device;
     c->cu_count = prop.multiProcessorCount;
     c->lds_per_block = prop.sharedMemPerBlock;
-    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
-        delete c;
+    if (slot_init(c.get(), c->main) != PP_OK) {
+        slot_free(c->main);
         return PP_ERR_HIP;
     }
-    for (auto &e : c->ev)
-        if (hipEventCreate(&e) != hipSuccess) {
-            delete c;
-            return PP_ERR_HIP;
-        }
-    *out = c;
+    *out = c.release();
     return PP_OK;
 }
 
@@ -898,17 +1157,7 @@ pp_status pp_spatstat_destroy(pp_ctx *c)
     if (!c)
         return PP_ERR_INVALID_ARG;
     (void)hipSetDevice(c->device);
-    (void)hipStreamSynchronize(c->stream);
-    if (c->scratch.p)
-        (void)hipFree(c->scratch.p);
-    if (c->jobbuf.p)
-        (void)hipFree(c->jobbuf.p);
-    if (c->cnbuf.p)
-        (void)hipFree(c->cnbuf.p);
-    for (auto &e : c->ev)
-        if (e)
-            (void)hipEventDestroy(e);
-    (void)hipStreamDestroy(c->stream);
+    slot_free(c->main);
     delete c;
     return PP_OK;
 }
@@ -936,16 +1185,36 @@ pp_status pp_cloud_from_device(pp_ctx *c, const float *d_xyz, const uint8_t *d_t
     return upload_common(c, d_xyz, d_type, d_s2, n, cell_width, hipMemcpyDeviceToDevice, out);
 }
 
+pp_status pp_cloud_add_cell_list(pp_ctx *c, pp_cloud *cl, float cell_width)
+{
+    if (!c || !cl || !(cell_width > 0))
+        return fail(c, PP_ERR_INVALID_ARG, "bad cloud or cell width");
+    if (!cl->resident)
+        return fail(c, PP_ERR_INVALID_ARG, "cloud is not resident (upload it with a cell width)");
+    PP_HIP(c, hipSetDevice(c->device));
+    CellList L;
+    pp_status st = build_celllist(c, c->main, cl, cell_width, L);
+    if (st != PP_OK) {
+        if (L.mem)
+            (void)hipFree(L.mem);
+        return st;
+    }
+    PP_HIP(c, hipStreamSynchronize(c->main.stream));
+    cl->lists.push_back(L);
+    return PP_OK;
+}
+
 pp_status pp_cloud_free(pp_ctx *c, pp_cloud *cl)
 {
     if (!c || !cl)
         return PP_ERR_INVALID_ARG;
     (void)hipSetDevice(c->device);
-    (void)hipStreamSynchronize(c->stream);
+    (void)hipDeviceSynchronize();
     if (cl->raw)
         (void)hipFree(cl->raw);
-    if (cl->cl.mem)
-        (void)hipFree(cl->cl.mem);
+    for (CellList &L : cl->lists)
+        if (L.mem)
+            (void)hipFree(L.mem);
     delete cl;
     return PP_OK;
 }
@@ -954,7 +1223,10 @@ pp_status pp_cloud_footprint(const pp_cloud *cl, uint64_t *bytes)
 {
     if (!cl || !bytes)
         return PP_ERR_INVALID_ARG;
-    *bytes = cl->raw_bytes + cl->cl.bytes;
+    size_t b = cl->raw_bytes;
+    for (const CellList &L : cl->lists)
+        b += L.bytes;
+    *bytes = b;
     return PP_OK;
 }
 
@@ -962,7 +1234,7 @@ pp_status pp_spatstat_run(pp_ctx *c, const pp_cloud *cl, const pp_job *job, pp_h
 {
     if (!c || !cl || !job || !out)
         return fail(c, PP_ERR_INVALID_ARG, "NULL argument");
-    return run_jobs(c, 1, &cl, job, out);
+    return run_jobs(c, c->main, 1, &cl, job, out);
 }
 
 pp_status pp_batch_run(pp_ctx *c, int32_t njobs, const pp_cloud *const *clouds, const pp_job *jobs,
@@ -974,20 +1246,132 @@ pp_status pp_batch_run(pp_ctx *c, int32_t njobs, const pp_cloud *const *clouds, 
         return fail(c, PP_ERR_INVALID_ARG, "at most 65535 jobs per batch");
     if (njobs == 0)
         return PP_OK;
-    return run_jobs(c, njobs, clouds, jobs, outs);
+    return run_jobs(c, c->main, njobs, clouds, jobs, outs);
 }
 
 pp_status pp_set_tuning(pp_ctx *c, int32_t centers_per_tile, int32_t cells_per_rmax)
 {
     if (!c)
         return PP_ERR_INVALID_ARG;
-    if (!(centers_per_tile == 0 || centers_per_tile == 8 || centers_per_tile == 16 ||
-          centers_per_tile == 32))
-        return fail(c, PP_ERR_INVALID_ARG, "centers_per_tile must be 0, 8, 16 or 32");
-    if (cells_per_rmax < 0 || cells_per_rmax > 4)
-        return fail(c, PP_ERR_INVALID_ARG, "cells_per_rmax must be 0..4");
+    const bool autot = centers_per_tile == -1 && cells_per_rmax == -1;
+    if (!autot && !(centers_per_tile == 0 || centers_per_tile == 8 || centers_per_tile == 16 ||
+                    centers_per_tile == 32))
+        return fail(c, PP_ERR_INVALID_ARG, "centers_per_tile must be 0, 8, 16 or 32 (-1: autotune)");
+    if (!autot && (cells_per_rmax < 0 || cells_per_rmax > 4))
+        return fail(c, PP_ERR_INVALID_ARG, "cells_per_rmax must be 0..4 (-1: autotune)");
+    std::lock_guard<std::mutex> lk(c->mu);
     c->tune_T = centers_per_tile;
     c->tune_r = cells_per_rmax;
+    return PP_OK;
+}
+
+pp_status pp_serve_create(pp_ctx *c, const pp_serve_opts *opts, pp_server **out)
+{
+    if (!c || !opts || !out)
+        return fail(c, PP_ERR_INVALID_ARG, "NULL argument");
+    if (opts->slots < 1 || opts->slots > 8 || opts->max_batch_jobs < 1 ||
+        opts->max_batch_jobs > 65535)
+        return fail(c, PP_ERR_INVALID_ARG, "slots must be 1..8, max_batch_jobs 1..65535");
+    PP_HIP(c, hipSetDevice(c->device));
+    std::unique_ptr<pp_server> s(new pp_server());
+    s->ctx = c;
+    s->opts = *opts;
+    s->budget = opts->hbm_budget_bytes;
+    if (s->budget == 0) {
+        size_t fr = 0, tot = 0;
+        PP_HIP(c, hipMemGetInfo(&fr, &tot));
+        s->budget = fr / 2;
+    }
+    for (int i = 0; i < opts->slots; ++i) {
+        s->slots.emplace_back(new Slot());
+        if (slot_init(c, *s->slots.back()) != PP_OK) {
+            for (auto &sl : s->slots)
+                slot_free(*sl);
+            return PP_ERR_HIP;
+        }
+    }
+    for (int i = 0; i < opts->slots; ++i)
+        s->workers.emplace_back(&pp_server::worker, s.get(), i);
+    *out = s.release();
+    return PP_OK;
+}
+
+pp_status pp_serve_submit(pp_server *s, const pp_cloud *cl, const pp_job *job, pp_hist_out *out,
+                          uint64_t *ticket)
+{
+    if (!s || !cl || !job || !out || !ticket)
+        return PP_ERR_INVALID_ARG;
+    if (!cl->resident)
+        return fail(s->ctx, PP_ERR_INVALID_ARG, "serving needs resident clouds");
+    Request r;
+    JobPlan p;
+    pp_status st = plan_job(s->ctx, job, p);
+    if (st != PP_OK)
+        return st;
+    p.r = rule_r(cl->density(), p.R);
+    r.cloud = cl;
+    r.job = *job;
+    const int nt = std::max(job->ntasks, 0);
+    r.trg.assign(job->trg_mask, job->trg_mask + 32 * (size_t)nt);
+    r.env.assign(job->env_mask, job->env_mask + 32 * (size_t)nt);
+    if (job->nk > 0 && job->kminus1_desc)
+        r.kd.assign(job->kminus1_desc, job->kminus1_desc + job->nk);
+    r.job.trg_mask = r.trg.data();
+    r.job.env_mask = r.env.data();
+    r.job.kminus1_desc = r.kd.empty() ? nullptr : r.kd.data();
+    r.out = out;
+    r.footprint = job_footprint(cl, p);
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (r.footprint > s->budget)
+        return fail(s->ctx, PP_ERR_ADMISSION_REJECTED, "request larger than the HBM budget");
+    r.ticket = s->next_ticket++;
+    *ticket = r.ticket;
+    ++s->submitted;
+    s->pending.push_back(std::move(r));
+    s->cv_work.notify_one();
+    return PP_OK;
+}
+
+pp_status pp_serve_poll(pp_server *s, uint64_t *tickets, int32_t *statuses, int32_t cap,
+                        int32_t *n)
+{
+    if (!s || !n || (cap > 0 && (!tickets || !statuses)))
+        return PP_ERR_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(s->mu);
+    const int32_t k = (int32_t)std::min<size_t>((size_t)std::max(cap, 0), s->done.size());
+    for (int32_t i = 0; i < k; ++i) {
+        tickets[i] = s->done[i].first;
+        statuses[i] = s->done[i].second;
+    }
+    s->done.erase(s->done.begin(), s->done.begin() + k);
+    *n = k;
+    return PP_OK;
+}
+
+pp_status pp_serve_drain(pp_server *s)
+{
+    if (!s)
+        return PP_ERR_INVALID_ARG;
+    std::unique_lock<std::mutex> lk(s->mu);
+    s->cv_done.wait(lk, [&] { return s->completed == s->submitted; });
+    return PP_OK;
+}
+
+pp_status pp_serve_destroy(pp_server *s)
+{
+    if (!s)
+        return PP_ERR_INVALID_ARG;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+    }
+    s->cv_work.notify_all();
+    for (auto &t : s->workers)
+        t.join();
+    (void)hipSetDevice(s->ctx->device);
+    for (auto &sl : s->slots)
+        slot_free(*sl);
+    delete s;
     return PP_OK;
 }
 

@@ -1,0 +1,75 @@
+"""End to end through the C++ host mirror: XML settings + task code -> gfx950
+path -> the reference's CSV files, whose counts must equal the oracle's."""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from paraprobe_amd import horderdist as H
+
+pytestmark = pytest.mark.gpu
+
+TYPES = {"Al": 1, "Sc": 2, "Zr": 3}
+XML = """<?xml version="1.0" encoding="utf-8"?>
+<Parameters>
+  <AnalysisSpatDistrType>1246</AnalysisSpatDistrType>
+  <DescrStatTasksCode>Al-Al;Sc-Sc;Al,Sc-Zr;Zr-Al,Sc</DescrStatTasksCode>
+  <SpatStatRadiusMin>0.0</SpatStatRadiusMin>
+  <SpatStatRadiusIncr>0.02</SpatStatRadiusIncr>
+  <SpatStatRadiusMax>1.2</SpatStatRadiusMax>
+  <SpatStatKNNOrder>1</SpatStatKNNOrder>
+  <SpatStatMKNNCode>1;3;10</SpatStatMKNNCode>
+</Parameters>
+"""
+
+
+def _read_counts(path, col=1, skip_head=5):
+    rows = [ln.split(";") for ln in open(path).read().splitlines()]
+    return rows
+
+
+def test_xml_to_csv(ctx, tmp_path):
+    print("test_xml_to_csv", flush=True)
+    rng = np.random.default_rng(12)
+    n = 6000
+    xyz = rng.uniform(0, 8, size=(n, 3)).astype(np.float32)
+    ty = rng.choice([1, 2, 3], size=n, p=[0.6, 0.3, 0.1]).astype(np.uint8)
+    s2 = rng.uniform(0, 4, size=n).astype(np.float32)
+    cl = ctx.upload(xyz, ty, s2)
+    try:
+        st = H.run_spatstat(ctx, cl, XML, 7, TYPES, 5000, 400.0, str(tmp_path))
+    finally:
+        cl.free()
+    assert st == 0
+    files = sorted(os.listdir(tmp_path))
+    tasks = [((1,), (1,)), ((2,), (2,)), ((1, 2), (3,)), ((3,), (1, 2))]
+    names = [("Al", "Al"), ("Sc", "Sc"), ("AlSc", "Zr"), ("Zr", "AlSc")]
+    ref, _ = O.spatstat(xyz, ty, s2, tasks, 0.0, 0.02, 1.2, 1 | 2 | 8 | 32, kvalues=[1, 3, 10])
+    nb = ref["rdf"].shape[1] - 2
+    for t, (a, b) in enumerate(names):
+        f = tmp_path / f"PARAPROBE.SimID.7.RDF.{a}.{b}.Original.csv"
+        rows = [ln.split(";") for ln in open(f).read().splitlines()]
+        # rows[7] lowest, rows[8 : 8 + nb] bins, rows[8 + nb] highest
+        got = [int(float(rows[7][1]))] + [int(float(r[1])) for r in rows[8:8 + nb]] + \
+              [int(float(rows[8 + nb][1]))]
+        np.testing.assert_array_equal(got, ref["rdf"][t])
+        f = tmp_path / f"PARAPROBE.SimID.7.1NN.{a}.{b}.Original.csv"
+        rows = [ln.split(";") for ln in open(f).read().splitlines()]
+        got = [int(float(rows[4][1]))] + [int(float(r[1])) for r in rows[5:5 + nb]] + \
+              [int(float(rows[5 + nb][1]))]
+        np.testing.assert_array_equal(got, ref["nn1"][t])
+        for kth, k in enumerate([10, 3, 1]):
+            f = tmp_path / f"PARAPROBE.SimID.7.MKNN{k}.{a}.{b}.Original.csv"
+            rows = [ln.split(";") for ln in open(f).read().splitlines()]
+            assert rows[0] == [f"MKNN{k}"]
+            got = [int(float(rows[4][1]))] + [int(float(r[1])) for r in rows[5:5 + nb]] + \
+                  [int(float(rows[5 + nb][1]))]
+            np.testing.assert_array_equal(got, ref["mknn"][t, kth])
+        f = tmp_path / f"PARAPROBE.SimID.7.NBCNTinRMAX.{a}.{b}.Original.csv"
+        rows = [ln.split(";") for ln in open(f).read().splitlines()][4:]
+        got = np.array([int(float(r[1])) for r in rows])
+        want = ref["cntnb"][t]
+        last = np.nonzero(want)[0].max() + 1
+        np.testing.assert_array_equal(got, want[:last])
+    assert len(files) == 4 * (1 + 1 + 3 + 1)
