@@ -15,6 +15,7 @@
 #include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <list>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -94,6 +95,16 @@ struct pp_cloud {
     bool resident = false;       // cell lists fixed at upload
     std::vector<CellList> lists; // resident widths, or the cached per-job list
     std::mutex mu;               // a non-resident cloud serves one call at a time
+    uint64_t type_count[256] = {};
+    // typed candidate lists: the ions of a type set, on the grid of one of
+    // the lists above (so the center tiles of that list index them directly)
+    struct SubList {
+        uint32_t mask[8];
+        const CellList *base;
+        CellList cl;
+    };
+    std::list<SubList> subs;
+    std::mutex sub_mu;
     double density() const
     {
         // ions per nm^3 inside the tip: the bounding box holds the tip with
@@ -322,6 +333,57 @@ std::vector<Row> make_rows(double w, double R)
 }
 
 
+// Cell list of n ions (device arrays xyz, type, s2) on grid g: key and rank
+// per ion, exclusive scan of the cell counts, scatter into cell-sorted float4.
+pp_status build_on_grid(pp_ctx *c, Slot &s, const float *xyz, const uint8_t *type,
+                        const float *s2, uint64_t n, const GridGeom &g, float w, CellList &L,
+                        size_t extra_scratch_bytes = 0)
+{
+    const uint64_t nc = g.ncells();
+    Carver m0(nullptr);
+    m0.take<uint32_t>(nc + 1);
+    m0.take<float4>(n);
+    m0.take<float>(n);
+    const size_t bytes = m0.off + 256;
+    if (L.mem && L.bytes < bytes) {
+        PP_HIP(c, hipStreamSynchronize(s.stream));
+        PP_HIP(c, hipFree(L.mem));
+        L.mem = nullptr;
+        L.bytes = 0;
+    }
+    if (!L.mem) {
+        PP_HIP(c, hipMalloc(&L.mem, bytes));
+        L.bytes = bytes;
+    }
+    Carver m(L.mem);
+    L.cellstart = m.take<uint32_t>(nc + 1);
+    L.pos = m.take<float4>(n);
+    L.s2 = m.take<float>(n);
+    L.g = g;
+    L.w = w;
+    // temporaries after the first extra_scratch_bytes of the scratch arena
+    Carver t0(nullptr);
+    t0.off = extra_scratch_bytes;
+    t0.take<uint32_t>(nc);
+    t0.take<uint32_t>(n);
+    t0.take<uint32_t>(n);
+    t0.take<uint32_t>(scan_workspace_words(nc));
+    pp_status st = ensure(c, s, s.scratch, t0.off + 256);
+    if (st != PP_OK)
+        return st;
+    Carver t(s.scratch.p);
+    t.off = extra_scratch_bytes;
+    uint32_t *count = t.take<uint32_t>(nc);
+    uint32_t *key = t.take<uint32_t>(n);
+    uint32_t *rank = t.take<uint32_t>(n);
+    uint32_t *ws = t.take<uint32_t>(scan_workspace_words(nc));
+    PP_HIP(c, hipMemsetAsync(count, 0, nc * sizeof(uint32_t), s.stream));
+    PP_HIP(c, launch_cellkeys(xyz, n, g, count, key, rank, s.stream));
+    PP_HIP(c, scan_exclusive(count, L.cellstart, nc, ws, s.stream));
+    PP_HIP(c, launch_scatter(xyz, type, s2, n, key, rank, L.cellstart, L.pos, L.s2, s.stream));
+    return PP_OK;
+}
+
 pp_status build_celllist(pp_ctx *c, Slot &s, pp_cloud *cl, float w, CellList &L)
 {
     const uint64_t n = cl->n;
@@ -347,46 +409,72 @@ pp_status build_celllist(pp_ctx *c, Slot &s, pp_cloud *cl, float w, CellList &L)
     g.inv_w = 1.0 / ww;
     for (int a = 0; a < 3; ++a)
         g.lo[a] = cl->lo[a];
-    const uint64_t nc = g.ncells();
-    Carver m0(nullptr);
-    m0.take<uint32_t>(nc + 1);
-    m0.take<float4>(n);
-    m0.take<float>(n);
-    const size_t bytes = m0.off + 256;
-    if (L.mem && L.bytes < bytes) {
-        PP_HIP(c, hipStreamSynchronize(s.stream));
-        PP_HIP(c, hipFree(L.mem));
-        L.mem = nullptr;
-        L.bytes = 0;
-    }
-    if (!L.mem) {
-        PP_HIP(c, hipMalloc(&L.mem, bytes));
-        L.bytes = bytes;
-    }
-    Carver m(L.mem);
-    L.cellstart = m.take<uint32_t>(nc + 1);
-    L.pos = m.take<float4>(n);
-    L.s2 = m.take<float>(n);
-    L.g = g;
-    L.w = w;
+    return build_on_grid(c, s, cl->xyz, cl->type, cl->s2, n, g, w, L);
+}
+
+// The typed candidate list of `mask` on the grid of `base` (built once per
+// cloud, base and type set; the caller holds no lock).
+pp_status get_sublist(pp_ctx *c, Slot &s, pp_cloud *cl, const CellList *base,
+                      const uint32_t mask[8], const CellList *&out)
+{
+    std::lock_guard<std::mutex> lk(cl->sub_mu);
+    for (const auto &sb : cl->subs)
+        if (sb.base == base && std::memcmp(sb.mask, mask, 32) == 0) {
+            out = &sb.cl;
+            return PP_OK;
+        }
+    const uint64_t n = cl->n;
     Carver t0(nullptr);
-    t0.take<uint32_t>(nc);
+    t0.take<uint32_t>(8);
     t0.take<uint32_t>(n);
-    t0.take<uint32_t>(n);
-    t0.take<uint32_t>(scan_workspace_words(nc));
+    t0.take<uint32_t>(n + 1);
+    t0.take<uint32_t>(scan_workspace_words(n));
+    const size_t head = t0.off;
+    t0.take<uint32_t>(4);
     pp_status st = ensure(c, s, s.scratch, t0.off + 256);
     if (st != PP_OK)
         return st;
     Carver t(s.scratch.p);
-    uint32_t *count = t.take<uint32_t>(nc);
-    uint32_t *key = t.take<uint32_t>(n);
-    uint32_t *rank = t.take<uint32_t>(n);
-    uint32_t *ws = t.take<uint32_t>(scan_workspace_words(nc));
-    PP_HIP(c, hipMemsetAsync(count, 0, nc * sizeof(uint32_t), s.stream));
-    PP_HIP(c, launch_cellkeys(cl->xyz, n, g, count, key, rank, s.stream));
-    PP_HIP(c, scan_exclusive(count, L.cellstart, nc, ws, s.stream));
-    PP_HIP(c, launch_scatter(cl->xyz, cl->type, cl->s2, n, key, rank, L.cellstart, L.pos, L.s2,
-                             s.stream));
+    uint32_t *dmask = t.take<uint32_t>(8);
+    uint32_t *flags = t.take<uint32_t>(n);
+    uint32_t *pos = t.take<uint32_t>(n + 1);
+    uint32_t *ws = t.take<uint32_t>(scan_workspace_words(n));
+    PP_HIP(c, hipMemcpyAsync(dmask, mask, 32, hipMemcpyHostToDevice, s.stream));
+    PP_HIP(c, launch_typeflags(cl->type, n, dmask, flags, s.stream));
+    PP_HIP(c, scan_exclusive(flags, pos, n, ws, s.stream));
+    uint32_t m = 0;
+    PP_HIP(c, hipMemcpyAsync(&m, pos + n, 4, hipMemcpyDeviceToHost, s.stream));
+    PP_HIP(c, hipStreamSynchronize(s.stream));
+    // gathered sub-cloud: kept in its own allocation until the list is built
+    Carver g0(nullptr);
+    g0.take<float>(3 * (size_t)m);
+    g0.take<uint8_t>(m);
+    g0.take<float>(m);
+    void *tmp = nullptr;
+    PP_HIP(c, hipMalloc(&tmp, g0.off + 256));
+    Carver gm(tmp);
+    float *sx = gm.take<float>(3 * (size_t)m);
+    uint8_t *sty = gm.take<uint8_t>(m);
+    float *ss2 = gm.take<float>(m);
+    hipError_t e = launch_gather(cl->xyz, cl->type, cl->s2, flags, pos, n, sx, sty, ss2, s.stream);
+    if (e != hipSuccess) {
+        (void)hipFree(tmp);
+        return fail(c, PP_ERR_HIP, std::string("gather: ") + hipGetErrorString(e));
+    }
+    cl->subs.emplace_back();
+    pp_cloud::SubList &sb = cl->subs.back();
+    std::memcpy(sb.mask, mask, 32);
+    sb.base = base;
+    st = build_on_grid(c, s, sx, sty, ss2, m, base->g, base->w, sb.cl, head);
+    (void)hipStreamSynchronize(s.stream);
+    (void)hipFree(tmp);
+    if (st != PP_OK) {
+        if (sb.cl.mem)
+            (void)hipFree(sb.cl.mem);
+        cl->subs.pop_back();
+        return st;
+    }
+    out = &sb.cl;
     return PP_OK;
 }
 
@@ -394,7 +482,8 @@ pp_status build_celllist(pp_ctx *c, Slot &s, pp_cloud *cl, float w, CellList &L)
 struct JobRun {
     JobPlan plan;
     const pp_cloud *cloud = nullptr;
-    const CellList *cl = nullptr;
+    const CellList *cl = nullptr;    // the full list: centers and tiles
+    const CellList *cand = nullptr;  // candidates: cl or a typed list on its grid
     std::vector<Row> rows;
     size_t ncells = 0;
     // device pointers (carved from the slot's scratch arena)
@@ -501,8 +590,9 @@ void fill_jobdev(JobRun &r)
 {
     const JobPlan &p = r.plan;
     JobDev &d = r.dev;
-    d.pos = r.cl->pos;
-    d.cellstart = r.cl->cellstart;
+    d.pos = r.cand->pos;
+    d.cellstart = r.cand->cellstart;
+    d.cpos = r.cl->pos;
     d.nx = r.cl->g.nx;
     d.ny = r.cl->g.ny;
     d.nz = r.cl->g.nz;
@@ -645,6 +735,13 @@ pp_status pick_celllist(pp_ctx *c, Slot &s, pp_cloud *cl, const JobPlan &p,
         cl->lists.emplace_back();
     CellList &L = cl->lists[0];
     if (!(L.mem && L.w == (float)want)) {
+        {
+            std::lock_guard<std::mutex> lk(cl->sub_mu);
+            for (auto &sb : cl->subs)
+                if (sb.cl.mem)
+                    (void)hipFree(sb.cl.mem);
+            cl->subs.clear();
+        }
         pp_status st = build_celllist(c, s, cl, (float)want, L);
         if (st != PP_OK)
             return st;
@@ -759,6 +856,24 @@ pp_status run_jobs(pp_ctx *c, Slot &sl, int njobs, const pp_cloud *const *clouds
                                      runs[i].cl);
         if (st != PP_OK)
             return st;
+        runs[i].cand = runs[i].cl;
+        {
+            // only the neighbour types are candidates when they are a minority
+            const JobPlan &p = runs[i].plan;
+            const pp_cloud *cl = clouds[i];
+            uint64_t inenv = 0;
+            uint32_t mask[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+            for (int e = 0; e < p.E; ++e) {
+                inenv += cl->type_count[p.env_type[e]];
+                mask[p.env_type[e] >> 5] |= 1u << (p.env_type[e] & 31);
+            }
+            if (cl->n > 0 && (double)inenv < 0.8 * (double)cl->n) {
+                st = get_sublist(c, sl, const_cast<pp_cloud *>(cl), runs[i].cl, mask,
+                                 runs[i].cand);
+                if (st != PP_OK)
+                    return st;
+            }
+        }
         runs[i].rows = make_rows(runs[i].cl->g.w, runs[i].plan.R);
         if ((int)runs[i].rows.size() > kMaxRows)
             return fail(c, PP_ERR_UNSUPPORTED, "stencil too large (cell width far below Rmax)");
@@ -932,6 +1047,8 @@ pp_status run_jobs(pp_ctx *c, Slot &sl, int njobs, const pp_cloud *const *clouds
         bool overflow = false;
         compose_outputs(runs[i], back[i].P, back[i].K, back[i].N1, back[i].CN, back[i].stats,
                         &outs[i], &overflow);
+        if (runs[i].cand != runs[i].cl)
+            outs[i].pairs_inball = -1;  // only neighbour types were scanned
         outs[i].ms_index = ms_idx;
         outs[i].ms_prep = ms_prep;
         outs[i].ms_query = ms_q;
@@ -1012,7 +1129,7 @@ pp_status upload_common(pp_ctx *c, const float *xyz, const uint8_t *type, const 
             hipMemcpyAsync(cl->type, type, n, kind, sl.stream) != hipSuccess ||
             hipMemcpyAsync(cl->s2, s2, 4 * n, kind, sl.stream) != hipSuccess)
             return cleanup(fail(c, PP_ERR_HIP, "input copy failed"));
-        if (ensure(c, sl, sl.scratch, 64) != PP_OK)
+        if (ensure(c, sl, sl.scratch, 64 + 1024) != PP_OK)
             return cleanup(PP_ERR_OOM_DEVICE);
         uint32_t *bb = (uint32_t *)sl.scratch.p;
         uint32_t h[6];
@@ -1020,6 +1137,15 @@ pp_status upload_common(pp_ctx *c, const float *xyz, const uint8_t *type, const 
             hipMemcpyAsync(h, bb, sizeof(h), hipMemcpyDeviceToHost, sl.stream) != hipSuccess ||
             hipStreamSynchronize(sl.stream) != hipSuccess)
             return cleanup(fail(c, PP_ERR_HIP, "bounding box failed"));
+        uint32_t *tc = bb + 8;
+        uint32_t htc[256];
+        if (hipMemsetAsync(tc, 0, 256 * 4, sl.stream) != hipSuccess ||
+            launch_typecount(cl->type, n, tc, sl.stream) != hipSuccess ||
+            hipMemcpyAsync(htc, tc, sizeof(htc), hipMemcpyDeviceToHost, sl.stream) != hipSuccess ||
+            hipStreamSynchronize(sl.stream) != hipSuccess)
+            return cleanup(fail(c, PP_ERR_HIP, "type count failed"));
+        for (int t = 0; t < 256; ++t)
+            cl->type_count[t] = htc[t];
         for (int a = 0; a < 3; ++a) {
             cl->lo[a] = ord_to_float(h[a]);
             cl->hi[a] = ord_to_float(h[3 + a]);
@@ -1215,6 +1341,9 @@ pp_status pp_cloud_free(pp_ctx *c, pp_cloud *cl)
     for (CellList &L : cl->lists)
         if (L.mem)
             (void)hipFree(L.mem);
+    for (auto &sb : cl->subs)
+        if (sb.cl.mem)
+            (void)hipFree(sb.cl.mem);
     delete cl;
     return PP_OK;
 }
@@ -1226,6 +1355,8 @@ pp_status pp_cloud_footprint(const pp_cloud *cl, uint64_t *bytes)
     size_t b = cl->raw_bytes;
     for (const CellList &L : cl->lists)
         b += L.bytes;
+    for (const auto &sb : cl->subs)
+        b += sb.cl.bytes;
     *bytes = b;
     return PP_OK;
 }

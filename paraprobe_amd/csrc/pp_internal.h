@@ -43,9 +43,11 @@ enum : uint32_t {
 // Everything a query kernel needs to know about one job; lives in device
 // memory as an array indexed by Tile::job.
 struct JobDev {
-    // cloud: cell-sorted ions (x, y, z, type in the bits of w) and cell offsets
+    // candidates: cell-sorted ions (x, y, z, type in the bits of w) and cell
+    // offsets -- every ion, or only the job's neighbour types on the same grid
     const float4 *pos;
     const uint32_t *cellstart;
+    const float4 *cpos;       // centers' cell-sorted ions (the full list)
     int nx, ny, nz;
     int nrows;
     const Row *rows;

@@ -281,6 +281,85 @@ hipError_t launch_scatter(const float *xyz, const uint8_t *type, const float *s2
 }
 
 // ---------------------------------------------------------------------------
+// typed candidate lists
+
+__global__ __launch_bounds__(256) void k_typeflags(const uint8_t *__restrict__ type, uint64_t n,
+                                                   const uint32_t *__restrict__ mask8,
+                                                   uint32_t *__restrict__ flags)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const uint32_t t = type[i];
+    flags[i] = (mask8[t >> 5] >> (t & 31)) & 1u;
+}
+
+hipError_t launch_typeflags(const uint8_t *type, uint64_t n, const uint32_t *mask8, uint32_t *flags,
+                            hipStream_t s)
+{
+    if (n == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_typeflags, dim3(grid_for(n, 256)), dim3(256), 0, s, type, n, mask8, flags);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void k_gather(const float *__restrict__ xyz,
+                                                const uint8_t *__restrict__ type,
+                                                const float *__restrict__ s2,
+                                                const uint32_t *__restrict__ flags,
+                                                const uint32_t *__restrict__ pos, uint64_t n,
+                                                float *__restrict__ oxyz,
+                                                uint8_t *__restrict__ otype,
+                                                float *__restrict__ os2)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !flags[i])
+        return;
+    const uint32_t d = pos[i];
+    oxyz[3 * (uint64_t)d] = xyz[3 * i];
+    oxyz[3 * (uint64_t)d + 1] = xyz[3 * i + 1];
+    oxyz[3 * (uint64_t)d + 2] = xyz[3 * i + 2];
+    otype[d] = type[i];
+    os2[d] = s2[i];
+}
+
+hipError_t launch_gather(const float *xyz, const uint8_t *type, const float *s2,
+                         const uint32_t *flags, const uint32_t *pos, uint64_t n, float *oxyz,
+                         uint8_t *otype, float *os2, hipStream_t s)
+{
+    if (n == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_gather, dim3(grid_for(n, 256)), dim3(256), 0, s, xyz, type, s2, flags, pos,
+                       n, oxyz, otype, os2);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void k_typecount(const uint8_t *__restrict__ type, uint64_t n,
+                                                   uint32_t *__restrict__ counts)
+{
+    __shared__ uint32_t h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        atomicAdd(&h[type[i]], 1u);
+    __syncthreads();
+    if (h[threadIdx.x])
+        atomicAdd(&counts[threadIdx.x], h[threadIdx.x]);
+}
+
+hipError_t launch_typecount(const uint8_t *type, uint64_t n, uint32_t *counts, hipStream_t s)
+{
+    if (n == 0)
+        return hipSuccess;
+    unsigned g = grid_for(n, 256);
+    if (g > 1024)
+        g = 1024;
+    hipLaunchKernelGGL(k_typecount, dim3(g), dim3(256), 0, s, type, n, counts);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // per-job centers and tiles
 
 __global__ __launch_bounds__(256) void k_flags(const float4 *__restrict__ pos,
@@ -577,7 +656,7 @@ __global__ __launch_bounds__(kBlock) void k_query(QArgs a)
 
         if (tid < cnt) {
             const uint32_t i = J.centers[t.first + tid];
-            const float4 p = J.pos[i];
+            const float4 p = J.cpos[i];
             s_cen[tid] = p;
             const uint32_t ty = __float_as_uint(p.w) & 0xFFu;
             const int eo = s_envoff[ty];

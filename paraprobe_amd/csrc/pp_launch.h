@@ -32,6 +32,16 @@ hipError_t launch_scatter(const float *xyz, const uint8_t *type, const float *s2
                           const uint32_t *key, const uint32_t *rank, const uint32_t *cellstart,
                           float4 *pos, float *s2sorted, hipStream_t s);
 
+// Typed candidate lists: flag the ions whose type is in a 256-bit mask,
+// and gather them (order kept) after an exclusive scan of the flags.
+hipError_t launch_typeflags(const uint8_t *type, uint64_t n, const uint32_t *mask8, uint32_t *flags,
+                            hipStream_t s);
+hipError_t launch_gather(const float *xyz, const uint8_t *type, const float *s2,
+                         const uint32_t *flags, const uint32_t *pos, uint64_t n, float *oxyz,
+                         uint8_t *otype, float *os2, hipStream_t s);
+// Per-type ion counts of a cloud (counts has 256 entries, zeroed by the caller).
+hipError_t launch_typecount(const uint8_t *type, uint64_t n, uint32_t *counts, hipStream_t s);
+
 // Per job: flag considered centers, compact them, count and write tiles.
 hipError_t launch_flags(const float4 *pos, const float *s2sorted, uint64_t n,
                         const int32_t *trgslot, float R2, uint32_t *flags, hipStream_t s);
