@@ -69,6 +69,168 @@ def cpu_baseline(tip, job, zwin, cores):
     return hist, st, dt, s2
 
 
+def _same(a, b) -> bool:
+    return all(np.array_equal(v, getattr(b, k)) for k, v in a.arrays().items())
+
+
+def batch_leg(ctx, args):
+    """256 clouds of uneven size, each with its own job: one persistent launch
+    (pp_batch_run) against the plain per-job launch loop (pp_spatstat_run)."""
+    from paraprobe_amd import workloads
+
+    t = time.perf_counter()
+    spec = workloads.batch_spec(args.batch_jobs)
+    tips = workloads.batch_clouds(spec)
+    gen_s = time.perf_counter() - t
+    clouds = [ctx.upload(tp.xyz, tp.types, tp.ion2surf_sqr) for tp in tips]
+    jobs = [it.job for it in spec]
+    nions = sum(tp.n for tp in tips)
+    del tips
+    if args.autotune:
+        ctx.set_tuning(-1, -1)
+    t = time.perf_counter()
+    ctx.run_batch(clouds, jobs)              # warmup: cell lists, typed lists, tuning
+    warm_s = time.perf_counter() - t
+    print(f"# batch: {len(jobs)} clouds, {nions} ions, generated in {gen_s:.1f} s, "
+          f"warmup {warm_s:.1f} s", file=sys.stderr, flush=True)
+    t = time.perf_counter()
+    plain = [ctx.run(c, j) for c, j in zip(clouds, jobs)]
+    plain_s = time.perf_counter() - t
+    t = time.perf_counter()
+    batched = ctx.run_batch(clouds, jobs)
+    batch_s = time.perf_counter() - t
+    parity = all(_same(a, b) for a, b in zip(plain, batched))
+    ctx.set_tuning(0, 0)
+    for c in clouds:
+        c.free()
+    print(f"# batch: plain loop {plain_s:.2f} s, batched {batch_s:.2f} s, parity {parity}",
+          file=sys.stderr, flush=True)
+    return {
+        "workload": f"{len(jobs)} clouds, ions log-uniform in [50k, 4M] ({nions} total), "
+                    "uniform or clustered frustum tips, one type pair each, Rmax U(1,5) nm, "
+                    "RDF + MKNN k=1..kmax (kmax in 1,10,50), bin 0.01 nm",
+        "autotune": bool(args.autotune),
+        "plain_jobs_per_s": len(jobs) / plain_s,
+        "batched_jobs_per_s": len(jobs) / batch_s,
+        "speedup": plain_s / batch_s,
+        "parity_batched_vs_plain": parity,
+    }
+
+
+def serve_leg(ctx, args):
+    """8 resident tips and a Poisson request stream: the continuous-batching
+    server against the plain per-request loop."""
+    import threading
+
+    from paraprobe_amd import workloads
+    from paraprobe_amd.capi import Server
+
+    t = time.perf_counter()
+    tips = workloads.serve_tips()
+    clouds = []
+    for tp in tips:
+        c = ctx.upload(tp.xyz, tp.types, tp.ion2surf_sqr, cell_width=1.0)
+        c.add_cell_list(1.3)
+        c.add_cell_list(0.6)
+        clouds.append(c)
+    nions = sum(tp.n for tp in tips)
+    del tips
+    M = args.serve_requests
+    stream = workloads.serve_stream(10_000)[:M]
+    print(f"# serve: 8 resident tips ({nions} ions) ready in {time.perf_counter() - t:.1f} s; "
+          f"{M} requests of the 10,000-request stream", file=sys.stderr, flush=True)
+    # build every (cloud, cell width, neighbour type) candidate list once, with
+    # jobs whose central type holds no ion (so no query work is done)
+    from paraprobe_amd import FAM_RDF, Job
+
+    for c in clouds:
+        for R in (1.5, 3.0, 5.0):
+            for e in (1, 2, 3):
+                ctx.run(c, Job([((7,), (e,))], 0.0, 0.01, R, FAM_RDF))
+    # plain loop, back to back: per-request service times
+    plain, svc = [], []
+    t0 = time.perf_counter()
+    for r in stream:
+        a = time.perf_counter()
+        plain.append(ctx.run(clouds[r.cloud], r.job))
+        svc.append(time.perf_counter() - a)
+    plain_s = time.perf_counter() - t0
+    plain_rate = M / plain_s
+
+    def served(rate):
+        """Runs the stream through the server; rate None submits everything at
+        once (capacity), otherwise Poisson arrivals at `rate` per second."""
+        gaps = np.random.default_rng(99).exponential(1.0 / rate, size=M) if rate else np.zeros(M)
+        arrive = np.cumsum(gaps)
+        done, lat, results = {}, {}, {}
+        with Server(ctx, slots=args.serve_slots, max_batch_jobs=args.serve_batch) as srv:
+            start = time.perf_counter()
+            tick = {}
+            stop = threading.Event()
+
+            def poller():
+                while not stop.is_set() or len(done) < M:
+                    for tk, st, res in srv.poll():
+                        done[tk] = time.perf_counter()
+                        results[tk] = (st, res)
+                    if len(done) >= M:
+                        break
+                    time.sleep(0.0002)
+
+            th = threading.Thread(target=poller)
+            th.start()
+            for i, r in enumerate(stream):
+                now = time.perf_counter() - start
+                if arrive[i] > now:
+                    time.sleep(arrive[i] - now)
+                tk = srv.submit(clouds[r.cloud], r.job)
+                tick[tk] = (i, start + arrive[i])
+            stop.set()
+            th.join()
+            end = max(done.values())
+        out = [None] * M
+        for tk, (i, a) in tick.items():
+            lat[i] = done[tk] - a
+            st, res = results[tk]
+            out[i] = res if st == 0 else None
+        return M / (end - start), np.array([lat[i] for i in range(M)]) * 1e3, out
+
+    cap_rate, _, res_cap = served(None)
+    lam = 0.7 * cap_rate
+    rate, lat_ms, res_open = served(lam)
+    parity = all(o is not None and _same(p, o) for p, o in zip(plain, res_cap)) and \
+        all(o is not None and _same(p, o) for p, o in zip(plain, res_open))
+    # the plain loop under the same arrivals: FIFO, one request at a time
+    arrive = np.cumsum(np.random.default_rng(99).exponential(1.0 / lam, size=M))
+    fin, plat = 0.0, []
+    for a, s_ in zip(arrive, svc):
+        fin = max(fin, a) + s_
+        plat.append(fin - a)
+    plat = np.array(plat) * 1e3
+    for c in clouds:
+        c.free()
+    print(f"# serve: plain {plain_rate:.1f} jobs/s, served capacity {cap_rate:.1f} jobs/s, "
+          f"open loop at {lam:.1f}/s: p50 {np.percentile(lat_ms, 50):.1f} ms "
+          f"p99 {np.percentile(lat_ms, 99):.1f} ms, parity {parity}", file=sys.stderr, flush=True)
+    return {
+        "workload": f"8 resident clustered frustum tips (1M-50M ions, {nions} total), first {M} "
+                    "requests of a 10,000-request stream: one type pair, Rmax U(1,5) nm, RDF or "
+                    "MKNN k=1..kmax (kmax U(1,100)), bin 0.01 nm",
+        "slots": args.serve_slots,
+        "max_batch_jobs": args.serve_batch,
+        "served_capacity_jobs_per_s": cap_rate,
+        "plain_jobs_per_s": plain_rate,
+        "speedup": cap_rate / plain_rate,
+        "poisson_rate_per_s": lam,
+        "served_jobs_per_s": rate,
+        "served_p50_ms": float(np.percentile(lat_ms, 50)),
+        "served_p99_ms": float(np.percentile(lat_ms, 99)),
+        "plain_p50_ms_same_arrivals": float(np.percentile(plat, 50)),
+        "plain_p99_ms_same_arrivals": float(np.percentile(plat, 99)),
+        "parity_served_vs_plain": parity,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -80,6 +242,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calib", action="store_true",
                     help="also run a 1 GiB device copy (HBM counter calibration)")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--batch-jobs", type=int, default=256)
+    ap.add_argument("--autotune", type=int, default=1)
+    ap.add_argument("--serve-requests", type=int, default=400)
+    ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-batch", type=int, default=32)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (every box has one MI355X); "
@@ -132,6 +301,11 @@ def main():
         gwin = ctx.run(cw, job)
         cw.free()
         cloud.free()
+        extra = {}
+        if not args.no_batch:
+            extra["batch"] = batch_leg(ctx, args)
+        if not args.no_serve:
+            extra["serve"] = serve_leg(ctx, args)
         if args.calib:
             # 1 GiB device-to-device copy: its FETCH_SIZE / WRITE_SIZE per
             # dispatch scale the counters of the query kernel to bytes
@@ -207,6 +381,7 @@ def main():
                       f"span), z-slab + KD-tree build {st['index_s']:.1f} s not timed; {model}",
             "parity_with_gpu": parity,
         }
+    line.update(extra)
     print(json.dumps(line), flush=True)
 
 

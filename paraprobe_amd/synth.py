@@ -106,3 +106,16 @@ def frustum_volume(r0: float, r1: float, h: float) -> float:
 
 
 ALL_PAIRS = [((a,), (b,)) for a in (1, 2, 3) for b in (1, 2, 3)]
+
+
+def scaled_frustum(n: int, seed: int, clustered: bool, density: float = 27.3) -> Tip:
+    """A frustum tip of the headline shape (r 40 -> 20 over h 250, i.e. radii
+    and height in the ratio 0.16 : 0.08 : 1) scaled to hold n ions at the given
+    density; clustered tips carry 50,000 clusters per 20M ions (B at 40%)."""
+    base = [0.95, 0.04, 0.01]
+    v = n / density
+    # V = pi h / 3 (r0^2 + r0 r1 + r1^2) with r0 = 0.16 h, r1 = 0.08 h
+    h = (v / (math.pi / 3.0 * (0.16 ** 2 + 0.16 * 0.08 + 0.08 ** 2))) ** (1.0 / 3.0)
+    nc = int(round(50_000 * n / 20_000_000)) if clustered else 0
+    return tip(n, seed, "frustum", 0.16 * h, 0.08 * h, h, base, nc, 1.0, 2.0,
+               cluster_fracs(base) if clustered else None)
