@@ -55,11 +55,44 @@ bool EdgeTable::build(const HistGeometry &g, std::string *why)
     edges.assign(C, 0u);
     const uint32_t top = 0x7F800000u;  // +inf: every finite non-negative d2 is below
     for (uint32_t c = 1; c < C; ++c) {
-        // smallest u in [0, top] with category(u) >= c (category(top) is highest)
-        uint32_t lo = 0, hi = top;
-        if (g.category_d2(f_of_bits(lo)) >= c) {
+        // smallest u in [0, top] with category(u) >= c (category(top) is
+        // highest).  Bracket it by galloping from the ideal edge position
+        // (start + (c - 1) / tmp)^2, then bisect: a few evaluations per edge.
+        if (g.category_d2(f_of_bits(0)) >= c) {
             edges[c] = 0;
             continue;
+        }
+        const double dc = g.start + (double)(c - 1) / g.tmp;
+        const float guess = (float)(dc * dc);
+        uint32_t u0;
+        std::memcpy(&u0, &guess, 4);
+        if (!(guess >= 0.f) || u0 > top)
+            u0 = top / 2;
+        uint32_t lo, hi;  // invariant: category(lo) < c <= category(hi)
+        if (g.category_d2(f_of_bits(u0)) >= c) {
+            hi = u0;
+            uint32_t step = 1;
+            for (;;) {
+                const uint32_t t = hi > step ? hi - step : 0;
+                if (t == 0 || g.category_d2(f_of_bits(t)) < c) {
+                    lo = t;
+                    break;
+                }
+                hi = t;
+                step = step < 0x40000000u ? step * 2 : step;
+            }
+        } else {
+            lo = u0;
+            uint32_t step = 1;
+            for (;;) {
+                const uint32_t t = (top - lo > step) ? lo + step : top;
+                if (t == top || g.category_d2(f_of_bits(t)) >= c) {
+                    hi = t;
+                    break;
+                }
+                lo = t;
+                step = step < 0x40000000u ? step * 2 : step;
+            }
         }
         while (hi - lo > 1) {
             const uint32_t mid = lo + (hi - lo) / 2;
