@@ -94,7 +94,10 @@ typedef struct pp_hist_out {
     int32_t tile_centers;       /* query-kernel configuration used: centers per tile, */
     int32_t cells_per_rmax;     /*   cell width as a fraction of Rmax (0: resident list), */
     int32_t counters16;         /*   16-bit per-center counters, */
-    int32_t percenter;          /*   per-center histograms */
+    int32_t percenter;          /*   per-center histograms: 0 none, 1 full range, 2 kNN range
+                                 *   only (with an exact full-range rerun of the centers
+                                 *   whose k-th neighbour lies beyond it) */
+    int64_t centers_rerun;      /* centers of that full-range rerun */
 } pp_hist_out;
 
 /* Context bound to one device; one per host thread (calls on a context are
@@ -144,6 +147,13 @@ pp_status pp_batch_run(pp_ctx *ctx, int32_t njobs, const pp_cloud *const *clouds
  * families) bucket times the candidate shapes and the winner is kept for the
  * bucket. */
 pp_status pp_set_tuning(pp_ctx *ctx, int32_t centers_per_tile, int32_t cells_per_rmax);
+
+/* kNN range of MKNN / 1NN jobs without CNTNB: per neighbour type, per-center
+ * histograms cover only the distance within which `factor` x k neighbours
+ * are expected at the cloud's mean density (default 3); centers with fewer
+ * than k neighbours there are re-run over the full range, so results do not
+ * depend on the factor.  0 = full-range per-center histograms always. */
+pp_status pp_set_knn_range(pp_ctx *ctx, float factor);
 
 /* ---- serving: a request loop over resident clouds (continuous batching) ----
  * Requests are queued by pp_serve_submit and admitted by their device-memory

@@ -48,6 +48,7 @@ EXPORTED = [
     "pp_spatstat_run",
     "pp_batch_run",
     "pp_set_tuning",
+    "pp_set_knn_range",
     "pp_cloud_add_cell_list",
     "pp_serve_create",
     "pp_serve_submit",
@@ -104,6 +105,7 @@ class _Out(C.Structure):
         ("cells_per_rmax", C.c_int32),
         ("counters16", C.c_int32),
         ("percenter", C.c_int32),
+        ("centers_rerun", C.c_int64),
     ]
 
 
@@ -133,6 +135,7 @@ def load_library() -> C.CDLL:
     lib.pp_spatstat_run.argtypes = [P, P, C.POINTER(_Job), C.POINTER(_Out)]
     lib.pp_batch_run.argtypes = [P, C.c_int32, C.POINTER(P), C.POINTER(_Job), C.POINTER(_Out)]
     lib.pp_set_tuning.argtypes = [P, C.c_int32, C.c_int32]
+    lib.pp_set_knn_range.argtypes = [P, C.c_float]
     lib.pp_cloud_add_cell_list.argtypes = [P, P, C.c_float]
     lib.pp_serve_create.argtypes = [P, C.POINTER(_ServeOpts), C.POINTER(P)]
     lib.pp_serve_submit.argtypes = [P, P, C.POINTER(_Job), C.POINTER(_Out), C.POINTER(C.c_uint64)]
@@ -207,6 +210,8 @@ class Result:
     ms_query: float
     ms_total: float
     kernel: str = ""
+    percenter: int = 0      # 0 none, 1 full-range per-center histograms, 2 kNN range only
+    centers_rerun: int = 0  # centers re-run over the full range (mode 2)
 
     def arrays(self) -> dict:
         return {k: getattr(self, k) for k in ("rdf", "nn1", "ripk", "mknn", "cntnb")
@@ -268,7 +273,9 @@ class _Prepared:
             ms_total=o.ms_total,
             kernel="k_query<%d,%s,%s> cells/Rmax=%d" % (
                 o.tile_centers, "true" if o.counters16 else "false",
-                "true" if o.percenter else "false", o.cells_per_rmax),
+                ("direct", "percenter", "hybrid")[o.percenter], o.cells_per_rmax),
+            percenter=o.percenter,
+            centers_rerun=o.centers_rerun,
         )
 
 
@@ -321,6 +328,11 @@ class SpatStat:
 
     def set_tuning(self, centers_per_tile: int = 0, cells_per_rmax: int = 0):
         self._check(self.lib.pp_set_tuning(self.ctx, centers_per_tile, cells_per_rmax))
+
+    def set_knn_range(self, factor: float = 3.0):
+        """kNN range factor (expected neighbours per k) of MKNN / 1NN jobs; 0
+        keeps full-range per-center histograms."""
+        self._check(self.lib.pp_set_knn_range(self.ctx, factor))
 
     def upload(self, xyz: np.ndarray, types: np.ndarray, ion2surf_sqr: np.ndarray,
                cell_width: float = 0.0) -> Cloud:

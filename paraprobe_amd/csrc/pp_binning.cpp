@@ -8,6 +8,8 @@ namespace pp {
 
 static constexpr double kEpsilon = 1.0e-6;  // src/PARAPROBE_Numerics.h:45
 
+static inline double ncat_of(const HistGeometry &g) { return (double)g.nbins + 2.0; }
+
 static inline float f_of_bits(uint32_t u)
 {
     float f;
@@ -134,7 +136,18 @@ bool EdgeTable::build(const HistGeometry &g, std::string *why)
     }
     idx_scale = (float)g.tmp;
     idx_off = (float)(1.5 - g.start * g.tmp);
+    idx_off1 = (float)(1.0 - g.start * g.tmp);
     return true;
+}
+
+float EdgeTable::amb_margin(const HistGeometry &g, double R) const
+{
+    // |y - ((d - start) tmp + 1)| <= |sqrt error| * scale + d |scale - tmp|
+    //   + |offset error| + fma rounding, each below 2^-22 relative to its own
+    // magnitude (2 ulp sqrt, half-ulp roundings); 5e-7 covers their sum
+    const double ymax = std::fabs(R) * g.tmp + std::fabs(1.0 - g.start * g.tmp) + ncat_of(g) + 1.0;
+    const double m = max_offset + 5e-7 * ymax + 1e-6;
+    return (float)std::min(m, 0.5);
 }
 
 } // namespace pp

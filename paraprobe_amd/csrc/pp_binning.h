@@ -43,6 +43,14 @@ struct EdgeTable {
     std::vector<uint32_t> edges;  // size ncat, edges[0] = 0 (unused)
     float idx_scale = 0, idx_off = 0;
     double max_offset = 0;        // worst distance of an edge from its ideal index
+    // Floor form of the same guess: y = fmaf(sqrt(d2), idx_scale, idx_off1)
+    // approximates the index coordinate (d - start) * tmp + 1 within
+    // y_err(R) (a v_sqrt_f32 of at most 2 ulp, the float scale and offset, one
+    // fma rounding).  Wherever y lies at least max_offset + y_err from every
+    // integer, floor(y) is the category; closer to an integer, the edge table
+    // decides.  amb_margin(R) = max_offset + y_err for distances up to R.
+    float idx_off1 = 0;
+    float amb_margin(const HistGeometry &g, double R) const;
     // Builds the table; returns false (with a reason) when the bins are too
     // narrow for a nearest-edge guess to be provably within one edge.
     bool build(const HistGeometry &g, std::string *why);

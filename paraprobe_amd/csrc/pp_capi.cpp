@@ -13,6 +13,8 @@
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <list>
@@ -67,6 +69,7 @@ struct Slot {
     DevBuf scratch;  // cell-list temporaries, job arenas
     DevBuf jobbuf;   // JobDev array + tile pointer table + tile offsets + queue
     DevBuf cnbuf;    // CNTNB counters
+    DevBuf kbuf;     // k-th neighbour categories of one launch chunk
     hipEvent_t ev[6] = {};
 };
 
@@ -79,6 +82,7 @@ struct pp_ctx {
     std::mutex mu;  // err, tuning state
     std::string err;
     int tune_T = 0, tune_r = 0;
+    double knn_factor = 3.0;  // kNN range: expected neighbours / k (0: full range)
     std::map<long, std::pair<int, int>> tuned;  // autotune bucket -> (T, cells per Rmax)
     Slot main;
 };
@@ -163,7 +167,7 @@ void slot_free(Slot &s)
 {
     if (s.stream)
         (void)hipStreamSynchronize(s.stream);
-    for (DevBuf *b : {&s.scratch, &s.jobbuf, &s.cnbuf})
+    for (DevBuf *b : {&s.scratch, &s.jobbuf, &s.cnbuf, &s.kbuf})
         if (b->p) {
             (void)hipFree(b->p);
             b->p = nullptr;
@@ -203,6 +207,22 @@ struct JobPlan {
     int kmax = 0, nku = 0;
     int cat0 = 0, cat_sentinel = 0;
     int r = 0;          // cells per Rmax of this job's cell list
+    // query mode: 0 pair counts only, 1 full-range per-center histograms,
+    // 2 hybrid (kNN-range per-center histograms + rerun of short centers)
+    int mode = 0;
+    int HK = 0, trunc_any = 0;
+    std::vector<int32_t> kinfo;      // [256][2]: kNN-range offset, threshold bits
+    std::vector<int32_t> ckoff;      // [kMaxEnv]
+    std::vector<uint32_t> task_ck;   // [ntasks]
+    int lmax = 1;                    // most tasks sharing one center type
+    std::vector<uint32_t> trg_ntask; // [ntrg]
+    std::vector<uint32_t> trg_tasks; // [ntrg][lmax]
+    // uint16 planes of k-th neighbour categories per center and task: the
+    // MKNN orders, then the 1NN
+    int nplanes() const
+    {
+        return ((families & PP_FAM_MKNN) ? nku : 0) + ((families & PP_FAM_1NN) ? 1 : 0);
+    }
 };
 
 pp_status plan_job(pp_ctx *c, const pp_job *j, JobPlan &p)
@@ -301,6 +321,20 @@ pp_status plan_job(pp_ctx *c, const pp_job *j, JobPlan &p)
             p.kth_to_u[q] = p.kidx_of_k[p.kdesc[q] + 1];
     }
     p.percenter = (fams & (PP_FAM_1NN | PP_FAM_CNTNB)) || ((fams & PP_FAM_MKNN) && p.nku > 0);
+    p.trg_ntask.assign(std::max(p.ntrg, 1), 0u);
+    p.lmax = 1;
+    for (int s = 0; s < p.ntrg; ++s) {
+        for (int k = 0; k < p.ntasks; ++k)
+            p.trg_ntask[s] += (p.task_trg[k] >> s) & 1u;
+        p.lmax = std::max(p.lmax, (int)p.trg_ntask[s]);
+    }
+    p.trg_tasks.assign((size_t)std::max(p.ntrg, 1) * p.lmax, 0u);
+    for (int s = 0; s < p.ntrg; ++s) {
+        int l = 0;
+        for (int k = 0; k < p.ntasks; ++k)
+            if ((p.task_trg[k] >> s) & 1u)
+                p.trg_tasks[(size_t)s * p.lmax + l++] = (uint32_t)k;
+    }
     p.cat0 = (int)p.hg.category(0.0f);
     p.cat_sentinel = (int)p.hg.category((float)((double)p.R + kEps));
     return PP_OK;
@@ -332,6 +366,94 @@ std::vector<Row> make_rows(double w, double R)
     return rows;
 }
 
+
+// Hybrid mode for kNN jobs: per neighbour type e a range rho_e in which the
+// k-th neighbour is expected with margin (factor * k neighbours expected at
+// the cloud's mean density), as a number of complete categories ck_e.  Pairs
+// within it also enter the center's own histogram; RDF pairs go straight to
+// the per-type counts.  Without CNTNB only; falls back to full-range
+// per-center histograms when the ranges save little.
+void plan_hybrid(const pp_cloud *cl, JobPlan &p, double factor)
+{
+    p.mode = p.percenter ? 1 : 0;
+    p.HK = 0;
+    p.trunc_any = 0;
+    p.kinfo.assign(512, 0);
+    const float mone = -1.0f;
+    int32_t mone_bits;
+    std::memcpy(&mone_bits, &mone, 4);
+    for (int t = 0; t < 256; ++t) {
+        p.kinfo[2 * t] = -1;
+        p.kinfo[2 * t + 1] = mone_bits;
+    }
+    p.ckoff.assign(kMaxEnv, 0);
+    p.task_ck.assign(std::max(p.ntasks, 1), 0u);
+    const bool mknn = (p.families & PP_FAM_MKNN) && p.nku > 0;
+    const bool knn = mknn || (p.families & PP_FAM_1NN);
+    if (!p.percenter || !knn || (p.families & PP_FAM_CNTNB) || !(factor > 0) || cl->n == 0 ||
+        p.E == 0)
+        return;
+    const double need = mknn ? (double)p.kmax : 1.0;
+    const double dens = cl->density();
+    const int catR = (int)p.hg.category_d2(p.R2);
+    std::vector<double> rho(p.E, 0.0);
+    for (int tk = 0; tk < p.ntasks; ++tk) {
+        if (!p.task_trg[tk])
+            continue;
+        double dS = 0;
+        for (int e = 0; e < p.E; ++e)
+            if ((p.task_env[tk] >> e) & 1u)
+                dS += dens * (double)cl->type_count[p.env_type[e]] / (double)cl->n;
+        const double rS = dS > 0 ? std::cbrt(factor * need / (4.18879020478639 * dS)) : (double)p.R;
+        for (int e = 0; e < p.E; ++e)
+            if ((p.task_env[tk] >> e) & 1u)
+                rho[e] = std::max(rho[e], rS);
+    }
+    std::vector<int> ck(p.E);
+    std::vector<char> tr(p.E, 0);
+    int HK = 0;
+    for (int e = 0; e < p.E; ++e) {
+        ck[e] = catR + 1;
+        if (rho[e] > 0 && rho[e] < (double)p.R) {
+            const float r2 = (float)(rho[e] * rho[e]);
+            const int cc = (int)p.hg.category_d2(r2);
+            if (cc >= 2 && cc <= catR) {
+                ck[e] = cc;
+                tr[e] = 1;
+            }
+        }
+        HK += ck[e];
+    }
+    if ((double)HK > 0.75 * (double)p.E * (double)p.C)
+        return;
+    int off = 0;
+    for (int e = 0; e < p.E; ++e) {
+        p.ckoff[e] = off;
+        const int t = p.env_type[e];
+        p.kinfo[2 * t] = off;
+        // truncated: d2 <= the float below edge ck_e, i.e. category < ck_e
+        const uint32_t thr = tr[e] ? p.et.edges[ck[e]] - 1u : [&] {
+            uint32_t b;
+            std::memcpy(&b, &p.R2, 4);
+            return b;
+        }();
+        p.kinfo[2 * t + 1] = (int32_t)thr;
+        off += ck[e];
+    }
+    for (int tk = 0; tk < p.ntasks; ++tk) {
+        int m = catR + 1;
+        bool t = false;
+        for (int e = 0; e < p.E; ++e)
+            if ((p.task_env[tk] >> e) & 1u) {
+                m = std::min(m, ck[e]);
+                t = t || tr[e];
+            }
+        p.task_ck[tk] = (uint32_t)m | (t ? 0x80000000u : 0u);
+        p.trunc_any |= t ? 1 : 0;
+    }
+    p.HK = HK;
+    p.mode = 2;
+}
 
 // Cell list of n ions (device arrays xyz, type, s2) on grid g: key and rank
 // per ion, exclusive scan of the cell counts, scatter into cell-sorted float4.
@@ -495,9 +617,12 @@ struct JobRun {
     int cn_cap = 0;
     // table offsets (words) inside `tables`
     size_t o_edges = 0, o_envoff = 0, o_trgslot = 0, o_ttrg = 0, o_tenv = 0, o_kidx = 0,
-           o_rows = 0, tables_words = 0;
+           o_tntask = 0, o_ttasks = 0, o_kinfo = 0, o_ckoff = 0, o_taskck = 0, o_rows = 0,
+           tables_words = 0;
     // read back after prep
     uint32_t h_counts[4] = {0, 0, 0, 0};  // considered, tiles, maxcand, pad
+    uint32_t considered = 0, rerun = 0;   // centers of the job, of its full-range rerun
+    int T_used = 0, c16_used = 0;
     std::vector<uint32_t> h_tables;       // host staging of the device tables
     JobDev dev{};
 };
@@ -527,13 +652,23 @@ void layout_job(JobRun &r, Carver &m)
     o += std::max(p.ntasks, 1);
     r.o_kidx = o;
     o += p.kmax + 1;
+    r.o_tntask = o;
+    o += p.trg_ntask.size();
+    r.o_ttasks = o;
+    o += p.trg_tasks.size();
+    r.o_kinfo = (o + 1) & ~(size_t)1;
+    o = r.o_kinfo + 512;
+    r.o_ckoff = o;
+    o += kMaxEnv;
+    r.o_taskck = o;
+    o += std::max(p.ntasks, 1);
     r.o_rows = (o + 1) & ~(size_t)1;
     o = r.o_rows + 2 * r.rows.size();
     r.tables_words = o;
     r.tables = m.take<uint32_t>(o);
     r.maxcand = m.take<uint32_t>(4);
     r.P = m.take<unsigned long long>((size_t)std::max(p.ntrg * p.E * p.C, 1));
-    r.stats = m.take<unsigned long long>(4);
+    r.stats = m.take<unsigned long long>(8);  // [4..7]: the full-range rerun's (unused)
     r.K = m.take<uint32_t>((size_t)std::max(p.ntasks * p.nku * p.C, 1));
     r.N1 = m.take<uint32_t>((size_t)std::max(p.ntasks * p.C, 1));
     // CN is carved later: its size follows from the stencil bound
@@ -572,11 +707,19 @@ pp_status prep_job(pp_ctx *c, Slot &sl, JobRun &r, int T)
     }
     for (int k = 0; k <= p.kmax; ++k)
         h[r.o_kidx + k] = (uint32_t)(p.kmax > 0 ? p.kidx_of_k[k] : -1);
+    std::memcpy(&h[r.o_tntask], p.trg_ntask.data(), p.trg_ntask.size() * 4);
+    std::memcpy(&h[r.o_ttasks], p.trg_tasks.data(), p.trg_tasks.size() * 4);
+    if (!p.kinfo.empty())
+        std::memcpy(&h[r.o_kinfo], p.kinfo.data(), 512 * 4);
+    if (!p.ckoff.empty())
+        std::memcpy(&h[r.o_ckoff], p.ckoff.data(), kMaxEnv * 4);
+    if (!p.task_ck.empty())
+        std::memcpy(&h[r.o_taskck], p.task_ck.data(), p.task_ck.size() * 4);
     if (!r.rows.empty())
         std::memcpy(&h[r.o_rows], r.rows.data(), r.rows.size() * sizeof(Row));
     PP_HIP(c, hipMemcpyAsync(r.tables, h.data(), h.size() * 4, hipMemcpyHostToDevice, s));
     PP_HIP(c, hipMemsetAsync(r.P, 0, sizeof(unsigned long long) * std::max(p.ntrg * p.E * p.C, 1), s));
-    PP_HIP(c, hipMemsetAsync(r.stats, 0, sizeof(unsigned long long) * 4, s));
+    PP_HIP(c, hipMemsetAsync(r.stats, 0, sizeof(unsigned long long) * 8, s));
     PP_HIP(c, hipMemsetAsync(r.K, 0, 4 * (size_t)std::max(p.ntasks * p.nku * p.C, 1), s));
     PP_HIP(c, hipMemsetAsync(r.N1, 0, 4 * (size_t)std::max(p.ntasks * p.C, 1), s));
     const int32_t *d_trgslot = (const int32_t *)(r.tables + r.o_trgslot);
@@ -604,6 +747,8 @@ void fill_jobdev(JobRun &r)
     d.edges = r.tables + r.o_edges;
     d.idx_scale = p.et.idx_scale;
     d.idx_off = p.et.idx_off;
+    d.idx_off1 = p.et.idx_off1;
+    d.amb = p.et.amb_margin(p.hg, p.R);
     d.cat0 = p.cat0;
     d.cat_sentinel = p.cat_sentinel;
     d.E = p.E;
@@ -624,6 +769,20 @@ void fill_jobdev(JobRun &r)
     d.cn_cap = r.cn_cap;
     d.percenter = p.percenter ? 1 : 0;
     d.stats = r.stats;
+    d.lmax = p.lmax;
+    d.trg_ntask = r.tables + r.o_tntask;
+    d.trg_tasks = r.tables + r.o_ttasks;
+    d.nplanes = p.nplanes();
+    d.KB = nullptr;  // set per launch chunk
+    d.KT = nullptr;
+    d.tile_base = 0;
+    d.kb_tiles = 0;
+    d.HK = p.HK;
+    d.trunc_any = p.trunc_any;
+    d.kinfo = (const int32_t *)(r.tables + r.o_kinfo);
+    d.ckoff = (const int32_t *)(r.tables + r.o_ckoff);
+    d.task_ck = r.tables + r.o_taskck;
+    d.unres = r.flags;  // reused once the center list is built
 }
 
 void compose_outputs(const JobRun &r, const std::vector<unsigned long long> &P,
@@ -761,6 +920,260 @@ struct Shape {
 pp_status run_jobs(pp_ctx *c, Slot &sl, int njobs, const pp_cloud *const *clouds,
                    const pp_job *jobs, pp_hist_out *outs, Shape force = Shape());
 
+constexpr size_t kKbChunkBytes = size_t(1) << 30;  // k-th neighbour buffer of one launch chunk
+constexpr int kReduceRows = 32768;                  // rows per reduction work unit
+
+// One query launch over the global tile range [g0, g1).
+struct KChunk {
+    uint32_t g0 = 0, g1 = 0;
+    std::vector<uint32_t> base, ntl;  // per job: first local tile and tile count in the chunk
+    size_t bytes = 0;                 // k-th neighbour buffer bytes
+};
+
+// Bytes of one tile's block of k-th neighbour categories and task ids.
+size_t kb_tile_bytes(const JobPlan &p, int T)
+{
+    const int np = p.nplanes();
+    if (np == 0)
+        return 0;
+    const size_t TR = (size_t)T * (size_t)p.lmax;
+    return TR * (2 * (size_t)np + 1);
+}
+
+// Splits the launch's tiles (job after job) into chunks whose k-th neighbour
+// blocks fit `budget` bytes (at least one tile per chunk).
+void plan_chunks(const std::vector<const JobPlan *> &plans, const std::vector<uint32_t> &tile_off,
+                 int T, size_t budget, std::vector<KChunk> &out)
+{
+    const int njobs = (int)plans.size();
+    const size_t slack = 512;  // alignment of a job's two arrays in the chunk buffer
+    out.clear();
+    KChunk cur;
+    cur.base.assign(njobs, 0u);
+    cur.ntl.assign(njobs, 0u);
+    auto close = [&](uint32_t g1) {
+        cur.g1 = g1;
+        out.push_back(cur);
+        cur.g0 = g1;
+        cur.bytes = 0;
+        std::fill(cur.base.begin(), cur.base.end(), 0u);
+        std::fill(cur.ntl.begin(), cur.ntl.end(), 0u);
+    };
+    for (int i = 0; i < njobs; ++i) {
+        const uint32_t n = tile_off[i + 1] - tile_off[i];
+        const size_t tb = kb_tile_bytes(*plans[i], T);
+        uint32_t done = 0;
+        while (done < n) {
+            uint32_t take = n - done;
+            if (tb) {
+                const size_t avail = budget > cur.bytes + slack ? budget - cur.bytes - slack : 0;
+                uint64_t fit = avail / tb;
+                if (fit == 0 && cur.bytes == 0)
+                    fit = 1;
+                if (fit == 0) {
+                    close(tile_off[i] + done);
+                    continue;
+                }
+                take = (uint32_t)std::min<uint64_t>(fit, take);
+                cur.bytes += (size_t)take * tb + slack;
+            }
+            if (cur.ntl[i] == 0)
+                cur.base[i] = done;
+            cur.ntl[i] += take;
+            done += take;
+            if (done < n)
+                close(tile_off[i] + done);
+        }
+    }
+    if (out.empty() || cur.g0 < tile_off[njobs])
+        close(tile_off[njobs]);
+}
+
+// Host images of device tables handed to asynchronous copies; they must
+// outlive the launches that read them.
+struct Keep {
+    std::deque<std::vector<JobDev>> devs;
+    std::deque<std::vector<uint32_t>> words;
+    std::deque<std::vector<const Tile *>> tiles;
+};
+
+// One query launch group: the jobs `ids` (one kernel mode) with their device
+// descriptors `base`, as launch chunks, each followed by the reduction of its
+// k-th neighbour categories.  May halve T (rebuilding the group's tiles) to
+// fit the LDS; returns the T used.
+pp_status launch_group(pp_ctx *c, Slot &sl, std::vector<JobRun> &runs, const std::vector<int> &ids,
+                       const std::vector<JobDev> &base, int mode, int &T, Keep &keep)
+{
+    const int nj = (int)ids.size();
+    if (nj == 0)
+        return PP_OK;
+    int maxC = 1, maxE = 1, maxTrg = 1, maxK = 0, maxRows = 1, maxHK = 1;
+    uint32_t maxcand = 0;
+    for (int q = 0; q < nj; ++q) {
+        const JobRun &r = runs[ids[q]];
+        const JobPlan &p = r.plan;
+        maxC = std::max(maxC, p.C);
+        maxE = std::max(maxE, p.E);
+        maxTrg = std::max(maxTrg, p.ntrg);
+        maxK = std::max(maxK, p.kmax);
+        maxRows = std::max(maxRows, (int)r.rows.size());
+        if (mode == 2)
+            maxHK = std::max(maxHK, p.HK);
+        maxcand = std::max(maxcand, r.h_counts[2]);
+    }
+    const int c16 = maxcand < 65535u ? 1 : 0;
+    size_t lds = query_lds_bytes(T, c16, mode, maxC, maxE, maxTrg, maxK, maxRows, maxHK);
+    while (lds > c->lds_per_block && T > 8) {
+        // smaller tiles: rebuild the tile lists
+        T /= 2;
+        for (int q = 0; q < nj; ++q) {
+            pp_status st = prep_tiles(c, sl, runs[ids[q]], T);
+            if (st != PP_OK)
+                return st;
+        }
+        PP_HIP(c, hipStreamSynchronize(sl.stream));
+        lds = query_lds_bytes(T, c16, mode, maxC, maxE, maxTrg, maxK, maxRows, maxHK);
+    }
+    if (lds > c->lds_per_block)
+        return fail(c, PP_ERR_UNSUPPORTED, "histograms too large for LDS (bins x types x tile)");
+    std::vector<uint32_t> tile_off(nj + 1, 0);
+    std::vector<const JobPlan *> plans(nj);
+    uint32_t total = 0;
+    for (int q = 0; q < nj; ++q) {
+        tile_off[q] = total;
+        total += runs[ids[q]].h_counts[1];
+        plans[q] = &runs[ids[q]].plan;
+    }
+    tile_off[nj] = total;
+    // launch chunks: the k-th neighbour categories of a chunk's tiles must
+    // fit the chunk buffer
+    std::vector<KChunk> chunks;
+    plan_chunks(plans, tile_off, T, kKbChunkBytes, chunks);
+    const size_t nch = chunks.size();
+    size_t kb_need = 0;
+    for (const KChunk &ch : chunks)
+        kb_need = std::max(kb_need, ch.bytes);
+    if (kb_need) {
+        pp_status st = ensure(c, sl, sl.kbuf, kb_need + 256);
+        if (st != PP_OK)
+            return st;
+    }
+    Carver jb0(nullptr);
+    jb0.take<JobDev>((size_t)nj * nch);
+    jb0.take<const Tile *>(nj);
+    jb0.take<uint32_t>(nj + 1);
+    jb0.take<uint32_t>((size_t)(nj + 1) * nch);
+    jb0.take<uint32_t>(kQueueWords);
+    pp_status st = ensure(c, sl, sl.jobbuf, jb0.off + 256);
+    if (st != PP_OK)
+        return st;
+    Carver jb(sl.jobbuf.p);
+    JobDev *d_jobs = jb.take<JobDev>((size_t)nj * nch);
+    const Tile **d_tiles = jb.take<const Tile *>(nj);
+    uint32_t *d_off = jb.take<uint32_t>(nj + 1);
+    uint32_t *d_uoff = jb.take<uint32_t>((size_t)(nj + 1) * nch);
+    uint32_t *d_queue = jb.take<uint32_t>(kQueueWords);
+    keep.devs.emplace_back((size_t)nj * nch);
+    std::vector<JobDev> &hj = keep.devs.back();
+    keep.words.emplace_back((size_t)(nj + 1) * nch, 0u);
+    std::vector<uint32_t> &huoff = keep.words.back();
+    keep.words.push_back(tile_off);
+    const std::vector<uint32_t> &htoff = keep.words.back();
+    keep.tiles.emplace_back(nj);
+    std::vector<const Tile *> &ht = keep.tiles.back();
+    uint32_t tpu = 1;
+    int red_lds = 0;
+    {
+        int maxTR = 1;
+        for (int q = 0; q < nj; ++q)
+            if (plans[q]->nplanes() > 0) {
+                maxTR = std::max(maxTR, T * plans[q]->lmax);
+                const int w = plans[q]->ntasks * plans[q]->C;
+                if (w <= kReduceLdsWords)
+                    red_lds = std::max(red_lds, w);
+            }
+        tpu = (uint32_t)std::max(1, kReduceRows / maxTR);
+    }
+    for (size_t k = 0; k < nch; ++k) {
+        const KChunk &ch = chunks[k];
+        Carver kc(sl.kbuf.p);
+        uint32_t units = 0;
+        for (int q = 0; q < nj; ++q) {
+            JobDev d = base[q];
+            const int np = plans[q]->nplanes();
+            huoff[k * (nj + 1) + q] = units;
+            if (np > 0 && ch.ntl[q] > 0) {
+                const size_t TR = (size_t)T * plans[q]->lmax;
+                d.KB = kc.take<uint16_t>((size_t)ch.ntl[q] * np * TR);
+                d.KT = kc.take<uint8_t>((size_t)ch.ntl[q] * TR);
+                d.tile_base = ch.base[q];
+                d.kb_tiles = ch.ntl[q];
+                units += (uint32_t)np * ((ch.ntl[q] + tpu - 1) / tpu);
+            }
+            hj[k * nj + q] = d;
+        }
+        huoff[k * (nj + 1) + nj] = units;
+    }
+    for (int q = 0; q < nj; ++q)
+        ht[q] = runs[ids[q]].tiles;
+    PP_HIP(c, hipMemcpyAsync(d_jobs, hj.data(), sizeof(JobDev) * hj.size(), hipMemcpyHostToDevice,
+                             sl.stream));
+    PP_HIP(c, hipMemcpyAsync(d_tiles, ht.data(), sizeof(Tile *) * nj, hipMemcpyHostToDevice,
+                             sl.stream));
+    PP_HIP(c, hipMemcpyAsync(d_off, htoff.data(), 4 * (nj + 1), hipMemcpyHostToDevice, sl.stream));
+    PP_HIP(c, hipMemcpyAsync(d_uoff, huoff.data(), 4 * huoff.size(), hipMemcpyHostToDevice,
+                             sl.stream));
+    int nb = 0;
+    PP_HIP(c, query_max_blocks_per_cu(T, c16, mode, lds, &nb));
+    if (nb < 1)
+        nb = 1;
+    if (std::getenv("PP_TRACE_LAUNCH"))
+        std::fprintf(stderr,
+                     "pp: query mode=%d T=%d c16=%d lds=%zu blocks/CU=%d jobs=%d tiles=%u chunks=%zu "
+                     "maxC=%d maxE=%d maxTrg=%d kmax=%d rows=%d HK=%d\n",
+                     mode, T, c16, lds, nb, nj, total, nch, maxC, maxE, maxTrg, maxK, maxRows,
+                     maxHK);
+    for (size_t k = 0; k < nch; ++k) {
+        const KChunk &ch = chunks[k];
+        QueryLaunch ql{};
+        ql.jobs = d_jobs + k * nj;
+        ql.njobs = nj;
+        ql.tile_off = d_off;
+        ql.tiles = d_tiles;
+        ql.first_tile = ch.g0;
+        ql.end_tile = ch.g1;
+        ql.queue = d_queue;
+        ql.T = T;
+        ql.c16 = c16;
+        ql.mode = mode;
+        ql.maxC = maxC;
+        ql.maxE = maxE;
+        ql.maxTrg = maxTrg;
+        ql.maxKmax = maxK;
+        ql.maxRows = maxRows;
+        ql.maxHK = maxHK;
+        const uint32_t want = (uint32_t)c->cu_count * (uint32_t)nb;
+        ql.grid = (int)std::max<uint32_t>(
+            1u, std::min<uint32_t>(want, std::max<uint32_t>(ch.g1 - ch.g0, 1u)));
+        // several tiles per queue atomic when tiles are many and cheap (the
+        // stencil bounds a tile's pair evaluations by maxcand * T)
+        ql.grab = (uint64_t)maxcand * (uint64_t)T < (1u << 17)
+                      ? std::max<uint32_t>(1u, std::min<uint32_t>(
+                                                   4u, (ch.g1 - ch.g0) / ((uint32_t)ql.grid * 16u)))
+                      : 1u;
+        PP_HIP(c, launch_query(ql, sl.stream));
+        const uint32_t units = huoff[k * (nj + 1) + nj];
+        if (units > 0)
+            PP_HIP(c, launch_kreduce(d_jobs + k * nj, nj, d_uoff + k * (nj + 1), units, T, tpu,
+                                     red_lds, sl.stream));
+    }
+    for (int q = 0; q < nj; ++q) {
+        runs[ids[q]].T_used = T;
+        runs[ids[q]].c16_used = c16;
+    }
+    return PP_OK;
+}
+
 // Times the candidate shapes on this job and keeps the fastest for its bucket.
 pp_status autotune(pp_ctx *c, Slot &sl, const pp_cloud *cl, const pp_job *job, const JobPlan &p,
                    long bucket)
@@ -795,10 +1208,12 @@ pp_status run_jobs(pp_ctx *c, Slot &sl, int njobs, const pp_cloud *const *clouds
         return fail(c, PP_ERR_INVALID_ARG, "outputs are NULL");
     std::vector<JobRun> runs(njobs);
     int tuneT, tuneR;
+    double knn_factor;
     {
         std::lock_guard<std::mutex> lk(c->mu);
         tuneT = c->tune_T;
         tuneR = c->tune_r;
+        knn_factor = c->knn_factor;
     }
     int T = 0;
     for (int i = 0; i < njobs; ++i) {
@@ -838,6 +1253,7 @@ pp_status run_jobs(pp_ctx *c, Slot &sl, int njobs, const pp_cloud *const *clouds
                 p.r = tuneR;
         }
         T = std::max(T, jT);
+        plan_hybrid(clouds[i], p, knn_factor);
     }
     PP_HIP(c, hipSetDevice(c->device));
     // a non-resident cloud's cached cell list is rebuilt per width: hold it
@@ -897,44 +1313,6 @@ pp_status run_jobs(pp_ctx *c, Slot &sl, int njobs, const pp_cloud *const *clouds
     }
     PP_HIP(c, hipStreamSynchronize(sl.stream));
     PP_HIP(c, hipEventRecord(sl.ev[2], sl.stream));
-    int maxC = 1, maxE = 1, maxTrg = 1, maxK = 0, maxRows = 1;
-    bool pc = false;
-    uint32_t maxcand = 0, total = 0;
-    std::vector<uint32_t> tile_off(njobs + 1, 0);
-    for (int i = 0; i < njobs; ++i) {
-        const JobPlan &p = runs[i].plan;
-        maxC = std::max(maxC, p.C);
-        maxE = std::max(maxE, p.E);
-        maxTrg = std::max(maxTrg, p.ntrg);
-        maxK = std::max(maxK, p.kmax);
-        maxRows = std::max(maxRows, (int)runs[i].rows.size());
-        pc |= p.percenter;
-        maxcand = std::max(maxcand, runs[i].h_counts[2]);
-        tile_off[i] = total;
-        total += runs[i].h_counts[1];
-    }
-    tile_off[njobs] = total;
-    const int c16 = maxcand < 65535u ? 1 : 0;
-    size_t lds = query_lds_bytes(T, c16, pc, maxC, maxE, maxTrg, maxK, maxRows);
-    while (lds > c->lds_per_block && T > 8) {
-        // smaller tiles: rebuild the tile lists
-        T /= 2;
-        for (int i = 0; i < njobs; ++i) {
-            st = prep_tiles(c, sl, runs[i], T);
-            if (st != PP_OK)
-                return st;
-        }
-        PP_HIP(c, hipStreamSynchronize(sl.stream));
-        total = 0;
-        for (int i = 0; i < njobs; ++i) {
-            tile_off[i] = total;
-            total += runs[i].h_counts[1];
-        }
-        tile_off[njobs] = total;
-        lds = query_lds_bytes(T, c16, pc, maxC, maxE, maxTrg, maxK, maxRows);
-    }
-    if (lds > c->lds_per_block)
-        return fail(c, PP_ERR_UNSUPPORTED, "histograms too large for LDS (bins x types x tile)");
     {
         // CNTNB counters: no neighbour count exceeds the largest stencil candidate count
         size_t words = 0;
@@ -954,56 +1332,78 @@ pp_status run_jobs(pp_ctx *c, Slot &sl, int njobs, const pp_cloud *const *clouds
             o += (w + 63) & ~(size_t)63;
             PP_HIP(c, hipMemsetAsync(r.CN, 0, w * 4, sl.stream));
             fill_jobdev(r);
+            r.considered = r.h_counts[0];
         }
     }
-    Carver jb0(nullptr);
-    jb0.take<JobDev>(njobs);
-    jb0.take<const Tile *>(njobs);
-    jb0.take<uint32_t>(njobs + 1);
-    jb0.take<uint32_t>(kXcds + 1);
-    st = ensure(c, sl, sl.jobbuf, jb0.off + 256);
+    // launch groups: pair-count and hybrid jobs in one launch, full-range
+    // per-center jobs in another, then the full-range rerun of the centers
+    // whose kNN range fell short
+    std::vector<int> gA, gB;
+    std::vector<JobDev> bA, bB;
+    int modeA = 0;
+    for (int i = 0; i < njobs; ++i) {
+        JobRun &r = runs[i];
+        if (r.plan.mode == 1) {
+            gB.push_back(i);
+            bB.push_back(r.dev);
+        } else {
+            gA.push_back(i);
+            bA.push_back(r.dev);
+            if (r.plan.mode == 2)
+                modeA = 2;
+            if (r.plan.mode == 2 && r.plan.trunc_any)
+                PP_HIP(c, hipMemsetAsync(r.flags, 0, 4 * r.cloud->n, sl.stream));
+        }
+    }
+    Keep keep;
+    int TA = T, TB = T;
+    PP_HIP(c, hipEventRecord(sl.ev[3], sl.stream));
+    st = launch_group(c, sl, runs, gA, bA, modeA, TA, keep);
     if (st != PP_OK)
         return st;
-    Carver jb(sl.jobbuf.p);
-    JobDev *d_jobs = jb.take<JobDev>(njobs);
-    const Tile **d_tiles = jb.take<const Tile *>(njobs);
-    uint32_t *d_off = jb.take<uint32_t>(njobs + 1);
-    uint32_t *d_queue = jb.take<uint32_t>(kXcds + 1);
-    std::vector<JobDev> hj(njobs);
-    std::vector<const Tile *> ht(njobs);
-    for (int i = 0; i < njobs; ++i) {
-        hj[i] = runs[i].dev;
-        ht[i] = runs[i].tiles;
+    st = launch_group(c, sl, runs, gB, bB, 1, TB, keep);
+    if (st != PP_OK)
+        return st;
+    {
+        std::vector<int> hy;
+        for (int i : gA)
+            if (runs[i].plan.mode == 2 && runs[i].plan.trunc_any)
+                hy.push_back(i);
+        if (!hy.empty()) {
+            for (int i : hy) {
+                JobRun &r = runs[i];
+                const uint64_t n = r.cloud->n;
+                PP_HIP(c, scan_exclusive(r.flags, r.cpos, n, r.ws, sl.stream));
+                PP_HIP(c, launch_compact(r.flags, r.cpos, n, r.centers, sl.stream));
+                st = prep_tiles(c, sl, r, TA);
+                if (st != PP_OK)
+                    return st;
+            }
+            PP_HIP(c, hipStreamSynchronize(sl.stream));
+            std::vector<int> gC;
+            std::vector<JobDev> bC;
+            for (int i : hy) {
+                JobRun &r = runs[i];
+                r.rerun = r.h_counts[0];
+                r.h_counts[0] = r.considered;
+                if (r.h_counts[1] == 0)
+                    continue;
+                JobDev d = r.dev;
+                d.families &= FAM_MKNN | FAM_1NN;
+                d.percenter = 1;
+                d.stats = r.stats + 4;
+                gC.push_back(i);
+                bC.push_back(d);
+            }
+            int TC = TA;
+            const int ta = TA;
+            st = launch_group(c, sl, runs, gC, bC, 1, TC, keep);
+            if (st != PP_OK)
+                return st;
+            for (int i : gC)
+                runs[i].T_used = ta;  // the job's main launch
+        }
     }
-    PP_HIP(c, hipMemcpyAsync(d_jobs, hj.data(), sizeof(JobDev) * njobs, hipMemcpyHostToDevice,
-                             sl.stream));
-    PP_HIP(c, hipMemcpyAsync(d_tiles, ht.data(), sizeof(Tile *) * njobs, hipMemcpyHostToDevice,
-                             sl.stream));
-    PP_HIP(c, hipMemcpyAsync(d_off, tile_off.data(), 4 * (njobs + 1), hipMemcpyHostToDevice,
-                             sl.stream));
-    int nb = 0;
-    PP_HIP(c, query_max_blocks_per_cu(T, c16, pc, lds, &nb));
-    if (nb < 1)
-        nb = 1;
-    QueryLaunch q{};
-    q.jobs = d_jobs;
-    q.njobs = njobs;
-    q.tile_off = d_off;
-    q.tiles = d_tiles;
-    q.total_tiles = total;
-    q.queue = d_queue;
-    q.T = T;
-    q.c16 = c16;
-    q.percenter = pc ? 1 : 0;
-    q.maxC = maxC;
-    q.maxE = maxE;
-    q.maxTrg = maxTrg;
-    q.maxKmax = maxK;
-    q.maxRows = maxRows;
-    const uint32_t want = (uint32_t)c->cu_count * (uint32_t)nb;
-    q.grid = (int)std::max<uint32_t>(1u, std::min<uint32_t>(want, std::max<uint32_t>(total, 1u)));
-    PP_HIP(c, hipEventRecord(sl.ev[3], sl.stream));
-    PP_HIP(c, launch_query(q, sl.stream));
     PP_HIP(c, hipEventRecord(sl.ev[4], sl.stream));
     // read back
     struct Back {
@@ -1053,10 +1453,11 @@ pp_status run_jobs(pp_ctx *c, Slot &sl, int njobs, const pp_cloud *const *clouds
         outs[i].ms_prep = ms_prep;
         outs[i].ms_query = ms_q;
         outs[i].ms_total = ms_total;
-        outs[i].tile_centers = T;
+        outs[i].tile_centers = runs[i].T_used;
         outs[i].cells_per_rmax = runs[i].cloud->resident ? 0 : runs[i].plan.r;
-        outs[i].counters16 = c16;
-        outs[i].percenter = pc ? 1 : 0;
+        outs[i].counters16 = runs[i].c16_used;
+        outs[i].percenter = runs[i].plan.mode;
+        outs[i].centers_rerun = runs[i].rerun;
         any_overflow |= overflow;
     }
     if (any_overflow)
@@ -1086,7 +1487,9 @@ size_t job_footprint(const pp_cloud *cl, const JobPlan &p)
     Carver m(nullptr);
     layout_job(r, m);
     size_t cn = (p.families & PP_FAM_CNTNB) ? (size_t)p.ntasks * 65536 * 4 : 0;
-    return m.off + cn + 4096;
+    // k-th neighbour chunk buffer: at most one tile per considered center
+    const size_t kb = std::min(kKbChunkBytes, (size_t)cl->n * kb_tile_bytes(p, 8) + 1024);
+    return m.off + cn + (p.nplanes() ? kb : 0) + 4096;
 }
 
 pp_status upload_common(pp_ctx *c, const float *xyz, const uint8_t *type, const float *s2,
@@ -1393,6 +1796,17 @@ pp_status pp_set_tuning(pp_ctx *c, int32_t centers_per_tile, int32_t cells_per_r
     std::lock_guard<std::mutex> lk(c->mu);
     c->tune_T = centers_per_tile;
     c->tune_r = cells_per_rmax;
+    return PP_OK;
+}
+
+pp_status pp_set_knn_range(pp_ctx *c, float factor)
+{
+    if (!c)
+        return PP_ERR_INVALID_ARG;
+    if (!(factor >= 0.f) || factor > 1e6f)
+        return fail(c, PP_ERR_INVALID_ARG, "kNN range factor must be >= 0");
+    std::lock_guard<std::mutex> lk(c->mu);
+    c->knn_factor = factor;
     return PP_OK;
 }
 

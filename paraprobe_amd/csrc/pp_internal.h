@@ -15,6 +15,10 @@ constexpr int kMaxTrg = 32;        // distinct center (target) types per job
 constexpr int kMaxTasks = 64;      // type-pair tasks per job
 constexpr int kMaxKmax = 4096;     // largest MKNN order
 constexpr int kXcds = 8;           // MI355X accelerator complex dies
+// Tile queues of a query launch: one counter per XCD, each on its own
+// 256-byte line (atomics on one line serialise at ~11 ns each).
+constexpr int kQueueStride = 64;
+constexpr int kQueueWords = kXcds * kQueueStride;
 
 // Stencil row: cells (cx-dxm .. cx+dxm, cy+dy, cz+dz) are contiguous in the
 // cell-sorted ion array because the cell key is x-fastest.
@@ -58,6 +62,9 @@ struct JobDev {
     int C;                    // nbins + 2 categories (lowest, bins, highest)
     const uint32_t *edges;    // C entries, edges[0] unused
     float idx_scale, idx_off;
+    // floor form: category = floor(fmaf(sqrt(d2), idx_scale, idx_off1))
+    // wherever that argument lies at least amb from an integer
+    float idx_off1, amb;
     int cat0;                 // category of d = 0 (self pair)
     int cat_sentinel;         // category of float(R + 1e-6)
     // types
@@ -80,6 +87,32 @@ struct JobDev {
     int cn_cap;
     int percenter;            // 1 when MKNN, 1NN or CNTNB need per-center histograms
     unsigned long long *stats;// [0] in-ball pairs (all types), [1] candidate evaluations
+    // tasks of each center type: trg_tasks[slot * lmax + l], l < trg_ntask[slot]
+    int lmax;
+    const uint32_t *trg_ntask;
+    const uint32_t *trg_tasks;
+    // k-th neighbour categories of the tiles of one launch chunk, reduced into
+    // K / N1 afterwards: per tile, nplanes planes of T * lmax uint16 (row
+    // c * lmax + l; plane nk is the 1NN when requested) and T * lmax task ids
+    // (0xFF: empty row).  Tile tl of the chunk is local tile tile_base + tl.
+    uint16_t *KB;
+    uint8_t *KT;
+    int nplanes;
+    uint32_t tile_base;
+    uint32_t kb_tiles;        // tiles of this job in the chunk
+    // kNN-range histograms (hybrid mode): per center, env slot e holds the
+    // categories [0, ck_e) at ckoff[e] (HK words in all); a pair enters them
+    // iff d2 <= kthr of its type, i.e. iff its category is below ck_e.
+    // kinfo[2 t] = ckoff of type t (or -1), kinfo[2 t + 1] = kthr bits.
+    // task_ck[tk] = categories complete for task tk, bit 31 set when that is
+    // short of the full range: a center with fewer than k neighbours there
+    // is flagged in unres (per sorted ion) and re-run over the full range.
+    int HK;
+    int trunc_any;
+    const int32_t *kinfo;
+    const int32_t *ckoff;
+    const uint32_t *task_ck;
+    uint32_t *unres;
 };
 
 } // namespace pp

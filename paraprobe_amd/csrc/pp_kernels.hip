@@ -492,13 +492,16 @@ hipError_t launch_tilewrite(const uint32_t *cellstart, const uint32_t *cpos, con
 // query kernel
 
 struct Lay {
-    int edges, envoff, trgslot, kidx, rows, rowchunk, cen, meta, P, H, total;
+    int edges, envoff, trgslot, kinfo, ckoff, kidx, rows, rowchunk, cen, meta, P, H, total;
 };
 
 __host__ __device__ inline int al4(int x) { return (x + 3) & ~3; }
 
-__host__ __device__ inline Lay make_layout(int T, int c16, int pc, int maxC, int maxE, int maxTrg,
-                                           int maxKmax, int maxRows)
+// LDS layout (uint32 words) of the query kernel.  mode: 0 pair counts per
+// center type only, 1 full-range per-center histograms, 2 kNN-range
+// per-center histograms of maxHK words.
+__host__ __device__ inline Lay make_layout(int T, int c16, int mode, int maxC, int maxE,
+                                           int maxTrg, int maxKmax, int maxRows, int maxHK)
 {
     Lay L;
     int o = 0;
@@ -508,8 +511,12 @@ __host__ __device__ inline Lay make_layout(int T, int c16, int pc, int maxC, int
     o += 256;
     L.trgslot = o;
     o += 256;
+    L.kinfo = o;
+    o += mode == 2 ? 512 : 0;
+    L.ckoff = o;
+    o += mode == 2 ? kMaxEnv : 0;
     L.kidx = o;
-    o += pc ? al4(maxKmax + 2) : 0;
+    o += mode ? al4(maxKmax + 2) : 0;
     L.rows = o;
     o += al4(2 * maxRows);
     L.rowchunk = o;
@@ -521,15 +528,17 @@ __host__ __device__ inline Lay make_layout(int T, int c16, int pc, int maxC, int
     L.P = o;
     o += al4(maxTrg * maxE * maxC);
     L.H = o;
-    o += pc ? al4((c16 ? (T + 1) / 2 : T) * maxE * maxC) : 0;
+    const int hc = c16 ? (T + 1) / 2 : T;
+    o += mode == 1 ? al4(hc * maxE * maxC) : mode == 2 ? al4(hc * maxHK) : 0;
     L.total = o;
     return L;
 }
 
-size_t query_lds_bytes(int T, int c16, int percenter, int maxC, int maxE, int maxTrg, int maxKmax,
-                       int maxRows)
+size_t query_lds_bytes(int T, int c16, int mode, int maxC, int maxE, int maxTrg, int maxKmax,
+                       int maxRows, int maxHK)
 {
-    return (size_t)make_layout(T, c16, percenter, maxC, maxE, maxTrg, maxKmax, maxRows).total * 4u;
+    return (size_t)make_layout(T, c16, mode, maxC, maxE, maxTrg, maxKmax, maxRows, maxHK).total *
+           4u;
 }
 
 struct QArgs {
@@ -537,9 +546,10 @@ struct QArgs {
     const uint32_t *tile_off;
     const Tile *const *tiles;
     uint32_t *queue;
-    uint32_t total_tiles;
+    uint32_t first_tile, end_tile;
+    uint32_t grab;
     int njobs;
-    int maxC, maxE, maxTrg, maxKmax, maxRows;
+    int maxC, maxE, maxTrg, maxKmax, maxRows, maxHK;
 };
 
 __device__ inline int find_job(const uint32_t *__restrict__ off, int njobs, uint32_t g)
@@ -557,6 +567,12 @@ __device__ inline int find_job(const uint32_t *__restrict__ off, int njobs, uint
 
 __device__ inline uint32_t rfl(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// XCC_ID hardware register (accelerator complex die of this wave)
+__device__ inline uint32_t xcc_id()
+{
+    return __builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & 0xFu;
+}
+
 typedef float fvec4 __attribute__((ext_vector_type(4)));
 typedef __attribute__((address_space(1))) const fvec4 gfvec4;
 
@@ -568,15 +584,77 @@ __device__ inline float4 load_g4(const gfvec4 *p, uint32_t i)
     return make_float4(v.x, v.y, v.z, v.w);
 }
 
-template <int T, bool C16, bool PC>
+// The next tile of one workgroup (thread 0).  The launch's tiles form blocks
+// of a.grab consecutive tiles; block b belongs to XCD b % 8, whose counter
+// hands its blocks out in order, so all XCDs sweep the same spatial front (the
+// ions of neighbouring tiles stay shared in the last-level cache) while each
+// counter sits on its own line.  A workgroup whose XCD has no blocks left
+// takes blocks of the next XCD.  Returns end_tile when every block is gone.
+__device__ inline uint32_t next_tile(const QArgs &a, uint32_t myx, int &qx, uint32_t &gnext,
+                                     uint32_t &left)
+{
+    if (left > 0) {
+        --left;
+        return gnext++;
+    }
+    const uint32_t span = a.end_tile - a.first_tile;
+    const uint32_t nblk = (span + a.grab - 1) / a.grab;
+    while (qx < kXcds) {
+        const uint32_t x = (myx + (uint32_t)qx) & (uint32_t)(kXcds - 1);
+        if (x < nblk) {
+            const uint32_t k = atomicAdd(&a.queue[x * kQueueStride], 1u);
+            const uint64_t b = (uint64_t)k * kXcds + x;
+            if (b < nblk) {
+                const uint32_t t0 = (uint32_t)b * a.grab;
+                const uint32_t n = span - t0 < a.grab ? span - t0 : a.grab;
+                left = n - 1;
+                gnext = a.first_tile + t0 + 1;
+                return a.first_tile + t0;
+            }
+        }
+        ++qx;
+    }
+    return a.end_tile;
+}
+
+// Category of an in-range squared distance: floor of the index coordinate
+// where it is clear of every integer by amb; when any active lane is closer
+// than that to a bin edge (rare), the wave settles every lane with the exact
+// edge table instead.  Branch-free apart from that wave-uniform test, and no
+// LDS read on the common path.
+__device__ inline int category_of(float d2, float isc, float ioff1, float amb, float Cm1f,
+                                  const uint32_t *s_edges)
+{
+    const float y = __builtin_fmaf(__builtin_amdgcn_sqrtf(d2), isc, ioff1);
+    const float f = __builtin_amdgcn_fractf(y);
+    const bool near = __builtin_fabsf(f - 0.5f) > 0.5f - amb;
+    if (__builtin_expect(__builtin_amdgcn_ballot_w64(near) != 0, 0)) {
+        const int q = (int)__builtin_amdgcn_fmed3f(y + 0.5f, 1.0f, Cm1f);
+        return q - (__float_as_uint(d2) < s_edges[q] ? 1 : 0);
+    }
+    return (int)__builtin_amdgcn_fmed3f(y, 0.0f, Cm1f);
+}
+
+// The persistent query kernel.  MODE 0: pair counts per (center type, env
+// type, category) only (RDF / RIPK).  MODE 1: full-range per-center
+// histograms, from which RDF, 1NN, MKNN and CNTNB are extracted per tile.
+// MODE 2: pair counts as in MODE 0 plus per-center histograms over the kNN
+// range only (see JobDev::HK); centers whose kNN range holds too few
+// neighbours are flagged for a MODE 1 rerun.
+template <int T, bool C16, int MODE>
 __global__ __launch_bounds__(kBlock) void k_query(QArgs a)
 {
+    constexpr bool PC = MODE == 1;
+    constexpr bool HY = MODE == 2;
     extern __shared__ uint32_t lds[];
     __shared__ uint32_t s_g, s_ctot;
-    const Lay L = make_layout(T, C16, PC, a.maxC, a.maxE, a.maxTrg, a.maxKmax, a.maxRows);
+    const Lay L = make_layout(T, C16, MODE, a.maxC, a.maxE, a.maxTrg, a.maxKmax, a.maxRows,
+                              a.maxHK);
     uint32_t *s_edges = lds + L.edges;
     int32_t *s_envoff = (int32_t *)(lds + L.envoff);
     int32_t *s_trgslot = (int32_t *)(lds + L.trgslot);
+    int2 *s_kinfo = (int2 *)(lds + L.kinfo);
+    int32_t *s_ckoff = (int32_t *)(lds + L.ckoff);
     int32_t *s_kidx = (int32_t *)(lds + L.kidx);
     uint32_t *s_rows = lds + L.rows;
     uint32_t *s_rowchunk = lds + L.rowchunk;
@@ -587,6 +665,10 @@ __global__ __launch_bounds__(kBlock) void k_query(QArgs a)
 
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
     int cur = -1;
+    // tile queues (one counter per XCD); thread 0 keeps the grab state
+    const uint32_t myx = xcc_id() & (uint32_t)(kXcds - 1);
+    int qx = 0;
+    uint32_t gnext = 0, left = 0;
     unsigned long long inball = 0;   // wave-uniform
     unsigned long long cand = 0;     // block-uniform
     unsigned long long pending = 0;  // pairs accumulated in s_P since its last flush
@@ -594,10 +676,10 @@ __global__ __launch_bounds__(kBlock) void k_query(QArgs a)
     for (;;) {
         __syncthreads();
         if (tid == 0)
-            s_g = atomicAdd(a.queue, 1u);
+            s_g = next_tile(a, myx, qx, gnext, left);
         __syncthreads();
         const uint32_t g = rfl(s_g);
-        const bool done = g >= a.total_tiles;
+        const bool done = g >= a.end_tile;
         const int job = done ? -1 : (int)rfl((uint32_t)find_job(a.tile_off, a.njobs, g));
 
         if (job != cur || (!done && pending > 0x7FFFFFFFull)) {
@@ -631,7 +713,13 @@ __global__ __launch_bounds__(kBlock) void k_query(QArgs a)
                     s_envoff[i] = J.envoff[i];
                     s_trgslot[i] = J.trgslot[i];
                 }
-                if (PC)
+                if (HY) {
+                    for (int i = tid; i < 512; i += kBlock)
+                        ((int32_t *)s_kinfo)[i] = J.kinfo[i];
+                    for (int i = tid; i < J.E; i += kBlock)
+                        s_ckoff[i] = J.ckoff[i];
+                }
+                if (PC || HY)
                     for (int i = tid; i <= J.kmax; i += kBlock)
                         s_kidx[i] = J.kidx_of_k ? J.kidx_of_k[i] : -1;
             }
@@ -649,6 +737,7 @@ __global__ __launch_bounds__(kBlock) void k_query(QArgs a)
         const int cnt = (int)rfl(t.count);
         const uint32_t key = rfl(t.key);
         const int C = J.C, E = J.E, EC = E * C;
+        const int HK = HY ? J.HK : 0;
         const int nrows = J.nrows;
         const int cx = (int)(key % (uint32_t)J.nx);
         const int cy = (int)((key / (uint32_t)J.nx) % (uint32_t)J.ny);
@@ -656,12 +745,18 @@ __global__ __launch_bounds__(kBlock) void k_query(QArgs a)
 
         if (tid < cnt) {
             const uint32_t i = J.centers[t.first + tid];
-            const float4 p = J.cpos[i];
-            s_cen[tid] = p;
+            float4 p = J.cpos[i];
             const uint32_t ty = __float_as_uint(p.w) & 0xFFu;
             const int eo = s_envoff[ty];
             const uint32_t es = eo >= 0 ? (uint32_t)(eo / C) : 0xFFu;
-            s_meta[2 * tid] = ((uint32_t)s_trgslot[ty] & 0xFFu) | (es << 8);
+            const uint32_t ts = (uint32_t)s_trgslot[ty] & 0xFFu;
+            s_meta[2 * tid] = ts | (es << 8);
+            if (!PC)
+                p.w = __int_as_float((int)ts * EC);  // pair-count row of the center type
+            s_cen[tid] = p;
+        } else if (!PC && tid < T) {
+            // absent centers of a partial tile: NaN coordinates compare false
+            s_cen[tid] = make_float4(__builtin_nanf(""), 0.f, 0.f, 0.f);
         }
         for (int r = tid; r < nrows; r += kBlock) {
             const Row rw = J.rows[r];
@@ -678,8 +773,8 @@ __global__ __launch_bounds__(kBlock) void k_query(QArgs a)
             s_rows[2 * r] = rs;
             s_rows[2 * r + 1] = re;
         }
-        if (PC) {
-            const int nh = (C16 ? (cnt + 1) / 2 : cnt) * EC;
+        if (PC || HY) {
+            const int nh = (C16 ? (cnt + 1) / 2 : cnt) * (PC ? EC : HK);
             for (int i = tid; i < nh; i += kBlock)
                 s_H[i] = 0u;
         }
@@ -716,24 +811,25 @@ __global__ __launch_bounds__(kBlock) void k_query(QArgs a)
 
         // ---------------- main candidate loop ----------------
         {
-            // the T centers in registers; absent centers of a partial tile
-            // get NaN coordinates, so their squared distances compare false
+            // the T centers in registers (no LDS read, and so no wait behind
+            // the histogram atomics, in the common path); absent centers of a
+            // partial tile have NaN coordinates, so their squared distances
+            // compare false
             float cxr[T], cyr[T], czr[T];
             int hb[T];
 #pragma unroll
             for (int c = 0; c < T; ++c) {
-                const float4 p = s_cen[c < cnt ? c : 0];
-                cxr[c] = c < cnt ? p.x : __builtin_nanf("");
+                const float4 p = s_cen[PC ? (c < cnt ? c : 0) : c];
+                cxr[c] = (PC && c >= cnt) ? __builtin_nanf("") : p.x;
                 cyr[c] = p.y;
                 czr[c] = p.z;
-                if (PC)
-                    hb[c] = (C16 ? (c >> 1) : c) * EC;
-                else
-                    hb[c] = (int)(s_meta[2 * (c < cnt ? c : 0)] & 0xFFu) * EC;
+                hb[c] = PC ? (C16 ? (c >> 1) : c) * EC : __float_as_int(p.w);
             }
             const float R2 = J.R2;
             const float isc = J.idx_scale, ioff = J.idx_off;
+            const float ioff1 = J.idx_off1, amb = J.amb;
             const float Cm1f = (float)(C - 1);
+            const bool rdf = (J.families & (FAM_RDF | FAM_RIPK)) != 0;
             const gfvec4 *__restrict__ pos = (const gfvec4 *)J.pos;
             uint32_t *s_acc = PC ? s_H : s_P;
             const int nch = (int)rfl(s_rowchunk[nrows]);
@@ -769,7 +865,8 @@ __global__ __launch_bounds__(kBlock) void k_query(QArgs a)
                     pn = load_g4(pos, okn ? jj : rs);
                 }
                 const float px = ok ? p.x : __builtin_nanf("");
-                const int woff = ok ? s_envoff[__float_as_uint(p.w) & 0xFFu] : -1;
+                const uint32_t ptype = __float_as_uint(p.w) & 0xFFu;
+                const int woff = ok ? s_envoff[ptype] : -1;
                 const bool envok = woff >= 0;
                 if (PC) {
                     // 1) all T squared distances (the reference's FP32 expression)
@@ -807,29 +904,55 @@ __global__ __launch_bounds__(kBlock) void k_query(QArgs a)
                                                    __HIP_MEMORY_SCOPE_WORKGROUP);
                         }
                     }
-                } else {
+                } else if (!HY) {
                     // direct mode (pair counts per center type): one center
                     // at a time keeps the register count, and so the number
                     // of resident wavefronts, higher
 #pragma unroll
                     for (int c = 0; c < T; ++c) {
-                        if (c < cnt) {
-                            const float dx = cxr[c] - px;
-                            const float dy = cyr[c] - p.y;
-                            const float dz = czr[c] - p.z;
-                            const float d2 = dx * dx + dy * dy + dz * dz;
-                            const bool in = d2 <= R2;
-                            my_in += (unsigned long long)__builtin_popcountll(
-                                __builtin_amdgcn_ballot_w64(in));
-                            if (in && envok) {
-                                const float sq = __builtin_amdgcn_sqrtf(d2);
-                                const int q1 = (int)__builtin_amdgcn_fmed3f(
-                                    __builtin_fmaf(sq, isc, ioff), 1.0f, Cm1f);
-                                const int cat = q1 - (__float_as_uint(d2) < s_edges[q1] ? 1 : 0);
-                                __hip_atomic_fetch_add(&s_acc[hb[c] + woff + cat], 1u,
+                        const float dx = cxr[c] - px;
+                        const float dy = cyr[c] - p.y;
+                        const float dz = czr[c] - p.z;
+                        const float d2 = dx * dx + dy * dy + dz * dz;
+                        const bool in = d2 <= R2;
+                        my_in += (unsigned long long)__builtin_popcountll(
+                            __builtin_amdgcn_ballot_w64(in));
+                        if (in && envok) {
+                            const int cat = category_of(d2, isc, ioff1, amb, Cm1f, s_edges);
+                            __hip_atomic_fetch_add(&s_acc[hb[c] + woff + cat], 1u,
+                                                   __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+                        }
+                    }
+                } else {
+                    // hybrid mode: the direct-mode pair counts, and the pairs
+                    // within the kNN range of their type also into the
+                    // center's own histogram
+                    const int2 ki = ok ? s_kinfo[ptype] : make_int2(-1, (int)0xBF800000u);
+                    const int kof = ki.x;
+                    const float kthr = __int_as_float(ki.y);  // -1: no kNN for this type
+                    const bool rdf_env = rdf && envok;
+                    const float bthr = rdf_env ? R2 : kthr;
+#pragma unroll
+                    for (int c = 0; c < T; ++c) {
+                        const float dx = cxr[c] - px;
+                        const float dy = cyr[c] - p.y;
+                        const float dz = czr[c] - p.z;
+                        const float d2 = dx * dx + dy * dy + dz * dz;
+                        const bool in = d2 <= R2;
+                        my_in += (unsigned long long)__builtin_popcountll(
+                            __builtin_amdgcn_ballot_w64(in));
+                        if (d2 <= bthr) {
+                            const int cat = category_of(d2, isc, ioff1, amb, Cm1f, s_edges);
+                            if (rdf_env)
+                                __hip_atomic_fetch_add(&s_P[hb[c] + woff + cat], 1u,
                                                        __ATOMIC_RELAXED,
                                                        __HIP_MEMORY_SCOPE_WORKGROUP);
-                            }
+                            if (d2 <= kthr)
+                                __hip_atomic_fetch_add(
+                                    &s_H[(C16 ? (c >> 1) : c) * HK + kof + cat],
+                                    (C16 && (c & 1)) ? 65536u : 1u, __ATOMIC_RELAXED,
+                                    __HIP_MEMORY_SCOPE_WORKGROUP);
                         }
                     }
                 }
@@ -838,6 +961,9 @@ __global__ __launch_bounds__(kBlock) void k_query(QArgs a)
         }
         __syncthreads();
 
+        const bool want_k = (J.families & FAM_MKNN) && J.nk > 0;
+        const bool want_1 = (J.families & FAM_1NN) != 0;
+        const bool want_cn = (J.families & FAM_CNTNB) != 0;
         if (!PC) {
             // remove the self pair (d = 0) each center added to its own type
             if (tid < cnt) {
@@ -847,12 +973,13 @@ __global__ __launch_bounds__(kBlock) void k_query(QArgs a)
                     __hip_atomic_fetch_add(&s_P[(m & 0xFFu) * EC + es * C + J.cat0], 0xFFFFFFFFu,
                                            __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             }
-            continue;
+            if (!HY || !(want_k || want_1))
+                continue;
         }
 
         // ---------------- per-center extraction ----------------
         const int cat0 = J.cat0;
-        if (J.families & (FAM_RDF | FAM_RIPK)) {
+        if (PC && (J.families & (FAM_RDF | FAM_RIPK))) {
             // pair counts per (center type, env type, category); each thread
             // owns distinct (env, category) entries, so no atomics are needed
             for (int idx = tid; idx < EC; idx += kBlock) {
@@ -868,48 +995,97 @@ __global__ __launch_bounds__(kBlock) void k_query(QArgs a)
                 }
             }
         }
-        const bool want_k = (J.families & FAM_MKNN) && J.nk > 0;
-        const bool want_1 = (J.families & FAM_1NN) != 0;
-        const bool want_cn = (J.families & FAM_CNTNB) != 0;
         if (want_k || want_1 || want_cn) {
             const int kmax = want_k ? J.kmax : 0;
-            for (int c = wid; c < cnt; c += kWaves) {
-                const uint32_t m = s_meta[2 * c];
+            const uint32_t need = want_k ? (uint32_t)kmax : 1u;
+            const int lmax = J.lmax;
+            // this tile's block of k-th neighbour rows in the chunk buffer
+            const uint64_t tl = (uint64_t)(g - a.tile_off[job] - J.tile_base);
+            const uint64_t TR = (uint64_t)T * (uint64_t)lmax;
+            uint16_t *kb = J.KB ? J.KB + tl * (uint64_t)J.nplanes * TR : nullptr;
+            uint8_t *kt = J.KT ? J.KT + tl * TR : nullptr;
+            for (int c = wid; c < T; c += kWaves) {
+                if (c >= cnt) {
+                    if (kt)
+                        for (int l = lane; l < lmax; l += kWave)
+                            kt[c * lmax + l] = 0xFFu;
+                    continue;
+                }
+                const uint32_t m = rfl(s_meta[2 * c]);
                 const uint32_t tc = m & 0xFFu;
                 const int es = (int)((m >> 8) & 0xFFu);
-                const int hoff = (C16 ? (c >> 1) : c) * EC;
+                const int hoff = (C16 ? (c >> 1) : c) * (PC ? EC : HK);
                 const int sh = (C16 && (c & 1)) ? 16 : 0;
-                for (int tk = 0; tk < J.ntasks; ++tk) {
-                    if (!((J.task_trg[tk] >> tc) & 1u))
+                // count of env slot e, category cat in this center's histogram
+                // (its own d = 0 pair removed)
+                auto hv = [&](int e, int cat) -> uint32_t {
+                    uint32_t w = s_H[hoff + (PC ? e * C : (int)rfl((uint32_t)s_ckoff[e])) + cat];
+                    w = C16 ? ((w >> sh) & 0xFFFFu) : w;
+                    if (e == es && cat == cat0)
+                        w -= 1u;
+                    return w;
+                };
+                const int L = (int)rfl(J.trg_ntask[tc]);
+                if (kt)
+                    for (int l = L + lane; l < lmax; l += kWave)
+                        kt[c * lmax + l] = 0xFFu;
+                if (HY && J.trunc_any) {
+                    // a task whose kNN range is short of the full range must
+                    // hold its k neighbours there, else the center is rerun
+                    bool short_range = false;
+                    for (int l = 0; l < L && !short_range; ++l) {
+                        const int tk = (int)rfl(J.trg_tasks[tc * lmax + l]);
+                        const uint32_t ckw = rfl(J.task_ck[tk]);
+                        if (!(ckw >> 31))
+                            continue;
+                        const int ck = (int)(ckw & 0x7FFFFFFFu);
+                        const uint32_t envset = rfl(J.task_env[tk]);
+                        uint32_t tot = 0;
+                        for (int cat = lane; cat < ck; cat += kWave)
+                            for (int e = 0; e < E; ++e)
+                                if ((envset >> e) & 1u)
+                                    tot += hv(e, cat);
+                        for (int d = kWave / 2; d > 0; d >>= 1)
+                            tot += __shfl_xor(tot, d, kWave);
+                        short_range = tot < need;
+                    }
+                    if (short_range) {
+                        if (lane == 0)
+                            J.unres[J.centers[t.first + c]] = 1u;
+                        if (kt)
+                            for (int l = lane; l < L; l += kWave)
+                                kt[c * lmax + l] = 0xFFu;
                         continue;
-                    const uint32_t envset = J.task_env[tk];
-                    if (want_k || want_1) {
+                    }
+                }
+                for (int l = 0; l < L; ++l) {
+                    const int tk = (int)rfl(J.trg_tasks[tc * lmax + l]);
+                    const int row = c * lmax + l;
+                    if (kt && lane == 0)
+                        kt[row] = (uint8_t)tk;
+                    const uint32_t envset = rfl(J.task_env[tk]);
+                    const int ck = HY ? (int)(rfl(J.task_ck[tk]) & 0x7FFFFFFFu) : C;
+                    if ((want_k || want_1) && kb) {
+                        // categories of the k-th nearest neighbours: a prefix
+                        // scan over the center's category counts, lane = category
                         uint32_t carry = 0;
-                        const uint32_t need = want_k ? (uint32_t)kmax : 1u;
-                        for (int cb = 0; cb < C; cb += kWave) {
+                        for (int cb = 0; cb < ck; cb += kWave) {
                             const int cat = cb + lane;
                             uint32_t v = 0;
-                            if (cat < C) {
-                                for (int e = 0; e < E; ++e) {
-                                    if (!((envset >> e) & 1u))
-                                        continue;
-                                    uint32_t w = s_H[hoff + e * C + cat];
-                                    w = C16 ? ((w >> sh) & 0xFFFFu) : w;
-                                    if (e == es && cat == cat0)
-                                        w -= 1u;
-                                    v += w;
-                                }
-                            }
+                            if (cat < ck)
+                                for (int e = 0; e < E; ++e)
+                                    if ((envset >> e) & 1u)
+                                        v += hv(e, cat);
                             const uint32_t incl = wave_incl_scan(v, lane);
                             const uint32_t cum = carry + incl, prev = cum - v;
                             if (want_1 && prev < 1u && cum >= 1u)
-                                atomicAdd(&J.N1[tk * C + cat], 1u);
+                                kb[(uint32_t)J.nk * (uint32_t)TR + (uint32_t)row] = (uint16_t)cat;
                             if (want_k) {
                                 const uint32_t hi = cum < (uint32_t)kmax ? cum : (uint32_t)kmax;
                                 for (uint32_t k = prev + 1u; k <= hi; ++k) {
                                     const int kid = s_kidx[k];
                                     if (kid >= 0)
-                                        atomicAdd(&J.K[((size_t)tk * J.nk + kid) * C + cat], 1u);
+                                        kb[(uint32_t)kid * (uint32_t)TR + (uint32_t)row] = (uint16_t)cat;
                                 }
                             }
                             carry += __shfl(incl, kWave - 1, kWave);
@@ -919,28 +1095,22 @@ __global__ __launch_bounds__(kBlock) void k_query(QArgs a)
                         // fewer than k env neighbours: R + EPSILON, i.e. the
                         // sentinel category (S.cpp:4093-4095, 4035-4036)
                         if (want_1 && carry == 0u && lane == 0)
-                            atomicAdd(&J.N1[tk * C + J.cat_sentinel], 1u);
+                            kb[(uint32_t)J.nk * (uint32_t)TR + (uint32_t)row] = (uint16_t)J.cat_sentinel;
                         if (want_k) {
                             for (int k = (int)carry + 1 + lane; k <= kmax; k += kWave) {
                                 const int kid = s_kidx[k];
                                 if (kid >= 0)
-                                    atomicAdd(&J.K[((size_t)tk * J.nk + kid) * C + J.cat_sentinel],
-                                              1u);
+                                    kb[(uint32_t)kid * (uint32_t)TR + (uint32_t)row] = (uint16_t)J.cat_sentinel;
                             }
                         }
                     }
-                    if (want_cn) {
+                    if (PC && want_cn) {
                         for (int e = 0; e < E; ++e) {
                             if (!((envset >> e) & 1u))
                                 continue;
                             uint32_t tot = 0;
-                            for (int cat = lane; cat < C; cat += kWave) {
-                                uint32_t w = s_H[hoff + e * C + cat];
-                                w = C16 ? ((w >> sh) & 0xFFFFu) : w;
-                                if (e == es && cat == cat0)
-                                    w -= 1u;
-                                tot += w;
-                            }
+                            for (int cat = lane; cat < C; cat += kWave)
+                                tot += hv(e, cat);
                             for (int d = kWave / 2; d > 0; d >>= 1)
                                 tot += __shfl_xor(tot, d, kWave);
                             if (lane == 0) {
@@ -957,72 +1127,154 @@ __global__ __launch_bounds__(kBlock) void k_query(QArgs a)
     }
 }
 
-template <int T, bool C16, bool PC>
+template <int T, bool C16, int MODE>
 static hipError_t launch_query_t(const QueryLaunch &q, hipStream_t s)
 {
-    const size_t lds = query_lds_bytes(T, C16, PC, q.maxC, q.maxE, q.maxTrg, q.maxKmax, q.maxRows);
+    const size_t lds = query_lds_bytes(T, C16, MODE, q.maxC, q.maxE, q.maxTrg, q.maxKmax,
+                                       q.maxRows, q.maxHK);
     QArgs a;
     a.jobs = q.jobs;
     a.tile_off = q.tile_off;
     a.tiles = q.tiles;
     a.queue = q.queue;
-    a.total_tiles = q.total_tiles;
+    a.first_tile = q.first_tile;
+    a.end_tile = q.end_tile;
+    a.grab = q.grab > 0 ? q.grab : 1u;
     a.njobs = q.njobs;
     a.maxC = q.maxC;
     a.maxE = q.maxE;
     a.maxTrg = q.maxTrg;
     a.maxKmax = q.maxKmax;
     a.maxRows = q.maxRows;
-    hipError_t e = hipFuncSetAttribute((const void *)k_query<T, C16, PC>,
+    a.maxHK = q.maxHK;
+    hipError_t e = hipFuncSetAttribute((const void *)k_query<T, C16, MODE>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess)
         return e;
-    hipLaunchKernelGGL((k_query<T, C16, PC>), dim3(q.grid), dim3(kBlock), lds, s, a);
+    hipLaunchKernelGGL((k_query<T, C16, MODE>), dim3(q.grid), dim3(kBlock), lds, s, a);
     return hipGetLastError();
 }
 
-template <int T, bool C16, bool PC>
+template <int T, bool C16, int MODE>
 static hipError_t occ_t(size_t lds, int *nb)
 {
-    hipError_t e = hipFuncSetAttribute((const void *)k_query<T, C16, PC>,
+    hipError_t e = hipFuncSetAttribute((const void *)k_query<T, C16, MODE>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess)
         return e;
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(nb, k_query<T, C16, PC>, kBlock, lds);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(nb, k_query<T, C16, MODE>, kBlock, lds);
 }
+
+#define PP_DISPATCH_T(FN, C16, MODE, ...)                                    \
+    do {                                                                     \
+        if (T == 8) return FN<8, C16, MODE>(__VA_ARGS__);                    \
+        if (T == 16) return FN<16, C16, MODE>(__VA_ARGS__);                  \
+        if (T == 32) return FN<32, C16, MODE>(__VA_ARGS__);                  \
+    } while (0)
 
 #define PP_DISPATCH(FN, ...)                                                 \
     do {                                                                     \
-        if (!percenter) {                                                    \
-            if (T == 8) return FN<8, false, false>(__VA_ARGS__);             \
-            if (T == 16) return FN<16, false, false>(__VA_ARGS__);           \
-            if (T == 32) return FN<32, false, false>(__VA_ARGS__);           \
-        } else if (c16) {                                                    \
-            if (T == 8) return FN<8, true, true>(__VA_ARGS__);               \
-            if (T == 16) return FN<16, true, true>(__VA_ARGS__);             \
-            if (T == 32) return FN<32, true, true>(__VA_ARGS__);             \
-        } else {                                                             \
-            if (T == 8) return FN<8, false, true>(__VA_ARGS__);              \
-            if (T == 16) return FN<16, false, true>(__VA_ARGS__);            \
-            if (T == 32) return FN<32, false, true>(__VA_ARGS__);            \
-        }                                                                    \
+        if (mode == 0)                                                       \
+            PP_DISPATCH_T(FN, true, 0, __VA_ARGS__);                         \
+        else if (mode == 1 && c16)                                           \
+            PP_DISPATCH_T(FN, true, 1, __VA_ARGS__);                         \
+        else if (mode == 1)                                                  \
+            PP_DISPATCH_T(FN, false, 1, __VA_ARGS__);                        \
+        else if (mode == 2 && c16)                                           \
+            PP_DISPATCH_T(FN, true, 2, __VA_ARGS__);                         \
+        else if (mode == 2)                                                  \
+            PP_DISPATCH_T(FN, false, 2, __VA_ARGS__);                        \
         return hipErrorInvalidValue;                                         \
     } while (0)
 
-hipError_t query_max_blocks_per_cu(int T, int c16, int percenter, size_t lds, int *nb)
+hipError_t query_max_blocks_per_cu(int T, int c16, int mode, size_t lds, int *nb)
 {
     PP_DISPATCH(occ_t, lds, nb);
 }
 
 hipError_t launch_query(const QueryLaunch &q, hipStream_t s)
 {
-    const int T = q.T, c16 = q.c16, percenter = q.percenter;
-    hipError_t e = hipMemsetAsync(q.queue, 0, sizeof(uint32_t) * (kXcds + 1), s);
+    const int T = q.T, c16 = q.c16, mode = q.mode;
+    hipError_t e = hipMemsetAsync(q.queue, 0, sizeof(uint32_t) * kQueueWords, s);
     if (e != hipSuccess)
         return e;
-    if (q.total_tiles == 0)
+    if (q.end_tile <= q.first_tile)
         return hipSuccess;
     PP_DISPATCH(launch_query_t, q, s);
+}
+
+// ---------------------------------------------------------------------------
+// k-th neighbour histograms of one query chunk
+
+// One work unit: one plane (an MKNN order, or the 1NN) of a block of
+// tiles_per_unit tiles of one job.  The unit's categories are counted per
+// (task, category) in LDS, then added to the job's K / N1 with one global
+// atomic per non-zero count.
+__global__ __launch_bounds__(256) void k_kreduce(const JobDev *__restrict__ jobs, int njobs,
+                                                 const uint32_t *__restrict__ unit_off, int T,
+                                                 uint32_t tpu, int lds_words)
+{
+    extern __shared__ uint32_t h[];
+    const uint32_t u = blockIdx.x;
+    const int j = find_job(unit_off, njobs, u);
+    const JobDev &J = jobs[j];
+    const uint32_t lu = u - unit_off[j];
+    const uint32_t nblk = (J.kb_tiles + tpu - 1) / tpu;
+    const int plane = (int)(lu / nblk);
+    const uint32_t t0 = (lu - (uint32_t)plane * nblk) * tpu;
+    const uint32_t t1 = t0 + tpu < J.kb_tiles ? t0 + tpu : J.kb_tiles;
+    const int C = J.C;
+    const uint32_t TR = (uint32_t)T * (uint32_t)J.lmax;
+    const int words = J.ntasks * C;
+    const bool use_lds = words <= lds_words;
+    const bool is1 = plane == J.nk;  // the 1NN plane follows the MKNN orders
+    uint32_t *dst = is1 ? J.N1 : J.K + (size_t)plane * C;
+    const size_t tstride = is1 ? (size_t)C : (size_t)J.nk * C;
+    if (use_lds) {
+        for (int i = threadIdx.x; i < words; i += blockDim.x)
+            h[i] = 0u;
+        __syncthreads();
+    }
+    const uint32_t n = (t1 - t0) * TR;
+    const uint16_t *kb = J.KB + ((size_t)t0 * J.nplanes + plane) * TR;
+    const uint8_t *kt = J.KT + (size_t)t0 * TR;
+    const size_t pstride = (size_t)J.nplanes * TR;
+    for (uint32_t e = threadIdx.x; e < n; e += blockDim.x) {
+        const uint32_t tt = e / TR, r = e - tt * TR;
+        const uint32_t tk = kt[(size_t)tt * TR + r];
+        if (tk == 0xFFu)
+            continue;
+        const uint32_t cat = kb[(size_t)tt * pstride + r];
+        if (use_lds)
+            atomicAdd(&h[tk * C + cat], 1u);
+        else
+            atomicAdd(&dst[tk * tstride + cat], 1u);
+    }
+    if (use_lds) {
+        __syncthreads();
+        for (int i = threadIdx.x; i < words; i += blockDim.x) {
+            const uint32_t v = h[i];
+            if (v) {
+                const int tk = i / C;
+                atomicAdd(&dst[tk * tstride + (i - tk * C)], v);
+            }
+        }
+    }
+}
+
+hipError_t launch_kreduce(const JobDev *jobs, int njobs, const uint32_t *unit_off, uint32_t units,
+                          int T, uint32_t tiles_per_unit, int lds_words, hipStream_t s)
+{
+    if (units == 0)
+        return hipSuccess;
+    const size_t lds = (size_t)lds_words * 4u;
+    hipError_t e = hipFuncSetAttribute((const void *)k_kreduce,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess)
+        return e;
+    hipLaunchKernelGGL(k_kreduce, dim3(units), dim3(256), lds, s, jobs, njobs, unit_off, T,
+                       tiles_per_unit, lds_words);
+    return hipGetLastError();
 }
 
 } // namespace pp

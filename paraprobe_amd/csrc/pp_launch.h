@@ -60,17 +60,28 @@ struct QueryLaunch {
     int njobs;
     const uint32_t *tile_off;  // device [njobs + 1] prefix of per-job tile counts
     const Tile *const *tiles;  // device [njobs] per-job tile arrays
-    uint32_t total_tiles;
-    uint32_t *queue;           // device counters (kXcds + 1), zeroed by the launcher
+    uint32_t first_tile;       // global tile range [first_tile, end_tile) of this launch
+    uint32_t end_tile;
+    uint32_t *queue;           // device counters (kQueueWords), zeroed by the launcher
+    uint32_t grab;             // tiles handed out per queue atomic
     int T;                     // centers per tile
     int c16;                   // 16-bit per-center counters
-    int percenter;             // per-center histograms (MKNN / 1NN / CNTNB)
-    int maxC, maxE, maxTrg, maxKmax, maxRows;
+    int mode;                  // 0 pair counts, 1 full-range per-center, 2 hybrid
+    int maxC, maxE, maxTrg, maxKmax, maxRows, maxHK;
     int grid;                  // workgroups (persistent)
 };
-size_t query_lds_bytes(int T, int c16, int percenter, int maxC, int maxE, int maxTrg,
-                       int maxKmax, int maxRows);
-hipError_t query_max_blocks_per_cu(int T, int c16, int percenter, size_t lds, int *nb);
+size_t query_lds_bytes(int T, int c16, int mode, int maxC, int maxE, int maxTrg, int maxKmax,
+                       int maxRows, int maxHK);
+hipError_t query_max_blocks_per_cu(int T, int c16, int mode, size_t lds, int *nb);
 hipError_t launch_query(const QueryLaunch &q, hipStream_t s);
+
+// Histograms of the k-th neighbour categories one query chunk wrote
+// (JobDev::KB / KT) into K and N1.  unit_off[njobs + 1] is the prefix of the
+// per-job work units (planes x tile blocks of tiles_per_unit tiles); lds_words
+// is the largest ntasks * C among the jobs that reduce through LDS.
+hipError_t launch_kreduce(const JobDev *jobs, int njobs, const uint32_t *unit_off,
+                          uint32_t units, int T, uint32_t tiles_per_unit, int lds_words,
+                          hipStream_t s);
+constexpr int kReduceLdsWords = 16384;  // ntasks * C above this: global atomics
 
 } // namespace pp

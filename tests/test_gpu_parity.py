@@ -69,6 +69,31 @@ def test_small_all_families(ctx, seed):
     _compare(ctx, xyz, ty, s2, job)
 
 
+@pytest.mark.parametrize("factor", [0.0, 0.3, 3.0, 50.0])
+def test_knn_range_does_not_change_results(ctx, factor):
+    """kNN jobs keep per-center histograms over a range where factor * k
+    neighbours are expected, re-running the centers short of k there; 0 means
+    full-range histograms.  Every factor must give the oracle's histograms;
+    0.3 forces many re-runs on this clustered tip."""
+    _say(f"test_knn_range factor={factor}")
+    base = [0.85, 0.12, 0.03]
+    tip = synth.tip(60_000, 5, "frustum", 12.0, 7.0, 35.0, base, nclusters=250, crmin=1.0,
+                    crmax=2.0, cfrac=synth.cluster_fracs(base))
+    jobs = [Job(synth.ALL_PAIRS + [((1, 2), (2, 3))], 0.0, 0.02, 2.5,
+                FAM_RDF | FAM_MKNN | FAM_1NN, kvalues=list(range(1, 41))),
+            Job(synth.ALL_PAIRS, 0.1, 0.01, 2.0, FAM_1NN)]
+    ctx.set_knn_range(factor)
+    try:
+        for job in jobs:
+            g, _, _ = _compare(ctx, tip.xyz, tip.types, tip.ion2surf_sqr, job)
+            if factor == 0.0:
+                assert g.percenter == 1
+            elif factor == 0.3 and job.families & FAM_MKNN:
+                assert g.percenter == 2 and g.centers_rerun > 0
+    finally:
+        ctx.set_knn_range(3.0)
+
+
 @pytest.mark.parametrize("T,r", [(8, 1), (16, 2), (32, 3), (16, 4), (8, 2), (32, 1)])
 def test_tuning_does_not_change_results(ctx, T, r):
     _say(f"test_tuning T={T} cells_per_rmax={r}")
