@@ -609,8 +609,12 @@ struct JobRun {
     std::vector<Row> rows;
     size_t ncells = 0;
     // device pointers (carved from the slot's scratch arena)
-    uint32_t *flags = nullptr, *cpos = nullptr, *centers = nullptr, *ntile = nullptr,
-             *toff = nullptr, *ws = nullptr, *tables = nullptr, *maxcand = nullptr;
+    // flags: the hybrid mode's rerun marks; sflags: one slot's selection;
+    // maxcand: [0] largest stencil candidate count, [1] centers, [2] tiles
+    uint32_t *flags = nullptr, *sflags = nullptr, *cpos = nullptr, *centers = nullptr,
+             *ntile = nullptr, *toff = nullptr, *ws = nullptr, *tables = nullptr,
+             *maxcand = nullptr;
+    const uint32_t *tile_src = nullptr;  // center selection of the current tiles
     Tile *tiles = nullptr;
     unsigned long long *P = nullptr, *stats = nullptr;
     uint32_t *K = nullptr, *N1 = nullptr, *CN = nullptr;
@@ -633,6 +637,7 @@ void layout_job(JobRun &r, Carver &m)
     const uint64_t n = r.cloud->n;
     const uint64_t nc = r.ncells;
     r.flags = m.take<uint32_t>(n);
+    r.sflags = m.take<uint32_t>(n);
     r.cpos = m.take<uint32_t>(n + 1);
     r.centers = m.take<uint32_t>(n);
     r.ntile = m.take<uint32_t>(nc);
@@ -674,19 +679,31 @@ void layout_job(JobRun &r, Carver &m)
     // CN is carved later: its size follows from the stencil bound
 }
 
+// Center list and tiles of a job, one target-type slot after the other
+// (every tile holds centers of one type), from the considered ions or from
+// the flags in r.tile_src; reads back centers, tiles and the candidate bound.
 pp_status prep_tiles(pp_ctx *c, Slot &sl, JobRun &r, int T)
 {
+    const JobPlan &p = r.plan;
     const uint64_t n = r.cloud->n;
     const uint64_t nc = r.ncells;
     hipStream_t s = sl.stream;
+    const int32_t *d_trgslot = (const int32_t *)(r.tables + r.o_trgslot);
+    const Row *d_rows = (const Row *)(r.tables + r.o_rows);
     PP_HIP(c, hipMemsetAsync(r.maxcand, 0, 16, s));
-    PP_HIP(c, launch_tilecount(r.cl->cellstart, r.cpos, nc, T, r.ntile, s));
-    PP_HIP(c, scan_exclusive(r.ntile, r.toff, nc, r.ws, s));
-    PP_HIP(c, launch_tilewrite(r.cl->cellstart, r.cpos, r.toff, nc, T, 0, r.cl->g,
-                               (const Row *)(r.tables + r.o_rows), (int)r.rows.size(), r.tiles,
-                               r.maxcand, s));
-    PP_HIP(c, hipMemcpyAsync(&r.h_counts[0], r.cpos + n, 4, hipMemcpyDeviceToHost, s));
-    PP_HIP(c, hipMemcpyAsync(&r.h_counts[1], r.toff + nc, 4, hipMemcpyDeviceToHost, s));
+    for (int slot = 0; slot < p.ntrg; ++slot) {
+        PP_HIP(c, launch_flags(r.cl->pos, r.cl->s2, n, d_trgslot, p.R2, slot, r.tile_src,
+                               r.sflags, s));
+        PP_HIP(c, scan_exclusive(r.sflags, r.cpos, n, r.ws, s));
+        PP_HIP(c, launch_compact(r.sflags, r.cpos, n, r.maxcand, r.centers, s));
+        PP_HIP(c, launch_tilecount(r.cl->cellstart, r.cpos, nc, T, r.ntile, s));
+        PP_HIP(c, scan_exclusive(r.ntile, r.toff, nc, r.ws, s));
+        PP_HIP(c, launch_tilewrite(r.cl->cellstart, r.cpos, r.toff, nc, T, slot, r.cl->g, d_rows,
+                                   (int)r.rows.size(), r.tiles, r.maxcand, s));
+        PP_HIP(c, launch_bump(r.cpos, n, r.toff, nc, r.maxcand, s));
+    }
+    PP_HIP(c, hipMemcpyAsync(&r.h_counts[0], r.maxcand + 1, 4, hipMemcpyDeviceToHost, s));
+    PP_HIP(c, hipMemcpyAsync(&r.h_counts[1], r.maxcand + 2, 4, hipMemcpyDeviceToHost, s));
     PP_HIP(c, hipMemcpyAsync(&r.h_counts[2], r.maxcand, 4, hipMemcpyDeviceToHost, s));
     return PP_OK;
 }
@@ -694,7 +711,6 @@ pp_status prep_tiles(pp_ctx *c, Slot &sl, JobRun &r, int T)
 pp_status prep_job(pp_ctx *c, Slot &sl, JobRun &r, int T)
 {
     const JobPlan &p = r.plan;
-    const uint64_t n = r.cloud->n;
     hipStream_t s = sl.stream;
     std::vector<uint32_t> &h = r.h_tables;  // lives until the job's results are back
     h.assign(r.tables_words, 0u);
@@ -722,10 +738,7 @@ pp_status prep_job(pp_ctx *c, Slot &sl, JobRun &r, int T)
     PP_HIP(c, hipMemsetAsync(r.stats, 0, sizeof(unsigned long long) * 8, s));
     PP_HIP(c, hipMemsetAsync(r.K, 0, 4 * (size_t)std::max(p.ntasks * p.nku * p.C, 1), s));
     PP_HIP(c, hipMemsetAsync(r.N1, 0, 4 * (size_t)std::max(p.ntasks * p.C, 1), s));
-    const int32_t *d_trgslot = (const int32_t *)(r.tables + r.o_trgslot);
-    PP_HIP(c, launch_flags(r.cl->pos, r.cl->s2, n, d_trgslot, p.R2, r.flags, s));
-    PP_HIP(c, scan_exclusive(r.flags, r.cpos, n, r.ws, s));
-    PP_HIP(c, launch_compact(r.flags, r.cpos, n, r.centers, s));
+    r.tile_src = nullptr;
     return prep_tiles(c, sl, r, T);
 }
 
@@ -1372,9 +1385,7 @@ pp_status run_jobs(pp_ctx *c, Slot &sl, int njobs, const pp_cloud *const *clouds
         if (!hy.empty()) {
             for (int i : hy) {
                 JobRun &r = runs[i];
-                const uint64_t n = r.cloud->n;
-                PP_HIP(c, scan_exclusive(r.flags, r.cpos, n, r.ws, sl.stream));
-                PP_HIP(c, launch_compact(r.flags, r.cpos, n, r.centers, sl.stream));
+                r.tile_src = r.flags;
                 st = prep_tiles(c, sl, r, TA);
                 if (st != PP_OK)
                     return st;
@@ -1788,9 +1799,10 @@ pp_status pp_set_tuning(pp_ctx *c, int32_t centers_per_tile, int32_t cells_per_r
     if (!c)
         return PP_ERR_INVALID_ARG;
     const bool autot = centers_per_tile == -1 && cells_per_rmax == -1;
-    if (!autot && !(centers_per_tile == 0 || centers_per_tile == 8 || centers_per_tile == 16 ||
-                    centers_per_tile == 32))
-        return fail(c, PP_ERR_INVALID_ARG, "centers_per_tile must be 0, 8, 16 or 32 (-1: autotune)");
+    if (!autot && !(centers_per_tile == 0 || centers_per_tile == 4 || centers_per_tile == 8 ||
+                    centers_per_tile == 16 || centers_per_tile == 32))
+        return fail(c, PP_ERR_INVALID_ARG,
+                    "centers_per_tile must be 0, 4, 8, 16 or 32 (-1: autotune)");
     if (!autot && (cells_per_rmax < 0 || cells_per_rmax > 4))
         return fail(c, PP_ERR_INVALID_ARG, "cells_per_rmax must be 0..4 (-1: autotune)");
     std::lock_guard<std::mutex> lk(c->mu);

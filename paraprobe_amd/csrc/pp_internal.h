@@ -26,13 +26,13 @@ struct Row {
     int16_t dy, dz, dxm, pad;
 };
 
-// One query tile: up to T considered centers of one cell of one job.
+// One query tile: up to T considered centers of one type in one cell of one job.
 struct Tile {
     uint32_t key;    // linear cell key
     uint32_t first;  // first entry in the job's center list
     uint16_t count;  // centers in this tile
-    uint16_t job;    // job index within the launch
-    uint32_t pad;
+    uint16_t job;    // unused (the launch's tile offsets locate the job)
+    uint32_t pad;    // target-type slot of the tile's centers
 };
 
 enum : uint32_t {

@@ -362,45 +362,54 @@ hipError_t launch_typecount(const uint8_t *type, uint64_t n, uint32_t *counts, h
 // ---------------------------------------------------------------------------
 // per-job centers and tiles
 
+// Center selection for one target type slot: considered (type a target of
+// some task and ion2surf >= R^2, src/PARAPROBE_SolverHdl.cpp:3846-3860), or
+// flagged in src when given, and of type slot `slot`.  Center lists and tiles
+// are built slot after slot, so every tile holds centers of one type.
 __global__ __launch_bounds__(256) void k_flags(const float4 *__restrict__ pos,
                                                const float *__restrict__ s2sorted, uint64_t n,
                                                const int32_t *__restrict__ trgslot, float R2,
+                                               int slot, const uint32_t *__restrict__ src,
                                                uint32_t *__restrict__ flags)
 {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n)
         return;
     const uint32_t t = __float_as_uint(pos[i].w) & 0xFFu;
-    // considered iff the type is a target of some task and ion2surf >= R^2
-    // (src/PARAPROBE_SolverHdl.cpp:3846-3860)
-    flags[i] = (trgslot[t] >= 0 && s2sorted[i] >= R2) ? 1u : 0u;
+    const int ts = trgslot[t];
+    const bool sel = src ? src[i] != 0u : (ts >= 0 && s2sorted[i] >= R2);
+    flags[i] = (sel && ts == slot) ? 1u : 0u;
 }
 
 hipError_t launch_flags(const float4 *pos, const float *s2sorted, uint64_t n,
-                        const int32_t *trgslot, float R2, uint32_t *flags, hipStream_t s)
+                        const int32_t *trgslot, float R2, int slot, const uint32_t *src,
+                        uint32_t *flags, hipStream_t s)
 {
     if (n == 0)
         return hipSuccess;
     hipLaunchKernelGGL(k_flags, dim3(grid_for(n, 256)), dim3(256), 0, s, pos, s2sorted, n, trgslot,
-                       R2, flags);
+                       R2, slot, src, flags);
     return hipGetLastError();
 }
 
+// centers[base[1] + cpos[i]] = i for the flagged ions (base[1]: centers of
+// the slots before)
 __global__ __launch_bounds__(256) void k_compact(const uint32_t *__restrict__ flags,
                                                  const uint32_t *__restrict__ cpos, uint64_t n,
+                                                 const uint32_t *__restrict__ base,
                                                  uint32_t *__restrict__ centers)
 {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n && flags[i])
-        centers[cpos[i]] = (uint32_t)i;
+        centers[base[1] + cpos[i]] = (uint32_t)i;
 }
 
 hipError_t launch_compact(const uint32_t *flags, const uint32_t *cpos, uint64_t n,
-                          uint32_t *centers, hipStream_t s)
+                          const uint32_t *base, uint32_t *centers, hipStream_t s)
 {
     if (n == 0)
         return hipSuccess;
-    hipLaunchKernelGGL(k_compact, dim3(grid_for(n, 256)), dim3(256), 0, s, flags, cpos, n,
+    hipLaunchKernelGGL(k_compact, dim3(grid_for(n, 256)), dim3(256), 0, s, flags, cpos, n, base,
                        centers);
     return hipGetLastError();
 }
@@ -425,28 +434,30 @@ hipError_t launch_tilecount(const uint32_t *cellstart, const uint32_t *cpos, uin
     return hipGetLastError();
 }
 
-__device__ inline void row_range(const Row &r, int cx, int cy, int cz, const GridGeom &g,
+// row range of stencil row rw around cell (cx, cy, cz): empty when outside
+__device__ inline void row_range(const Row &rw, int cx, int cy, int cz, const GridGeom &g,
                                  const uint32_t *__restrict__ cellstart, uint32_t &rs, uint32_t &re)
 {
-    const int y = cy + r.dy, z = cz + r.dz;
-    if (y < 0 || y >= g.ny || z < 0 || z >= g.nz) {
-        rs = re = 0;
+    rs = re = 0;
+    const int y = cy + rw.dy, z = cz + rw.dz;
+    if (y < 0 || y >= g.ny || z < 0 || z >= g.nz)
         return;
-    }
-    const int x0 = cx - r.dxm < 0 ? 0 : cx - r.dxm;
-    const int x1 = cx + r.dxm >= g.nx ? g.nx - 1 : cx + r.dxm;
-    const uint32_t k0 = ((uint32_t)z * (uint32_t)g.ny + (uint32_t)y) * (uint32_t)g.nx + (uint32_t)x0;
+    const int x0 = cx - rw.dxm < 0 ? 0 : cx - rw.dxm;
+    const int x1 = cx + rw.dxm >= g.nx ? g.nx - 1 : cx + rw.dxm;
+    const uint64_t k0 = ((uint64_t)z * g.ny + y) * g.nx + x0;
     rs = cellstart[k0];
     re = cellstart[k0 + (uint32_t)(x1 - x0) + 1u];
 }
 
+// Tiles of one slot: base[1] centers and base[2] tiles precede it; base[0]
+// is the largest candidate count of any tile's stencil.
 __global__ __launch_bounds__(256) void k_tilewrite(const uint32_t *__restrict__ cellstart,
                                                    const uint32_t *__restrict__ cpos,
                                                    const uint32_t *__restrict__ toff,
-                                                   uint64_t ncells, int T, int job, GridGeom g,
+                                                   uint64_t ncells, int T, int slot, GridGeom g,
                                                    const Row *__restrict__ rows, int nrows,
                                                    Tile *__restrict__ tiles,
-                                                   uint32_t *__restrict__ maxcand)
+                                                   uint32_t *__restrict__ base)
 {
     const uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ncells)
@@ -455,15 +466,15 @@ __global__ __launch_bounds__(256) void k_tilewrite(const uint32_t *__restrict__ 
     if (hi == lo)
         return;
     const uint32_t cnt = hi - lo;
-    const uint32_t t0 = toff[c];
+    const uint32_t cb = base[1], t0 = base[2] + toff[c];
     for (uint32_t m = 0; m * (uint32_t)T < cnt; ++m) {
         Tile t;
         t.key = (uint32_t)c;
-        t.first = lo + m * (uint32_t)T;
+        t.first = cb + lo + m * (uint32_t)T;
         const uint32_t left = cnt - m * (uint32_t)T;
         t.count = (uint16_t)(left < (uint32_t)T ? left : (uint32_t)T);
-        t.job = (uint16_t)job;
-        t.pad = 0;
+        t.job = 0;
+        t.pad = (uint32_t)slot;
         tiles[t0 + m] = t;
     }
     // candidates of this cell's stencil bound every center's neighbour count
@@ -476,15 +487,32 @@ __global__ __launch_bounds__(256) void k_tilewrite(const uint32_t *__restrict__ 
         row_range(rows[r], cx, cy, cz, g, cellstart, rs, re);
         tot += re - rs;
     }
-    atomicMax(maxcand, tot);
+    atomicMax(&base[0], tot);
 }
 
 hipError_t launch_tilewrite(const uint32_t *cellstart, const uint32_t *cpos, const uint32_t *toff,
-                            uint64_t ncells, int T, int job, GridGeom g, const Row *rows, int nrows,
-                            Tile *tiles, uint32_t *maxcand, hipStream_t s)
+                            uint64_t ncells, int T, int slot, GridGeom g, const Row *rows,
+                            int nrows, Tile *tiles, uint32_t *base, hipStream_t s)
 {
     hipLaunchKernelGGL(k_tilewrite, dim3(grid_for(ncells, 256)), dim3(256), 0, s, cellstart, cpos,
-                       toff, ncells, T, job, g, rows, nrows, tiles, maxcand);
+                       toff, ncells, T, slot, g, rows, nrows, tiles, base);
+    return hipGetLastError();
+}
+
+// base[1] += centers of the slot just written (cpos[n]), base[2] += its tiles
+__global__ void k_bump(const uint32_t *__restrict__ cpos, uint64_t n,
+                       const uint32_t *__restrict__ toff, uint64_t ncells, uint32_t *base)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        base[1] += cpos[n];
+        base[2] += toff[ncells];
+    }
+}
+
+hipError_t launch_bump(const uint32_t *cpos, uint64_t n, const uint32_t *toff, uint64_t ncells,
+                       uint32_t *base, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_bump, dim3(1), dim3(64), 0, s, cpos, n, toff, ncells, base);
     return hipGetLastError();
 }
 
@@ -526,7 +554,7 @@ __host__ __device__ inline Lay make_layout(int T, int c16, int mode, int maxC, i
     L.meta = o;
     o += al4(2 * T);
     L.P = o;
-    o += al4(maxTrg * maxE * maxC);
+    o += al4(maxE * maxC);  // pair counts of one center type (tiles hold one type)
     L.H = o;
     const int hc = c16 ? (T + 1) / 2 : T;
     o += mode == 1 ? al4(hc * maxE * maxC) : mode == 2 ? al4(hc * maxHK) : 0;
@@ -641,8 +669,10 @@ __device__ inline int category_of(float d2, float isc, float ioff1, float amb, f
 // MODE 2: pair counts as in MODE 0 plus per-center histograms over the kNN
 // range only (see JobDev::HK); centers whose kNN range holds too few
 // neighbours are flagged for a MODE 1 rerun.
+// Pair-count mode with 8 centers per tile fits 72 VGPRs without spills: ask
+// for 7 resident waves per SIMD.
 template <int T, bool C16, int MODE>
-__global__ __launch_bounds__(kBlock) void k_query(QArgs a)
+__global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : 1) void k_query(QArgs a)
 {
     constexpr bool PC = MODE == 1;
     constexpr bool HY = MODE == 2;
@@ -681,17 +711,30 @@ __global__ __launch_bounds__(kBlock) void k_query(QArgs a)
         const uint32_t g = rfl(s_g);
         const bool done = g >= a.end_tile;
         const int job = done ? -1 : (int)rfl((uint32_t)find_job(a.tile_off, a.njobs, g));
+        // the tile: cell key, first center, center count, target-type slot
+        uint32_t t_key = 0, t_first = 0, t_count = 0;
+        int ts = 0;
+        if (!done) {
+            const Tile *tp = a.tiles[job] + (g - a.tile_off[job]);
+            t_key = rfl(tp->key);
+            t_first = rfl(tp->first);
+            t_count = rfl((uint32_t)tp->count);
+            ts = (int)rfl(tp->pad);
+        }
+        // s_P holds the pair counts of one (job, center type)
+        const int pkey = done ? -1 : job * kMaxTrg + ts;
 
-        if (job != cur || (!done && pending > 0x7FFFFFFFull)) {
+        if (pkey != cur || (!done && pending > 0x7FFFFFFFull)) {
             if (cur >= 0) {
-                // flush the accumulated pair counts and statistics of the job
-                const JobDev &P0 = a.jobs[cur];
-                const int n = P0.ntrg * P0.E * P0.C;
+                // flush the accumulated pair counts and statistics
+                const JobDev &P0 = a.jobs[cur / kMaxTrg];
+                const int n = P0.E * P0.C;
                 if (P0.families & (FAM_RDF | FAM_RIPK)) {
+                    unsigned long long *dst = P0.P + (size_t)(cur % kMaxTrg) * n;
                     for (int i = tid; i < n; i += kBlock) {
                         const uint32_t v = s_P[i];
                         if (v)
-                            atomicAdd(&P0.P[i], (unsigned long long)v);
+                            atomicAdd(&dst[i], (unsigned long long)v);
                     }
                 }
                 if (lane == 0 && inball)
@@ -706,7 +749,7 @@ __global__ __launch_bounds__(kBlock) void k_query(QArgs a)
             if (done)
                 break;
             const JobDev &J = a.jobs[job];
-            if (job != cur) {
+            if (cur < 0 || job != cur / kMaxTrg) {
                 for (int i = tid; i < J.C; i += kBlock)
                     s_edges[i] = J.edges[i];
                 for (int i = tid; i < 256; i += kBlock) {
@@ -723,19 +766,18 @@ __global__ __launch_bounds__(kBlock) void k_query(QArgs a)
                     for (int i = tid; i <= J.kmax; i += kBlock)
                         s_kidx[i] = J.kidx_of_k ? J.kidx_of_k[i] : -1;
             }
-            const int n = J.ntrg * J.E * J.C;
+            const int n = J.E * J.C;
             for (int i = tid; i < n; i += kBlock)
                 s_P[i] = 0u;
-            cur = job;
+            cur = pkey;
             __syncthreads();
         }
         if (done)
             break;
 
         const JobDev &J = a.jobs[job];
-        const Tile t = a.tiles[job][g - a.tile_off[job]];
-        const int cnt = (int)rfl(t.count);
-        const uint32_t key = rfl(t.key);
+        const int cnt = (int)t_count;
+        const uint32_t key = t_key;
         const int C = J.C, E = J.E, EC = E * C;
         const int HK = HY ? J.HK : 0;
         const int nrows = J.nrows;
@@ -743,20 +785,22 @@ __global__ __launch_bounds__(kBlock) void k_query(QArgs a)
         const int cy = (int)((key / (uint32_t)J.nx) % (uint32_t)J.ny);
         const int cz = (int)(key / ((uint32_t)J.nx * (uint32_t)J.ny));
 
-        if (tid < cnt) {
-            const uint32_t i = J.centers[t.first + tid];
-            float4 p = J.cpos[i];
-            const uint32_t ty = __float_as_uint(p.w) & 0xFFu;
-            const int eo = s_envoff[ty];
-            const uint32_t es = eo >= 0 ? (uint32_t)(eo / C) : 0xFFu;
-            const uint32_t ts = (uint32_t)s_trgslot[ty] & 0xFFu;
-            s_meta[2 * tid] = ts | (es << 8);
-            if (!PC)
-                p.w = __int_as_float((int)ts * EC);  // pair-count row of the center type
-            s_cen[tid] = p;
-        } else if (!PC && tid < T) {
+        if (tid < T) {
             // absent centers of a partial tile: NaN coordinates compare false
-            s_cen[tid] = make_float4(__builtin_nanf(""), 0.f, 0.f, 0.f);
+            float px = __builtin_nanf(""), py = 0.f, pz = 0.f;
+            uint32_t es = 0xFFu;
+            if (tid < cnt) {
+                const float4 *cp = J.cpos + J.centers[t_first + tid];
+                px = cp->x;
+                py = cp->y;
+                pz = cp->z;
+                const int eo = s_envoff[__float_as_uint(cp->w) & 0xFFu];
+                // env slot = eo / C exactly (eo is a multiple of C below 2^24)
+                if (eo >= 0)
+                    es = (uint32_t)__builtin_rintf((float)eo * __builtin_amdgcn_rcpf((float)C));
+            }
+            s_meta[2 * tid] = (uint32_t)ts | (es << 8);
+            s_cen[tid] = make_float4(px, py, pz, 0.f);
         }
         for (int r = tid; r < nrows; r += kBlock) {
             const Row rw = J.rows[r];
@@ -816,14 +860,12 @@ __global__ __launch_bounds__(kBlock) void k_query(QArgs a)
             // partial tile have NaN coordinates, so their squared distances
             // compare false
             float cxr[T], cyr[T], czr[T];
-            int hb[T];
 #pragma unroll
             for (int c = 0; c < T; ++c) {
-                const float4 p = s_cen[PC ? (c < cnt ? c : 0) : c];
-                cxr[c] = (PC && c >= cnt) ? __builtin_nanf("") : p.x;
+                const float4 p = s_cen[c];
+                cxr[c] = p.x;
                 cyr[c] = p.y;
                 czr[c] = p.z;
-                hb[c] = PC ? (C16 ? (c >> 1) : c) * EC : __float_as_int(p.w);
             }
             const float R2 = J.R2;
             const float isc = J.idx_scale, ioff = J.idx_off;
@@ -900,7 +942,8 @@ __global__ __launch_bounds__(kBlock) void k_query(QArgs a)
                         if (in && envok) {
                             const int cat = qi[c] - (__float_as_uint(d2[c]) < ed[c] ? 1 : 0);
                             const uint32_t inc = (C16 && (c & 1)) ? 65536u : 1u;
-                            __hip_atomic_fetch_add(&s_acc[hb[c] + woff + cat], inc, __ATOMIC_RELAXED,
+                            __hip_atomic_fetch_add(&s_acc[(C16 ? (c >> 1) : c) * EC + woff + cat],
+                                                   inc, __ATOMIC_RELAXED,
                                                    __HIP_MEMORY_SCOPE_WORKGROUP);
                         }
                     }
@@ -910,18 +953,20 @@ __global__ __launch_bounds__(kBlock) void k_query(QArgs a)
                     // of resident wavefronts, higher
 #pragma unroll
                     for (int c = 0; c < T; ++c) {
-                        const float dx = cxr[c] - px;
-                        const float dy = cyr[c] - p.y;
-                        const float dz = czr[c] - p.z;
-                        const float d2 = dx * dx + dy * dy + dz * dz;
-                        const bool in = d2 <= R2;
-                        my_in += (unsigned long long)__builtin_popcountll(
-                            __builtin_amdgcn_ballot_w64(in));
-                        if (in && envok) {
-                            const int cat = category_of(d2, isc, ioff1, amb, Cm1f, s_edges);
-                            __hip_atomic_fetch_add(&s_acc[hb[c] + woff + cat], 1u,
-                                                   __ATOMIC_RELAXED,
-                                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+                        if (c < cnt) {  // uniform: skips a partial tile's absent centers
+                            const float dx = cxr[c] - px;
+                            const float dy = cyr[c] - p.y;
+                            const float dz = czr[c] - p.z;
+                            const float d2 = dx * dx + dy * dy + dz * dz;
+                            const bool in = d2 <= R2;
+                            my_in += (unsigned long long)__builtin_popcountll(
+                                __builtin_amdgcn_ballot_w64(in));
+                            if (in && envok) {
+                                const int cat = category_of(d2, isc, ioff1, amb, Cm1f, s_edges);
+                                __hip_atomic_fetch_add(&s_acc[woff + cat], 1u,
+                                                       __ATOMIC_RELAXED,
+                                                       __HIP_MEMORY_SCOPE_WORKGROUP);
+                            }
                         }
                     }
                 } else {
@@ -935,24 +980,26 @@ __global__ __launch_bounds__(kBlock) void k_query(QArgs a)
                     const float bthr = rdf_env ? R2 : kthr;
 #pragma unroll
                     for (int c = 0; c < T; ++c) {
-                        const float dx = cxr[c] - px;
-                        const float dy = cyr[c] - p.y;
-                        const float dz = czr[c] - p.z;
-                        const float d2 = dx * dx + dy * dy + dz * dz;
-                        const bool in = d2 <= R2;
-                        my_in += (unsigned long long)__builtin_popcountll(
-                            __builtin_amdgcn_ballot_w64(in));
-                        if (d2 <= bthr) {
-                            const int cat = category_of(d2, isc, ioff1, amb, Cm1f, s_edges);
-                            if (rdf_env)
-                                __hip_atomic_fetch_add(&s_P[hb[c] + woff + cat], 1u,
-                                                       __ATOMIC_RELAXED,
-                                                       __HIP_MEMORY_SCOPE_WORKGROUP);
-                            if (d2 <= kthr)
-                                __hip_atomic_fetch_add(
-                                    &s_H[(C16 ? (c >> 1) : c) * HK + kof + cat],
-                                    (C16 && (c & 1)) ? 65536u : 1u, __ATOMIC_RELAXED,
-                                    __HIP_MEMORY_SCOPE_WORKGROUP);
+                        if (c < cnt) {  // uniform: skips a partial tile's absent centers
+                            const float dx = cxr[c] - px;
+                            const float dy = cyr[c] - p.y;
+                            const float dz = czr[c] - p.z;
+                            const float d2 = dx * dx + dy * dy + dz * dz;
+                            const bool in = d2 <= R2;
+                            my_in += (unsigned long long)__builtin_popcountll(
+                                __builtin_amdgcn_ballot_w64(in));
+                            if (d2 <= bthr) {
+                                const int cat = category_of(d2, isc, ioff1, amb, Cm1f, s_edges);
+                                if (rdf_env)
+                                    __hip_atomic_fetch_add(&s_P[woff + cat], 1u,
+                                                           __ATOMIC_RELAXED,
+                                                           __HIP_MEMORY_SCOPE_WORKGROUP);
+                                if (d2 <= kthr)
+                                    __hip_atomic_fetch_add(
+                                        &s_H[(C16 ? (c >> 1) : c) * HK + kof + cat],
+                                        (C16 && (c & 1)) ? 65536u : 1u, __ATOMIC_RELAXED,
+                                        __HIP_MEMORY_SCOPE_WORKGROUP);
+                            }
                         }
                     }
                 }
@@ -970,7 +1017,7 @@ __global__ __launch_bounds__(kBlock) void k_query(QArgs a)
                 const uint32_t m = s_meta[2 * tid];
                 const uint32_t es = (m >> 8) & 0xFFu;
                 if (es != 0xFFu)
-                    __hip_atomic_fetch_add(&s_P[(m & 0xFFu) * EC + es * C + J.cat0], 0xFFFFFFFFu,
+                    __hip_atomic_fetch_add(&s_P[es * C + J.cat0], 0xFFFFFFFFu,
                                            __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             }
             if (!HY || !(want_k || want_1))
@@ -991,7 +1038,7 @@ __global__ __launch_bounds__(kBlock) void k_query(QArgs a)
                     const uint32_t m = s_meta[2 * c];
                     if ((int)((m >> 8) & 0xFFu) == e && cat == cat0)
                         v -= 1u;
-                    s_P[(m & 0xFFu) * EC + idx] += v;
+                    s_P[idx] += v;
                 }
             }
         }
@@ -1051,7 +1098,7 @@ __global__ __launch_bounds__(kBlock) void k_query(QArgs a)
                     }
                     if (short_range) {
                         if (lane == 0)
-                            J.unres[J.centers[t.first + c]] = 1u;
+                            J.unres[J.centers[t_first + c]] = 1u;
                         if (kt)
                             for (int l = lane; l < L; l += kWave)
                                 kt[c * lmax + l] = 0xFFu;
@@ -1167,6 +1214,7 @@ static hipError_t occ_t(size_t lds, int *nb)
 
 #define PP_DISPATCH_T(FN, C16, MODE, ...)                                    \
     do {                                                                     \
+        if (T == 4) return FN<4, C16, MODE>(__VA_ARGS__);                    \
         if (T == 8) return FN<8, C16, MODE>(__VA_ARGS__);                    \
         if (T == 16) return FN<16, C16, MODE>(__VA_ARGS__);                  \
         if (T == 32) return FN<32, C16, MODE>(__VA_ARGS__);                  \

@@ -42,17 +42,24 @@ hipError_t launch_gather(const float *xyz, const uint8_t *type, const float *s2,
 // Per-type ion counts of a cloud (counts has 256 entries, zeroed by the caller).
 hipError_t launch_typecount(const uint8_t *type, uint64_t n, uint32_t *counts, hipStream_t s);
 
-// Per job: flag considered centers, compact them, count and write tiles.
+// Per job and target-type slot: flag the slot's centers (considered, or
+// flagged in src), compact them after the centers of the slots before
+// (base[1]), count and write the slot's tiles after base[2] tiles (tiles
+// carry the slot), track the largest stencil candidate count in base[0], and
+// advance base[1] / base[2].
 hipError_t launch_flags(const float4 *pos, const float *s2sorted, uint64_t n,
-                        const int32_t *trgslot, float R2, uint32_t *flags, hipStream_t s);
+                        const int32_t *trgslot, float R2, int slot, const uint32_t *src,
+                        uint32_t *flags, hipStream_t s);
 hipError_t launch_compact(const uint32_t *flags, const uint32_t *cpos, uint64_t n,
-                          uint32_t *centers, hipStream_t s);
+                          const uint32_t *base, uint32_t *centers, hipStream_t s);
 hipError_t launch_tilecount(const uint32_t *cellstart, const uint32_t *cpos, uint64_t ncells,
                             int T, uint32_t *ntile, hipStream_t s);
 hipError_t launch_tilewrite(const uint32_t *cellstart, const uint32_t *cpos,
-                            const uint32_t *toff, uint64_t ncells, int T, int job,
+                            const uint32_t *toff, uint64_t ncells, int T, int slot,
                             GridGeom g, const Row *rows, int nrows, Tile *tiles,
-                            uint32_t *maxcand, hipStream_t s);
+                            uint32_t *base, hipStream_t s);
+hipError_t launch_bump(const uint32_t *cpos, uint64_t n, const uint32_t *toff, uint64_t ncells,
+                       uint32_t *base, hipStream_t s);
 
 // Query launch over all jobs of a launch.
 struct QueryLaunch {
