@@ -537,8 +537,7 @@ __host__ __device__ inline Lay make_layout(int T, int c16, int mode, int maxC, i
     o += al4(maxC);
     L.envoff = o;
     o += 256;
-    L.trgslot = o;
-    o += 256;
+    L.trgslot = o;  // unused: tiles carry their center type
     L.kinfo = o;
     o += mode == 2 ? 512 : 0;
     L.ckoff = o;
@@ -682,7 +681,6 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : 1) void k_query
                               a.maxHK);
     uint32_t *s_edges = lds + L.edges;
     int32_t *s_envoff = (int32_t *)(lds + L.envoff);
-    int32_t *s_trgslot = (int32_t *)(lds + L.trgslot);
     int2 *s_kinfo = (int2 *)(lds + L.kinfo);
     int32_t *s_ckoff = (int32_t *)(lds + L.ckoff);
     int32_t *s_kidx = (int32_t *)(lds + L.kidx);
@@ -752,10 +750,8 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : 1) void k_query
             if (cur < 0 || job != cur / kMaxTrg) {
                 for (int i = tid; i < J.C; i += kBlock)
                     s_edges[i] = J.edges[i];
-                for (int i = tid; i < 256; i += kBlock) {
+                for (int i = tid; i < 256; i += kBlock)
                     s_envoff[i] = J.envoff[i];
-                    s_trgslot[i] = J.trgslot[i];
-                }
                 if (HY) {
                     for (int i = tid; i < 512; i += kBlock)
                         ((int32_t *)s_kinfo)[i] = J.kinfo[i];
@@ -1076,35 +1072,10 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : 1) void k_query
                 if (kt)
                     for (int l = L + lane; l < lmax; l += kWave)
                         kt[c * lmax + l] = 0xFFu;
-                if (HY && J.trunc_any) {
-                    // a task whose kNN range is short of the full range must
-                    // hold its k neighbours there, else the center is rerun
-                    bool short_range = false;
-                    for (int l = 0; l < L && !short_range; ++l) {
-                        const int tk = (int)rfl(J.trg_tasks[tc * lmax + l]);
-                        const uint32_t ckw = rfl(J.task_ck[tk]);
-                        if (!(ckw >> 31))
-                            continue;
-                        const int ck = (int)(ckw & 0x7FFFFFFFu);
-                        const uint32_t envset = rfl(J.task_env[tk]);
-                        uint32_t tot = 0;
-                        for (int cat = lane; cat < ck; cat += kWave)
-                            for (int e = 0; e < E; ++e)
-                                if ((envset >> e) & 1u)
-                                    tot += hv(e, cat);
-                        for (int d = kWave / 2; d > 0; d >>= 1)
-                            tot += __shfl_xor(tot, d, kWave);
-                        short_range = tot < need;
-                    }
-                    if (short_range) {
-                        if (lane == 0)
-                            J.unres[J.centers[t_first + c]] = 1u;
-                        if (kt)
-                            for (int l = lane; l < L; l += kWave)
-                                kt[c * lmax + l] = 0xFFu;
-                        continue;
-                    }
-                }
+                // a task whose kNN range is short of the full range must hold
+                // its k neighbours there, else the center is rerun over the
+                // full range (its rows here are then marked empty)
+                bool short_range = false;
                 for (int l = 0; l < L; ++l) {
                     const int tk = (int)rfl(J.trg_tasks[tc * lmax + l]);
                     const int row = c * lmax + l;
@@ -1139,6 +1110,10 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : 1) void k_query
                             if (carry >= need)
                                 break;
                         }
+                        if (HY && carry < need && (rfl(J.task_ck[tk]) >> 31)) {
+                            short_range = true;
+                            break;
+                        }
                         // fewer than k env neighbours: R + EPSILON, i.e. the
                         // sentinel category (S.cpp:4093-4095, 4035-4036)
                         if (want_1 && carry == 0u && lane == 0)
@@ -1168,6 +1143,13 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : 1) void k_query
                             }
                         }
                     }
+                }
+                if (HY && short_range && lane == 0) {
+                    // the lane that wrote the rows' task ids empties them again
+                    J.unres[J.centers[t_first + c]] = 1u;
+                    if (kt)
+                        for (int l = 0; l < L; ++l)
+                            kt[c * lmax + l] = 0xFFu;
                 }
             }
         }
