@@ -247,7 +247,7 @@ def main():
     ap.add_argument("--batch-jobs", type=int, default=256)
     ap.add_argument("--autotune", type=int, default=1)
     ap.add_argument("--serve-requests", type=int, default=400)
-    ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-slots", type=int, default=1)
     ap.add_argument("--serve-batch", type=int, default=32)
     args = ap.parse_args()
     if args.gpus != 1:

@@ -1348,39 +1348,31 @@ pp_status run_jobs(pp_ctx *c, Slot &sl, int njobs, const pp_cloud *const *clouds
             r.considered = r.h_counts[0];
         }
     }
-    // launch groups: pair-count and hybrid jobs in one launch, full-range
-    // per-center jobs in another, then the full-range rerun of the centers
-    // whose kNN range fell short
-    std::vector<int> gA, gB;
-    std::vector<JobDev> bA, bB;
-    int modeA = 0;
+    // one launch group per kernel mode (pair counts, hybrid, full-range per
+    // center), then the full-range rerun of the centers whose kNN range fell
+    // short
+    std::vector<int> g[3];
+    std::vector<JobDev> b[3];
     for (int i = 0; i < njobs; ++i) {
         JobRun &r = runs[i];
-        if (r.plan.mode == 1) {
-            gB.push_back(i);
-            bB.push_back(r.dev);
-        } else {
-            gA.push_back(i);
-            bA.push_back(r.dev);
-            if (r.plan.mode == 2)
-                modeA = 2;
-            if (r.plan.mode == 2 && r.plan.trunc_any)
-                PP_HIP(c, hipMemsetAsync(r.flags, 0, 4 * r.cloud->n, sl.stream));
-        }
+        g[r.plan.mode].push_back(i);
+        b[r.plan.mode].push_back(r.dev);
+        if (r.plan.mode == 2 && r.plan.trunc_any)
+            PP_HIP(c, hipMemsetAsync(r.flags, 0, 4 * r.cloud->n, sl.stream));
     }
     Keep keep;
-    int TA = T, TB = T;
+    int Tm[3] = {T, T, T};
     PP_HIP(c, hipEventRecord(sl.ev[3], sl.stream));
-    st = launch_group(c, sl, runs, gA, bA, modeA, TA, keep);
-    if (st != PP_OK)
-        return st;
-    st = launch_group(c, sl, runs, gB, bB, 1, TB, keep);
-    if (st != PP_OK)
-        return st;
+    for (int mode : {0, 2, 1}) {
+        st = launch_group(c, sl, runs, g[mode], b[mode], mode, Tm[mode], keep);
+        if (st != PP_OK)
+            return st;
+    }
+    const int TA = Tm[2];
     {
         std::vector<int> hy;
-        for (int i : gA)
-            if (runs[i].plan.mode == 2 && runs[i].plan.trunc_any)
+        for (int i : g[2])
+            if (runs[i].plan.trunc_any)
                 hy.push_back(i);
         if (!hy.empty()) {
             for (int i : hy) {
@@ -1595,7 +1587,36 @@ struct Request {
     std::vector<int64_t> kd;
     pp_hist_out *out = nullptr;
     size_t footprint = 0;
+    double work = 0;  // estimated candidate pair evaluations
+    std::chrono::steady_clock::time_point arrival;
 };
+
+// Candidate pair evaluations a job is expected to make: centers of its
+// target types times the candidates of a stencil at the cloud's mean
+// density (neighbour types only when they are a minority), per-center
+// histogram modes weighted up.
+double job_work(const pp_cloud *cl, const JobPlan &p)
+{
+    if (cl->n == 0)
+        return 0.0;
+    double centers = 0, envn = 0;
+    for (int s = 0; s < p.ntrg; ++s)
+        centers += (double)cl->type_count[p.trg_type[s]];
+    for (int e = 0; e < p.E; ++e)
+        envn += (double)cl->type_count[p.env_type[e]];
+    const double frac = envn < 0.8 * (double)cl->n ? envn / (double)cl->n : 1.0;
+    const double R = p.R;
+    const double cand = cl->density() * frac * 4.18879020478639 * R * R * R * 2.5;
+    return centers * cand * (p.percenter ? 1.5 : 1.0);
+}
+
+// Work that one serving batch may gather (about 20 ms of query kernel), so
+// that small requests share launches and large ones do not hold back the
+// requests behind them.
+constexpr double kBatchWork = 3.0e10;
+// Batches are formed shortest estimated work first among the admitted
+// requests; one that has waited this long goes first, in arrival order.
+constexpr double kMaxWaitMs = 100.0;
 
 } // namespace
 
@@ -1613,6 +1634,45 @@ struct pp_server {
     uint64_t next_ticket = 1, submitted = 0, completed = 0;
     std::vector<std::pair<uint64_t, int32_t>> done;
 
+    // Moves the next batch out of `pending` (called with mu held): requests
+    // that waited kMaxWaitMs or more first (arrival order), then the smallest
+    // estimated work; each must fit the remaining budget, and the batch stops
+    // at max_batch_jobs or about kBatchWork.  Returns false if nothing fits.
+    bool pick(std::vector<Request> &batch, size_t &fp)
+    {
+        const auto now = std::chrono::steady_clock::now();
+        double work = 0;
+        while ((int)batch.size() < opts.max_batch_jobs && !pending.empty()) {
+            int best = -1;
+            bool best_old = false;
+            for (size_t k = 0; k < pending.size(); ++k) {
+                const Request &r = pending[k];
+                if (used + r.footprint > budget)
+                    continue;
+                if (!batch.empty() && work + r.work > kBatchWork)
+                    continue;
+                const bool old =
+                    std::chrono::duration<double, std::milli>(now - r.arrival).count() >=
+                    kMaxWaitMs;
+                if (best < 0 || (old && !best_old) ||
+                    (old == best_old && !old && r.work < pending[best].work)) {
+                    best = (int)k;
+                    best_old = old;
+                }
+                if (old)
+                    break;  // the oldest waiting request: take it now
+            }
+            if (best < 0)
+                break;
+            used += pending[best].footprint;
+            fp += pending[best].footprint;
+            work += pending[best].work;
+            batch.push_back(std::move(pending[best]));
+            pending.erase(pending.begin() + best);
+        }
+        return !batch.empty();
+    }
+
     void worker(int i)
     {
         (void)hipSetDevice(ctx->device);
@@ -1622,21 +1682,9 @@ struct pp_server {
             size_t fp = 0;
             {
                 std::unique_lock<std::mutex> lk(mu);
-                cv_work.wait(lk, [&] {
-                    return (stop && pending.empty()) ||
-                           (!pending.empty() && used + pending.front().footprint <= budget);
-                });
-                if (pending.empty())
+                cv_work.wait(lk, [&] { return (stop && pending.empty()) || pick(batch, fp); });
+                if (batch.empty())
                     return;  // stop requested and nothing left
-                // every admitted request waiting now joins this batch (FIFO,
-                // stopping at the first one the budget cannot hold yet)
-                while (!pending.empty() && (int)batch.size() < opts.max_batch_jobs &&
-                       used + pending.front().footprint <= budget) {
-                    used += pending.front().footprint;
-                    fp += pending.front().footprint;
-                    batch.push_back(std::move(pending.front()));
-                    pending.pop_front();
-                }
             }
             std::vector<const pp_cloud *> cls;
             std::vector<pp_job> jobs;
@@ -1878,6 +1926,8 @@ pp_status pp_serve_submit(pp_server *s, const pp_cloud *cl, const pp_job *job, p
     r.job.kminus1_desc = r.kd.empty() ? nullptr : r.kd.data();
     r.out = out;
     r.footprint = job_footprint(cl, p);
+    r.work = job_work(cl, p);
+    r.arrival = std::chrono::steady_clock::now();
     std::lock_guard<std::mutex> lk(s->mu);
     if (r.footprint > s->budget)
         return fail(s->ctx, PP_ERR_ADMISSION_REJECTED, "request larger than the HBM budget");

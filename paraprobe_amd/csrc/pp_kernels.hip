@@ -533,8 +533,8 @@ __host__ __device__ inline Lay make_layout(int T, int c16, int mode, int maxC, i
 {
     Lay L;
     int o = 0;
-    L.edges = o;
-    o += al4(maxC);
+    L.edges = o;  // the hybrid mode reads the (rarely needed) edges from global memory
+    o += mode == 2 ? 0 : al4(maxC);
     L.envoff = o;
     o += 256;
     L.trgslot = o;  // unused: tiles carry their center type
@@ -668,10 +668,10 @@ __device__ inline int category_of(float d2, float isc, float ioff1, float amb, f
 // MODE 2: pair counts as in MODE 0 plus per-center histograms over the kNN
 // range only (see JobDev::HK); centers whose kNN range holds too few
 // neighbours are flagged for a MODE 1 rerun.
-// Pair-count mode with 8 centers per tile fits 72 VGPRs without spills: ask
-// for 7 resident waves per SIMD.
+// Resident waves per SIMD asked of the compiler for 8-center tiles: 7 in the
+// pair-count mode (72 VGPRs, no spills), 6 in the hybrid mode.
 template <int T, bool C16, int MODE>
-__global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : 1) void k_query(QArgs a)
+__global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : (T == 8 && MODE == 2) ? 6 : 1) void k_query(QArgs a)
 {
     constexpr bool PC = MODE == 1;
     constexpr bool HY = MODE == 2;
@@ -748,8 +748,9 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : 1) void k_query
                 break;
             const JobDev &J = a.jobs[job];
             if (cur < 0 || job != cur / kMaxTrg) {
-                for (int i = tid; i < J.C; i += kBlock)
-                    s_edges[i] = J.edges[i];
+                if (!HY)
+                    for (int i = tid; i < J.C; i += kBlock)
+                        s_edges[i] = J.edges[i];
                 for (int i = tid; i < 256; i += kBlock)
                     s_envoff[i] = J.envoff[i];
                 if (HY) {
@@ -974,6 +975,7 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : 1) void k_query
                     const float kthr = __int_as_float(ki.y);  // -1: no kNN for this type
                     const bool rdf_env = rdf && envok;
                     const float bthr = rdf_env ? R2 : kthr;
+                    const uint32_t *gedges = J.edges;
 #pragma unroll
                     for (int c = 0; c < T; ++c) {
                         if (c < cnt) {  // uniform: skips a partial tile's absent centers
@@ -985,7 +987,7 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : 1) void k_query
                             my_in += (unsigned long long)__builtin_popcountll(
                                 __builtin_amdgcn_ballot_w64(in));
                             if (d2 <= bthr) {
-                                const int cat = category_of(d2, isc, ioff1, amb, Cm1f, s_edges);
+                                const int cat = category_of(d2, isc, ioff1, amb, Cm1f, gedges);
                                 if (rdf_env)
                                     __hip_atomic_fetch_add(&s_P[woff + cat], 1u,
                                                            __ATOMIC_RELAXED,
