@@ -142,7 +142,7 @@ pp_status pp_batch_run(pp_ctx *ctx, int32_t njobs, const pp_cloud *const *clouds
                        const pp_job *jobs, pp_hist_out *outs);
 
 /* Query-kernel tuning: centers per tile (4, 8, 16 or 32) and the cell width as a
- * fraction of Rmax (cells per Rmax: 1..4).  0 = built-in choice (density
+ * fraction of Rmax (cells per Rmax: 1..8).  0 = built-in choice (density
  * rule); -1 for both = autotune: the first job of each (density x Rmax^3,
  * families) bucket times the candidate shapes and the winner is kept for the
  * bucket. */

@@ -626,6 +626,9 @@ struct JobRun {
     // read back after prep
     uint32_t h_counts[4] = {0, 0, 0, 0};  // considered, tiles, maxcand, pad
     uint32_t considered = 0, rerun = 0;   // centers of the job, of its full-range rerun
+    // in-ball pairs counted by the kernel; otherwise every candidate is a
+    // neighbour type of the RDF, whose pair counts sum to the same total
+    bool count_in = true;
     int T_used = 0, c16_used = 0;
     std::vector<uint32_t> h_tables;       // host staging of the device tables
     JobDev dev{};
@@ -796,6 +799,7 @@ void fill_jobdev(JobRun &r)
     d.ckoff = (const int32_t *)(r.tables + r.o_ckoff);
     d.task_ck = r.tables + r.o_taskck;
     d.unres = r.flags;  // reused once the center list is built
+    d.count_in = r.count_in ? 1 : 0;
 }
 
 void compose_outputs(const JobRun &r, const std::vector<unsigned long long> &P,
@@ -811,6 +815,11 @@ void compose_outputs(const JobRun &r, const std::vector<unsigned long long> &P,
     // the kernel's ballot count includes each considered center's self pair
     // (d = 0, always in its own cell); the reference excludes it by index
     o->pairs_inball = (int64_t)stats[0] - (int64_t)r.h_counts[0];
+    if (!r.count_in) {
+        o->pairs_inball = 0;
+        for (unsigned long long v : P)
+            o->pairs_inball += (int64_t)v;
+    }
     o->pairs_candidate = (int64_t)stats[1];
     *overflow = stats[2] != 0;
     if (p.families & (PP_FAM_RDF | PP_FAM_RIPK)) {
@@ -870,12 +879,13 @@ void compose_outputs(const JobRun &r, const std::vector<unsigned long long> &P,
 // kernel shape: centers per tile and cells per Rmax
 
 // Density rule: cells of Rmax / r with r as large as possible (fewer
-// candidates per in-ball pair: 6.3 at r = 1, 3.7 at 2, 2.3 at 4) while a cell
-// still holds ~16 ions, so that stencil rows fill 64-lane chunks.
+// candidates per in-ball pair: 6.3 at r = 1, 3.7 at 2, 2.3 at 4, 1.9 at 5)
+// while a cell still holds ~16 ions, so that stencil rows fill 64-lane
+// chunks; beyond 5 the per-tile and per-row work outweighs the saving.
 int rule_r(double density, double R)
 {
     const int r = (int)std::floor(R * std::cbrt(std::max(density, 1e-9) / 16.0));
-    return std::min(4, std::max(1, r));
+    return std::min(5, std::max(1, r));
 }
 
 long tune_bucket(double density, const JobPlan &p)
@@ -1195,7 +1205,7 @@ pp_status autotune(pp_ctx *c, Slot &sl, const pp_cloud *cl, const pp_job *job, c
     double best = 1e300;
     std::pair<int, int> win(8, rule_r(cl->density(), p.R));
     for (int T : {8, 16})
-        for (int r = 1; r <= 4; ++r) {
+        for (int r = 1; r <= 5; ++r) {
             std::memset(&scratch, 0, sizeof(scratch));
             Shape f;
             f.T = T;
@@ -1302,6 +1312,16 @@ pp_status run_jobs(pp_ctx *c, Slot &sl, int njobs, const pp_cloud *const *clouds
                 if (st != PP_OK)
                     return st;
             }
+        }
+        {
+            // the RDF pair counts give the in-ball total when every ion type
+            // of the cloud is a neighbour type of the job
+            const JobPlan &p = runs[i].plan;
+            bool all_env = runs[i].cand == runs[i].cl;
+            for (int t = 0; t < 256 && all_env; ++t)
+                if (clouds[i]->type_count[t] > 0 && p.envoff[t] < 0)
+                    all_env = false;
+            runs[i].count_in = !(all_env && (p.families & (PP_FAM_RDF | PP_FAM_RIPK)));
         }
         runs[i].rows = make_rows(runs[i].cl->g.w, runs[i].plan.R);
         if ((int)runs[i].rows.size() > kMaxRows)
@@ -1851,8 +1871,8 @@ pp_status pp_set_tuning(pp_ctx *c, int32_t centers_per_tile, int32_t cells_per_r
                     centers_per_tile == 16 || centers_per_tile == 32))
         return fail(c, PP_ERR_INVALID_ARG,
                     "centers_per_tile must be 0, 4, 8, 16 or 32 (-1: autotune)");
-    if (!autot && (cells_per_rmax < 0 || cells_per_rmax > 4))
-        return fail(c, PP_ERR_INVALID_ARG, "cells_per_rmax must be 0..4 (-1: autotune)");
+    if (!autot && (cells_per_rmax < 0 || cells_per_rmax > 8))
+        return fail(c, PP_ERR_INVALID_ARG, "cells_per_rmax must be 0..8 (-1: autotune)");
     std::lock_guard<std::mutex> lk(c->mu);
     c->tune_T = centers_per_tile;
     c->tune_r = cells_per_rmax;

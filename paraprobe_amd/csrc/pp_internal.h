@@ -87,6 +87,8 @@ struct JobDev {
     int cn_cap;
     int percenter;            // 1 when MKNN, 1NN or CNTNB need per-center histograms
     unsigned long long *stats;// [0] in-ball pairs (all types), [1] candidate evaluations
+    int count_in;             // count in-ball pairs in stats[0]; 0 when every candidate is
+                              // a neighbour type and the pair counts give the total
     // tasks of each center type: trg_tasks[slot * lmax + l], l < trg_ntask[slot]
     int lmax;
     const uint32_t *trg_ntask;

@@ -869,6 +869,7 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : (T == 8 && MODE
             const float ioff1 = J.idx_off1, amb = J.amb;
             const float Cm1f = (float)(C - 1);
             const bool rdf = (J.families & (FAM_RDF | FAM_RIPK)) != 0;
+            const bool count_in = J.count_in != 0;
             const gfvec4 *__restrict__ pos = (const gfvec4 *)J.pos;
             uint32_t *s_acc = PC ? s_H : s_P;
             const int nch = (int)rfl(s_rowchunk[nrows]);
@@ -907,6 +908,8 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : (T == 8 && MODE
                 const uint32_t ptype = __float_as_uint(p.w) & 0xFFu;
                 const int woff = ok ? s_envoff[ptype] : -1;
                 const bool envok = woff >= 0;
+                // this candidate's pair-count row (only dereferenced when envok)
+                uint32_t *acc_row = (PC ? s_H : s_P) + woff;
                 if (PC) {
                     // 1) all T squared distances (the reference's FP32 expression)
                     float d2[T];
@@ -934,8 +937,9 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : (T == 8 && MODE
     #pragma unroll
                     for (int c = 0; c < T; ++c) {
                         const bool in = d2[c] <= R2;
-                        my_in += (unsigned long long)__builtin_popcountll(
-                            __builtin_amdgcn_ballot_w64(in));
+                        if (count_in)
+                            my_in += (unsigned long long)__builtin_popcountll(
+                                __builtin_amdgcn_ballot_w64(in));
                         if (in && envok) {
                             const int cat = qi[c] - (__float_as_uint(d2[c]) < ed[c] ? 1 : 0);
                             const uint32_t inc = (C16 && (c & 1)) ? 65536u : 1u;
@@ -956,11 +960,12 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : (T == 8 && MODE
                             const float dz = czr[c] - p.z;
                             const float d2 = dx * dx + dy * dy + dz * dz;
                             const bool in = d2 <= R2;
-                            my_in += (unsigned long long)__builtin_popcountll(
-                                __builtin_amdgcn_ballot_w64(in));
+                            if (count_in)
+                                my_in += (unsigned long long)__builtin_popcountll(
+                                    __builtin_amdgcn_ballot_w64(in));
                             if (in && envok) {
                                 const int cat = category_of(d2, isc, ioff1, amb, Cm1f, s_edges);
-                                __hip_atomic_fetch_add(&s_acc[woff + cat], 1u,
+                                __hip_atomic_fetch_add(acc_row + cat, 1u,
                                                        __ATOMIC_RELAXED,
                                                        __HIP_MEMORY_SCOPE_WORKGROUP);
                             }
@@ -973,6 +978,7 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : (T == 8 && MODE
                     const int2 ki = ok ? s_kinfo[ptype] : make_int2(-1, (int)0xBF800000u);
                     const int kof = ki.x;
                     const float kthr = __int_as_float(ki.y);  // -1: no kNN for this type
+                    uint32_t *krow = s_H + kof;  // only dereferenced when d2 <= kthr
                     const bool rdf_env = rdf && envok;
                     const float bthr = rdf_env ? R2 : kthr;
                     const uint32_t *gedges = J.edges;
@@ -983,18 +989,18 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : (T == 8 && MODE
                             const float dy = cyr[c] - p.y;
                             const float dz = czr[c] - p.z;
                             const float d2 = dx * dx + dy * dy + dz * dz;
-                            const bool in = d2 <= R2;
-                            my_in += (unsigned long long)__builtin_popcountll(
-                                __builtin_amdgcn_ballot_w64(in));
+                            if (count_in)
+                                my_in += (unsigned long long)__builtin_popcountll(
+                                    __builtin_amdgcn_ballot_w64(d2 <= R2));
                             if (d2 <= bthr) {
                                 const int cat = category_of(d2, isc, ioff1, amb, Cm1f, gedges);
                                 if (rdf_env)
-                                    __hip_atomic_fetch_add(&s_P[woff + cat], 1u,
+                                    __hip_atomic_fetch_add(acc_row + cat, 1u,
                                                            __ATOMIC_RELAXED,
                                                            __HIP_MEMORY_SCOPE_WORKGROUP);
                                 if (d2 <= kthr)
                                     __hip_atomic_fetch_add(
-                                        &s_H[(C16 ? (c >> 1) : c) * HK + kof + cat],
+                                        krow + (C16 ? (c >> 1) : c) * HK + cat,
                                         (C16 && (c & 1)) ? 65536u : 1u, __ATOMIC_RELAXED,
                                         __HIP_MEMORY_SCOPE_WORKGROUP);
                             }
