@@ -880,12 +880,26 @@ void compose_outputs(const JobRun &r, const std::vector<unsigned long long> &P,
 
 // Density rule: cells of Rmax / r with r as large as possible (fewer
 // candidates per in-ball pair: 6.3 at r = 1, 3.7 at 2, 2.3 at 4, 1.9 at 5)
-// while a cell still holds ~16 ions, so that stencil rows fill 64-lane
-// chunks; beyond 5 the per-tile and per-row work outweighs the saving.
+// while a cell still holds ~16 candidates, so that stencil rows fill 64-lane
+// chunks; capped at 4, beyond which the per-tile and per-row work mostly
+// cancels the saving (the autotuner also tries 5).
+// Ions per nm^3 among a job's candidates: the neighbour types alone when they
+// are a minority (the typed candidate list), else every ion.
+double cand_density(const pp_cloud *cl, const JobPlan &p)
+{
+    if (cl->n == 0)
+        return 0.0;
+    double envn = 0;
+    for (int e = 0; e < p.E; ++e)
+        envn += (double)cl->type_count[p.env_type[e]];
+    const double frac = envn / (double)cl->n;
+    return cl->density() * (frac < 0.8 ? frac : 1.0);
+}
+
 int rule_r(double density, double R)
 {
     const int r = (int)std::floor(R * std::cbrt(std::max(density, 1e-9) / 16.0));
-    return std::min(5, std::max(1, r));
+    return std::min(4, std::max(1, r));
 }
 
 long tune_bucket(double density, const JobPlan &p)
@@ -1203,7 +1217,7 @@ pp_status autotune(pp_ctx *c, Slot &sl, const pp_cloud *cl, const pp_job *job, c
 {
     pp_hist_out scratch;
     double best = 1e300;
-    std::pair<int, int> win(8, rule_r(cl->density(), p.R));
+    std::pair<int, int> win(8, rule_r(cand_density(cl, p), p.R));
     for (int T : {8, 16})
         for (int r = 1; r <= 5; ++r) {
             std::memset(&scratch, 0, sizeof(scratch));
@@ -1248,14 +1262,14 @@ pp_status run_jobs(pp_ctx *c, Slot &sl, int njobs, const pp_cloud *const *clouds
         runs[i].cloud = clouds[i];
         JobPlan &p = runs[i].plan;
         int jT = 8;
-        p.r = rule_r(clouds[i]->density(), p.R);
+        p.r = rule_r(cand_density(clouds[i], p), p.R);
         if (force.T > 0 || force.r > 0) {
             if (force.T > 0)
                 jT = force.T;
             if (force.r > 0)
                 p.r = force.r;
         } else if (tuneT < 0) {
-            const long b = tune_bucket(clouds[i]->density(), p);
+            const long b = tune_bucket(cand_density(clouds[i], p), p);
             bool have;
             {
                 std::lock_guard<std::mutex> lk(c->mu);
@@ -1933,7 +1947,7 @@ pp_status pp_serve_submit(pp_server *s, const pp_cloud *cl, const pp_job *job, p
     pp_status st = plan_job(s->ctx, job, p);
     if (st != PP_OK)
         return st;
-    p.r = rule_r(cl->density(), p.R);
+    p.r = rule_r(cand_density(cl, p), p.R);
     r.cloud = cl;
     r.job = *job;
     const int nt = std::max(job->ntasks, 0);
