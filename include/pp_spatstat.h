@@ -148,6 +148,19 @@ pp_status pp_batch_run(pp_ctx *ctx, int32_t njobs, const pp_cloud *const *clouds
  * bucket. */
 pp_status pp_set_tuning(pp_ctx *ctx, int32_t centers_per_tile, int32_t cells_per_rmax);
 
+/* Squared distance of every ion to a triangulated tip surface, the ion2surf
+ * input of pp_cloud_upload: replaces solver::surface_distancing2
+ * (src/PARAPROBE_SolverHdl.cpp:312-484).  Each ion gets the smallest
+ * closestPointOnTriangle value (src/PARAPROBE_Math.cpp:88-171) over the
+ * triangles whose vertex box overlaps its box [p - R, p + R], or fl32(R * R)
+ * when that is smaller or there is none; pass R = max(SpatStatRadiusMax,
+ * SurfaceCellsCarvingRadius).  xyz: n x 3 floats, triangles: ntri x 9 floats
+ * (three vertices), out: n floats; all host memory.  Zero-area triangles give
+ * NaN distances, which are skipped (the reference's result for them depends
+ * on its R-tree's candidate order). */
+pp_status pp_ion2surf(pp_ctx *ctx, const float *xyz, uint64_t n, const float *triangles,
+                      uint64_t ntri, float R, float *ion2surf_sqr);
+
 /* kNN range of MKNN / 1NN jobs without CNTNB: per neighbour type, per-center
  * histograms cover only the distance within which `factor` x k neighbours
  * are expected at the cloud's mean density (default 3); centers with fewer

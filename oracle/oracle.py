@@ -56,6 +56,9 @@ def _load():
             C.c_int64, P, P, P, P, P, C.c_float, C.c_float, C.c_float, C.c_int64, C.c_int32, P,
         ]
         lib.pp_oracle_binedge_report.restype = C.c_int
+        lib.pp_oracle_closest_tri.argtypes = [P, P]
+        lib.pp_oracle_closest_tri.restype = C.c_float
+        lib.pp_oracle_ion2surf.argtypes = [P, C.c_int64, P, C.c_int64, C.c_float, P]
         _lib = lib
     return _lib
 
@@ -146,3 +149,24 @@ def binedge_report(xyz, types, ion2surf, trg, env, rmin, rincr, rmax, center_str
         raise ValueError("bad arguments")
     return {"pairs": int(out[0]), "bin_differs_from_fp64": int(out[1]),
             "within_one_ulp_of_edge": int(out[2]), "ball_membership_flips": int(out[3])}
+
+
+def closest_tri(tri, p) -> float:
+    """FP32 squared distance from p to one triangle (9 floats), the reference's
+    region logic and operation order."""
+    lib = _load()
+    t = np.ascontiguousarray(tri, dtype=np.float32).reshape(9)
+    q = np.ascontiguousarray(p, dtype=np.float32).reshape(3)
+    return float(lib.pp_oracle_closest_tri(t.ctypes.data, q.ctypes.data))
+
+
+def ion2surf(xyz, triangles, R) -> np.ndarray:
+    """Squared ion-to-hull distance capped at fl32(R*R), by brute force over
+    the triangles in index order."""
+    lib = _load()
+    xyz = np.ascontiguousarray(xyz, dtype=np.float32).reshape(-1, 3)
+    tri = np.ascontiguousarray(triangles, dtype=np.float32).reshape(-1, 9)
+    out = np.empty(xyz.shape[0], np.float32)
+    lib.pp_oracle_ion2surf(xyz.ctypes.data, xyz.shape[0], tri.ctypes.data, tri.shape[0],
+                           C.c_float(R), out.ctypes.data)
+    return out

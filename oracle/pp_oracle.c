@@ -540,3 +540,122 @@ int pp_oracle_binedge_report(int64_t n, const float *xyz, const uint8_t *type, c
     out[3] = ball_flip;
     return 0;
 }
+
+/* ------------------------------------------------------------------------
+ * Ion-to-surface squared distance (the ion2surf producer).
+ *
+ * pp_oracle_closest_tri restates mathHdl::closestPointOnTriangle
+ * (src/PARAPROBE_Math.cpp:88-171): the FP32 squared distance from p to the
+ * triangle t (9 floats, vertices 1..3), with the reference's region logic,
+ * its CLAMP/MIN/MAX ternaries (src/PARAPROBE_Abinitio.h:47-49, a NaN clamps
+ * to hi) and its operation order (1.f/det then two multiplies; a - 2*b + c).
+ */
+static float orc_clamp01(float x)
+{
+    const float m = (x < 0.f) ? 0.f : x;
+    return (m < 1.f) ? m : 1.f;
+}
+
+float pp_oracle_closest_tri(const float *t, const float *p)
+{
+    const float e0u = t[3] - t[0], e0v = t[4] - t[1], e0w = t[5] - t[2];
+    const float e1u = t[6] - t[0], e1v = t[7] - t[1], e1w = t[8] - t[2];
+    const float v0u = t[0] - p[0], v0v = t[1] - p[1], v0w = t[2] - p[2];
+    const float a = e0u * e0u + e0v * e0v + e0w * e0w;
+    const float b = e0u * e1u + e0v * e1v + e0w * e1w;
+    const float c = e1u * e1u + e1v * e1v + e1w * e1w;
+    const float d = e0u * v0u + e0v * v0v + e0w * v0w;
+    const float e = e1u * v0u + e1v * v0v + e1w * v0w;
+    const float det = a * c - b * b;
+    float s = b * e - c * d;
+    float u = b * d - a * e;
+    if (s + u < det) {
+        if (s < 0.f) {
+            if (u < 0.f) {
+                if (d < 0.f) {
+                    s = orc_clamp01(-d / a);
+                    u = 0.f;
+                } else {
+                    s = 0.f;
+                    u = orc_clamp01(-e / c);
+                }
+            } else {
+                s = 0.f;
+                u = orc_clamp01(-e / c);
+            }
+        } else if (u < 0.f) {
+            s = orc_clamp01(-d / a);
+            u = 0.f;
+        } else {
+            const float inv = 1.f / det;
+            s *= inv;
+            u *= inv;
+        }
+    } else if (s < 0.f) {
+        const float t0 = b + d, t1 = c + e;
+        if (t1 > t0) {
+            s = orc_clamp01((t1 - t0) / (a - 2.f * b + c));
+            u = 1.f - s;
+        } else {
+            u = orc_clamp01(-e / c);
+            s = 0.f;
+        }
+    } else if (u < 0.f) {
+        if (a + d > b + e) {
+            s = orc_clamp01((c + e - b - d) / (a - 2.f * b + c));
+            u = 1.f - s;
+        } else {
+            s = orc_clamp01(-e / c);
+            u = 0.f;
+        }
+    } else {
+        s = orc_clamp01((c + e - b - d) / (a - 2.f * b + c));
+        u = 1.f - s;
+    }
+    const float qx = t[0] + s * e0u + u * e1u;
+    const float qy = t[1] + s * e0v + u * e1v;
+    const float qz = t[2] + s * e0w + u * e1w;
+    const float dx = qx - p[0], dy = qy - p[1], dz = qz - p[2];
+    return dx * dx + dy * dy + dz * dz;
+}
+
+/*
+ * pp_oracle_ion2surf restates solver::surface_distancing2
+ * (src/PARAPROBE_SolverHdl.cpp:312-484) by brute force over the triangles in
+ * index order: start at fl32(R*R); a triangle is a candidate iff its vertex
+ * box (fminf/fmaxf, SolverHdl.cpp:2716-2721) overlaps the float box
+ * [p - R, p + R] with closed bounds (src/PARAPROBE_AABBTree.cpp:112-123);
+ * a candidate's value replaces the running best unless it is larger
+ * (SolverHdl.cpp:425-433).
+ */
+void pp_oracle_ion2surf(const float *xyz, int64_t n, const float *tri, int64_t ntri, float R,
+                        float *out)
+{
+    const float RSQR = R * R;
+    float *box = (float *)malloc(sizeof(float) * 6 * (size_t)(ntri > 0 ? ntri : 1));
+    for (int64_t t = 0; t < ntri; ++t)
+        for (int k = 0; k < 3; ++k) {
+            const float *v = tri + 9 * t;
+            box[6 * t + k] = fminf(fminf(v[k], v[3 + k]), v[6 + k]);
+            box[6 * t + 3 + k] = fmaxf(fmaxf(v[k], v[3 + k]), v[6 + k]);
+        }
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t i = 0; i < n; ++i) {
+        const float *p = xyz + 3 * i;
+        const float lo[3] = {p[0] - R, p[1] - R, p[2] - R};
+        const float hi[3] = {p[0] + R, p[1] + R, p[2] + R};
+        float best = RSQR;
+        for (int64_t t = 0; t < ntri; ++t) {
+            const float *b = box + 6 * t;
+            if (hi[0] < b[0] || lo[0] > b[3] || hi[1] < b[1] || lo[1] > b[4] || hi[2] < b[2] ||
+                lo[2] > b[5])
+                continue;
+            const float v = pp_oracle_closest_tri(tri + 9 * t, p);
+            if (v > best)
+                continue;
+            best = v;
+        }
+        out[i] = best;
+    }
+    free(box);
+}

@@ -55,6 +55,7 @@ EXPORTED = [
     "pp_serve_poll",
     "pp_serve_drain",
     "pp_serve_destroy",
+    "pp_ion2surf",
 ]
 
 
@@ -142,6 +143,7 @@ def load_library() -> C.CDLL:
     lib.pp_serve_poll.argtypes = [P, P, P, C.c_int32, C.POINTER(C.c_int32)]
     lib.pp_serve_drain.argtypes = [P]
     lib.pp_serve_destroy.argtypes = [P]
+    lib.pp_ion2surf.argtypes = [P, P, C.c_uint64, P, C.c_uint64, C.c_float, P]
     for name in EXPORTED:
         if name != "pp_last_error_string":
             getattr(lib, name).restype = C.c_int
@@ -333,6 +335,17 @@ class SpatStat:
         """kNN range factor (expected neighbours per k) of MKNN / 1NN jobs; 0
         keeps full-range per-center histograms."""
         self._check(self.lib.pp_set_knn_range(self.ctx, factor))
+
+    def ion2surf(self, xyz: np.ndarray, triangles: np.ndarray, R: float) -> np.ndarray:
+        """Squared distance of every ion to the triangle hull, capped at
+        fl32(R * R) (solver::surface_distancing2); R is max(SpatStatRadiusMax,
+        SurfaceCellsCarvingRadius) in the reference."""
+        xyz = np.ascontiguousarray(xyz, dtype=np.float32).reshape(-1, 3)
+        tri = np.ascontiguousarray(triangles, dtype=np.float32).reshape(-1, 9)
+        out = np.empty(xyz.shape[0], np.float32)
+        self._check(self.lib.pp_ion2surf(self.ctx, xyz.ctypes.data, xyz.shape[0], tri.ctypes.data,
+                                         tri.shape[0], C.c_float(R), out.ctypes.data))
+        return out
 
     def upload(self, xyz: np.ndarray, types: np.ndarray, ion2surf_sqr: np.ndarray,
                cell_width: float = 0.0) -> Cloud:

@@ -1893,6 +1893,111 @@ pp_status pp_set_tuning(pp_ctx *c, int32_t centers_per_tile, int32_t cells_per_r
     return PP_OK;
 }
 
+pp_status pp_ion2surf(pp_ctx *c, const float *xyz, uint64_t n, const float *tri, uint64_t ntri,
+                      float R, float *out)
+{
+    if (!c)
+        return PP_ERR_INVALID_ARG;
+    if (!(R > 0.f) || !std::isfinite(R))
+        return fail(c, PP_ERR_INVALID_ARG, "R must be positive and finite");
+    if ((n > 0 && (!xyz || !out)) || (ntri > 0 && !tri))
+        return fail(c, PP_ERR_INVALID_ARG, "NULL input or output arrays");
+    if (n >= 0xFFFFFFF0ull || ntri >= 0xFFFFFFF0ull)
+        return fail(c, PP_ERR_INVALID_ARG, "more than 2^32 - 16 ions or triangles");
+    if (n == 0)
+        return PP_OK;
+    if (ntri == 0) {
+        // no hull: every ion keeps fl32(R * R)
+        const float r2 = R * R;
+        std::fill(out, out + n, r2);
+        return PP_OK;
+    }
+    PP_HIP(c, hipSetDevice(c->device));
+    Slot &sl = c->main;
+    hipStream_t s = sl.stream;
+    // device buffers of this call
+    Carver m0(nullptr);
+    m0.take<float>(3 * n);
+    m0.take<float>(n);
+    m0.take<float>(9 * ntri + 1);
+    m0.take<float>(6 * ntri + 1);
+    m0.take<uint32_t>(8);
+    void *mem = nullptr;
+    PP_HIP(c, hipMalloc(&mem, m0.off + 256));
+    std::unique_ptr<void, void (*)(void *)> own(mem, [](void *p) { (void)hipFree(p); });
+    Carver m(mem);
+    float *d_xyz = m.take<float>(3 * n);
+    float *d_out = m.take<float>(n);
+    float *d_tri = m.take<float>(9 * ntri + 1);
+    float *d_box = m.take<float>(6 * ntri + 1);
+    uint32_t *d_bb = m.take<uint32_t>(8);
+    PP_HIP(c, hipMemcpyAsync(d_xyz, xyz, 12 * n, hipMemcpyHostToDevice, s));
+    PP_HIP(c, hipMemcpyAsync(d_tri, tri, 36 * ntri, hipMemcpyHostToDevice, s));
+    // grid over the ions and the triangle vertices, cells of at least R
+    uint32_t hb[12];
+    PP_HIP(c, launch_bbox(d_xyz, n, d_bb, s));
+    PP_HIP(c, hipMemcpyAsync(hb, d_bb, 24, hipMemcpyDeviceToHost, s));
+    PP_HIP(c, hipStreamSynchronize(s));
+    PP_HIP(c, launch_bbox(d_tri, 3 * ntri, d_bb, s));
+    PP_HIP(c, hipMemcpyAsync(hb + 6, d_bb, 24, hipMemcpyDeviceToHost, s));
+    PP_HIP(c, hipStreamSynchronize(s));
+    double lo[3], hi[3];
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = std::min(ord_to_float(hb[a]), ord_to_float(hb[6 + a]));
+        hi[a] = std::max(ord_to_float(hb[3 + a]), ord_to_float(hb[9 + a]));
+        if (!std::isfinite(lo[a]) || !std::isfinite(hi[a]))
+            return fail(c, PP_ERR_INVALID_ARG, "non-finite coordinates");
+    }
+    double w = R;
+    {
+        double vol = 1.0;
+        for (int a = 0; a < 3; ++a)
+            vol *= std::max(hi[a] - lo[a], 1e-3);
+        w = std::max(w, std::cbrt(vol / 4.0e6));
+    }
+    GridGeom g{};
+    for (;;) {
+        g.nx = (int)std::floor((hi[0] - lo[0]) / w) + 1;
+        g.ny = (int)std::floor((hi[1] - lo[1]) / w) + 1;
+        g.nz = (int)std::floor((hi[2] - lo[2]) / w) + 1;
+        if (g.ncells() <= (1ull << 24))
+            break;
+        w *= 1.25;
+    }
+    g.w = w;
+    g.inv_w = 1.0 / w;
+    for (int a = 0; a < 3; ++a)
+        g.lo[a] = lo[a];
+    const uint64_t nc = g.ncells();
+    Carver t0(nullptr);
+    t0.take<uint32_t>(nc + 1);
+    t0.take<uint32_t>(nc + 1);
+    t0.take<uint32_t>(scan_workspace_words(nc));
+    void *tm = nullptr;
+    PP_HIP(c, hipMalloc(&tm, t0.off + 256));
+    std::unique_ptr<void, void (*)(void *)> own_t(tm, [](void *p) { (void)hipFree(p); });
+    Carver t(tm);
+    uint32_t *count = t.take<uint32_t>(nc + 1);
+    uint32_t *cellstart = t.take<uint32_t>(nc + 1);
+    uint32_t *ws = t.take<uint32_t>(scan_workspace_words(nc));
+    PP_HIP(c, hipMemsetAsync(count, 0, 4 * (nc + 1), s));
+    PP_HIP(c, launch_tribox(d_tri, ntri, R, g, d_box, count, s));
+    PP_HIP(c, scan_exclusive(count, cellstart, nc, ws, s));
+    uint32_t entries = 0;
+    PP_HIP(c, hipMemcpyAsync(&entries, cellstart + nc, 4, hipMemcpyDeviceToHost, s));
+    PP_HIP(c, hipStreamSynchronize(s));
+    void *lm = nullptr;
+    PP_HIP(c, hipMalloc(&lm, 4 * (size_t)std::max<uint32_t>(entries, 1) + 256));
+    std::unique_ptr<void, void (*)(void *)> own_l(lm, [](void *p) { (void)hipFree(p); });
+    uint32_t *list = (uint32_t *)lm;
+    PP_HIP(c, hipMemcpyAsync(count, cellstart, 4 * (nc + 1), hipMemcpyDeviceToDevice, s));
+    PP_HIP(c, launch_trifill(d_box, ntri, R, g, count, list, s));
+    PP_HIP(c, launch_ion2surf(d_xyz, n, d_tri, d_box, g, cellstart, list, R, d_out, s));
+    PP_HIP(c, hipMemcpyAsync(out, d_out, 4 * n, hipMemcpyDeviceToHost, s));
+    PP_HIP(c, hipStreamSynchronize(s));
+    return PP_OK;
+}
+
 pp_status pp_set_knn_range(pp_ctx *c, float factor)
 {
     if (!c)

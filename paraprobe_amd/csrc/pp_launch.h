@@ -91,4 +91,15 @@ hipError_t launch_kreduce(const JobDev *jobs, int njobs, const uint32_t *unit_of
                           hipStream_t s);
 constexpr int kReduceLdsWords = 16384;  // ntasks * C above this: global atomics
 
+// Ion-to-surface squared distances (pp_surfdist.hip): triangle boxes and
+// per-cell triangle counts on grid g, the cell lists (cursor starts as a copy
+// of the cell offsets), and one thread per ion.
+hipError_t launch_tribox(const float *tri, uint64_t ntri, float R, GridGeom g, float *box,
+                         uint32_t *count, hipStream_t s);
+hipError_t launch_trifill(const float *box, uint64_t ntri, float R, GridGeom g, uint32_t *cursor,
+                          uint32_t *list, hipStream_t s);
+hipError_t launch_ion2surf(const float *xyz, uint64_t n, const float *tri, const float *box,
+                           GridGeom g, const uint32_t *cellstart, const uint32_t *list, float R,
+                           float *out, hipStream_t s);
+
 } // namespace pp

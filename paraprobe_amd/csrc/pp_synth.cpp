@@ -217,6 +217,63 @@ int pp_synth_tip(uint64_t n, uint64_t seed, int32_t shape, double r0, double r1,
     return 0;
 }
 
+// Triangulated hull of the cylinder / frustum solid of pp_synth_tip (the
+// input surface_distancing2 needs; the reference builds it with CGAL alpha
+// shapes, src/PARAPROBE_SolverHdl.cpp:2477-2690): the lateral surface as nseg
+// x nz quads split into two triangles, and both caps as fans around their
+// centre.  Vertices lie on the exact surface.  Writes 9 floats per triangle;
+// returns the triangle count (2 nseg nz + 2 nseg), or -1 for bad parameters.
+int64_t pp_synth_hull(int32_t shape, double r0, double r1, double h, int32_t nseg, int32_t nz,
+                      float *tri, int64_t cap)
+{
+    if (nseg < 3 || nz < 1 || h <= 0.0 || r0 <= 0.0 || (shape == 1 && r1 <= 0.0))
+        return -1;
+    const int64_t ntri = 2 * (int64_t)nseg * nz + 2 * (int64_t)nseg;
+    if (!tri)
+        return ntri;
+    if (cap < ntri)
+        return -1;
+    Solid sol{shape, r0, shape == 0 ? r0 : r1, h};
+    const double two_pi = 6.283185307179586476925;
+    auto vert = [&](int i, int k, float *o) {
+        const double z = h * (double)k / (double)nz;
+        const double r = sol.radius_at(z);
+        const double a = two_pi * (double)(i % nseg) / (double)nseg;
+        o[0] = (float)(r * std::cos(a));
+        o[1] = (float)(r * std::sin(a));
+        o[2] = (float)z;
+    };
+    int64_t w = 0;
+    for (int k = 0; k < nz; ++k)
+        for (int i = 0; i < nseg; ++i) {
+            float a[3], b[3], c[3], d[3];
+            vert(i, k, a);
+            vert(i + 1, k, b);
+            vert(i + 1, k + 1, c);
+            vert(i, k + 1, d);
+            const float *t0[3] = {a, b, c}, *t1[3] = {a, c, d};
+            for (const float *const *t : {t0, t1}) {
+                for (int v = 0; v < 3; ++v)
+                    for (int q = 0; q < 3; ++q)
+                        tri[9 * w + 3 * v + q] = t[v][q];
+                ++w;
+            }
+        }
+    for (int cap_k : {0, nz})
+        for (int i = 0; i < nseg; ++i) {
+            float a[3], b[3];
+            vert(i, cap_k, a);
+            vert(i + 1, cap_k, b);
+            const float ctr[3] = {0.f, 0.f, (float)(h * (double)cap_k / (double)nz)};
+            const float *t[3] = {ctr, a, b};
+            for (int v = 0; v < 3; ++v)
+                for (int q = 0; q < 3; ++q)
+                    tri[9 * w + 3 * v + q] = t[v][q];
+            ++w;
+        }
+    return w;
+}
+
 // Simple cubic / fcc lattice block for closed-form tests: ions at
 // a*(i + basis) for i in [0,nx)x[0,ny)x[0,nz); fcc uses the 4-atom basis.
 // Returns the number of points written (at most cap).

@@ -34,6 +34,9 @@ def _load():
         lib.pp_synth_lattice.argtypes = [C.c_int32, C.c_double, C.c_int32, C.c_int32, C.c_int32,
                                          C.c_void_p, C.c_int64]
         lib.pp_synth_lattice.restype = C.c_int64
+        lib.pp_synth_hull.argtypes = [C.c_int32, C.c_double, C.c_double, C.c_double, C.c_int32,
+                                      C.c_int32, C.c_void_p, C.c_int64]
+        lib.pp_synth_hull.restype = C.c_int64
         _lib = lib
     return _lib
 
@@ -74,6 +77,21 @@ def lattice(fcc: bool, a: float, nx: int, ny: int, nz: int) -> np.ndarray:
     out = np.empty((cap, 3), dtype=np.float32)
     w = lib.pp_synth_lattice(1 if fcc else 0, a, nx, ny, nz, out.ctypes.data, cap)
     return out[:w]
+
+
+def hull(shape: str, r0: float, r1: float, h: float, nseg: int, nz: int) -> np.ndarray:
+    """Closed triangle hull of a cylinder / frustum (z from 0 to h, radius r0
+    at z = 0, r1 at z = h): nseg x nz lateral quads as two triangles each, both
+    caps as fans.  Returns (ntri, 9) float32 vertex coordinates."""
+    lib = _load()
+    s = 0 if shape == "cylinder" else 1
+    nt = lib.pp_synth_hull(s, r0, r1, h, nseg, nz, None, 0)
+    if nt < 0:
+        raise ValueError("bad hull parameters")
+    out = np.empty((nt, 9), dtype=np.float32)
+    if lib.pp_synth_hull(s, r0, r1, h, nseg, nz, out.ctypes.data, nt) != nt:
+        raise ValueError("bad hull parameters")
+    return out
 
 
 # ---------------------------------------------------------------------------
