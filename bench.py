@@ -231,6 +231,71 @@ def serve_leg(ctx, args):
     }
 
 
+def surf_leg(ctx, tip, hip, args):
+    """ion2surf, the producer of the per-ion surface distances: the 20M ions
+    against a 512 x 256 triangulated frustum hull (263,168 triangles), R =
+    max(SpatStatRadiusMax 5, SurfaceCellsCarvingRadius 2) nm, inputs and output
+    resident in HBM; the reference's structure (triangle-box tree query per
+    ion) timed on the host cores on every 4th ion."""
+    from paraprobe_amd import synth
+
+    tri = synth.hull("frustum", 40.0, 20.0, 250.0, 512, 256)
+    R = 5.0
+    n, nt = tip.n, tri.shape[0]
+    ptr = [ctypes.c_void_p() for _ in range(3)]
+    sizes = [12 * n, 36 * nt, 4 * n]
+    for p_, b in zip(ptr, sizes):
+        if hip.hipMalloc(ctypes.byref(p_), ctypes.c_size_t(b)) != 0:
+            raise SystemExit("bench: hipMalloc failed")
+    try:
+        hip.hipMemcpy(ptr[0], tip.xyz.ctypes.data_as(ctypes.c_void_p), ctypes.c_size_t(sizes[0]), 1)
+        hip.hipMemcpy(ptr[1], tri.ctypes.data_as(ctypes.c_void_p), ctypes.c_size_t(sizes[1]), 1)
+        ctx.ion2surf_device(ptr[0].value, n, ptr[1].value, nt, R, ptr[2].value)
+        hip.hipDeviceSynchronize()
+        reps = max(1, args.steps)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            ctx.ion2surf_device(ptr[0].value, n, ptr[1].value, nt, R, ptr[2].value)
+        hip.hipDeviceSynchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / reps
+        got = np.empty(n, np.float32)
+        hip.hipMemcpy(got.ctypes.data_as(ctypes.c_void_p), ptr[2], ctypes.c_size_t(sizes[2]), 2)
+    finally:
+        for p_ in ptr:
+            hip.hipFree(p_)
+    t0 = time.perf_counter()
+    host = ctx.ion2surf(tip.xyz, tri, R)
+    host_ms = (time.perf_counter() - t0) * 1e3
+    out = {
+        "workload": "frustum-20M ions vs 512x256 frustum hull (263,168 triangles), R 5 nm",
+        "ions_per_s": n / (ms / 1e3),
+        "ms": ms,
+        "ms_host_buffers": host_ms,
+        "ions_within_R": int((got < np.float32(R * R)).sum()),
+        "parity_device_vs_host_entry": bool(np.array_equal(got.view(np.uint32),
+                                                           host.view(np.uint32))),
+    }
+    print(f"# ion2surf: {ms:.1f} ms resident, {host_ms:.1f} ms with host buffers",
+          file=sys.stderr, flush=True)
+    if not args.no_cpu:
+        from oracle import cpu_baseline as cb
+
+        model, cores = host_cpu()
+        stride = 4
+        v, build_s, query_s = cb.ion2surf(tip.xyz, tri, R, nthreads=cores, stride=stride)
+        out["cpu_baseline"] = {
+            "value": v.size / query_s,
+            "unit": "ions/s",
+            "cores": cores,
+            "kind": "port",
+            "sample": f"every {stride}th ion ({v.size} ions), triangle-box tree built in "
+                      f"{build_s:.2f} s (not timed), query {query_s:.1f} s; {model}",
+            "parity_with_gpu": bool(np.array_equal(v.view(np.uint32),
+                                                   got[::stride].view(np.uint32))),
+        }
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -244,6 +309,7 @@ def main():
                     help="also run a 1 GiB device copy (HBM counter calibration)")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-surf", action="store_true")
     ap.add_argument("--batch-jobs", type=int, default=256)
     ap.add_argument("--autotune", type=int, default=1)
     ap.add_argument("--serve-requests", type=int, default=400)
@@ -302,6 +368,8 @@ def main():
         cw.free()
         cloud.free()
         extra = {}
+        if not args.no_surf:
+            extra["ion2surf"] = surf_leg(ctx, tip, hip, args)
         if not args.no_batch:
             extra["batch"] = batch_leg(ctx, args)
         if not args.no_serve:

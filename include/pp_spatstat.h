@@ -155,11 +155,18 @@ pp_status pp_set_tuning(pp_ctx *ctx, int32_t centers_per_tile, int32_t cells_per
  * triangles whose vertex box overlaps its box [p - R, p + R], or fl32(R * R)
  * when that is smaller or there is none; pass R = max(SpatStatRadiusMax,
  * SurfaceCellsCarvingRadius).  xyz: n x 3 floats, triangles: ntri x 9 floats
- * (three vertices), out: n floats; all host memory.  Zero-area triangles give
- * NaN distances, which are skipped (the reference's result for them depends
- * on its R-tree's candidate order). */
+ * (three vertices), out: n floats; all host memory.  The expression is the
+ * reference's, quirks included (its region-6 far branch clamps -e/c where
+ * the geometry asks for -d/a).  A NaN distance (only from arithmetic that
+ * overflows) is skipped; the reference's result then depends on its R-tree's
+ * candidate order. */
 pp_status pp_ion2surf(pp_ctx *ctx, const float *xyz, uint64_t n, const float *triangles,
                       uint64_t ntri, float R, float *ion2surf_sqr);
+
+/* pp_ion2surf on device memory (the same device as ctx): d_xyz, d_triangles
+ * and d_out stay on the GPU, so the result can feed pp_cloud_from_device. */
+pp_status pp_ion2surf_device(pp_ctx *ctx, const float *d_xyz, uint64_t n,
+                             const float *d_triangles, uint64_t ntri, float R, float *d_out);
 
 /* kNN range of MKNN / 1NN jobs without CNTNB: per neighbour type, per-center
  * histograms cover only the distance within which `factor` x k neighbours

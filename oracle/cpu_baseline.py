@@ -27,6 +27,9 @@ def _load():
             C.c_int32, P, C.c_int32, P, P, P, P, P, C.c_int64, P, P,
         ]
         lib.pp_cpu_spatstat.restype = C.c_int
+        lib.pp_cpu_ion2surf.argtypes = [C.c_int64, P, C.c_int64, P, C.c_float, C.c_int32,
+                                        C.c_int64, P, P]
+        lib.pp_cpu_ion2surf.restype = C.c_int
         _lib = lib
     return _lib
 
@@ -75,3 +78,20 @@ def spatstat(xyz, types, ion2surf, tasks, rmin, rincr, rmax, families, kvalues=(
             keep[name] = out[name]
     return keep, {"considered": int(stats[0]), "discarded": int(stats[1]),
                   "inball": int(stats[2]), "index_s": float(tim[0]), "compute_s": float(tim[1])}
+
+
+def ion2surf(xyz, triangles, R, nthreads=0, stride=1):
+    """Every stride-th ion's squared hull distance by a triangle-box tree
+    query per ion (the reference's surface_distancing2 structure).  Returns
+    (values, build_s, query_s)."""
+    lib = _load()
+    xyz = np.ascontiguousarray(xyz, dtype=np.float32).reshape(-1, 3)
+    tri = np.ascontiguousarray(triangles, dtype=np.float32).reshape(-1, 9)
+    m = (xyz.shape[0] + stride - 1) // stride
+    out = np.empty(m, np.float32)
+    tim = np.zeros(2, np.float64)
+    st = lib.pp_cpu_ion2surf(xyz.shape[0], xyz.ctypes.data, tri.shape[0], tri.ctypes.data,
+                             C.c_float(R), nthreads, stride, out.ctypes.data, tim.ctypes.data)
+    if st != 0:
+        raise ValueError("cpu baseline rejected the arguments")
+    return out, float(tim[0]), float(tim[1])

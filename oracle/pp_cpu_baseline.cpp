@@ -464,4 +464,187 @@ int pp_cpu_spatstat(int64_t n, const float *xyz, const uint8_t *type, const floa
     return 0;
 }
 
+// ---------------------------------------------------------------------------
+// Ion-to-surface squared distance, timed the reference's way
+// (solver::surface_distancing2, src/PARAPROBE_SolverHdl.cpp:312-484): a
+// bounding-volume tree over the triangles' vertex boxes, one leaf per
+// triangle, queried per ion with the float box [p - R, p + R] by a stack walk
+// (Tree::query, src/PARAPROBE_AABBTree.cpp:331-368), the candidates' closest-
+// point values folded into a running best that starts at fl32(R * R), ions
+// spread over the threads with schedule(dynamic).  The reference inserts the
+// triangles one by one with surface-area costs (Tree::insertLeaf,
+// AABBTree.cpp:383-510); this tree is built top-down by median splits, which
+// queries at least as fast.  Returns the build time in timings[0] and the
+// query time in timings[1].
+int pp_cpu_ion2surf(int64_t n, const float *xyz, int64_t ntri, const float *tri, float R,
+                    int32_t nthreads, int64_t stride, float *out, double *timings)
+{
+    if (n < 0 || ntri < 0 || !(R > 0.f) || stride < 1)
+        return -1;
+    if (nthreads > 0)
+        omp_set_num_threads(nthreads);
+    const double t0 = omp_get_wtime();
+    struct Node {
+        float lo[3], hi[3];
+        int32_t left, right;  // children, or (-1, triangle) for a leaf
+    };
+    std::vector<Node> nodes;
+    std::vector<float> box(6 * (size_t)ntri);
+    std::vector<int32_t> ids((size_t)ntri);
+    for (int64_t t = 0; t < ntri; ++t) {
+        const float *v = tri + 9 * t;
+        for (int k = 0; k < 3; ++k) {
+            box[6 * t + k] = std::fmin(std::fmin(v[k], v[3 + k]), v[6 + k]);
+            box[6 * t + 3 + k] = std::fmax(std::fmax(v[k], v[3 + k]), v[6 + k]);
+        }
+        ids[t] = (int32_t)t;
+    }
+    nodes.reserve(2 * (size_t)ntri + 1);
+    // iterative top-down build over ids[b, e)
+    struct Job {
+        int64_t b, e;
+        int32_t node;
+    };
+    std::vector<Job> work;
+    if (ntri > 0) {
+        nodes.push_back(Node{});
+        work.push_back({0, ntri, 0});
+    }
+    while (!work.empty()) {
+        Job j = work.back();
+        work.pop_back();
+        Node nd;
+        for (int k = 0; k < 3; ++k) {
+            nd.lo[k] = INFINITY;
+            nd.hi[k] = -INFINITY;
+        }
+        for (int64_t i = j.b; i < j.e; ++i)
+            for (int k = 0; k < 3; ++k) {
+                nd.lo[k] = std::fmin(nd.lo[k], box[6 * ids[i] + k]);
+                nd.hi[k] = std::fmax(nd.hi[k], box[6 * ids[i] + 3 + k]);
+            }
+        if (j.e - j.b == 1) {
+            nd.left = -1;
+            nd.right = ids[j.b];
+            nodes[j.node] = nd;
+            continue;
+        }
+        int ax = 0;
+        for (int k = 1; k < 3; ++k)
+            if (nd.hi[k] - nd.lo[k] > nd.hi[ax] - nd.lo[ax])
+                ax = k;
+        const int64_t mid = (j.b + j.e) / 2;
+        std::nth_element(ids.begin() + j.b, ids.begin() + mid, ids.begin() + j.e,
+                         [&](int32_t a, int32_t b) {
+                             return box[6 * a + ax] + box[6 * a + 3 + ax] <
+                                    box[6 * b + ax] + box[6 * b + 3 + ax];
+                         });
+        nd.left = (int32_t)nodes.size();
+        nd.right = nd.left + 1;
+        nodes.push_back(Node{});
+        nodes.push_back(Node{});
+        nodes[j.node] = nd;
+        work.push_back({j.b, mid, nd.left});
+        work.push_back({mid, j.e, nd.right});
+    }
+    const double t1 = omp_get_wtime();
+    const float RSQR = R * R;
+    auto closest = [](const float *t, const float *p) {
+        auto clamp01 = [](float x) {
+            const float m = (x < 0.f) ? 0.f : x;
+            return (m < 1.f) ? m : 1.f;
+        };
+        const float e0u = t[3] - t[0], e0v = t[4] - t[1], e0w = t[5] - t[2];
+        const float e1u = t[6] - t[0], e1v = t[7] - t[1], e1w = t[8] - t[2];
+        const float v0u = t[0] - p[0], v0v = t[1] - p[1], v0w = t[2] - p[2];
+        const float a = e0u * e0u + e0v * e0v + e0w * e0w;
+        const float b = e0u * e1u + e0v * e1v + e0w * e1w;
+        const float c = e1u * e1u + e1v * e1v + e1w * e1w;
+        const float d = e0u * v0u + e0v * v0v + e0w * v0w;
+        const float e = e1u * v0u + e1v * v0v + e1w * v0w;
+        const float det = a * c - b * b;
+        float s = b * e - c * d, u = b * d - a * e;
+        if (s + u < det) {
+            if (s < 0.f) {
+                if (u < 0.f && d < 0.f) {
+                    s = clamp01(-d / a);
+                    u = 0.f;
+                } else {
+                    s = 0.f;
+                    u = clamp01(-e / c);
+                }
+            } else if (u < 0.f) {
+                s = clamp01(-d / a);
+                u = 0.f;
+            } else {
+                const float inv = 1.f / det;
+                s *= inv;
+                u *= inv;
+            }
+        } else if (s < 0.f) {
+            if (c + e > b + d) {
+                s = clamp01(((c + e) - (b + d)) / (a - 2.f * b + c));
+                u = 1.f - s;
+            } else {
+                u = clamp01(-e / c);
+                s = 0.f;
+            }
+        } else if (u < 0.f) {
+            if (a + d > b + e) {
+                s = clamp01((c + e - b - d) / (a - 2.f * b + c));
+                u = 1.f - s;
+            } else {
+                s = clamp01(-e / c);
+                u = 0.f;
+            }
+        } else {
+            s = clamp01((c + e - b - d) / (a - 2.f * b + c));
+            u = 1.f - s;
+        }
+        const float dx = (t[0] + s * e0u + u * e1u) - p[0];
+        const float dy = (t[1] + s * e0v + u * e1v) - p[1];
+        const float dz = (t[2] + s * e0w + u * e1w) - p[2];
+        return dx * dx + dy * dy + dz * dz;
+    };
+    const int64_t m = (n + stride - 1) / stride;
+#pragma omp parallel
+    {
+        std::vector<int32_t> stack;
+        stack.reserve(256);
+#pragma omp for schedule(dynamic, 64)
+        for (int64_t q = 0; q < m; ++q) {
+            const int64_t i = q * stride;
+            const float *p = xyz + 3 * i;
+            const float lo[3] = {p[0] - R, p[1] - R, p[2] - R};
+            const float hi[3] = {p[0] + R, p[1] + R, p[2] + R};
+            float best = RSQR;
+            stack.clear();
+            if (!nodes.empty())
+                stack.push_back(0);
+            while (!stack.empty()) {
+                const Node &nd = nodes[stack.back()];
+                stack.pop_back();
+                if (hi[0] < nd.lo[0] || lo[0] > nd.hi[0] || hi[1] < nd.lo[1] || lo[1] > nd.hi[1] ||
+                    hi[2] < nd.lo[2] || lo[2] > nd.hi[2])
+                    continue;
+                if (nd.left < 0) {
+                    const float v = closest(tri + 9 * (int64_t)nd.right, p);
+                    if (!(v > best))
+                        best = v;
+                } else {
+                    stack.push_back(nd.left);
+                    stack.push_back(nd.right);
+                }
+            }
+            out[q] = best;
+        }
+    }
+    const double t2 = omp_get_wtime();
+    if (timings) {
+        timings[0] = t1 - t0;
+        timings[1] = t2 - t1;
+    }
+    return 0;
+}
+
 } // extern "C"

@@ -56,6 +56,7 @@ EXPORTED = [
     "pp_serve_drain",
     "pp_serve_destroy",
     "pp_ion2surf",
+    "pp_ion2surf_device",
 ]
 
 
@@ -144,6 +145,7 @@ def load_library() -> C.CDLL:
     lib.pp_serve_drain.argtypes = [P]
     lib.pp_serve_destroy.argtypes = [P]
     lib.pp_ion2surf.argtypes = [P, P, C.c_uint64, P, C.c_uint64, C.c_float, P]
+    lib.pp_ion2surf_device.argtypes = [P, P, C.c_uint64, P, C.c_uint64, C.c_float, P]
     for name in EXPORTED:
         if name != "pp_last_error_string":
             getattr(lib, name).restype = C.c_int
@@ -346,6 +348,13 @@ class SpatStat:
         self._check(self.lib.pp_ion2surf(self.ctx, xyz.ctypes.data, xyz.shape[0], tri.ctypes.data,
                                          tri.shape[0], C.c_float(R), out.ctypes.data))
         return out
+
+    def ion2surf_device(self, d_xyz: int, n: int, d_triangles: int, ntri: int, R: float,
+                        d_out: int):
+        """ion2surf on device pointers (n x 3 and ntri x 9 floats in, n floats out)."""
+        self._check(self.lib.pp_ion2surf_device(self.ctx, C.c_void_p(d_xyz), n,
+                                                C.c_void_p(d_triangles), ntri, C.c_float(R),
+                                                C.c_void_p(d_out)))
 
     def upload(self, xyz: np.ndarray, types: np.ndarray, ion2surf_sqr: np.ndarray,
                cell_width: float = 0.0) -> Cloud:

@@ -1893,8 +1893,12 @@ pp_status pp_set_tuning(pp_ctx *c, int32_t centers_per_tile, int32_t cells_per_r
     return PP_OK;
 }
 
-pp_status pp_ion2surf(pp_ctx *c, const float *xyz, uint64_t n, const float *tri, uint64_t ntri,
-                      float R, float *out)
+namespace {
+
+// ion2surf of n ions against ntri triangles; dev: the three arrays are device
+// memory, else host memory (staged through one device allocation).
+pp_status ion2surf_run(pp_ctx *c, const float *xyz, uint64_t n, const float *tri, uint64_t ntri,
+                       float R, float *out, bool dev)
 {
     if (!c)
         return PP_ERR_INVALID_ARG;
@@ -1906,33 +1910,45 @@ pp_status pp_ion2surf(pp_ctx *c, const float *xyz, uint64_t n, const float *tri,
         return fail(c, PP_ERR_INVALID_ARG, "more than 2^32 - 16 ions or triangles");
     if (n == 0)
         return PP_OK;
-    if (ntri == 0) {
-        // no hull: every ion keeps fl32(R * R)
-        const float r2 = R * R;
-        std::fill(out, out + n, r2);
-        return PP_OK;
-    }
     PP_HIP(c, hipSetDevice(c->device));
     Slot &sl = c->main;
     hipStream_t s = sl.stream;
-    // device buffers of this call
+    const float r2 = R * R;
+    if (ntri == 0 && !dev) {
+        // no hull: every ion keeps fl32(R * R)
+        std::fill(out, out + n, r2);
+        return PP_OK;
+    }
     Carver m0(nullptr);
-    m0.take<float>(3 * n);
-    m0.take<float>(n);
-    m0.take<float>(9 * ntri + 1);
+    if (!dev) {
+        m0.take<float>(3 * n);
+        m0.take<float>(n);
+        m0.take<float>(9 * ntri + 1);
+    }
     m0.take<float>(6 * ntri + 1);
     m0.take<uint32_t>(8);
     void *mem = nullptr;
     PP_HIP(c, hipMalloc(&mem, m0.off + 256));
     std::unique_ptr<void, void (*)(void *)> own(mem, [](void *p) { (void)hipFree(p); });
     Carver m(mem);
-    float *d_xyz = m.take<float>(3 * n);
-    float *d_out = m.take<float>(n);
-    float *d_tri = m.take<float>(9 * ntri + 1);
+    const float *d_xyz = xyz, *d_tri = tri;
+    float *d_out = out;
+    if (!dev) {
+        float *x = m.take<float>(3 * n);
+        d_out = m.take<float>(n);
+        float *t = m.take<float>(9 * ntri + 1);
+        PP_HIP(c, hipMemcpyAsync(x, xyz, 12 * n, hipMemcpyHostToDevice, s));
+        PP_HIP(c, hipMemcpyAsync(t, tri, 36 * ntri, hipMemcpyHostToDevice, s));
+        d_xyz = x;
+        d_tri = t;
+    }
     float *d_box = m.take<float>(6 * ntri + 1);
     uint32_t *d_bb = m.take<uint32_t>(8);
-    PP_HIP(c, hipMemcpyAsync(d_xyz, xyz, 12 * n, hipMemcpyHostToDevice, s));
-    PP_HIP(c, hipMemcpyAsync(d_tri, tri, 36 * ntri, hipMemcpyHostToDevice, s));
+    if (ntri == 0) {
+        PP_HIP(c, launch_fill(d_out, n, r2, s));
+        PP_HIP(c, hipStreamSynchronize(s));
+        return PP_OK;
+    }
     // grid over the ions and the triangle vertices, cells of at least R
     uint32_t hb[12];
     PP_HIP(c, launch_bbox(d_xyz, n, d_bb, s));
@@ -1948,12 +1964,16 @@ pp_status pp_ion2surf(pp_ctx *c, const float *xyz, uint64_t n, const float *tri,
         if (!std::isfinite(lo[a]) || !std::isfinite(hi[a]))
             return fail(c, PP_ERR_INVALID_ARG, "non-finite coordinates");
     }
-    double w = R;
+    // cells of at least R/2 holding a few hundred ions: a tile is one cell's
+    // ions against the triangles whose R-inflated box touches the cell
+    double w = 0.5 * (double)R, M = 0.0;
     {
         double vol = 1.0;
-        for (int a = 0; a < 3; ++a)
+        for (int a = 0; a < 3; ++a) {
             vol *= std::max(hi[a] - lo[a], 1e-3);
-        w = std::max(w, std::cbrt(vol / 4.0e6));
+            M = std::max(M, std::max(std::fabs(lo[a]), std::fabs(hi[a])));
+        }
+        w = std::max(w, std::cbrt(256.0 * vol / (double)n));
     }
     GridGeom g{};
     for (;;) {
@@ -1972,30 +1992,79 @@ pp_status pp_ion2surf(pp_ctx *c, const float *xyz, uint64_t n, const float *tri,
     Carver t0(nullptr);
     t0.take<uint32_t>(nc + 1);
     t0.take<uint32_t>(nc + 1);
-    t0.take<uint32_t>(scan_workspace_words(nc));
+    t0.take<uint32_t>(nc + 1);
+    t0.take<uint32_t>(scan_workspace_words(std::max<uint64_t>(nc, n)));
+    t0.take<uint32_t>(n);
+    t0.take<uint32_t>(n);
+    t0.take<float4>(n);
     void *tm = nullptr;
     PP_HIP(c, hipMalloc(&tm, t0.off + 256));
     std::unique_ptr<void, void (*)(void *)> own_t(tm, [](void *p) { (void)hipFree(p); });
     Carver t(tm);
     uint32_t *count = t.take<uint32_t>(nc + 1);
-    uint32_t *cellstart = t.take<uint32_t>(nc + 1);
-    uint32_t *ws = t.take<uint32_t>(scan_workspace_words(nc));
+    uint32_t *tcs = t.take<uint32_t>(nc + 1);
+    uint32_t *ics = t.take<uint32_t>(nc + 1);
+    uint32_t *ws = t.take<uint32_t>(scan_workspace_words(std::max<uint64_t>(nc, n)));
+    uint32_t *key = t.take<uint32_t>(n);
+    uint32_t *rank = t.take<uint32_t>(n);
+    float4 *ions = t.take<float4>(n);
+    // ions in cell order
+    PP_HIP(c, hipMemsetAsync(count, 0, 4 * (nc + 1), s));
+    PP_HIP(c, launch_cellkeys(d_xyz, n, g, count, key, rank, s));
+    PP_HIP(c, scan_exclusive(count, ics, nc, ws, s));
+    PP_HIP(c, launch_ionscatter(d_xyz, n, key, rank, ics, ions, s));
+    // triangle lists per cell
     PP_HIP(c, hipMemsetAsync(count, 0, 4 * (nc + 1), s));
     PP_HIP(c, launch_tribox(d_tri, ntri, R, g, d_box, count, s));
-    PP_HIP(c, scan_exclusive(count, cellstart, nc, ws, s));
-    uint32_t entries = 0;
-    PP_HIP(c, hipMemcpyAsync(&entries, cellstart + nc, 4, hipMemcpyDeviceToHost, s));
+    PP_HIP(c, scan_exclusive(count, tcs, nc, ws, s));
+    std::vector<uint32_t> h_ics(nc + 1), h_tcs(nc + 1);
+    PP_HIP(c, hipMemcpyAsync(h_ics.data(), ics, 4 * (nc + 1), hipMemcpyDeviceToHost, s));
+    PP_HIP(c, hipMemcpyAsync(h_tcs.data(), tcs, 4 * (nc + 1), hipMemcpyDeviceToHost, s));
     PP_HIP(c, hipStreamSynchronize(s));
+    const uint32_t entries = h_tcs[nc];
+    std::vector<SurfTile> tiles;
+    for (uint64_t k = 0; k < nc; ++k) {
+        if (h_tcs[k + 1] == h_tcs[k])
+            continue;
+        for (uint32_t f = h_ics[k]; f < h_ics[k + 1]; f += 256)
+            tiles.push_back({(uint32_t)k, f, std::min<uint32_t>(256, h_ics[k + 1] - f), 0});
+    }
     void *lm = nullptr;
-    PP_HIP(c, hipMalloc(&lm, 4 * (size_t)std::max<uint32_t>(entries, 1) + 256));
+    PP_HIP(c, hipMalloc(&lm, 4 * (size_t)std::max<uint32_t>(entries, 1) +
+                                 sizeof(SurfTile) * (tiles.size() + 1) + 512));
     std::unique_ptr<void, void (*)(void *)> own_l(lm, [](void *p) { (void)hipFree(p); });
-    uint32_t *list = (uint32_t *)lm;
-    PP_HIP(c, hipMemcpyAsync(count, cellstart, 4 * (nc + 1), hipMemcpyDeviceToDevice, s));
+    Carver l(lm);
+    uint32_t *list = l.take<uint32_t>(std::max<uint32_t>(entries, 1));
+    SurfTile *d_tiles = l.take<SurfTile>(tiles.size() + 1);
+    PP_HIP(c, hipMemcpyAsync(count, tcs, 4 * (nc + 1), hipMemcpyDeviceToDevice, s));
     PP_HIP(c, launch_trifill(d_box, ntri, R, g, count, list, s));
-    PP_HIP(c, launch_ion2surf(d_xyz, n, d_tri, d_box, g, cellstart, list, R, d_out, s));
-    PP_HIP(c, hipMemcpyAsync(out, d_out, 4 * n, hipMemcpyDeviceToHost, s));
+    if (!tiles.empty())
+        PP_HIP(c, hipMemcpyAsync(d_tiles, tiles.data(), sizeof(SurfTile) * tiles.size(),
+                                 hipMemcpyHostToDevice, s));
+    PP_HIP(c, launch_fill(d_out, n, r2, s));
+    // delta bounds how far a computed closest point can stray outside the
+    // triangle's box: a few roundings of coordinates up to M in magnitude
+    const float delta = (float)(std::ldexp(M, -16));
+    PP_HIP(c, launch_surftile(ions, d_tiles, (uint32_t)tiles.size(), tcs, list, d_tri, d_box, R,
+                              delta, d_out, s));
+    if (!dev)
+        PP_HIP(c, hipMemcpyAsync(out, d_out, 4 * n, hipMemcpyDeviceToHost, s));
     PP_HIP(c, hipStreamSynchronize(s));
     return PP_OK;
+}
+
+} // namespace
+
+pp_status pp_ion2surf(pp_ctx *c, const float *xyz, uint64_t n, const float *tri, uint64_t ntri,
+                      float R, float *out)
+{
+    return ion2surf_run(c, xyz, n, tri, ntri, R, out, false);
+}
+
+pp_status pp_ion2surf_device(pp_ctx *c, const float *d_xyz, uint64_t n, const float *d_tri,
+                             uint64_t ntri, float R, float *d_out)
+{
+    return ion2surf_run(c, d_xyz, n, d_tri, ntri, R, d_out, true);
 }
 
 pp_status pp_set_knn_range(pp_ctx *c, float factor)

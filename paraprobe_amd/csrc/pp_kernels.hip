@@ -190,11 +190,24 @@ __global__ __launch_bounds__(256) void k_bbox(const float *__restrict__ xyz, uin
             mx[a] = t1 > mx[a] ? t1 : mx[a];
         }
     }
-    if ((threadIdx.x & (kWave - 1)) == 0) {
+    // one atomic per value and workgroup (atomics on one address serialise)
+    __shared__ uint32_t s_mm[kWaves][6];
+    const int wid = threadIdx.x / kWave;
+    if ((threadIdx.x & (kWave - 1)) == 0)
         for (int a = 0; a < 3; ++a) {
-            atomicMin(&out6[a], mn[a]);
-            atomicMax(&out6[3 + a], mx[a]);
+            s_mm[wid][a] = mn[a];
+            s_mm[wid][3 + a] = mx[a];
         }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int a = threadIdx.x;
+        uint32_t v = s_mm[0][a];
+        for (int w = 1; w < kWaves; ++w)
+            v = a < 3 ? (s_mm[w][a] < v ? s_mm[w][a] : v) : (s_mm[w][a] > v ? s_mm[w][a] : v);
+        if (a < 3)
+            atomicMin(&out6[a], v);
+        else
+            atomicMax(&out6[a], v);
     }
 }
 
@@ -205,8 +218,8 @@ hipError_t launch_bbox(const float *xyz, uint64_t n, uint32_t *out6, hipStream_t
     if (e != hipSuccess)
         return e;
     unsigned g = grid_for(n, 256);
-    if (g > 4096)
-        g = 4096;
+    if (g > 1024)
+        g = 1024;
     hipLaunchKernelGGL(k_bbox, dim3(g), dim3(256), 0, s, xyz, n, out6);
     return hipGetLastError();
 }

@@ -93,13 +93,23 @@ constexpr int kReduceLdsWords = 16384;  // ntasks * C above this: global atomics
 
 // Ion-to-surface squared distances (pp_surfdist.hip): triangle boxes and
 // per-cell triangle counts on grid g, the cell lists (cursor starts as a copy
-// of the cell offsets), and one thread per ion.
+// of the cell offsets), ions in cell order, and one workgroup per tile of
+// ions; launch_fill sets n floats to v.
+hipError_t launch_fill(float *out, uint64_t n, float v, hipStream_t s);
 hipError_t launch_tribox(const float *tri, uint64_t ntri, float R, GridGeom g, float *box,
                          uint32_t *count, hipStream_t s);
 hipError_t launch_trifill(const float *box, uint64_t ntri, float R, GridGeom g, uint32_t *cursor,
                           uint32_t *list, hipStream_t s);
-hipError_t launch_ion2surf(const float *xyz, uint64_t n, const float *tri, const float *box,
-                           GridGeom g, const uint32_t *cellstart, const uint32_t *list, float R,
-                           float *out, hipStream_t s);
+// One tile = up to 256 ions of one cell (ions[first, first + count) in cell
+// order) tested against that cell's triangles.
+struct SurfTile {
+    uint32_t cell, first, count, pad;
+};
+hipError_t launch_ionscatter(const float *xyz, uint64_t n, const uint32_t *key,
+                             const uint32_t *rank, const uint32_t *cellstart, float4 *ions,
+                             hipStream_t s);
+hipError_t launch_surftile(const float4 *ions, const SurfTile *tiles, uint32_t ntiles,
+                           const uint32_t *tcellstart, const uint32_t *list, const float *tri,
+                           const float *box, float R, float delta, float *out, hipStream_t s);
 
 } // namespace pp

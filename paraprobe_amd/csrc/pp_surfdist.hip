@@ -37,81 +37,65 @@ __device__ inline float ref_clamp01(float x) { return ref_min(ref_max(x, 0.f), 1
 } // namespace
 
 // mathHdl::closestPointOnTriangle (src/PARAPROBE_Math.cpp:88-171): squared
-// distance from (sx, sy, sz) to triangle t[0..8], same operations in the same
-// order.
-__host__ __device__ inline float closest_sqr(const float *t, float sx, float sy, float sz)
+// distance from (px, py, pz) to the triangle with vertex 1 at (x1, y1, z1)
+// and edges e0 = V2 - V1, e1 = V3 - V1 as the reference computes them.  The
+// seven regions are resolved with selects instead of branches (lanes of a
+// wavefront hit different regions), but each region's values come from the
+// same operations in the same order as the reference, so the result is the
+// same float:
+//   region 0 (s+t < det, s >= 0, t >= 0): s, t times 1.f/det
+//   regions 4 (d < 0) and 5:     s = CLAMP(-d/a), t = 0
+//   regions 4 (d >= 0), 3, 2b:   s = 0, t = CLAMP(-e/c)
+//   region 6b:                   s = CLAMP(-e/c), t = 0 (the reference's
+//                                choice; the geometry would use -d/a)
+//   region 2a:                   s = CLAMP(((c+e) - (b+d)) / denom), t = 1-s
+//   regions 6a and 1:            s = CLAMP((((c+e) - b) - d) / denom), t = 1-s
+// with denom = (a - 2b) + c and CLAMP(x) = MIN(MAX(x, 0), 1) as ternaries
+// (src/PARAPROBE_Abinitio.h:47-49: a NaN clamps to 1).
+__device__ inline float closest_sqr(float x1, float y1, float z1, float e0u, float e0v, float e0w,
+                                    float e1u, float e1v, float e1w, float px, float py, float pz)
 {
-    const float e0u = t[3] - t[0], e0v = t[4] - t[1], e0w = t[5] - t[2];
-    const float e1u = t[6] - t[0], e1v = t[7] - t[1], e1w = t[8] - t[2];
-    const float v0u = t[0] - sx, v0v = t[1] - sy, v0w = t[2] - sz;
+    const float v0u = x1 - px, v0v = y1 - py, v0w = z1 - pz;
     const float a = e0u * e0u + e0v * e0v + e0w * e0w;
     const float b = e0u * e1u + e0v * e1v + e0w * e1w;
     const float c = e1u * e1u + e1v * e1v + e1w * e1w;
     const float d = e0u * v0u + e0v * v0v + e0w * v0w;
     const float e = e1u * v0u + e1v * v0v + e1w * v0w;
     const float det = a * c - b * b;
-    float s = b * e - c * d;
-    float tt = b * d - a * e;
-    auto clamp01 = [](float x) {
-        const float m = (x < 0.f) ? 0.f : x;
-        return (m < 1.f) ? m : 1.f;
-    };
-    if (s + tt < det) {
-        if (s < 0.f) {
-            if (tt < 0.f) {  // region 4
-                if (d < 0.f) {
-                    s = clamp01(-d / a);
-                    tt = 0.f;
-                } else {
-                    s = 0.f;
-                    tt = clamp01(-e / c);
-                }
-            } else {  // region 3
-                s = 0.f;
-                tt = clamp01(-e / c);
-            }
-        } else if (tt < 0.f) {  // region 5
-            s = clamp01(-d / a);
-            tt = 0.f;
-        } else {  // region 0
-            const float invDet = 1.f / det;
-            s *= invDet;
-            tt *= invDet;
-        }
+    const float s = b * e - c * d;
+    const float t = b * d - a * e;
+    const bool inner = s + t < det;
+    const bool sneg = s < 0.f, tneg = t < 0.f;
+    const float tmp0 = b + d, tmp1 = c + e;
+    const bool r0 = inner && !sneg && !tneg;
+    const bool da = inner && ((sneg && tneg && d < 0.f) || (!sneg && tneg));
+    const bool r6 = !inner && !sneg && tneg;
+    const bool r6a = r6 && (a + d > b + e);
+    const bool r2a = !inner && sneg && (tmp1 > tmp0);
+    const bool ec = (inner && sneg && !(tneg && d < 0.f)) || (!inner && sneg && !r2a) ||
+                    (r6 && !r6a);
+    const float denom = a - 2.f * b + c;
+    const float n1 = c + e - b - d;
+    const float num = r0 ? 1.f : (da ? -d : (ec ? -e : (r2a ? tmp1 - tmp0 : n1)));
+    const float den = r0 ? det : (da ? a : (ec ? c : denom));
+    const float q = num / den;
+    const float m = (q < 0.f) ? 0.f : q;
+    const float cq = (m < 1.f) ? m : 1.f;
+    // s-only: regions 4a, 5, 6b; t-only: 4b, 3, 2b; both from cq: 2a, 6a, 1
+    const bool t_only = ec && !r6;
+    const bool s_only = da || (ec && r6);
+    float ss, tt;
+    if (r0) {
+        ss = s * q;
+        tt = t * q;
     } else {
-        if (s < 0.f) {  // region 2
-            const float tmp0 = b + d;
-            const float tmp1 = c + e;
-            if (tmp1 > tmp0) {
-                const float numer = tmp1 - tmp0;
-                const float denom = a - 2.f * b + c;
-                s = clamp01(numer / denom);
-                tt = 1.f - s;
-            } else {
-                tt = clamp01(-e / c);
-                s = 0.f;
-            }
-        } else if (tt < 0.f) {  // region 6
-            if (a + d > b + e) {
-                const float numer = c + e - b - d;
-                const float denom = a - 2.f * b + c;
-                s = clamp01(numer / denom);
-                tt = 1.f - s;
-            } else {
-                s = clamp01(-e / c);
-                tt = 0.f;
-            }
-        } else {  // region 1
-            const float numer = c + e - b - d;
-            const float denom = a - 2.f * b + c;
-            s = clamp01(numer / denom);
-            tt = 1.f - s;
-        }
+        ss = t_only ? 0.f : cq;
+        tt = t_only ? cq : (s_only ? 0.f : 1.f - cq);
     }
-    const float cx = t[0] + s * e0u + tt * e1u;
-    const float cy = t[1] + s * e0v + tt * e1v;
-    const float cz = t[2] + s * e0w + tt * e1w;
-    const float dx = cx - sx, dy = cy - sy, dz = cz - sz;
+    const float cx = x1 + ss * e0u + tt * e1u;
+    const float cy = y1 + ss * e0v + tt * e1v;
+    const float cz = z1 + ss * e0w + tt * e1w;
+    const float dx = cx - px, dy = cy - py, dz = cz - pz;
     return dx * dx + dy * dy + dz * dz;
 }
 
@@ -173,44 +157,121 @@ __global__ __launch_bounds__(256) void k_trifill(const float *__restrict__ box, 
                 list[atomicAdd(&cursor[((uint64_t)z * g.ny + y) * g.nx + x], 1u)] = (uint32_t)i;
 }
 
-// One thread per ion: the candidates of its cell, the reference's overlap
-// test against the float box [p - R, p + R], the smallest squared distance
-// not above R^2 (NaN values of degenerate triangles are skipped).
-__global__ __launch_bounds__(256) void k_ion2surf(const float *__restrict__ xyz, uint64_t n,
-                                                  const float *__restrict__ tri,
-                                                  const float *__restrict__ box, GridGeom g,
-                                                  const uint32_t *__restrict__ cellstart,
-                                                  const uint32_t *__restrict__ list, float R,
-                                                  float *__restrict__ out)
+// Ions into cell order: (x, y, z, original index as bits).
+__global__ __launch_bounds__(256) void k_ionscatter(const float *__restrict__ xyz, uint64_t n,
+                                                    const uint32_t *__restrict__ key,
+                                                    const uint32_t *__restrict__ rank,
+                                                    const uint32_t *__restrict__ cellstart,
+                                                    float4 *__restrict__ ions)
 {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n)
         return;
-    const float px = xyz[3 * i], py = xyz[3 * i + 1], pz = xyz[3 * i + 2];
+    ions[cellstart[key[i]] + rank[i]] =
+        make_float4(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], __uint_as_float((uint32_t)i));
+}
+
+// One workgroup per tile (up to 256 ions of one cell, one ion per thread):
+// the cell's triangles pass through LDS in chunks of 256, and every lane
+// keeps the smallest closest-point value not above R^2 over the triangles
+// that pass the reference's overlap test against its float box [p - R,
+// p + R].  A triangle is first tested against a lower bound of its value:
+// the computed closest point lies in the triangle's box widened by delta
+// (its barycentric coordinates stay in the simplex up to a few roundings;
+// delta = 2^-16 max|coordinate| covers them with a wide margin), so the value
+// is at least the squared distance from p to that widened box (from the box
+// centre and half extents plus delta).  A bound above best * (1 + 2^-18)
+// proves the value is above best, where the reference would not take it
+// either; only lanes whose bound passes run the exact overlap test and the
+// closest-point expression.  NaN values (only from overflowing arithmetic)
+// are skipped.
+__global__ __launch_bounds__(256) void k_surftile(const float4 *__restrict__ ions,
+                                                  const SurfTile *__restrict__ tiles,
+                                                  const uint32_t *__restrict__ tcellstart,
+                                                  const uint32_t *__restrict__ list,
+                                                  const float *__restrict__ tri,
+                                                  const float *__restrict__ box, float R,
+                                                  float delta, float *__restrict__ out)
+{
+    // per triangle of the chunk, 24 floats: centre (3), half extent + delta
+    // (3), box (6), vertex 1 (3), edges e0 and e1 (6), padding (3)
+    __shared__ float4 s_r[256][6];
+    const SurfTile tl = tiles[blockIdx.x];
+    const int tid = threadIdx.x;
+    const bool act = tid < (int)tl.count;
+    // inactive lanes sit at infinity: every bound fails for them
+    const float4 p = act ? ions[tl.first + tid]
+                         : make_float4(__builtin_inff(), __builtin_inff(), __builtin_inff(), 0.f);
+    const float px = p.x, py = p.y, pz = p.z;
     const float RSQR = R * R;
     const float qlx = px - R, qly = py - R, qlz = pz - R;
     const float qux = px + R, quy = py + R, quz = pz + R;
-    int c[3];
-    const float p[3] = {px, py, pz};
-    for (int k = 0; k < 3; ++k) {
-        const int nk = k == 0 ? g.nx : (k == 1 ? g.ny : g.nz);
-        int v = (int)floor(((double)p[k] - g.lo[k]) * g.inv_w);
-        c[k] = v < 0 ? 0 : (v >= nk ? nk - 1 : v);
+    constexpr float kGrow = 1.0f + 0x1p-18f;
+    float best = RSQR, bestg = RSQR * kGrow;
+    const uint32_t ts = __builtin_amdgcn_readfirstlane(tcellstart[tl.cell]);
+    const uint32_t te = __builtin_amdgcn_readfirstlane(tcellstart[tl.cell + 1]);
+    for (uint32_t base = ts; base < te; base += 256) {
+        const uint32_t m = __builtin_amdgcn_readfirstlane((te - base) < 256u ? (te - base) : 256u);
+        __syncthreads();
+        if ((uint32_t)tid < m) {
+            const uint64_t t = list[base + tid];
+            const float *bx = box + 6 * t;
+            const float b0 = bx[0], b1 = bx[1], b2 = bx[2], b3 = bx[3], b4 = bx[4], b5 = bx[5];
+            const float *v = tri + 9 * t;
+            const float v0 = v[0], v1 = v[1], v2 = v[2], v3 = v[3], v4 = v[4], v5 = v[5];
+            const float v6 = v[6], v7 = v[7], v8 = v[8];
+            s_r[tid][0] = make_float4(0.5f * (b0 + b3), 0.5f * (b1 + b4), 0.5f * (b2 + b5),
+                                      0.5f * (b3 - b0) + delta);
+            s_r[tid][1] = make_float4(0.5f * (b4 - b1) + delta, 0.5f * (b5 - b2) + delta, b0, b1);
+            s_r[tid][2] = make_float4(b2, b3, b4, b5);
+            // vertex 1 and the two edges as the reference forms them
+            s_r[tid][3] = make_float4(v0, v1, v2, v3 - v0);
+            s_r[tid][4] = make_float4(v4 - v1, v5 - v2, v6 - v0, v7 - v1);
+            s_r[tid][5] = make_float4(v8 - v2, 0.f, 0.f, 0.f);
+        }
+        __syncthreads();
+#pragma unroll 2
+        for (uint32_t j = 0; j < m; ++j) {
+            const float4 r0 = s_r[j][0], r1 = s_r[j][1];
+            const float gx = fmaxf(fabsf(px - r0.x) - r0.w, 0.f);
+            const float gy = fmaxf(fabsf(py - r0.y) - r1.x, 0.f);
+            const float gz = fmaxf(fabsf(pz - r0.z) - r1.y, 0.f);
+            const bool pass = !(gx * gx + gy * gy + gz * gz > bestg);
+            if (__builtin_amdgcn_ballot_w64(pass)) {
+                const float4 r2 = s_r[j][2];
+                // the reference's overlap test, evaluated without short cuts
+                const bool out_of = (qux < r1.z) | (qlx > r2.y) | (quy < r1.w) | (qly > r2.z) |
+                                    (quz < r2.x) | (qlz > r2.w);
+                const bool need = pass & act & !out_of;
+                if (__builtin_amdgcn_ballot_w64(need)) {
+                    const float4 r3 = s_r[j][3], r4 = s_r[j][4], r5 = s_r[j][5];
+                    const float v = closest_sqr(r3.x, r3.y, r3.z, r3.w, r4.x, r4.y, r4.z, r4.w,
+                                                r5.x, px, py, pz);
+                    if (need && v <= best) {
+                        best = v;
+                        bestg = v * kGrow;
+                    }
+                }
+            }
+        }
     }
-    const uint64_t cell = ((uint64_t)c[2] * g.ny + c[1]) * g.nx + c[0];
-    float best = RSQR;
-    for (uint32_t q = cellstart[cell]; q < cellstart[cell + 1]; ++q) {
-        const uint32_t t = list[q];
-        const float *b = box + 6 * (uint64_t)t;
-        const bool overlap = !(qux < b[0] || qlx > b[3] || quy < b[1] || qly > b[4] ||
-                               quz < b[2] || qlz > b[5]);
-        if (!overlap)
-            continue;
-        const float v = closest_sqr(tri + 9 * (uint64_t)t, px, py, pz);
-        if (v <= best)  // the reference keeps v unless v > best
-            best = v;
-    }
-    out[i] = best;
+    if (act)
+        out[__float_as_uint(p.w)] = best;
+}
+
+__global__ __launch_bounds__(256) void k_fill(float *__restrict__ out, uint64_t n, float v)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        out[i] = v;
+}
+
+hipError_t launch_fill(float *out, uint64_t n, float v, hipStream_t s)
+{
+    if (n == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_fill, dim3(blocks_for(n, 256)), dim3(256), 0, s, out, n, v);
+    return hipGetLastError();
 }
 
 hipError_t launch_tribox(const float *tri, uint64_t ntri, float R, GridGeom g, float *box,
@@ -233,14 +294,25 @@ hipError_t launch_trifill(const float *box, uint64_t ntri, float R, GridGeom g, 
     return hipGetLastError();
 }
 
-hipError_t launch_ion2surf(const float *xyz, uint64_t n, const float *tri, const float *box,
-                           GridGeom g, const uint32_t *cellstart, const uint32_t *list, float R,
-                           float *out, hipStream_t s)
+hipError_t launch_ionscatter(const float *xyz, uint64_t n, const uint32_t *key,
+                             const uint32_t *rank, const uint32_t *cellstart, float4 *ions,
+                             hipStream_t s)
 {
     if (n == 0)
         return hipSuccess;
-    hipLaunchKernelGGL(k_ion2surf, dim3(blocks_for(n, 256)), dim3(256), 0, s, xyz, n, tri, box, g,
-                       cellstart, list, R, out);
+    hipLaunchKernelGGL(k_ionscatter, dim3(blocks_for(n, 256)), dim3(256), 0, s, xyz, n, key, rank,
+                       cellstart, ions);
+    return hipGetLastError();
+}
+
+hipError_t launch_surftile(const float4 *ions, const SurfTile *tiles, uint32_t ntiles,
+                           const uint32_t *tcellstart, const uint32_t *list, const float *tri,
+                           const float *box, float R, float delta, float *out, hipStream_t s)
+{
+    if (ntiles == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_surftile, dim3(ntiles), dim3(256), 0, s, ions, tiles, tcellstart, list,
+                       tri, box, R, delta, out);
     return hipGetLastError();
 }
 

@@ -102,3 +102,32 @@ def test_ion2surf_headline_tip_properties(ctx):
     # a nm to the cancellation in det = a c - b^2 (the reference's behaviour)
     assert np.abs(d[near] - exact[near]).max() < 0.1
     assert (got[far] == np.float32(4.0)).all()
+
+
+def test_ion2surf_device_entry_matches_host_entry(ctx):
+    import ctypes
+
+    hip = ctypes.CDLL("libamdhip64.so.7")
+    t = synth.tip(50_000, 21, "cylinder", 10.0, 10.0, 30.0, [1.0])
+    tri = synth.hull("cylinder", 10.0, 10.0, 30.0, 64, 24)
+    n, nt = t.n, tri.shape[0]
+    ptr = [ctypes.c_void_p() for _ in range(3)]
+    sizes = [12 * n, 36 * nt, 4 * n]
+    try:
+        for p, b in zip(ptr, sizes):
+            assert hip.hipMalloc(ctypes.byref(p), ctypes.c_size_t(b)) == 0
+        assert hip.hipMemcpy(ptr[0], t.xyz.ctypes.data_as(ctypes.c_void_p),
+                             ctypes.c_size_t(sizes[0]), 1) == 0
+        assert hip.hipMemcpy(ptr[1], tri.ctypes.data_as(ctypes.c_void_p),
+                             ctypes.c_size_t(sizes[1]), 1) == 0
+        got = np.empty(n, np.float32)
+        for R, ntri in ((2.0, nt), (3.0, 0)):
+            _say(f"ion2surf device R={R} ntri={ntri}")
+            ctx.ion2surf_device(ptr[0].value, n, ptr[1].value, ntri, R, ptr[2].value)
+            assert hip.hipMemcpy(got.ctypes.data_as(ctypes.c_void_p), ptr[2],
+                                 ctypes.c_size_t(sizes[2]), 2) == 0
+            assert _bits_equal(got, ctx.ion2surf(t.xyz, tri[:ntri], R))
+    finally:
+        for p in ptr:
+            if p.value:
+                hip.hipFree(p)
