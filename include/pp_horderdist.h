@@ -22,6 +22,7 @@
 
 #ifdef __cplusplus
 #include <map>
+#include <random>
 #include <string>
 #include <vector>
 
@@ -96,6 +97,46 @@ private:
     std::vector<TypeSpecDescrStat> spatstat_tasks;
 };
 
+/* rndlabeler (S.cpp:2809-3000): label randomisation of the whole tip.  The
+ * generator is a std::mt19937 seeded with Settings::RndDescrStatsPRNGSeed
+ * ((unsigned) -1) and advanced by Settings::RndDescrStatsPRNGDiscard (700000)
+ * at construction (Settings.cpp:166-167, S.cpp:2809-2816); reset() keeps its
+ * state.  The global type array is the cloud's types in upload order and is
+ * permuted with std::shuffle, so a cloud uploaded in the reference's order
+ * (z-slabs by thread, KD-tree order inside each) gets the reference's
+ * randomised labels when both use the same C++ standard library.  Applying the
+ * labels builds a relabelled cloud on the device (pp_cloud_relabel); the
+ * original cloud is left untouched. */
+class RndLabeler {
+public:
+    static constexpr uint32_t kSeed = 4294967295u;   /* (unsigned int) -1 */
+    static constexpr uint32_t kDiscard = 700000u;
+    explicit RndLabeler(pp_ctx *ctx, uint32_t seed = kSeed, uint32_t discard = kDiscard);
+    ~RndLabeler();
+    RndLabeler(const RndLabeler &) = delete;
+    RndLabeler &operator=(const RndLabeler &) = delete;
+    /* S.cpp:2825-2835: clears the type arrays, keeps the generator. */
+    void reset();
+    /* S.cpp:2850-2880: the cloud's types in upload order. */
+    pp_status learn_original_types_global(const pp_cloud *cloud);
+    /* S.cpp:2882-2904: shuffles a copy once. */
+    void shuffle_types_mt19937();
+    /* S.cpp:2907-2942: the relabelled cloud (available through cloud()). */
+    pp_status apply_shuffled_types_global(const pp_cloud *cloud);
+    /* S.cpp:2951-2983: drops the relabelled cloud. */
+    void reset_original_types_global();
+    bool is_shuffled() const { return shuffled; }
+    bool is_applied() const { return applied; }
+    const pp_cloud *cloud() const { return rnd_; }
+    std::vector<uint8_t> initial_iontypid, temporary_iontypid;
+
+private:
+    pp_ctx *ctx_;
+    std::mt19937 urng;
+    bool shuffled = false, applied = false;
+    pp_cloud *rnd_ = nullptr;
+};
+
 /* CSV writers (S.cpp:4299-4453).  kind: 1 RDF, 2 1NN, 3 RIPK, 4 MKNN.
  * hist holds [lowest, bins[nbins], highest]. */
 bool report_apriori_descrstat2(const std::string &path, int kind, const std::string &metric,
@@ -130,8 +171,20 @@ int32_t pph_write_descrstat2(const char *path, int32_t kind, const char *metric,
                              float rincr, float rmax, const int64_t *hist, uint64_t nions_inside,
                              float volume_inside);
 int32_t pph_write_descrstat3(const char *path, const int64_t *cnts, int64_t len);
+/* SpatStatAdditionalLabelRandomization of the XML text: 1 when it reads 1
+ * (Settings.cpp:496-501), 0 otherwise, -1 for malformed settings. */
+int32_t pph_label_randomization(const char *xml);
+/* std::shuffle of n types with a std::mt19937 seeded `seed` and advanced by
+ * `discard` (the rndlabeler's shuffle); out may equal in. */
+int32_t pph_shuffle_types(uint32_t seed, uint32_t discard, const uint8_t *in, uint64_t n,
+                          uint8_t *out);
+/* count outputs of a std::mt19937 seeded `seed` after `discard` draws. */
+int32_t pph_mt19937(uint32_t seed, uint64_t discard, uint32_t *out, int32_t count);
 /* End to end: settings XML text + type registry + a cloud already uploaded ->
- * GPU computation -> the reference's CSV files in outdir.  Returns pp_status. */
+ * GPU computation -> the reference's CSV files in outdir (solver::
+ * characterize_spatstat, S.cpp:745-776: the original labels, then, when
+ * SpatStatAdditionalLabelRandomization is 1, the randomised ones into
+ * *.Rndmized.csv).  Returns pp_status. */
 int32_t pph_run_spatstat(pp_ctx *ctx, const pp_cloud *cloud, const char *xml, uint32_t simid,
                          int32_t ntypes, const char *const *type_names, const uint32_t *type_ids,
                          uint64_t nions_inside, float volume_inside, const char *outdir);

@@ -125,6 +125,19 @@ pp_status pp_cloud_from_device(pp_ctx *ctx, const float *d_xyz, const uint8_t *d
                                const float *d_ion2surf_sqr, uint64_t n, float cell_width,
                                pp_cloud **out);
 pp_status pp_cloud_free(pp_ctx *ctx, pp_cloud *cloud);
+/* Number of ions of a cloud (the sum over the z-slabs the reference forms in
+ * rndlabeler::learn_original_types_global, S.cpp:2856-2859). */
+pp_status pp_cloud_count(const pp_cloud *cloud, uint64_t *n);
+/* Copies the cloud's ion types (n bytes, upload order) to host memory:
+ * rndlabeler::learn_original_types_global (S.cpp:2850-2880). */
+pp_status pp_cloud_types(pp_ctx *ctx, const pp_cloud *cloud, uint8_t *types);
+/* A new cloud with src's positions and ion2surf (copied on the device) and
+ * the given host types (n bytes, src's upload order), resident with src's
+ * cell widths when src is resident: the relabelled tip of
+ * rndlabeler::apply_shuffled_types_global (S.cpp:2907-2942).  Free it with
+ * pp_cloud_free (rndlabeler::reset_original_types_global, S.cpp:2951-2983). */
+pp_status pp_cloud_relabel(pp_ctx *ctx, const pp_cloud *src, const uint8_t *types,
+                           pp_cloud **out);
 /* Adds one more resident cell list of the given width to a resident cloud;
  * each job then uses the resident width that suits its Rmax best. */
 pp_status pp_cloud_add_cell_list(pp_ctx *ctx, pp_cloud *cloud, float cell_width);

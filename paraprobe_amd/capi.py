@@ -57,6 +57,9 @@ EXPORTED = [
     "pp_serve_destroy",
     "pp_ion2surf",
     "pp_ion2surf_device",
+    "pp_cloud_count",
+    "pp_cloud_types",
+    "pp_cloud_relabel",
 ]
 
 
@@ -146,6 +149,9 @@ def load_library() -> C.CDLL:
     lib.pp_serve_destroy.argtypes = [P]
     lib.pp_ion2surf.argtypes = [P, P, C.c_uint64, P, C.c_uint64, C.c_float, P]
     lib.pp_ion2surf_device.argtypes = [P, P, C.c_uint64, P, C.c_uint64, C.c_float, P]
+    lib.pp_cloud_count.argtypes = [P, C.POINTER(C.c_uint64)]
+    lib.pp_cloud_types.argtypes = [P, P, P]
+    lib.pp_cloud_relabel.argtypes = [P, P, P, C.POINTER(P)]
     for name in EXPORTED:
         if name != "pp_last_error_string":
             getattr(lib, name).restype = C.c_int
@@ -291,6 +297,23 @@ class Cloud:
 
     def add_cell_list(self, width: float):
         self._ctx._check(self._ctx.lib.pp_cloud_add_cell_list(self._ctx.ctx, self.handle, width))
+
+    def types(self) -> np.ndarray:
+        """The cloud's ion types in upload order."""
+        out = np.empty(self.n, np.uint8)
+        self._ctx._check(self._ctx.lib.pp_cloud_types(self._ctx.ctx, self.handle,
+                                                      out.ctypes.data))
+        return out
+
+    def relabel(self, types: np.ndarray) -> "Cloud":
+        """A new cloud with these positions and ion2surf and the given types."""
+        types = np.ascontiguousarray(types, dtype=np.uint8)
+        if types.shape[0] != self.n:
+            raise ValueError("one type per ion")
+        h = C.c_void_p()
+        self._ctx._check(self._ctx.lib.pp_cloud_relabel(self._ctx.ctx, self.handle,
+                                                        types.ctypes.data, C.byref(h)))
+        return Cloud(self._ctx, h, self.n)
 
     def footprint(self) -> int:
         b = C.c_uint64()

@@ -1844,6 +1844,55 @@ pp_status pp_cloud_free(pp_ctx *c, pp_cloud *cl)
     return PP_OK;
 }
 
+pp_status pp_cloud_count(const pp_cloud *cl, uint64_t *n)
+{
+    if (!cl || !n)
+        return PP_ERR_INVALID_ARG;
+    *n = cl->n;
+    return PP_OK;
+}
+
+pp_status pp_cloud_types(pp_ctx *c, const pp_cloud *cl, uint8_t *types)
+{
+    if (!c || !cl || (cl->n > 0 && !types))
+        return fail(c, PP_ERR_INVALID_ARG, "NULL argument");
+    if (cl->n == 0)
+        return PP_OK;
+    PP_HIP(c, hipSetDevice(c->device));
+    PP_HIP(c, hipMemcpyAsync(types, cl->type, cl->n, hipMemcpyDeviceToHost, c->main.stream));
+    PP_HIP(c, hipStreamSynchronize(c->main.stream));
+    return PP_OK;
+}
+
+pp_status pp_cloud_relabel(pp_ctx *c, const pp_cloud *src, const uint8_t *types, pp_cloud **out)
+{
+    if (!c || !src || !out || (src->n > 0 && !types))
+        return fail(c, PP_ERR_INVALID_ARG, "NULL argument");
+    PP_HIP(c, hipSetDevice(c->device));
+    const uint64_t n = src->n;
+    void *d = nullptr;
+    PP_HIP(c, hipMalloc(&d, n + 256));
+    std::unique_ptr<void, void (*)(void *)> own(d, [](void *p) { (void)hipFree(p); });
+    if (n > 0) {
+        PP_HIP(c, hipMemcpyAsync(d, types, n, hipMemcpyHostToDevice, c->main.stream));
+        PP_HIP(c, hipStreamSynchronize(c->main.stream));
+    }
+    const float w = src->resident && !src->lists.empty() ? src->lists[0].w : 0.f;
+    pp_cloud *cl = nullptr;
+    pp_status st = upload_common(c, src->xyz, (const uint8_t *)d, src->s2, n, w,
+                                 hipMemcpyDeviceToDevice, &cl);
+    if (st != PP_OK)
+        return st;
+    if (src->resident)
+        for (size_t i = 1; i < src->lists.size(); ++i)
+            if ((st = pp_cloud_add_cell_list(c, cl, src->lists[i].w)) != PP_OK) {
+                pp_cloud_free(c, cl);
+                return st;
+            }
+    *out = cl;
+    return PP_OK;
+}
+
 pp_status pp_cloud_footprint(const pp_cloud *cl, uint64_t *bytes)
 {
     if (!cl || !bytes)

@@ -13,7 +13,10 @@ from . import capi
 _PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_PKG, "libpp_horderdist.so")
 EXPORTED = ["pph_parse_tasks", "pph_parse_mknn_code", "pph_read_settings",
-            "pph_write_descrstat2", "pph_write_descrstat3", "pph_run_spatstat"]
+            "pph_write_descrstat2", "pph_write_descrstat3", "pph_run_spatstat",
+            "pph_label_randomization", "pph_shuffle_types", "pph_mt19937"]
+RND_SEED = 4294967295     # Settings::RndDescrStatsPRNGSeed, (unsigned int) -1
+RND_DISCARD = 700000      # Settings::RndDescrStatsPRNGDiscard
 _lib = None
 
 
@@ -40,6 +43,12 @@ def load():
         lib.pph_run_spatstat.argtypes = [P, P, C.c_char_p, C.c_uint32, C.c_int32, P, P,
                                          C.c_uint64, C.c_float, C.c_char_p]
         lib.pph_run_spatstat.restype = C.c_int32
+        lib.pph_label_randomization.argtypes = [C.c_char_p]
+        lib.pph_label_randomization.restype = C.c_int32
+        lib.pph_shuffle_types.argtypes = [C.c_uint32, C.c_uint32, P, C.c_uint64, P]
+        lib.pph_shuffle_types.restype = C.c_int32
+        lib.pph_mt19937.argtypes = [C.c_uint32, C.c_uint64, P, C.c_int32]
+        lib.pph_mt19937.restype = C.c_int32
         _lib = lib
     return _lib
 
@@ -106,3 +115,23 @@ def run_spatstat(ctx, cloud, xml: str, simid: int, types: dict, nions_inside: in
     names, ids = _registry(types)
     return lib.pph_run_spatstat(ctx.ctx, cloud.handle, xml.encode(), simid, len(types), names,
                                 ids.ctypes.data, nions_inside, volume_inside, outdir.encode())
+
+
+def label_randomization(xml: str) -> int:
+    return load().pph_label_randomization(xml.encode())
+
+
+def shuffle_types(types, seed: int = RND_SEED, discard: int = RND_DISCARD) -> np.ndarray:
+    """The rndlabeler's std::shuffle of a type array (std::mt19937, seed, discard)."""
+    t = np.ascontiguousarray(types, dtype=np.uint8)
+    out = np.empty_like(t)
+    if load().pph_shuffle_types(seed, discard, t.ctypes.data, t.shape[0], out.ctypes.data):
+        raise ValueError("bad arguments")
+    return out
+
+
+def mt19937(seed: int, discard: int, count: int) -> np.ndarray:
+    out = np.empty(count, np.uint32)
+    if load().pph_mt19937(seed, discard, out.ctypes.data, count):
+        raise ValueError("bad arguments")
+    return out

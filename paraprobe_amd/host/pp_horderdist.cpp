@@ -403,6 +403,76 @@ bool report_apriori_descrstat3(const std::string &path, const int64_t *cnts, int
 }
 
 // ---------------------------------------------------------------------------
+// label randomisation (S.cpp:2809-3000)
+
+RndLabeler::RndLabeler(pp_ctx *ctx, uint32_t seed, uint32_t discard) : ctx_(ctx)
+{
+    urng.seed(seed);
+    urng.discard(discard);
+}
+
+RndLabeler::~RndLabeler() { reset_original_types_global(); }
+
+void RndLabeler::reset()
+{
+    initial_iontypid.clear();
+    temporary_iontypid.clear();
+    shuffled = false;
+    applied = false;
+}
+
+pp_status RndLabeler::learn_original_types_global(const pp_cloud *cloud)
+{
+    if (shuffled) {
+        std::cerr << "WARNING::Refusing to randomize iontypes as they currently are randomized "
+                     "already!"
+                  << std::endl;
+        return PP_OK;
+    }
+    uint64_t n = 0;
+    if (!cloud || pp_cloud_count(cloud, &n) != PP_OK)
+        return PP_ERR_INVALID_ARG;
+    initial_iontypid.assign(n, 0);
+    return pp_cloud_types(ctx_, cloud, initial_iontypid.data());
+}
+
+void RndLabeler::shuffle_types_mt19937()
+{
+    if (shuffled)
+        return;
+    temporary_iontypid = initial_iontypid;
+    std::shuffle(temporary_iontypid.begin(), temporary_iontypid.end(), urng);
+    shuffled = true;
+}
+
+pp_status RndLabeler::apply_shuffled_types_global(const pp_cloud *cloud)
+{
+    if (!shuffled || applied)
+        return PP_OK;
+    uint64_t n = 0;
+    if (!cloud || pp_cloud_count(cloud, &n) != PP_OK)
+        return PP_ERR_INVALID_ARG;
+    if (temporary_iontypid.size() != n) {
+        std::cerr << "WARNING::We have not consistently learned type of all ions, therefore "
+                     "performing no randomization!"
+                  << std::endl;
+        return PP_ERR_INVALID_ARG;
+    }
+    pp_status st = pp_cloud_relabel(ctx_, cloud, temporary_iontypid.data(), &rnd_);
+    if (st == PP_OK)
+        applied = true;
+    return st;
+}
+
+void RndLabeler::reset_original_types_global()
+{
+    if (rnd_)
+        pp_cloud_free(ctx_, rnd_);
+    rnd_ = nullptr;
+    applied = false;
+}
+
+// ---------------------------------------------------------------------------
 // driver
 
 Horderdist::Horderdist(pp_ctx *ctx, const SpatstatSettings &s, const PeriodicTable &pse)
@@ -667,7 +737,56 @@ int32_t pph_run_spatstat(pp_ctx *ctx, const pp_cloud *cloud, const char *xml, ui
     pph::VolumeMeta m;
     m.nions_inside = nions_inside;
     m.volume_inside = volume_inside;
-    return h.compute_generic_spatstat3(cloud, m, outdir ? outdir : "");
+    const std::string dir = outdir ? outdir : "";
+    pp_status st = h.compute_generic_spatstat3(cloud, m, dir);
+    if (st != PP_OK || !s.add_label_rnd)
+        return st;
+    // randomise the labels over the whole tip, characterise again, restore
+    pph::RndLabeler rnd(ctx);
+    rnd.reset();
+    if ((st = rnd.learn_original_types_global(cloud)) != PP_OK)
+        return st;
+    rnd.shuffle_types_mt19937();
+    if ((st = rnd.apply_shuffled_types_global(cloud)) != PP_OK)
+        return st;
+    st = h.compute_generic_spatstat3(rnd.cloud(), m, dir, true);
+    rnd.reset_original_types_global();
+    return st;
+}
+
+int32_t pph_label_randomization(const char *xml)
+{
+    if (!xml)
+        return -1;
+    pph::SpatstatSettings s;
+    std::string why;
+    if (!s.read_xml_text(xml, &why))
+        return -1;
+    return s.add_label_rnd ? 1 : 0;
+}
+
+int32_t pph_shuffle_types(uint32_t seed, uint32_t discard, const uint8_t *in, uint64_t n,
+                          uint8_t *out)
+{
+    if (n > 0 && (!in || !out))
+        return 1;
+    std::mt19937 urng(seed);
+    urng.discard(discard);
+    std::vector<uint8_t> v(in, in + n);
+    std::shuffle(v.begin(), v.end(), urng);
+    std::copy(v.begin(), v.end(), out);
+    return 0;
+}
+
+int32_t pph_mt19937(uint32_t seed, uint64_t discard, uint32_t *out, int32_t count)
+{
+    if (count < 0 || (count > 0 && !out))
+        return 1;
+    std::mt19937 urng(seed);
+    urng.discard(discard);
+    for (int32_t i = 0; i < count; ++i)
+        out[i] = (uint32_t)urng();
+    return 0;
 }
 
 } // extern "C"

@@ -73,3 +73,81 @@ def test_xml_to_csv(ctx, tmp_path):
         last = np.nonzero(want)[0].max() + 1
         np.testing.assert_array_equal(got, want[:last])
     assert len(files) == 4 * (1 + 1 + 3 + 1)
+
+
+def _rdf_from_csv(path, nb):
+    rows = [ln.split(";") for ln in open(path).read().splitlines()]
+    return [int(float(rows[7][1]))] + [int(float(r[1])) for r in rows[8:8 + nb]] + \
+           [int(float(rows[8 + nb][1]))]
+
+
+def test_label_randomisation_rerun(ctx, tmp_path):
+    """SpatStatAdditionalLabelRandomization = 1: the original labels, then the
+    labels shuffled by the rndlabeler's std::mt19937 + std::shuffle in upload
+    order, written to *.Rndmized.csv; the cloud keeps its own labels."""
+    print("test_label_randomisation_rerun", flush=True)
+    rng = np.random.default_rng(21)
+    n = 8000
+    xyz = rng.uniform(0, 9, size=(n, 3)).astype(np.float32)
+    # clustered Sc so that the randomised RDF visibly differs
+    ty = np.where(np.linalg.norm(xyz - 4.5, axis=1) < 2.0, 2, 1).astype(np.uint8)
+    ty[rng.random(n) < 0.05] = 3
+    s2 = rng.uniform(0, 4, size=n).astype(np.float32)
+    xml = XML.replace("</Parameters>",
+                      "  <SpatStatAdditionalLabelRandomization>1"
+                      "</SpatStatAdditionalLabelRandomization>\n</Parameters>")
+    xml = xml.replace("<AnalysisSpatDistrType>1246</AnalysisSpatDistrType>",
+                      "<AnalysisSpatDistrType>14</AnalysisSpatDistrType>")
+    cl = ctx.upload(xyz, ty, s2, cell_width=1.2)
+    try:
+        st = H.run_spatstat(ctx, cl, xml, 9, TYPES, 7000, 700.0, str(tmp_path))
+        assert st == 0
+        assert np.array_equal(cl.types(), ty)
+    finally:
+        cl.free()
+    tasks = [((1,), (1,)), ((2,), (2,)), ((1, 2), (3,)), ((3,), (1, 2))]
+    names = [("Al", "Al"), ("Sc", "Sc"), ("AlSc", "Zr"), ("Zr", "AlSc")]
+    shuffled = H.shuffle_types(ty)
+    ref, _ = O.spatstat(xyz, ty, s2, tasks, 0.0, 0.02, 1.2, 1 | 8, kvalues=[1, 3, 10])
+    rnd, _ = O.spatstat(xyz, shuffled, s2, tasks, 0.0, 0.02, 1.2, 1 | 8, kvalues=[1, 3, 10])
+    nb = ref["rdf"].shape[1] - 2
+    for t, (a, b) in enumerate(names):
+        got = _rdf_from_csv(tmp_path / f"PARAPROBE.SimID.9.RDF.{a}.{b}.Original.csv", nb)
+        np.testing.assert_array_equal(got, ref["rdf"][t])
+        got = _rdf_from_csv(tmp_path / f"PARAPROBE.SimID.9.RDF.{a}.{b}.Rndmized.csv", nb)
+        np.testing.assert_array_equal(got, rnd["rdf"][t])
+        for kth, k in enumerate([10, 3, 1]):
+            f = tmp_path / f"PARAPROBE.SimID.9.MKNN{k}.{a}.{b}.Rndmized.csv"
+            rows = [ln.split(";") for ln in open(f).read().splitlines()]
+            got = [int(float(rows[4][1]))] + [int(float(r[1])) for r in rows[5:5 + nb]] + \
+                  [int(float(rows[5 + nb][1]))]
+            np.testing.assert_array_equal(got, rnd["mknn"][t, kth])
+    assert not np.array_equal(ref["rdf"][1], rnd["rdf"][1])  # Sc-Sc loses its clustering
+    assert len(os.listdir(tmp_path)) == 2 * 4 * (1 + 3)
+
+
+def test_relabelled_cloud_equals_fresh_upload(ctx):
+    print("test_relabelled_cloud_equals_fresh_upload", flush=True)
+    from paraprobe_amd import FAM_1NN, FAM_RDF, Job
+
+    rng = np.random.default_rng(22)
+    n = 20000
+    xyz = rng.uniform(0, 12, size=(n, 3)).astype(np.float32)
+    ty = rng.choice([1, 2, 3], size=n, p=[0.5, 0.3, 0.2]).astype(np.uint8)
+    s2 = rng.uniform(0, 6, size=n).astype(np.float32)
+    new = H.shuffle_types(ty, seed=5)
+    job = Job([((1,), (2,)), ((2, 3), (1, 2, 3))], 0.0, 0.01, 1.5, FAM_RDF | FAM_1NN)
+    for width in (0.0, 1.5):
+        src = ctx.upload(xyz, ty, s2, cell_width=width)
+        if width:
+            src.add_cell_list(0.75)
+        rel = src.relabel(new)
+        fresh = ctx.upload(xyz, new, s2)
+        try:
+            assert np.array_equal(rel.types(), new)
+            a, b = ctx.run(rel, job), ctx.run(fresh, job)
+            for k, v in a.arrays().items():
+                np.testing.assert_array_equal(v, getattr(b, k))
+        finally:
+            for c in (src, rel, fresh):
+                c.free()
