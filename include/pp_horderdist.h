@@ -36,6 +36,25 @@ struct PeriodicTable {
     PeriodicTable();
     void add_iontype(const std::string &name, unsigned id);
     unsigned max_type_id() const;
+    /* Ranging (src/PARAPROBE_PeriodicTable.cpp:65-91, 119-176, 324-420):
+     * mq2ion[type id] holds that type's closed [lo, hi] mass-to-charge ranges;
+     * Unknown (0) has the dummy range [0, 0.001]. */
+    struct MQRange {
+        float lo, hi;
+    };
+    std::vector<std::vector<MQRange>> mq2ion;
+    std::map<unsigned, std::string> ion_names;
+    bool rangefile_loaded = false;
+    /* read_rangefile2: single and molecular ion types from the [Ranges]
+     * lines of an RRNG file, in order of first appearance. */
+    bool read_rangefile2(const std::string &path);
+    bool read_rangefile2_text(const std::string &text);
+    void add_iontype_single_or_molecular(const std::vector<std::string> &tokens, MQRange r);
+    /* mq2ionname: the first type (id ascending) with a range holding mq. */
+    unsigned mq2ionname(float mq) const;
+    /* The ranges in mq2ionname's order, for pp_pos_decode. */
+    void flat_ranges(std::vector<float> &lo, std::vector<float> &hi,
+                     std::vector<uint8_t> &type) const;
 };
 
 /* TypeSpecDescrStat (src/PARAPROBE_PeriodicTable.h:100-127). */
@@ -61,6 +80,10 @@ struct SpatstatSettings {
          do_cntnb = false, add_label_rnd = false;
     std::string tasks_code;
     float rmin = 0.f, rincr = 0.1f, rmax = 2.f;
+    /* SurfaceCellsCarvingRadius (read only with AnalysisVolumeTessellation 1,
+     * Settings.cpp:132, 410-419, 533-536); the surface distance uses
+     * max(rmax, carving_radius) (S.cpp:387). */
+    float carving_radius = 2.f;
     size_t knn_order = 0;
     std::string mknn_code;
     /* Reads <Parameters> children from an XML text (the keys the reference reads). */
@@ -171,6 +194,20 @@ int32_t pph_write_descrstat2(const char *path, int32_t kind, const char *metric,
                              float rincr, float rmax, const int64_t *hist, uint64_t nions_inside,
                              float volume_inside);
 int32_t pph_write_descrstat3(const char *path, const int64_t *cnts, int64_t len);
+/* Reads an RRNG file (read_rangefile2).  Returns the number of ion types
+ * (Unknown included) or -1; writes up to cap ranges in mq2ionname's order
+ * (lo, hi, type) and *nranges, and the type names (names_stride bytes each,
+ * type id order) when names is not NULL. */
+int32_t pph_read_rangefile(const char *path, float *lo, float *hi, uint8_t *type, int32_t cap,
+                           int32_t *nranges, char *names, int32_t names_stride);
+/* End to end from files: settings XML text, a POS file, an RRNG file (NULL:
+ * no ranging) and a triangle hull (ntri = 0: none) -> the POS decoded, ranged
+ * and distanced on the GPU (pp_cloud_from_pos with R = max(SpatStatRadiusMax,
+ * SurfaceCellsCarvingRadius)) -> pph_run_spatstat with the range file's ion
+ * types.  Returns pp_status. */
+int32_t pph_run_pos(pp_ctx *ctx, const char *xml, const char *pos_path, const char *rrng_path,
+                    const float *triangles, uint64_t ntri, uint32_t simid,
+                    uint64_t nions_inside, float volume_inside, const char *outdir);
 /* SpatStatAdditionalLabelRandomization of the XML text: 1 when it reads 1
  * (Settings.cpp:496-501), 0 otherwise, -1 for malformed settings. */
 int32_t pph_label_randomization(const char *xml);

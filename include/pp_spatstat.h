@@ -181,6 +181,39 @@ pp_status pp_ion2surf(pp_ctx *ctx, const float *xyz, uint64_t n, const float *tr
 pp_status pp_ion2surf_device(pp_ctx *ctx, const float *d_xyz, uint64_t n,
                              const float *d_triangles, uint64_t ntri, float R, float *d_out);
 
+/* POS records to positions and ranged ion types on the GPU: replaces the
+ * record conversion of solverHdl::load_pos_sequentially
+ * (S.cpp:6797-6930, src/PARAPROBE_EPOSEndianness.h:45-84: 16-byte records of
+ * four big-endian floats x, y, z, mass-to-charge) and solverHdl::
+ * identify_ions / PeriodicTable::mq2ionname (S.cpp:7340-7438,
+ * src/PARAPROBE_PeriodicTable.cpp:65-91).  nbytes must be a multiple of 16.
+ * Ranges are tested in the given order (the reference's: type id ascending,
+ * Unknown's dummy range [0, 0.001] first); an ion takes the type of the first
+ * closed range [lo, hi] holding its mass-to-charge (a NaN matches the first
+ * range), else type 0.  nranges = 0 (no range file) gives type 0 everywhere.
+ * Outputs: xyz (3 floats per ion), mq (may be NULL), type (1 byte per ion);
+ * all host memory here. */
+pp_status pp_pos_decode(pp_ctx *ctx, const void *pos, uint64_t nbytes, const float *range_lo,
+                        const float *range_hi, const uint8_t *range_type, int32_t nranges,
+                        float *xyz, float *mq, uint8_t *type);
+/* Same with pos, xyz, mq and type in device memory (pos 16-byte aligned);
+ * the ranges stay host arrays.  The outputs can feed pp_ion2surf_device and
+ * pp_cloud_from_device directly. */
+pp_status pp_pos_decode_device(pp_ctx *ctx, const void *d_pos, uint64_t nbytes,
+                               const float *range_lo, const float *range_hi,
+                               const uint8_t *range_type, int32_t nranges, float *d_xyz,
+                               float *d_mq, uint8_t *d_type);
+
+/* POS bytes + ranges + triangle hull -> a cloud, entirely on the device:
+ * pp_pos_decode, then pp_ion2surf with R (an empty hull gives every ion
+ * fl32(R * R), as solver::surface_distancing2 does without a hull,
+ * S.cpp:333-358), then the cloud as pp_cloud_upload would make it.  pos and
+ * triangles are host memory. */
+pp_status pp_cloud_from_pos(pp_ctx *ctx, const void *pos, uint64_t nbytes, const float *range_lo,
+                            const float *range_hi, const uint8_t *range_type, int32_t nranges,
+                            const float *triangles, uint64_t ntri, float R, float cell_width,
+                            pp_cloud **out);
+
 /* kNN range of MKNN / 1NN jobs without CNTNB: per neighbour type, per-center
  * histograms cover only the distance within which `factor` x k neighbours
  * are expected at the cloud's mean density (default 3); centers with fewer

@@ -59,6 +59,7 @@ def _load():
         lib.pp_oracle_closest_tri.argtypes = [P, P]
         lib.pp_oracle_closest_tri.restype = C.c_float
         lib.pp_oracle_ion2surf.argtypes = [P, C.c_int64, P, C.c_int64, C.c_float, P]
+        lib.pp_oracle_pos_decode.argtypes = [P, C.c_int64, P, P, P, C.c_int32, P, P, P]
         _lib = lib
     return _lib
 
@@ -170,3 +171,24 @@ def ion2surf(xyz, triangles, R) -> np.ndarray:
     lib.pp_oracle_ion2surf(xyz.ctypes.data, xyz.shape[0], tri.ctypes.data, tri.shape[0],
                            C.c_float(R), out.ctypes.data)
     return out
+
+
+def pos_decode(pos: bytes, ranges=None):
+    """POS bytes -> (xyz, mq, types) by the reference's byte reversal and
+    first-matching closed range."""
+    lib = _load()
+    buf = np.frombuffer(pos, dtype=np.uint8)
+    n = buf.shape[0] // 16
+    if ranges is None:
+        lo = hi = np.zeros(0, np.float32)
+        rt = np.zeros(0, np.uint8)
+    else:
+        lo, hi, rt = (np.ascontiguousarray(ranges[0], np.float32),
+                      np.ascontiguousarray(ranges[1], np.float32),
+                      np.ascontiguousarray(ranges[2], np.uint8))
+    xyz = np.empty((n, 3), np.float32)
+    mq = np.empty(n, np.float32)
+    ty = np.empty(n, np.uint8)
+    lib.pp_oracle_pos_decode(buf.ctypes.data, n, lo.ctypes.data, hi.ctypes.data, rt.ctypes.data,
+                             lo.shape[0], xyz.ctypes.data, mq.ctypes.data, ty.ctypes.data)
+    return xyz, mq, ty

@@ -659,3 +659,42 @@ void pp_oracle_ion2surf(const float *xyz, int64_t n, const float *tri, int64_t n
     }
     free(box);
 }
+
+/* ------------------------------------------------------------------------
+ * POS records and ranging.
+ *
+ * pp_oracle_pos_decode restates pos(pos_be_2_stl_le)
+ * (src/PARAPROBE_EPOSEndianness.h:45-84: each of the four big-endian floats
+ * of a 16-byte record has its bytes reversed) and PeriodicTable::mq2ionname
+ * (src/PARAPROBE_PeriodicTable.cpp:65-91): the first range in the given order
+ * with !(mq < lo) && !(mq > hi) gives the type, else 0.
+ */
+void pp_oracle_pos_decode(const uint8_t *rec, int64_t n, const float *lo, const float *hi,
+                          const uint8_t *rtype, int32_t nranges, float *xyz, float *mq,
+                          uint8_t *type)
+{
+    for (int64_t i = 0; i < n; ++i) {
+        float v[4];
+        for (int k = 0; k < 4; ++k) {
+            uint8_t b[4];
+            for (int q = 0; q < 4; ++q)
+                b[q] = rec[16 * i + 4 * k + 3 - q];
+            memcpy(&v[k], b, 4);
+        }
+        xyz[3 * i] = v[0];
+        xyz[3 * i + 1] = v[1];
+        xyz[3 * i + 2] = v[2];
+        if (mq)
+            mq[i] = v[3];
+        uint8_t t = 0;
+        for (int32_t r = 0; r < nranges; ++r) {
+            if (v[3] < lo[r])
+                continue;
+            if (v[3] > hi[r])
+                continue;
+            t = rtype[r];
+            break;
+        }
+        type[i] = t;
+    }
+}

@@ -60,6 +60,9 @@ EXPORTED = [
     "pp_cloud_count",
     "pp_cloud_types",
     "pp_cloud_relabel",
+    "pp_pos_decode",
+    "pp_pos_decode_device",
+    "pp_cloud_from_pos",
 ]
 
 
@@ -152,6 +155,10 @@ def load_library() -> C.CDLL:
     lib.pp_cloud_count.argtypes = [P, C.POINTER(C.c_uint64)]
     lib.pp_cloud_types.argtypes = [P, P, P]
     lib.pp_cloud_relabel.argtypes = [P, P, P, C.POINTER(P)]
+    lib.pp_pos_decode.argtypes = [P, P, C.c_uint64, P, P, P, C.c_int32, P, P, P]
+    lib.pp_pos_decode_device.argtypes = [P, P, C.c_uint64, P, P, P, C.c_int32, P, P, P]
+    lib.pp_cloud_from_pos.argtypes = [P, P, C.c_uint64, P, P, P, C.c_int32, P, C.c_uint64,
+                                      C.c_float, C.c_float, C.POINTER(P)]
     for name in EXPORTED:
         if name != "pp_last_error_string":
             getattr(lib, name).restype = C.c_int
@@ -289,6 +296,15 @@ class _Prepared:
         )
 
 
+def _ranges(ranges):
+    if ranges is None:
+        z = np.zeros(1, np.float32)
+        return z[:0], z[:0], np.zeros(0, np.uint8)
+    lo, hi, rt = ranges
+    return (np.ascontiguousarray(lo, dtype=np.float32), np.ascontiguousarray(hi, dtype=np.float32),
+            np.ascontiguousarray(rt, dtype=np.uint8))
+
+
 class Cloud:
     def __init__(self, ctx: "SpatStat", handle: C.c_void_p, n: int):
         self._ctx = ctx
@@ -378,6 +394,45 @@ class SpatStat:
         self._check(self.lib.pp_ion2surf_device(self.ctx, C.c_void_p(d_xyz), n,
                                                 C.c_void_p(d_triangles), ntri, C.c_float(R),
                                                 C.c_void_p(d_out)))
+
+    def pos_decode(self, pos: bytes, ranges=None, with_mq: bool = True):
+        """POS bytes (16 per ion, big-endian x, y, z, mq) -> (xyz, mq, types);
+        ranges = (lo, hi, type) arrays in the reference's order, or None."""
+        buf = np.frombuffer(pos, dtype=np.uint8)
+        n = buf.shape[0] // 16
+        lo, hi, rt = _ranges(ranges)
+        xyz = np.empty((n, 3), np.float32)
+        mq = np.empty(n, np.float32) if with_mq else None
+        ty = np.empty(n, np.uint8)
+        self._check(self.lib.pp_pos_decode(self.ctx, buf.ctypes.data, buf.shape[0],
+                                           lo.ctypes.data, hi.ctypes.data, rt.ctypes.data,
+                                           lo.shape[0], xyz.ctypes.data,
+                                           None if mq is None else mq.ctypes.data,
+                                           ty.ctypes.data))
+        return xyz, mq, ty
+
+    def cloud_from_pos(self, pos: bytes, ranges, triangles, R: float,
+                       cell_width: float = 0.0) -> Cloud:
+        """POS bytes + ranges + hull -> a cloud, decoded and distanced on the GPU."""
+        buf = np.frombuffer(pos, dtype=np.uint8)
+        lo, hi, rt = _ranges(ranges)
+        tri = np.ascontiguousarray(triangles if triangles is not None else
+                                   np.zeros((0, 9), np.float32), dtype=np.float32).reshape(-1, 9)
+        h = C.c_void_p()
+        self._check(self.lib.pp_cloud_from_pos(self.ctx, buf.ctypes.data, buf.shape[0],
+                                               lo.ctypes.data, hi.ctypes.data, rt.ctypes.data,
+                                               lo.shape[0], tri.ctypes.data, tri.shape[0],
+                                               C.c_float(R), C.c_float(cell_width), C.byref(h)))
+        return Cloud(self, h, buf.shape[0] // 16)
+
+    def pos_decode_device(self, d_pos: int, nbytes: int, ranges, d_xyz: int, d_mq: int,
+                          d_type: int):
+        lo, hi, rt = _ranges(ranges)
+        self._check(self.lib.pp_pos_decode_device(self.ctx, C.c_void_p(d_pos), nbytes,
+                                                  lo.ctypes.data, hi.ctypes.data,
+                                                  rt.ctypes.data, lo.shape[0], C.c_void_p(d_xyz),
+                                                  C.c_void_p(d_mq) if d_mq else None,
+                                                  C.c_void_p(d_type)))
 
     def upload(self, xyz: np.ndarray, types: np.ndarray, ion2surf_sqr: np.ndarray,
                cell_width: float = 0.0) -> Cloud:

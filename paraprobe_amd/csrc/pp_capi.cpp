@@ -2116,6 +2116,137 @@ pp_status pp_ion2surf_device(pp_ctx *c, const float *d_xyz, uint64_t n, const fl
     return ion2surf_run(c, d_xyz, n, d_tri, ntri, R, d_out, true);
 }
 
+namespace {
+
+// POS decoding and ranging; dev: pos and the outputs are device memory.
+pp_status pos_decode_run(pp_ctx *c, const void *pos, uint64_t nbytes, const float *lo,
+                         const float *hi, const uint8_t *rt, int32_t nranges, float *xyz,
+                         float *mq, uint8_t *type, bool dev)
+{
+    if (!c)
+        return PP_ERR_INVALID_ARG;
+    if (nbytes % 16 != 0)
+        return fail(c, PP_ERR_INVALID_ARG,
+                    "POS size is not an integer multiple of the 16-byte record");
+    const uint64_t n = nbytes / 16;
+    if (n > 0xFFFFFFFDull)
+        return fail(c, PP_ERR_INVALID_ARG, "more than 2^32 - 3 ions");
+    if (nranges < 0 || nranges > kMaxRanges)
+        return fail(c, PP_ERR_INVALID_ARG, "0 to 4096 ranges");
+    if (nranges > 0 && (!lo || !hi || !rt))
+        return fail(c, PP_ERR_INVALID_ARG, "NULL range arrays");
+    if (n > 0 && (!pos || !xyz || !type))
+        return fail(c, PP_ERR_INVALID_ARG, "NULL input or output arrays");
+    if (dev && ((uintptr_t)pos & 15u))
+        return fail(c, PP_ERR_INVALID_ARG, "device POS buffer must be 16-byte aligned");
+    if (n == 0)
+        return PP_OK;
+    PP_HIP(c, hipSetDevice(c->device));
+    hipStream_t s = c->main.stream;
+    Carver m0(nullptr);
+    m0.take<float>(2 * (size_t)nranges + 1);
+    m0.take<uint8_t>((size_t)nranges + 1);
+    if (!dev) {
+        m0.take<uint32_t>(4 * n);
+        m0.take<float>(3 * n);
+        m0.take<float>(n);
+        m0.take<uint8_t>(n);
+    }
+    void *mem = nullptr;
+    PP_HIP(c, hipMalloc(&mem, m0.off + 256));
+    std::unique_ptr<void, void (*)(void *)> own(mem, [](void *p) { (void)hipFree(p); });
+    Carver m(mem);
+    float *d_rng = m.take<float>(2 * (size_t)nranges + 1);
+    uint8_t *d_rt = m.take<uint8_t>((size_t)nranges + 1);
+    if (nranges > 0) {
+        PP_HIP(c, hipMemcpyAsync(d_rng, lo, 4 * (size_t)nranges, hipMemcpyHostToDevice, s));
+        PP_HIP(c, hipMemcpyAsync(d_rng + nranges, hi, 4 * (size_t)nranges, hipMemcpyHostToDevice,
+                                 s));
+        PP_HIP(c, hipMemcpyAsync(d_rt, rt, (size_t)nranges, hipMemcpyHostToDevice, s));
+    }
+    const uint32_t *d_pos = (const uint32_t *)pos;
+    float *d_xyz = xyz, *d_mq = mq;
+    uint8_t *d_type = type;
+    if (!dev) {
+        uint32_t *p = m.take<uint32_t>(4 * n);
+        d_xyz = m.take<float>(3 * n);
+        float *q = m.take<float>(n);
+        d_type = m.take<uint8_t>(n);
+        d_mq = mq ? q : nullptr;
+        PP_HIP(c, hipMemcpyAsync(p, pos, nbytes, hipMemcpyHostToDevice, s));
+        d_pos = p;
+    }
+    PP_HIP(c, launch_posdecode(d_pos, n, d_rng, d_rng + nranges, d_rt, nranges, d_xyz, d_mq,
+                               d_type, s));
+    if (!dev) {
+        PP_HIP(c, hipMemcpyAsync(xyz, d_xyz, 12 * n, hipMemcpyDeviceToHost, s));
+        if (mq)
+            PP_HIP(c, hipMemcpyAsync(mq, d_mq, 4 * n, hipMemcpyDeviceToHost, s));
+        PP_HIP(c, hipMemcpyAsync(type, d_type, n, hipMemcpyDeviceToHost, s));
+    }
+    PP_HIP(c, hipStreamSynchronize(s));
+    return PP_OK;
+}
+
+} // namespace
+
+pp_status pp_pos_decode(pp_ctx *c, const void *pos, uint64_t nbytes, const float *lo,
+                        const float *hi, const uint8_t *rt, int32_t nranges, float *xyz,
+                        float *mq, uint8_t *type)
+{
+    return pos_decode_run(c, pos, nbytes, lo, hi, rt, nranges, xyz, mq, type, false);
+}
+
+pp_status pp_pos_decode_device(pp_ctx *c, const void *d_pos, uint64_t nbytes, const float *lo,
+                               const float *hi, const uint8_t *rt, int32_t nranges, float *d_xyz,
+                               float *d_mq, uint8_t *d_type)
+{
+    return pos_decode_run(c, d_pos, nbytes, lo, hi, rt, nranges, d_xyz, d_mq, d_type, true);
+}
+
+pp_status pp_cloud_from_pos(pp_ctx *c, const void *pos, uint64_t nbytes, const float *lo,
+                            const float *hi, const uint8_t *rt, int32_t nranges,
+                            const float *tri, uint64_t ntri, float R, float cell_width,
+                            pp_cloud **out)
+{
+    if (!c || !out || (nbytes > 0 && !pos) || (ntri > 0 && !tri))
+        return fail(c, PP_ERR_INVALID_ARG, "NULL argument");
+    if (nbytes % 16 != 0)
+        return fail(c, PP_ERR_INVALID_ARG,
+                    "POS size is not an integer multiple of the 16-byte record");
+    if (ntri >= 0xFFFFFFF0ull)
+        return fail(c, PP_ERR_INVALID_ARG, "more than 2^32 - 16 triangles");
+    const uint64_t n = nbytes / 16;
+    PP_HIP(c, hipSetDevice(c->device));
+    hipStream_t s = c->main.stream;
+    Carver m0(nullptr);
+    m0.take<uint32_t>(4 * n + 4);
+    m0.take<float>(3 * n + 1);
+    m0.take<uint8_t>(n + 1);
+    m0.take<float>(n + 1);
+    m0.take<float>(9 * ntri + 1);
+    void *mem = nullptr;
+    PP_HIP(c, hipMalloc(&mem, m0.off + 256));
+    std::unique_ptr<void, void (*)(void *)> own(mem, [](void *p) { (void)hipFree(p); });
+    Carver m(mem);
+    uint32_t *d_pos = m.take<uint32_t>(4 * n + 4);
+    float *d_xyz = m.take<float>(3 * n + 1);
+    uint8_t *d_type = m.take<uint8_t>(n + 1);
+    float *d_s2 = m.take<float>(n + 1);
+    float *d_tri = m.take<float>(9 * ntri + 1);
+    if (n > 0)
+        PP_HIP(c, hipMemcpyAsync(d_pos, pos, nbytes, hipMemcpyHostToDevice, s));
+    if (ntri > 0)
+        PP_HIP(c, hipMemcpyAsync(d_tri, tri, 36 * ntri, hipMemcpyHostToDevice, s));
+    pp_status st = pos_decode_run(c, d_pos, nbytes, lo, hi, rt, nranges, d_xyz, nullptr, d_type,
+                                  true);
+    if (st != PP_OK)
+        return st;
+    if ((st = ion2surf_run(c, d_xyz, n, d_tri, ntri, R, d_s2, true)) != PP_OK)
+        return st;
+    return upload_common(c, d_xyz, d_type, d_s2, n, cell_width, hipMemcpyDeviceToDevice, out);
+}
+
 pp_status pp_set_knn_range(pp_ctx *c, float factor)
 {
     if (!c)

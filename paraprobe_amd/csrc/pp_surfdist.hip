@@ -230,7 +230,6 @@ __global__ __launch_bounds__(256) void k_surftile(const float4 *__restrict__ ion
             s_r[tid][5] = make_float4(v8 - v2, 0.f, 0.f, 0.f);
         }
         __syncthreads();
-#pragma unroll 2
         for (uint32_t j = 0; j < m; ++j) {
             const float4 r0 = s_r[j][0], r1 = s_r[j][1];
             const float gx = fmaxf(fabsf(px - r0.x) - r0.w, 0.f);

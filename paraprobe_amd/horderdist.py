@@ -14,7 +14,8 @@ _PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_PKG, "libpp_horderdist.so")
 EXPORTED = ["pph_parse_tasks", "pph_parse_mknn_code", "pph_read_settings",
             "pph_write_descrstat2", "pph_write_descrstat3", "pph_run_spatstat",
-            "pph_label_randomization", "pph_shuffle_types", "pph_mt19937"]
+            "pph_label_randomization", "pph_shuffle_types", "pph_mt19937",
+            "pph_read_rangefile", "pph_run_pos"]
 RND_SEED = 4294967295     # Settings::RndDescrStatsPRNGSeed, (unsigned int) -1
 RND_DISCARD = 700000      # Settings::RndDescrStatsPRNGDiscard
 _lib = None
@@ -49,6 +50,12 @@ def load():
         lib.pph_shuffle_types.restype = C.c_int32
         lib.pph_mt19937.argtypes = [C.c_uint32, C.c_uint64, P, C.c_int32]
         lib.pph_mt19937.restype = C.c_int32
+        lib.pph_read_rangefile.argtypes = [C.c_char_p, P, P, P, C.c_int32,
+                                           C.POINTER(C.c_int32), P, C.c_int32]
+        lib.pph_read_rangefile.restype = C.c_int32
+        lib.pph_run_pos.argtypes = [P, C.c_char_p, C.c_char_p, C.c_char_p, P, C.c_uint64,
+                                    C.c_uint32, C.c_uint64, C.c_float, C.c_char_p]
+        lib.pph_run_pos.restype = C.c_int32
         _lib = lib
     return _lib
 
@@ -135,3 +142,31 @@ def mt19937(seed: int, discard: int, count: int) -> np.ndarray:
     if load().pph_mt19937(seed, discard, out.ctypes.data, count):
         raise ValueError("bad arguments")
     return out
+
+
+def read_rangefile(path: str, cap: int = 4096):
+    """RRNG file -> (type names by id, (lo, hi, type) ranges in ranging order)."""
+    lib = load()
+    lo = np.empty(cap, np.float32)
+    hi = np.empty(cap, np.float32)
+    ty = np.empty(cap, np.uint8)
+    nr = C.c_int32(0)
+    stride = 64
+    names = C.create_string_buffer(256 * stride)
+    nt = lib.pph_read_rangefile(path.encode(), lo.ctypes.data, hi.ctypes.data, ty.ctypes.data,
+                                cap, C.byref(nr), names, stride)
+    if nt < 0:
+        raise ValueError(f"cannot read range file {path}")
+    raw = names.raw
+    nm = [raw[i * stride:(i + 1) * stride].split(b"\0", 1)[0].decode() for i in range(nt)]
+    k = min(nr.value, cap)
+    return nm, (lo[:k].copy(), hi[:k].copy(), ty[:k].copy())
+
+
+def run_pos(ctx, xml: str, pos_path: str, rrng_path, triangles, simid: int,
+            nions_inside: int, volume_inside: float, outdir: str) -> int:
+    tri = np.ascontiguousarray(triangles if triangles is not None else
+                               np.zeros((0, 9), np.float32), dtype=np.float32).reshape(-1, 9)
+    return load().pph_run_pos(ctx.ctx, xml.encode(), pos_path.encode(),
+                              rrng_path.encode() if rrng_path else None, tri.ctypes.data,
+                              tri.shape[0], simid, nions_inside, volume_inside, outdir.encode())

@@ -97,6 +97,133 @@ unsigned PeriodicTable::max_type_id() const
     return m;
 }
 
+// ---------------------------------------------------------------------------
+// ranging (src/PARAPROBE_PeriodicTable.cpp)
+
+namespace {
+
+// boost::split(tokens, line, boost::is_any_of(" \t")): every delimiter ends a
+// token, so runs of delimiters give empty tokens
+std::vector<std::string> split_blank_tab(const std::string &line)
+{
+    std::vector<std::string> out(1);
+    for (char ch : line) {
+        if (ch == ' ' || ch == '\t')
+            out.emplace_back();
+        else
+            out.back().push_back(ch);
+    }
+    return out;
+}
+
+} // namespace
+
+void PeriodicTable::add_iontype_single_or_molecular(const std::vector<std::string> &in, MQRange r)
+{
+    // element names found in the tokens between "Vol:..." and the last one,
+    // concatenated in the table's key order; one-letter names need "X:"
+    std::string cand;
+    for (const auto &kv : nuclids) {
+        const std::string &key = kv.first;
+        const bool single = key.size() == 1;
+        for (size_t t = 3; t + 1 < in.size(); ++t) {
+            const bool hit = single ? in[t].find(key + ":") != std::string::npos
+                                    : in[t].find(key) != std::string::npos;
+            if (hit) {
+                cand += key;
+                break;
+            }
+        }
+    }
+    auto it = ion_types.find(cand);
+    if (it == ion_types.end()) {
+        const unsigned id = (unsigned)ion_types.size();
+        ion_types[cand] = id;
+        ion_names[id] = cand;
+        if (mq2ion.size() <= id)
+            mq2ion.resize(id + 1);
+        mq2ion[id].push_back(r);
+    } else {
+        if (mq2ion.size() <= it->second)
+            mq2ion.resize(it->second + 1);
+        mq2ion[it->second].push_back(r);
+    }
+}
+
+bool PeriodicTable::read_rangefile2_text(const std::string &text)
+{
+    ion_types.clear();
+    ion_names.clear();
+    mq2ion.assign(1, std::vector<MQRange>{{0.0f, (float)0.001}});  // Unknown's dummy range
+    ion_types["Unknown"] = 0;
+    ion_names[0] = "Unknown";
+    std::istringstream f(text);
+    std::string ln;
+    while (f.good()) {
+        std::getline(f, ln);
+        if (ln.find("[Ranges]") != std::string::npos)
+            break;
+    }
+    while (f.good()) {
+        std::getline(f, ln);
+        const std::vector<std::string> tk = split_blank_tab(ln);
+        if (tk.size() >= 5 && tk[0].find("Range") != std::string::npos &&
+            tk[2].find("Vol") != std::string::npos) {
+            std::istringstream line(ln);
+            std::string piece;
+            std::getline(line, piece, '=');
+            std::getline(line, piece, ' ');
+            const float lo = std::stof(piece);
+            std::getline(line, piece, ' ');
+            const float hi = std::stof(piece);
+            add_iontype_single_or_molecular(tk, MQRange{lo, hi});
+        }
+    }
+    for (auto &v : mq2ion)
+        std::stable_sort(v.begin(), v.end(),
+                         [](const MQRange &a, const MQRange &b) { return a.hi < b.lo; });
+    rangefile_loaded = true;
+    return true;
+}
+
+bool PeriodicTable::read_rangefile2(const std::string &path)
+{
+    std::ifstream f(path);
+    if (!f.is_open()) {
+        rangefile_loaded = false;
+        return false;
+    }
+    std::stringstream ss;
+    ss << f.rdbuf();
+    return read_rangefile2_text(ss.str());
+}
+
+unsigned PeriodicTable::mq2ionname(float mq) const
+{
+    if (rangefile_loaded)
+        for (size_t i = 0; i < mq2ion.size(); ++i)
+            for (const MQRange &r : mq2ion[i])
+                if (!(mq < r.lo) && !(mq > r.hi))
+                    return (unsigned)i;
+    return 0;
+}
+
+void PeriodicTable::flat_ranges(std::vector<float> &lo, std::vector<float> &hi,
+                                std::vector<uint8_t> &type) const
+{
+    lo.clear();
+    hi.clear();
+    type.clear();
+    if (!rangefile_loaded)
+        return;
+    for (size_t i = 0; i < mq2ion.size(); ++i)
+        for (const MQRange &r : mq2ion[i]) {
+            lo.push_back(r.lo);
+            hi.push_back(r.hi);
+            type.push_back((uint8_t)i);
+        }
+}
+
 bool TypeSpecDescrStat::define_iontask4(const std::string &command, const PeriodicTable &pse)
 {
     if (command.empty())
@@ -262,6 +389,11 @@ bool SpatstatSettings::read_xml_text(const std::string &xml, std::string *why)
             if ((it = kv.find("SpatStatAdditionalLabelRandomization")) != kv.end())
                 add_label_rnd = std::stoul(it->second) == 1;
         }
+        // the carving radius is read only for a Voronoi tessellation run
+        if ((it = kv.find("AnalysisVolumeTessellation")) != kv.end() &&
+            std::stoul(it->second) == 1 &&
+            (it = kv.find("SurfaceCellsCarvingRadius")) != kv.end())
+            carving_radius = std::stof(it->second);
     } catch (const std::exception &e) {
         if (why)
             *why = std::string("malformed setting: ") + e.what();
@@ -751,6 +883,100 @@ int32_t pph_run_spatstat(pp_ctx *ctx, const pp_cloud *cloud, const char *xml, ui
         return st;
     st = h.compute_generic_spatstat3(rnd.cloud(), m, dir, true);
     rnd.reset_original_types_global();
+    return st;
+}
+
+int32_t pph_read_rangefile(const char *path, float *lo, float *hi, uint8_t *type, int32_t cap,
+                           int32_t *nranges, char *names, int32_t names_stride)
+{
+    if (!path)
+        return -1;
+    pph::PeriodicTable pse;
+    try {
+        if (!pse.read_rangefile2(path))
+            return -1;
+    } catch (const std::exception &) {
+        return -1;
+    }
+    std::vector<float> l, h;
+    std::vector<uint8_t> t;
+    pse.flat_ranges(l, h, t);
+    const int32_t nr = (int32_t)l.size();
+    if (nranges)
+        *nranges = nr;
+    for (int32_t r = 0; r < std::min(nr, cap); ++r) {
+        if (lo)
+            lo[r] = l[r];
+        if (hi)
+            hi[r] = h[r];
+        if (type)
+            type[r] = t[r];
+    }
+    const int32_t ntypes = (int32_t)pse.ion_names.size();
+    if (names && names_stride > 0)
+        for (const auto &kv : pse.ion_names) {
+            char *dst = names + (size_t)kv.first * names_stride;
+            std::strncpy(dst, kv.second.c_str(), names_stride - 1);
+            dst[names_stride - 1] = 0;
+        }
+    return ntypes;
+}
+
+int32_t pph_run_pos(pp_ctx *ctx, const char *xml, const char *pos_path, const char *rrng_path,
+                    const float *triangles, uint64_t ntri, uint32_t simid, uint64_t nions_inside,
+                    float volume_inside, const char *outdir)
+{
+    if (!ctx || !xml || !pos_path)
+        return PP_ERR_INVALID_ARG;
+    pph::SpatstatSettings s;
+    std::string why;
+    if (!s.read_xml_text(xml, &why) || !s.check(&why)) {
+        std::cerr << "ERROR::" << why << std::endl;
+        return PP_ERR_INVALID_ARG;
+    }
+    // load_pos_sequentially (S.cpp:6797-6930): the whole file, 16 bytes a record
+    std::ifstream f(pos_path, std::ios::binary);
+    if (!f.is_open()) {
+        std::cerr << "ERROR::File " << pos_path << " cannot be read!" << std::endl;
+        return PP_ERR_INVALID_ARG;
+    }
+    std::vector<char> bytes((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+    if (bytes.size() % 16 != 0) {
+        std::cerr << "ERROR::Size of file " << pos_path
+                  << " is not an integer multiple of expected datatype" << std::endl;
+        return PP_ERR_INVALID_ARG;
+    }
+    // identify_ions (S.cpp:7340-7438): a range file, or every ion Unknown
+    pph::PeriodicTable pse;
+    try {
+        if (rrng_path && !pse.read_rangefile2(rrng_path))
+            std::cerr << "WARNING::Unable to load RRNG file " << rrng_path << std::endl;
+    } catch (const std::exception &e) {
+        std::cerr << "ERROR::malformed range file: " << e.what() << std::endl;
+        return PP_ERR_INVALID_ARG;
+    }
+    if (pse.ion_types.size() > 256) {
+        std::cerr << "ERROR::more than 256 ion types" << std::endl;
+        return PP_ERR_UNSUPPORTED;
+    }
+    std::vector<float> lo, hi;
+    std::vector<uint8_t> rt;
+    pse.flat_ranges(lo, hi, rt);
+    const float R = std::max(s.rmax, s.carving_radius);
+    pp_cloud *cl = nullptr;
+    pp_status st = pp_cloud_from_pos(ctx, bytes.data(), bytes.size(), lo.data(), hi.data(),
+                                     rt.data(), (int32_t)lo.size(), triangles, ntri, R, 0.f, &cl);
+    if (st != PP_OK)
+        return st;
+    std::vector<const char *> names;
+    std::vector<uint32_t> ids;
+    for (const auto &kv : pse.ion_types) {
+        names.push_back(kv.first.c_str());
+        ids.push_back(kv.second);
+    }
+    st = (pp_status)pph_run_spatstat(ctx, cl, xml, simid, (int32_t)names.size(), names.data(),
+                                     ids.data(), nions_inside, volume_inside, outdir);
+    pp_cloud_free(ctx, cl);
     return st;
 }
 
