@@ -296,6 +296,72 @@ def surf_leg(ctx, tip, hip, args):
     return out
 
 
+POS_RANGES = (  # Unknown's dummy range, then Al, Fe, Si, Sc as in a four-range RRNG file
+    np.array([0.0, 26.7, 55.9, 27.8, 44.8], np.float32),
+    np.array([0.001, 26.9, 56.0, 28.0, 50.0], np.float32),
+    np.array([0, 1, 2, 3, 4], np.uint8),
+)
+
+
+def pos_leg(ctx, tip, hip, args):
+    """POS records -> positions, mass-to-charge and ranged types: the 20M ions
+    as big-endian 16-byte records resident in HBM, decoded into device
+    buffers; the reference's sequential per-record loop (byte reversal +
+    mq2ionname) timed on one host core."""
+    mq = np.select([tip.types == 1, tip.types == 2], [26.8, 55.95], 47.0).astype(np.float32)
+    rec = np.concatenate([tip.xyz, mq[:, None]], axis=1).astype(">f4").tobytes()
+    n = tip.n
+    ptr = [ctypes.c_void_p() for _ in range(4)]
+    sizes = [16 * n, 12 * n, 4 * n, n]
+    for p_, b in zip(ptr, sizes):
+        if hip.hipMalloc(ctypes.byref(p_), ctypes.c_size_t(b)) != 0:
+            raise SystemExit("bench: hipMalloc failed")
+    try:
+        src = np.frombuffer(rec, np.uint8)
+        hip.hipMemcpy(ptr[0], src.ctypes.data_as(ctypes.c_void_p), ctypes.c_size_t(sizes[0]), 1)
+        args_ = (ptr[0].value, 16 * n, POS_RANGES, ptr[1].value, ptr[2].value, ptr[3].value)
+        ctx.pos_decode_device(*args_)
+        hip.hipDeviceSynchronize()
+        reps = max(5, args.steps)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            ctx.pos_decode_device(*args_)
+        hip.hipDeviceSynchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / reps
+        ty = np.empty(n, np.uint8)
+        hip.hipMemcpy(ty.ctypes.data_as(ctypes.c_void_p), ptr[3], ctypes.c_size_t(n), 2)
+    finally:
+        for p_ in ptr:
+            hip.hipFree(p_)
+    gbs = 33.0 * n / (ms / 1e3) / 1e9
+    out = {
+        "workload": "frustum-20M ions as POS records (320 MB) -> xyz, mq, type; 5 ranges",
+        "ions_per_s": n / (ms / 1e3),
+        "ms": ms,
+        "gb_per_s": gbs,
+        "roofline_frac_hbm": gbs / 8000.0,
+        "bytes_per_ion": 33,
+    }
+    print(f"# pos decode: {ms:.2f} ms, {gbs:.0f} GB/s", file=sys.stderr, flush=True)
+    if not args.no_cpu:
+        from oracle import oracle as O
+
+        m = n
+        t0 = time.perf_counter()
+        _, _, oty = O.pos_decode(rec[:16 * m], POS_RANGES)
+        cs = time.perf_counter() - t0
+        out["cpu_baseline"] = {
+            "value": m / cs,
+            "unit": "ions/s",
+            "cores": 1,
+            "kind": "port",
+            "sample": f"all {m} records, the reference's sequential loop (byte reversal, "
+                      f"first matching range) in C, {cs:.2f} s",
+            "parity_with_gpu": bool(np.array_equal(oty, ty[:m])),
+        }
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -310,6 +376,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-surf", action="store_true")
+    ap.add_argument("--no-pos", action="store_true")
     ap.add_argument("--batch-jobs", type=int, default=256)
     ap.add_argument("--autotune", type=int, default=1)
     ap.add_argument("--serve-requests", type=int, default=400)
@@ -370,6 +437,8 @@ def main():
         extra = {}
         if not args.no_surf:
             extra["ion2surf"] = surf_leg(ctx, tip, hip, args)
+        if not args.no_pos:
+            extra["pos_decode"] = pos_leg(ctx, tip, hip, args)
         if not args.no_batch:
             extra["batch"] = batch_leg(ctx, args)
         if not args.no_serve:
