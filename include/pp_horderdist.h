@@ -112,6 +112,9 @@ public:
     const std::vector<TypeSpecDescrStat> &tasks() const { return spatstat_tasks; }
     pp_hist_out last;                       /* statistics of the last run */
     std::vector<int64_t> rdf, nn1, ripk, mknn, cntnb;
+    /* NPCORR counts [task][edge^3] (x fastest) of the last run, edge = npcorr_edge */
+    std::vector<uint32_t> npcorr;
+    int64_t npcorr_edge = 0;
 
 private:
     pp_ctx *ctx_;

@@ -50,7 +50,7 @@ typedef enum pp_status {
 #define PP_FAM_1NN (1u << 1)
 #define PP_FAM_RIPK (1u << 2)
 #define PP_FAM_MKNN (1u << 3)
-#define PP_FAM_NPCORR (1u << 4) /* not provided: returns PP_ERR_UNSUPPORTED */
+#define PP_FAM_NPCORR (1u << 4) /* through pp_npcorr_run, not pp_spatstat_run */
 #define PP_FAM_CNTNB (1u << 5)
 
 typedef struct pp_ctx pp_ctx;
@@ -180,6 +180,26 @@ pp_status pp_ion2surf(pp_ctx *ctx, const float *xyz, uint64_t n, const float *tr
  * and d_out stay on the GPU, so the result can feed pp_cloud_from_device. */
 pp_status pp_ion2surf_device(pp_ctx *ctx, const float *d_xyz, uint64_t n,
                              const float *d_triangles, uint64_t ntri, float R, float *d_out);
+
+/* NPCORR, the n-point correlation family (AnalysisSpatDistrType digit 5;
+ * S.cpp:3899-3960, npc3d, src/PARAPROBE_NPointCorrelations3D.cpp:50-207).
+ * pp_npcorr_geometry gives the edge length of its cubic histogram of
+ * npc3d(rmax, rincr): 1 + 2 (ceilf(rmax / rincr) + 1), or -1 when edge^3
+ * exceeds 503^3 (the reference then skips the statistics).
+ * pp_npcorr_run counts, per task, the difference vector (neighbour - center)
+ * of the knn_order-th (0-based, SpatStatKNNOrder) nearest env-type neighbour
+ * of every considered center of a central type that has more than knn_order
+ * neighbours of any type within rmax, binned to the nearest cell (halves away
+ * from zero) when its length is <= rmax; counts is [ntasks][edge^3] uint32
+ * (npc3d::cnts), x fastest.  Equal distances are ordered by the smaller
+ * (u, v, w) (the reference leaves them to nth_element over its KD-tree
+ * order).  knn_order must be below 512.  *centers_binned (may be NULL)
+ * receives the number of counts added. */
+int64_t pp_npcorr_geometry(float rmax, float rincr);
+pp_status pp_npcorr_run(pp_ctx *ctx, const pp_cloud *cloud, int32_t ntasks,
+                        const uint8_t *trg_mask, const uint8_t *env_mask, float rmax,
+                        float rincr, int64_t knn_order, uint32_t *counts,
+                        int64_t *centers_binned);
 
 /* POS records to positions and ranged ion types on the GPU: replaces the
  * record conversion of solverHdl::load_pos_sequentially

@@ -60,6 +60,9 @@ def _load():
         lib.pp_oracle_closest_tri.restype = C.c_float
         lib.pp_oracle_ion2surf.argtypes = [P, C.c_int64, P, C.c_int64, C.c_float, P]
         lib.pp_oracle_pos_decode.argtypes = [P, C.c_int64, P, P, P, C.c_int32, P, P, P]
+        lib.pp_oracle_npcorr.argtypes = [C.c_int64, P, P, P, C.c_int32, P, P, C.c_float,
+                                         C.c_float, C.c_int64, P]
+        lib.pp_oracle_npcorr.restype = C.c_int
         _lib = lib
     return _lib
 
@@ -192,3 +195,25 @@ def pos_decode(pos: bytes, ranges=None):
     lib.pp_oracle_pos_decode(buf.ctypes.data, n, lo.ctypes.data, hi.ctypes.data, rt.ctypes.data,
                              lo.shape[0], xyz.ctypes.data, mq.ctypes.data, ty.ctypes.data)
     return xyz, mq, ty
+
+
+def npcorr(xyz, types, ion2surf, tasks, rmax, rincr, knn_order):
+    """NPCORR counts [ntasks][edge][edge][edge] (z, y, x) by brute force."""
+    lib = _load()
+    xyz = np.ascontiguousarray(xyz, dtype=np.float32).reshape(-1, 3)
+    types = np.ascontiguousarray(types, dtype=np.uint8)
+    s2 = np.ascontiguousarray(ion2surf, dtype=np.float32)
+    T = len(tasks)
+    trg = np.zeros((max(T, 1), 32), np.uint8)
+    env = np.zeros((max(T, 1), 32), np.uint8)
+    for i, (t, e) in enumerate(tasks):
+        trg[i], env[i] = _mask(t), _mask(e)
+    half = int(np.ceil(np.float32(rmax) / np.float32(rincr)) + np.float32(1.0))
+    edge = 1 + 2 * half
+    out = np.zeros((T, edge, edge, edge), np.uint32)
+    st = lib.pp_oracle_npcorr(xyz.shape[0], xyz.ctypes.data, types.ctypes.data, s2.ctypes.data,
+                              T, trg.ctypes.data, env.ctypes.data, rmax, rincr, knn_order,
+                              out.ctypes.data)
+    if st != 0:
+        raise MemoryError("oracle npcorr")
+    return out

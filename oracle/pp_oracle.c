@@ -698,3 +698,97 @@ void pp_oracle_pos_decode(const uint8_t *rec, int64_t n, const float *lo, const 
         type[i] = t;
     }
 }
+
+/* ------------------------------------------------------------------------
+ * NPCORR (the n-point correlation family, S.cpp:3899-3960, npc3d::add
+ * src/PARAPROBE_NPointCorrelations3D.cpp:165-207), by brute force.
+ *
+ * For every considered center (ion2surf >= fl32(R*R)) and every task whose
+ * central types hold its type: when more than K ions of any type lie within R
+ * (self excluded), the env-type neighbours within R are ordered by
+ * d = sqrtf((u*u + v*v) + w*w), (u, v, w) = neighbour - center, ties by the
+ * smaller u, then v, then w (the reference leaves ties to nth_element); the
+ * K-th one is binned when d <= R: per component t = u * (1.f / width), index
+ * (int)(t + 0.5) or (int)(t - 0.5) in double, plus half.  counts is
+ * [ntasks][edge^3], x fastest.
+ */
+typedef struct {
+    float d, u, v, w;
+} onpc;
+
+static int onpc_cmp(const void *pa, const void *pb)
+{
+    const onpc *a = (const onpc *)pa, *b = (const onpc *)pb;
+    if (a->d != b->d)
+        return a->d < b->d ? -1 : 1;
+    if (a->u != b->u)
+        return a->u < b->u ? -1 : 1;
+    if (a->v != b->v)
+        return a->v < b->v ? -1 : 1;
+    if (a->w != b->w)
+        return a->w < b->w ? -1 : 1;
+    return 0;
+}
+
+int pp_oracle_npcorr(int64_t n, const float *xyz, const uint8_t *type, const float *ion2surf,
+                     int32_t ntasks, const uint8_t *trg, const uint8_t *env, float rmax,
+                     float rincr, int64_t K, uint32_t *counts)
+{
+    const float realhalf = ceilf(rmax / rincr) + 1.f;
+    const int64_t half = (int64_t)realhalf, edge = 1 + 2 * half;
+    const int64_t nxyz = edge * edge * edge;
+    const float R2 = rmax * rmax, inv = 1.f / rincr;
+    memset(counts, 0, sizeof(uint32_t) * (size_t)(ntasks * nxyz));
+    onpc *buf = (onpc *)malloc(sizeof(onpc) * (size_t)(n > 0 ? n : 1));
+    if (!buf)
+        return -1;
+    for (int64_t i = 0; i < n; ++i) {
+        const float *p = xyz + 3 * i;
+        if (!(ion2surf[i] >= R2))
+            continue;
+        int64_t ntot = 0;
+        for (int64_t j = 0; j < n; ++j) {
+            if (j == i)
+                continue;
+            const float dx = p[0] - xyz[3 * j], dy = p[1] - xyz[3 * j + 1], dz = p[2] - xyz[3 * j + 2];
+            const float d2 = dx * dx + dy * dy + dz * dz;
+            ntot += d2 <= R2;
+        }
+        if (ntot <= K)
+            continue;
+        for (int32_t t = 0; t < ntasks; ++t) {
+            if (!((trg[32 * t + type[i] / 8] >> (type[i] % 8)) & 1))
+                continue;
+            int64_t m = 0;
+            for (int64_t j = 0; j < n; ++j) {
+                if (j == i || !((env[32 * t + type[j] / 8] >> (type[j] % 8)) & 1))
+                    continue;
+                const float dx = p[0] - xyz[3 * j], dy = p[1] - xyz[3 * j + 1], dz = p[2] - xyz[3 * j + 2];
+                const float d2 = dx * dx + dy * dy + dz * dz;
+                if (!(d2 <= R2))
+                    continue;
+                onpc e;
+                e.u = xyz[3 * j] - p[0];
+                e.v = xyz[3 * j + 1] - p[1];
+                e.w = xyz[3 * j + 2] - p[2];
+                e.d = sqrtf(e.u * e.u + e.v * e.v + e.w * e.w);
+                buf[m++] = e;
+            }
+            if (m <= K)
+                continue;
+            qsort(buf, (size_t)m, sizeof(onpc), onpc_cmp);
+            const onpc k = buf[K];
+            if (!(k.d <= rmax))
+                continue;
+            const float c[3] = {k.u * inv, k.v * inv, k.w * inv};
+            int64_t ix[3];
+            for (int q = 0; q < 3; ++q) {
+                const int disc = (c[q] >= 0.f) ? (int)((double)c[q] + 0.5) : (int)((double)c[q] - 0.5);
+                ix[q] = disc + half;
+            }
+            counts[t * nxyz + ix[0] + ix[1] * edge + ix[2] * edge * edge] += 1;
+        }
+    }
+    free(buf);
+    return 0;
+}

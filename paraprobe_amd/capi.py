@@ -63,6 +63,8 @@ EXPORTED = [
     "pp_pos_decode",
     "pp_pos_decode_device",
     "pp_cloud_from_pos",
+    "pp_npcorr_geometry",
+    "pp_npcorr_run",
 ]
 
 
@@ -157,6 +159,10 @@ def load_library() -> C.CDLL:
     lib.pp_cloud_relabel.argtypes = [P, P, P, C.POINTER(P)]
     lib.pp_pos_decode.argtypes = [P, P, C.c_uint64, P, P, P, C.c_int32, P, P, P]
     lib.pp_pos_decode_device.argtypes = [P, P, C.c_uint64, P, P, P, C.c_int32, P, P, P]
+    lib.pp_npcorr_geometry.argtypes = [C.c_float, C.c_float]
+    lib.pp_npcorr_geometry.restype = C.c_int64
+    lib.pp_npcorr_run.argtypes = [P, P, C.c_int32, P, P, C.c_float, C.c_float, C.c_int64, P,
+                                  C.POINTER(C.c_int64)]
     lib.pp_cloud_from_pos.argtypes = [P, P, C.c_uint64, P, P, P, C.c_int32, P, C.c_uint64,
                                       C.c_float, C.c_float, C.POINTER(P)]
     for name in EXPORTED:
@@ -394,6 +400,23 @@ class SpatStat:
         self._check(self.lib.pp_ion2surf_device(self.ctx, C.c_void_p(d_xyz), n,
                                                 C.c_void_p(d_triangles), ntri, C.c_float(R),
                                                 C.c_void_p(d_out)))
+
+    def npcorr(self, cloud: Cloud, tasks, rmax: float, rincr: float, knn_order: int):
+        """NPCORR counts [ntasks][edge][edge][edge] (z, y, x) uint32."""
+        edge = self.lib.pp_npcorr_geometry(rmax, rincr)
+        if edge < 0:
+            raise PPError(6, "NPCORR histogram too large")
+        T = len(tasks)
+        trg = np.zeros((max(T, 1), 32), np.uint8)
+        env = np.zeros((max(T, 1), 32), np.uint8)
+        for i, (t, e) in enumerate(tasks):
+            trg[i], env[i] = type_mask(t), type_mask(e)
+        out = np.zeros((T, edge, edge, edge), np.uint32)
+        nb = C.c_int64(0)
+        self._check(self.lib.pp_npcorr_run(self.ctx, cloud.handle, T, trg.ctypes.data,
+                                           env.ctypes.data, rmax, rincr, knn_order,
+                                           out.ctypes.data, C.byref(nb)))
+        return out
 
     def pos_decode(self, pos: bytes, ranges=None, with_mq: bool = True):
         """POS bytes (16 per ion, big-endian x, y, z, mq) -> (xyz, mq, types);

@@ -230,7 +230,7 @@ pp_status plan_job(pp_ctx *c, const pp_job *j, JobPlan &p)
     if (!j)
         return fail(c, PP_ERR_INVALID_ARG, "job is NULL");
     if (j->families & PP_FAM_NPCORR)
-        return fail(c, PP_ERR_UNSUPPORTED, "NPCORR (n-point correlation) is not provided");
+        return fail(c, PP_ERR_UNSUPPORTED, "NPCORR runs through pp_npcorr_run");
     const uint32_t fams = j->families & (PP_FAM_RDF | PP_FAM_1NN | PP_FAM_RIPK | PP_FAM_MKNN |
                                          PP_FAM_CNTNB);
     if (fams == 0)
@@ -2245,6 +2245,98 @@ pp_status pp_cloud_from_pos(pp_ctx *c, const void *pos, uint64_t nbytes, const f
     if ((st = ion2surf_run(c, d_xyz, n, d_tri, ntri, R, d_s2, true)) != PP_OK)
         return st;
     return upload_common(c, d_xyz, d_type, d_s2, n, cell_width, hipMemcpyDeviceToDevice, out);
+}
+
+int64_t pp_npcorr_geometry(float rmax, float rincr)
+{
+    if (!(rmax > 0.f) || !(rincr > 0.f) || !std::isfinite(rmax) || !std::isfinite(rincr))
+        return -1;
+    // npc3d::npc3d (NPointCorrelations3D.cpp:50-60): float ceil of the float
+    // quotient, one guard layer, odd edge
+    const float realhalf = std::ceil(rmax / rincr) + 1.f;
+    if (!(realhalf < 1e6f))
+        return -1;
+    const int64_t half = (int64_t)realhalf;
+    const int64_t edge = 1 + 2 * half;
+    // MAXIMUM_NPC3D_BIN_RESOLUTION (src/PARAPROBE_Numerics.h:73; S.cpp:3706-3711)
+    if (edge * edge * edge > 503ll * 503ll * 503ll)
+        return -1;
+    return edge;
+}
+
+pp_status pp_npcorr_run(pp_ctx *c, const pp_cloud *cl, int32_t ntasks, const uint8_t *trg_mask,
+                        const uint8_t *env_mask, float rmax, float rincr, int64_t knn_order,
+                        uint32_t *counts, int64_t *centers_binned)
+{
+    if (!c || !cl || ntasks < 0 || ntasks > kMaxTasks || (ntasks > 0 && (!trg_mask || !env_mask)))
+        return fail(c, PP_ERR_INVALID_ARG, "bad NPCORR arguments");
+    if (knn_order < 0 || knn_order >= kNpcCap / 2)
+        return fail(c, PP_ERR_UNSUPPORTED, "NPCORR: SpatStatKNNOrder must be below 512");
+    const int64_t edge = pp_npcorr_geometry(rmax, rincr);
+    if (edge < 0)
+        return fail(c, PP_ERR_UNSUPPORTED,
+                    "Computation of 2-point statistics would require too large buffers");
+    if (ntasks > 0 && !counts)
+        return fail(c, PP_ERR_INVALID_ARG, "NULL counts");
+    const uint64_t nxyz = (uint64_t)edge * edge * edge;
+    if (centers_binned)
+        *centers_binned = 0;
+    if (ntasks == 0)
+        return PP_OK;
+    std::fill(counts, counts + (size_t)ntasks * nxyz, 0u);
+    if (cl->n == 0)
+        return PP_OK;
+    PP_HIP(c, hipSetDevice(c->device));
+    Slot &sl = c->main;
+    hipStream_t s = sl.stream;
+    // a cell list at least R wide: the 27 cells around a center hold its ball
+    CellList L;
+    pp_status st = build_celllist(c, sl, const_cast<pp_cloud *>(cl),
+                                  rmax * (1.0f + std::ldexp(1.0f, -10)), L);
+    std::unique_ptr<void, void (*)(void *)> own_l(L.mem, [](void *p) {
+        if (p)
+            (void)hipFree(p);
+    });
+    if (st != PP_OK)
+        return st;
+    void *mem = nullptr;
+    const size_t cbytes = 4 * (size_t)ntasks * nxyz;
+    PP_HIP(c, hipMalloc(&mem, cbytes + 2 * 32 * (size_t)ntasks + 512));
+    std::unique_ptr<void, void (*)(void *)> own(mem, [](void *p) { (void)hipFree(p); });
+    Carver m(mem);
+    uint32_t *d_counts = m.take<uint32_t>((size_t)ntasks * nxyz);
+    uint32_t *d_trg = m.take<uint32_t>(8 * (size_t)ntasks);
+    uint32_t *d_env = m.take<uint32_t>(8 * (size_t)ntasks);
+    PP_HIP(c, hipMemsetAsync(d_counts, 0, cbytes, s));
+    PP_HIP(c, hipMemcpyAsync(d_trg, trg_mask, 32 * (size_t)ntasks, hipMemcpyHostToDevice, s));
+    PP_HIP(c, hipMemcpyAsync(d_env, env_mask, 32 * (size_t)ntasks, hipMemcpyHostToDevice, s));
+    NpcArgs a{};
+    a.pos = L.pos;
+    a.s2 = L.s2;
+    a.cellstart = L.cellstart;
+    a.g = L.g;
+    a.n = cl->n;
+    a.R2 = rmax * rmax;
+    a.ntasks = ntasks;
+    a.trg = d_trg;
+    a.env = d_env;
+    a.K = (uint32_t)knn_order;
+    a.maxlen = rmax;
+    a.inv_width = 1.f / rincr;
+    a.half = (int)((edge - 1) / 2);
+    a.edge = (uint64_t)edge;
+    a.nxyz = nxyz;
+    a.counts = d_counts;
+    PP_HIP(c, launch_npcorr(a, s));
+    PP_HIP(c, hipMemcpyAsync(counts, d_counts, cbytes, hipMemcpyDeviceToHost, s));
+    PP_HIP(c, hipStreamSynchronize(s));
+    if (centers_binned) {
+        int64_t tot = 0;
+        for (size_t i = 0; i < (size_t)ntasks * nxyz; ++i)
+            tot += counts[i];
+        *centers_binned = tot;
+    }
+    return PP_OK;
 }
 
 pp_status pp_set_knn_range(pp_ctx *c, float factor)

@@ -119,4 +119,25 @@ hipError_t launch_posdecode(const uint32_t *rec, uint64_t n, const float *lo, co
                             const uint8_t *rtype, int nranges, float *xyz, float *mq,
                             uint8_t *type, hipStream_t s);
 
+// NPCORR (pp_npcorr.hip): one wavefront per center over a cell list at least
+// R wide; per-task counts of the K-th env neighbour's binned difference vector.
+constexpr int kNpcCap = 1024;  // LDS entries per wavefront; K < kNpcCap / 2
+struct NpcArgs {
+    const float4 *pos;          // cell-sorted (x, y, z, type bits)
+    const float *s2;            // cell-sorted ion2surf
+    const uint32_t *cellstart;  // ncells + 1
+    GridGeom g;
+    uint64_t n;
+    float R2;                   // fl32(R * R)
+    int ntasks;
+    const uint32_t *trg, *env;  // [ntasks][8] 256-bit type masks
+    uint32_t K;                 // SpatStatKNNOrder (0-based)
+    float maxlen;               // npc3d fvec_maxlen (SpatStatRadiusMax)
+    float inv_width;            // fl32(1 / SpatStatRadiusIncr)
+    int half;                   // halfedgelen
+    uint64_t edge, nxyz;        // edgelen, edgelen^3
+    uint32_t *counts;           // [ntasks][nxyz]
+};
+hipError_t launch_npcorr(const NpcArgs &a, hipStream_t s);
+
 } // namespace pp

@@ -627,12 +627,13 @@ pp_status Horderdist::compute_generic_spatstat3(const pp_cloud *cloud, const Vol
         std::cerr << "WARNING::No spatial statistics task to do" << std::endl;
         return PP_ERR_INVALID_ARG;
     }
-    if (fam & PP_FAM_NPCORR) {
-        std::cerr << "WARNING::n-point correlations are not part of this GPU path, skipping them"
+    // avoid that in case of 2-point statistics the containers get too large:
+    // the reference then skips the whole computation (S.cpp:3705-3712)
+    if ((fam & PP_FAM_NPCORR) && pp_npcorr_geometry(s_.rmax, s_.rincr) < 0) {
+        std::cerr << "WARNING::Computation of 2-point statistics would require too large "
+                     "buffers, skipping!"
                   << std::endl;
-        fam &= ~PP_FAM_NPCORR;
-        if (fam == 0)
-            return PP_ERR_UNSUPPORTED;
+        return PP_OK;
     }
     std::vector<int64_t> kdesc;
     if (fam & PP_FAM_MKNN) {
@@ -680,6 +681,25 @@ pp_status Horderdist::compute_generic_spatstat3(const pp_cloud *cloud, const Vol
     last.mknn = mknn.empty() ? nullptr : mknn.data();
     last.cntnb = cntnb.empty() ? nullptr : cntnb.data();
     last.cntnb_cap = (fam & PP_FAM_CNTNB) ? cap : 0;
+    npcorr.clear();
+    npcorr_edge = 0;
+    if (fam & PP_FAM_NPCORR) {
+        // n-point correlations: their own kernel; the counts stay in npcorr
+        // (the reference writes them to HDF5, S.cpp:4254-4288)
+        npcorr_edge = pp_npcorr_geometry(s_.rmax, s_.rincr);
+        npcorr.assign((size_t)T * npcorr_edge * npcorr_edge * npcorr_edge, 0u);
+        st = pp_npcorr_run(ctx_, cloud, T, trg.data(), env.data(), s_.rmax, s_.rincr,
+                           (int64_t)s_.knn_order, npcorr.data(), nullptr);
+        if (st != PP_OK) {
+            std::cerr << "ERROR::n-point correlations on the GPU failed: "
+                      << pp_last_error_string(ctx_) << std::endl;
+            return st;
+        }
+        fam &= ~PP_FAM_NPCORR;
+        job.families = fam;
+        if (fam == 0)
+            return PP_OK;
+    }
     st = pp_spatstat_run(ctx_, cloud, &job, &last);
     if (st != PP_OK) {
         std::cerr << "ERROR::spatial statistics on the GPU failed: " << pp_last_error_string(ctx_)

@@ -1,0 +1,96 @@
+"""GPU parity of NPCORR (n-point correlations) against the brute-force oracle:
+identical uint32 counts for random and clustered clouds, several K, tasks and
+bin widths, and a lattice where every pick is a tie."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from paraprobe_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+@pytest.mark.parametrize("K,width,R", [(0, 0.05, 1.0), (3, 0.1, 1.2), (40, 0.2, 1.5),
+                                       (300, 0.1, 2.5)])
+def test_npcorr_matches_oracle(ctx, K, width, R):
+    rng = np.random.default_rng(K + 1)
+    n = 6000
+    xyz = rng.uniform(0, 7, size=(n, 3)).astype(np.float32)
+    ty = rng.choice([1, 2, 3], size=n, p=[0.6, 0.3, 0.1]).astype(np.uint8)
+    s2 = rng.uniform(0, 8, size=n).astype(np.float32)
+    tasks = [((1,), (1,)), ((1, 2), (3,)), ((3,), (1, 2, 3))]
+    _say(f"npcorr K={K} width={width} R={R}")
+    cloud = ctx.upload(xyz, ty, s2)
+    try:
+        got = ctx.npcorr(cloud, tasks, R, width, K)
+    finally:
+        cloud.free()
+    want = O.npcorr(xyz, ty, s2, tasks, R, width, K)
+    assert got.shape == want.shape
+    assert np.array_equal(got, want)
+    assert got.sum() > 0
+
+
+def test_npcorr_lattice_ties_and_clusters(ctx):
+    a = 0.404
+    xyz = synth.lattice(True, a, 9, 9, 9)
+    n = xyz.shape[0]
+    ty = np.ones(n, np.uint8)
+    ty[::7] = 2
+    s2 = np.full(n, 100.0, np.float32)
+    tasks = [((1,), (1,)), ((2,), (1, 2))]
+    for K in (0, 11, 12):
+        _say(f"npcorr fcc K={K}")
+        cloud = ctx.upload(xyz, ty, s2)
+        try:
+            got = ctx.npcorr(cloud, tasks, 1.0, 0.05, K)
+        finally:
+            cloud.free()
+        assert np.array_equal(got, O.npcorr(xyz, ty, s2, tasks, 1.0, 0.05, K))
+    t = synth.tip(8000, 5, "cylinder", 6.0, 6.0, 10.0, [0.9, 0.1], 20, 0.5, 1.0, [0.4, 0.6])
+    _say("npcorr clustered")
+    cloud = ctx.upload(t.xyz, t.types, t.ion2surf_sqr)
+    try:
+        got = ctx.npcorr(cloud, [((2,), (2,)), ((1,), (2,))], 1.5, 0.1, 2)
+    finally:
+        cloud.free()
+    assert np.array_equal(got, O.npcorr(t.xyz, t.types, t.ion2surf_sqr,
+                                        [((2,), (2,)), ((1,), (2,))], 1.5, 0.1, 2))
+
+
+def test_npcorr_through_the_host_driver(ctx, tmp_path):
+    """AnalysisSpatDistrType 15: RDF files as before, NPCORR counts in memory;
+    the RDF must not change when NPCORR rides along."""
+    from paraprobe_amd import horderdist as H
+
+    rng = np.random.default_rng(9)
+    n = 5000
+    xyz = rng.uniform(0, 7, size=(n, 3)).astype(np.float32)
+    ty = rng.choice([1, 2], size=n, p=[0.7, 0.3]).astype(np.uint8)
+    s2 = rng.uniform(0, 5, size=n).astype(np.float32)
+    xml = """<Parameters>
+  <AnalysisSpatDistrType>{}</AnalysisSpatDistrType>
+  <DescrStatTasksCode>Al-Al;Sc-Al</DescrStatTasksCode>
+  <SpatStatRadiusMin>0.0</SpatStatRadiusMin>
+  <SpatStatRadiusIncr>0.1</SpatStatRadiusIncr>
+  <SpatStatRadiusMax>1.2</SpatStatRadiusMax>
+  <SpatStatKNNOrder>2</SpatStatKNNOrder>
+</Parameters>"""
+    cl = ctx.upload(xyz, ty, s2)
+    try:
+        for code, sub in (("1", "a"), ("15", "b")):
+            d = tmp_path / sub
+            d.mkdir()
+            _say(f"host driver AnalysisSpatDistrType={code}")
+            assert H.run_spatstat(ctx, cl, xml.format(code), 1, {"Al": 1, "Sc": 2}, n, 343.0,
+                                  str(d)) == 0
+        for f in sorted(p.name for p in (tmp_path / "a").iterdir()):
+            assert (tmp_path / "a" / f).read_bytes() == (tmp_path / "b" / f).read_bytes()
+        got = ctx.npcorr(cl, [((1,), (1,)), ((2,), (1,))], 1.2, 0.1, 1)
+    finally:
+        cl.free()
+    assert np.array_equal(got, O.npcorr(xyz, ty, s2, [((1,), (1,)), ((2,), (1,))], 1.2, 0.1, 1))
