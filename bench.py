@@ -362,6 +362,59 @@ def pos_leg(ctx, tip, hip, args):
     return out
 
 
+def npcorr_leg(ctx, tip, args):
+    """NPCORR with the reference's settings fixture values (tasks A-A; B-B;
+    A-B; B-A, SpatStatRadiusMax 2.5, SpatStatRadiusIncr 0.05, SpatStatKNNOrder
+    100, i.e. the 100th neighbour's difference vector in a 103^3 cube) on the
+    resident 20M-ion tip; the reference's structure (z-slabs, KD-trees,
+    nth_element) timed on the host cores for the centers of a 2 nm z-window."""
+    tasks = [((1,), (1,)), ((2,), (2,)), ((1,), (2,)), ((2,), (1,))]
+    R, w, K = 2.5, 0.05, 99
+    cloud = ctx.upload(tip.xyz, tip.types, tip.ion2surf_sqr)
+    try:
+        ctx.npcorr(cloud, tasks, R, w, K)
+        reps = 2
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            got = ctx.npcorr(cloud, tasks, R, w, K)
+        ms = (time.perf_counter() - t0) * 1e3 / reps
+        zwin = (124.0, 126.0)
+        s2w = tip.ion2surf_sqr.copy()
+        zz = tip.xyz[:, 2]
+        s2w[(zz < zwin[0]) | (zz >= zwin[1])] = 0.0
+        cw = ctx.upload(tip.xyz, tip.types, s2w)
+        gwin = ctx.npcorr(cw, tasks, R, w, K)
+        cw.free()
+    finally:
+        cloud.free()
+    considered = int((tip.ion2surf_sqr >= np.float32(R) * np.float32(R)).sum())
+    out = {
+        "workload": "frustum-20M, NPCORR A-A;B-B;A-B;B-A, Rmax 2.5 nm, bin 0.05 nm (103^3 cube), "
+                    "100th neighbour",
+        "ms": ms,
+        "centers_per_s": considered / (ms / 1e3),
+        "vectors_binned": int(got.sum()),
+    }
+    print(f"# npcorr: {ms:.1f} ms", file=sys.stderr, flush=True)
+    if not args.no_cpu:
+        from oracle import cpu_baseline as cb
+
+        model, cores = host_cpu()
+        cnt, idx_s, comp_s = cb.npcorr(tip.xyz, tip.types, s2w, tasks, R, w, K, nthreads=cores)
+        nwin = int(((zz >= zwin[0]) & (zz < zwin[1]) &
+                    (tip.ion2surf_sqr >= np.float32(R) * np.float32(R))).sum())
+        out["cpu_baseline"] = {
+            "value": nwin / comp_s,
+            "unit": "centers/s",
+            "cores": cores,
+            "kind": "port",
+            "sample": f"centers with z in [{zwin[0]}, {zwin[1]}) nm ({nwin} considered); compute "
+                      f"{comp_s:.1f} s timed, z-slab + KD-tree build {idx_s:.1f} s not; {model}",
+            "parity_with_gpu": bool(np.array_equal(cnt, gwin)),
+        }
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -377,6 +430,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-surf", action="store_true")
     ap.add_argument("--no-pos", action="store_true")
+    ap.add_argument("--no-npcorr", action="store_true")
     ap.add_argument("--batch-jobs", type=int, default=256)
     ap.add_argument("--autotune", type=int, default=1)
     ap.add_argument("--serve-requests", type=int, default=400)
@@ -439,6 +493,8 @@ def main():
             extra["ion2surf"] = surf_leg(ctx, tip, hip, args)
         if not args.no_pos:
             extra["pos_decode"] = pos_leg(ctx, tip, hip, args)
+        if not args.no_npcorr:
+            extra["npcorr"] = npcorr_leg(ctx, tip, args)
         if not args.no_batch:
             extra["batch"] = batch_leg(ctx, args)
         if not args.no_serve:

@@ -30,6 +30,9 @@ def _load():
         lib.pp_cpu_ion2surf.argtypes = [C.c_int64, P, C.c_int64, P, C.c_float, C.c_int32,
                                         C.c_int64, P, P]
         lib.pp_cpu_ion2surf.restype = C.c_int
+        lib.pp_cpu_npcorr.argtypes = [C.c_int64, P, P, P, C.c_int32, P, P, C.c_float, C.c_float,
+                                      C.c_int64, C.c_int32, P, P]
+        lib.pp_cpu_npcorr.restype = C.c_int
         _lib = lib
     return _lib
 
@@ -92,6 +95,30 @@ def ion2surf(xyz, triangles, R, nthreads=0, stride=1):
     tim = np.zeros(2, np.float64)
     st = lib.pp_cpu_ion2surf(xyz.shape[0], xyz.ctypes.data, tri.shape[0], tri.ctypes.data,
                              C.c_float(R), nthreads, stride, out.ctypes.data, tim.ctypes.data)
+    if st != 0:
+        raise ValueError("cpu baseline rejected the arguments")
+    return out, float(tim[0]), float(tim[1])
+
+
+def npcorr(xyz, types, ion2surf, tasks, rmax, rincr, knn_order, nthreads=0):
+    """NPCORR counts [ntasks][edge][edge][edge] the reference's way (z-slabs,
+    KD-trees, nth_element); returns (counts, index_s, compute_s)."""
+    lib = _load()
+    xyz = np.ascontiguousarray(xyz, dtype=np.float32).reshape(-1, 3)
+    types = np.ascontiguousarray(types, dtype=np.uint8)
+    s2 = np.ascontiguousarray(ion2surf, dtype=np.float32)
+    T = len(tasks)
+    trg = np.zeros((max(T, 1), 32), np.uint8)
+    env = np.zeros((max(T, 1), 32), np.uint8)
+    for i, (t, e) in enumerate(tasks):
+        trg[i], env[i] = _mask(t), _mask(e)
+    half = int(np.ceil(np.float32(rmax) / np.float32(rincr)) + np.float32(1.0))
+    edge = 1 + 2 * half
+    out = np.zeros((T, edge, edge, edge), np.uint32)
+    tim = np.zeros(2, np.float64)
+    st = lib.pp_cpu_npcorr(xyz.shape[0], xyz.ctypes.data, types.ctypes.data, s2.ctypes.data, T,
+                           trg.ctypes.data, env.ctypes.data, rmax, rincr, knn_order, nthreads,
+                           out.ctypes.data, tim.ctypes.data)
     if st != 0:
         raise ValueError("cpu baseline rejected the arguments")
     return out, float(tim[0]), float(tim[1])

@@ -179,6 +179,19 @@ struct KdTree {
     void range(const P3 &target, size_t self, const std::vector<P3> &sorted, float r2,
                std::vector<Nbor> &out) const
     {
+        visit(target, self, sorted, r2,
+              [&](const P3 &q, float d) { out.push_back(Nbor{std::sqrt(d), q.m}); });
+    }
+    // the same query returning the neighbours themselves (the _p3dm1 variants)
+    void range_p3(const P3 &target, size_t self, const std::vector<P3> &sorted, float r2,
+                  std::vector<P3> &out) const
+    {
+        visit(target, self, sorted, r2, [&](const P3 &q, float) { out.push_back(q); });
+    }
+    template <class F>
+    void visit(const P3 &target, size_t self, const std::vector<P3> &sorted, float r2,
+               F &&emit) const
+    {
         if (nodes.empty())
             return;
         struct TT {
@@ -210,7 +223,7 @@ struct KdTree {
                     if (d > r2)
                         continue;
                     if (i != self)
-                        out.push_back(Nbor{std::sqrt(d), q.m});
+                        emit(q, d);
                 }
                 --top;
                 continue;
@@ -259,34 +272,18 @@ float lerpf(float v0, float v1, float t) { return (1.f - t) * v0 + t * v1; }
 
 } // namespace
 
-extern "C" {
-
-// Returns 0 on success.  timings[0] = index build seconds (slabs + KD-trees),
-// timings[1] = compute seconds (the DescrStatsThreadparallelCompute span).
-int pp_cpu_spatstat(int64_t n, const float *xyz, const uint8_t *type, const float *ion2surf,
-                    int32_t ntasks, const uint8_t *trg_mask, const uint8_t *env_mask, float rmin,
-                    float rincr, float rmax, uint32_t families, int32_t nk,
-                    const int64_t *kminus1_desc, int32_t nthreads, int64_t *rdf, int64_t *nn1,
-                    int64_t *ripk, int64_t *mknn, int64_t *cntnb, int64_t cntnb_cap,
-                    int64_t *stats, double *timings)
+// decompositor::loadpartitioning (S.cpp:1819-1934) and build_local_kdtree
+// (S.cpp:1593-1663): one z-slab per thread at the ion-count quantiles, each
+// with its KD-tree, ions and ion2surf in leaf order.
+static void build_regions(int64_t n, const float *xyz, const uint8_t *type,
+                          const float *ion2surf, int nthreads, std::vector<Region> &reg)
 {
-    if (nthreads <= 0)
-        nthreads = omp_get_max_threads();
-    const bool doRDF = families & 1u, do1NN = families & 2u, doRIPK = families & 4u,
-               doMKNN = (families & 8u) && nk > 0, doCNT = families & 32u;
-    Hist proto(rmin, rincr, rmax);
-    if (proto.nbins == 0)
-        return 1;
-    const int64_t C = (int64_t)proto.nbins + 2;
-    const float R = rmax, RSQR = R * R;
-    const double t0 = omp_get_wtime();
-
     // ---- loadpartitioning: quantile z-slabs
     std::vector<float> zs((size_t)n);
     for (int64_t i = 0; i < n; ++i)
         zs[i] = xyz[3 * i + 2];
     std::sort(zs.begin(), zs.end());
-    std::vector<Region> reg(nthreads);
+    reg.assign(nthreads, Region());
     for (int t = 0; t < nthreads; ++t) {
         auto quant = [&](int tid) {
             const float q = (float)tid / (float)nthreads;
@@ -321,6 +318,31 @@ int pp_cpu_spatstat(int64_t n, const float *xyz, const uint8_t *type, const floa
             r.s2.push_back(ls2[perm[k]]);
         }
     }
+}
+
+extern "C" {
+
+// Returns 0 on success.  timings[0] = index build seconds (slabs + KD-trees),
+// timings[1] = compute seconds (the DescrStatsThreadparallelCompute span).
+int pp_cpu_spatstat(int64_t n, const float *xyz, const uint8_t *type, const float *ion2surf,
+                    int32_t ntasks, const uint8_t *trg_mask, const uint8_t *env_mask, float rmin,
+                    float rincr, float rmax, uint32_t families, int32_t nk,
+                    const int64_t *kminus1_desc, int32_t nthreads, int64_t *rdf, int64_t *nn1,
+                    int64_t *ripk, int64_t *mknn, int64_t *cntnb, int64_t cntnb_cap,
+                    int64_t *stats, double *timings)
+{
+    if (nthreads <= 0)
+        nthreads = omp_get_max_threads();
+    const bool doRDF = families & 1u, do1NN = families & 2u, doRIPK = families & 4u,
+               doMKNN = (families & 8u) && nk > 0, doCNT = families & 32u;
+    Hist proto(rmin, rincr, rmax);
+    if (proto.nbins == 0)
+        return 1;
+    const int64_t C = (int64_t)proto.nbins + 2;
+    const float R = rmax, RSQR = R * R;
+    const double t0 = omp_get_wtime();
+    std::vector<Region> reg;
+    build_regions(n, xyz, type, ion2surf, nthreads, reg);
     const double t1 = omp_get_wtime();
 
     // ---- compute_generic_spatstat3
@@ -638,6 +660,110 @@ int pp_cpu_ion2surf(int64_t n, const float *xyz, int64_t ntri, const float *tri,
             }
             out[q] = best;
         }
+    }
+    const double t2 = omp_get_wtime();
+    if (timings) {
+        timings[0] = t1 - t0;
+        timings[1] = t2 - t1;
+    }
+    return 0;
+}
+
+// NPCORR, the n-point correlation branch of the center loop (S.cpp:3899-3960)
+// with npc3d::add (src/PARAPROBE_NPointCorrelations3D.cpp:165-207): the
+// neighbours within R as positions (the _p3dm1 range queries, the other slabs
+// probed when the ball crosses the own slab), per task the env ones as
+// (difference vector, sqrt length), nth_element for the K-th, binned.  Equal
+// lengths are ordered by the smaller (u, v, w) so that the result is
+// deterministic (the reference leaves them to nth_element over its KD-tree
+// order).  counts is [ntasks][edge^3]; timings as for pp_cpu_spatstat.
+int pp_cpu_npcorr(int64_t n, const float *xyz, const uint8_t *type, const float *ion2surf,
+                  int32_t ntasks, const uint8_t *trg_mask, const uint8_t *env_mask, float rmax,
+                  float rincr, int64_t K, int32_t nthreads, uint32_t *counts, double *timings)
+{
+    if (nthreads <= 0)
+        nthreads = omp_get_max_threads();
+    const float realhalf = std::ceil(rmax / rincr) + 1.f;
+    const int64_t half = (int64_t)realhalf, edge = 1 + 2 * half, nxyz = edge * edge * edge;
+    const float R = rmax, RSQR = R * R, inv = 1.f / rincr;
+    const double t0 = omp_get_wtime();
+    std::vector<Region> reg;
+    build_regions(n, xyz, type, ion2surf, nthreads, reg);
+    const double t1 = omp_get_wtime();
+    std::memset(counts, 0, sizeof(uint32_t) * (size_t)(ntasks * nxyz));
+    struct NP {
+        float u, v, w, d;
+    };
+    auto less = [](const NP &a, const NP &b) {
+        if (a.d != b.d)
+            return a.d < b.d;
+        if (a.u != b.u)
+            return a.u < b.u;
+        if (a.v != b.v)
+            return a.v < b.v;
+        return a.w < b.w;
+    };
+#pragma omp parallel num_threads(nthreads)
+    {
+        std::vector<uint32_t> m_cnt((size_t)(ntasks * nxyz), 0u);
+        std::vector<P3> nb;
+        std::vector<NP> tmp;
+        for (int thr = 0; thr < nthreads; ++thr) {
+            const Region &cur = reg[thr];
+#pragma omp for schedule(dynamic, 1) nowait
+            for (size_t i = 0; i < cur.ions.size(); ++i) {
+                const P3 me = cur.ions[i];
+                if (!(cur.s2[i] >= RSQR))
+                    continue;
+                bool any = false;
+                for (int t = 0; t < ntasks; ++t)
+                    any |= has_type(trg_mask + 32 * t, me.m);
+                if (!any)
+                    continue;
+                nb.clear();
+                cur.tree.range_p3(me, i, cur.ions, RSQR, nb);
+                if (!((me.z - R) > cur.zmi && (me.z + R) < cur.zmx)) {
+                    for (int o = thr + 1; o < nthreads; ++o)
+                        reg[o].tree.range_p3(me, (size_t)-1, reg[o].ions, RSQR, nb);
+                    for (int o = thr - 1; o > -1; --o)
+                        reg[o].tree.range_p3(me, (size_t)-1, reg[o].ions, RSQR, nb);
+                }
+                if (nb.size() <= (size_t)K)
+                    continue;
+                for (int t = 0; t < ntasks; ++t) {
+                    if (!has_type(trg_mask + 32 * t, me.m))
+                        continue;
+                    tmp.clear();
+                    for (const P3 &q : nb)
+                        if (has_type(env_mask + 32 * t, q.m)) {
+                            NP e;
+                            e.u = q.x - me.x;
+                            e.v = q.y - me.y;
+                            e.w = q.z - me.z;
+                            e.d = std::sqrt(e.u * e.u + e.v * e.v + e.w * e.w);
+                            tmp.push_back(e);
+                        }
+                    if (tmp.size() <= (size_t)K)
+                        continue;
+                    std::nth_element(tmp.begin(), tmp.begin() + K, tmp.end(), less);
+                    const NP k = tmp[(size_t)K];
+                    if (!(k.d <= rmax))
+                        continue;
+                    const float c[3] = {k.u * inv, k.v * inv, k.w * inv};
+                    int64_t ix[3];
+                    for (int q = 0; q < 3; ++q) {
+                        const int disc = (c[q] >= 0.f) ? (int)((double)c[q] + 0.5)
+                                                       : (int)((double)c[q] - 0.5);
+                        ix[q] = disc + half;
+                    }
+                    m_cnt[(size_t)(t * nxyz + ix[0] + ix[1] * edge + ix[2] * edge * edge)] += 1;
+                }
+            }
+#pragma omp barrier
+        }
+#pragma omp critical
+        for (size_t b = 0; b < m_cnt.size(); ++b)
+            counts[b] += m_cnt[b];
     }
     const double t2 = omp_get_wtime();
     if (timings) {

@@ -95,3 +95,20 @@ def test_total_counts_against_kdtree():
                 want[t] += 1
     # points within 1e-5 of R can flip either way in float; none at this seed
     assert [int(got[t].sum()) for t in range(2)] == want
+
+
+def test_cpu_baseline_equals_oracle():
+    """The reference-structured CPU baseline (z-slabs, KD-trees, nth_element)
+    gives the oracle's counts at 1 and 4 threads."""
+    from oracle import cpu_baseline as cb
+
+    rng = np.random.default_rng(4)
+    n = 4000
+    xyz = rng.uniform(0, 6, size=(n, 3)).astype(np.float32)
+    ty = rng.choice([1, 2], size=n, p=[0.7, 0.3]).astype(np.uint8)
+    s2 = rng.uniform(0, 4, size=n).astype(np.float32)
+    tasks = [((1,), (1, 2)), ((2,), (2,))]
+    want = O.npcorr(xyz, ty, s2, tasks, 1.3, 0.1, 5)
+    for th in (1, 4):
+        got, _, _ = cb.npcorr(xyz, ty, s2, tasks, 1.3, 0.1, 5, nthreads=th)
+        assert np.array_equal(got, want)
