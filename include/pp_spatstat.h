@@ -193,7 +193,7 @@ pp_status pp_ion2surf_device(pp_ctx *ctx, const float *d_xyz, uint64_t n,
  * from zero) when its length is <= rmax; counts is [ntasks][edge^3] uint32
  * (npc3d::cnts), x fastest.  Equal distances are ordered by the smaller
  * (u, v, w) (the reference leaves them to nth_element over its KD-tree
- * order).  knn_order must be below 512.  *centers_binned (may be NULL)
+ * order).  knn_order must be below 448.  *centers_binned (may be NULL)
  * receives the number of counts added. */
 int64_t pp_npcorr_geometry(float rmax, float rincr);
 pp_status pp_npcorr_run(pp_ctx *ctx, const pp_cloud *cloud, int32_t ntasks,

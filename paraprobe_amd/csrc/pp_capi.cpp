@@ -2270,8 +2270,8 @@ pp_status pp_npcorr_run(pp_ctx *c, const pp_cloud *cl, int32_t ntasks, const uin
 {
     if (!c || !cl || ntasks < 0 || ntasks > kMaxTasks || (ntasks > 0 && (!trg_mask || !env_mask)))
         return fail(c, PP_ERR_INVALID_ARG, "bad NPCORR arguments");
-    if (knn_order < 0 || knn_order >= kNpcCap / 2)
-        return fail(c, PP_ERR_UNSUPPORTED, "NPCORR: SpatStatKNNOrder must be below 512");
+    if (knn_order < 0 || knn_order >= kNpcCap - 64)
+        return fail(c, PP_ERR_UNSUPPORTED, "NPCORR: SpatStatKNNOrder must be below 448");
     const int64_t edge = pp_npcorr_geometry(rmax, rincr);
     if (edge < 0)
         return fail(c, PP_ERR_UNSUPPORTED,
@@ -2289,10 +2289,11 @@ pp_status pp_npcorr_run(pp_ctx *c, const pp_cloud *cl, int32_t ntasks, const uin
     PP_HIP(c, hipSetDevice(c->device));
     Slot &sl = c->main;
     hipStream_t s = sl.stream;
-    // a cell list at least R wide: the 27 cells around a center hold its ball
+    // a cell list at least R/2 wide: the 5 x 5 x 5 cells around a center
+    // hold its ball
     CellList L;
     pp_status st = build_celllist(c, sl, const_cast<pp_cloud *>(cl),
-                                  rmax * (1.0f + std::ldexp(1.0f, -10)), L);
+                                  0.5f * rmax * (1.0f + std::ldexp(1.0f, -10)), L);
     std::unique_ptr<void, void (*)(void *)> own_l(L.mem, [](void *p) {
         if (p)
             (void)hipFree(p);
@@ -2323,6 +2324,7 @@ pp_status pp_npcorr_run(pp_ctx *c, const pp_cloud *cl, int32_t ntasks, const uin
     a.K = (uint32_t)knn_order;
     a.maxlen = rmax;
     a.inv_width = 1.f / rincr;
+    a.inv_bucket = (float)kNpcBuckets / rmax;
     a.half = (int)((edge - 1) / 2);
     a.edge = (uint64_t)edge;
     a.nxyz = nxyz;

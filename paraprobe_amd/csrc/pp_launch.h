@@ -120,8 +120,9 @@ hipError_t launch_posdecode(const uint32_t *rec, uint64_t n, const float *lo, co
                             uint8_t *type, hipStream_t s);
 
 // NPCORR (pp_npcorr.hip): one wavefront per center over a cell list at least
-// R wide; per-task counts of the K-th env neighbour's binned difference vector.
-constexpr int kNpcCap = 1024;  // LDS entries per wavefront; K < kNpcCap / 2
+// R/2 wide; per-task counts of the K-th env neighbour's binned difference vector.
+constexpr int kNpcCap = 512;      // LDS entries per wavefront; K < kNpcCap - 64
+constexpr int kNpcBuckets = 512;  // distance buckets per wavefront
 struct NpcArgs {
     const float4 *pos;          // cell-sorted (x, y, z, type bits)
     const float *s2;            // cell-sorted ion2surf
@@ -134,6 +135,7 @@ struct NpcArgs {
     uint32_t K;                 // SpatStatKNNOrder (0-based)
     float maxlen;               // npc3d fvec_maxlen (SpatStatRadiusMax)
     float inv_width;            // fl32(1 / SpatStatRadiusIncr)
+    float inv_bucket;           // kNpcBuckets / R: distance bucket of d = d * inv_bucket
     int half;                   // halfedgelen
     uint64_t edge, nxyz;        // edgelen, edgelen^3
     uint32_t *counts;           // [ntasks][nxyz]

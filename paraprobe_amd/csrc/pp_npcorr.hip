@@ -15,12 +15,16 @@
 // to that order; here ties are broken by the smaller (u, v, w),
 // lexicographically, numerically.
 //
-// One wavefront per center over the 27 cells of a cell list at least R wide.
-// The wave keeps the candidates for the K-th neighbour in an LDS buffer
-// (16 bytes each: d bits, then u, v, w as order-preserving integers): env
-// neighbours are appended with a ballot compaction, and whenever the buffer
-// fills up it is sorted (bitonic, 1024 entries) and cut to its K + 1
-// smallest.  K must be below kNpcCap / 2.
+// One wavefront per center over the 5 x 5 x 5 cells of a cell list at least
+// R/2 wide.  For each task the wave first counts the env neighbours per
+// distance bucket (kNpcBuckets buckets of width R / kNpcBuckets over
+// d = sqrtf(d2), in LDS), finds the bucket that holds the K-th one, then
+// gathers that bucket's neighbours (16 bytes each: d bits, then u, v, w as
+// order-preserving integers) and ranks them; buckets are monotone in d, so
+// every neighbour of a lower bucket precedes them.  When the bucket holds
+// more than kNpcCap neighbours (shells of a perfect lattice) the wave falls
+// back to a buffer of candidates that is sorted (bitonic) and cut to the
+// K + 1 smallest whenever it fills up.  K must be below kNpcCap - 64.
 #include "pp_launch.h"
 
 #pragma clang fp contract(off)
@@ -87,8 +91,10 @@ __device__ void wave_sort(uint4 *buf, uint32_t cnt, int lane)
 __global__ __launch_bounds__(256) void k_npcorr(NpcArgs a)
 {
     __shared__ uint4 s_buf[kWaves][kNpcCap];
+    __shared__ uint32_t s_hist[kWaves][kNpcBuckets];
     const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
     uint4 *buf = s_buf[wid];
+    uint32_t *hist = s_hist[wid];
     const uint64_t ci = (uint64_t)blockIdx.x * kWaves + wid;
     if (ci >= a.n)
         return;
@@ -105,93 +111,172 @@ __global__ __launch_bounds__(256) void k_npcorr(NpcArgs a)
     const int cx = cell_of(me.x, a.g.lo[0], a.g.inv_w, a.g.nx);
     const int cy = cell_of(me.y, a.g.lo[1], a.g.inv_w, a.g.ny);
     const int cz = cell_of(me.z, a.g.lo[2], a.g.inv_w, a.g.nz);
-    const int x0 = cx > 0 ? cx - 1 : 0, x1 = cx + 1 < a.g.nx ? cx + 1 : cx;
-    // all neighbours within R, any type, self excluded by index
-    uint32_t ntot = 0;
-    for (int dz = -1; dz <= 1; ++dz) {
-        const int z = cz + dz;
-        if (z < 0 || z >= a.g.nz)
-            continue;
-        for (int dy = -1; dy <= 1; ++dy) {
-            const int y = cy + dy;
-            if (y < 0 || y >= a.g.ny)
-                continue;
-            const uint64_t row = ((uint64_t)z * a.g.ny + y) * a.g.nx;
-            const uint32_t b = a.cellstart[row + x0], e = a.cellstart[row + x1 + 1];
-            for (uint32_t j = b + lane; j < e; j += kWave) {
-                const float4 p = a.pos[j];
-                const float dxx = me.x - p.x, dyy = me.y - p.y, dzz = me.z - p.z;
-                const float d2 = dxx * dxx + dyy * dyy + dzz * dzz;
-                ntot += (d2 <= a.R2 && j != ci) ? 1u : 0u;
-            }
-        }
-    }
-    for (int o = kWave / 2; o > 0; o >>= 1)
-        ntot += __shfl_xor(ntot, o, kWave);
-    const uint32_t K = a.K;
-    if (ntot <= K)
-        return;
-    for (int t = 0; t < a.ntasks; ++t) {
-        if (!((a.trg[t * 8 + (mt >> 5)] >> (mt & 31)) & 1u))
-            continue;
-        const uint32_t *env = a.env + t * 8;
-        uint32_t cnt = 0;
-        for (int dz = -1; dz <= 1; ++dz) {
+    const int x0 = cx > 1 ? cx - 2 : 0, x1 = cx + 2 < a.g.nx ? cx + 2 : a.g.nx - 1;
+    // visits every candidate of the stencil: f(j, p) per lane, j < e valid
+    auto sweep = [&](auto &&f) {
+        for (int dz = -2; dz <= 2; ++dz) {
             const int z = cz + dz;
             if (z < 0 || z >= a.g.nz)
                 continue;
-            for (int dy = -1; dy <= 1; ++dy) {
+            for (int dy = -2; dy <= 2; ++dy) {
                 const int y = cy + dy;
                 if (y < 0 || y >= a.g.ny)
                     continue;
                 const uint64_t row = ((uint64_t)z * a.g.ny + y) * a.g.nx;
                 const uint32_t b = a.cellstart[row + x0], e = a.cellstart[row + x1 + 1];
-                for (uint32_t j0 = b; j0 < e; j0 += kWave) {
-                    const uint32_t j = j0 + lane;
-                    bool take = false;
-                    uint4 key = make_uint4(0, 0, 0, 0);
-                    if (j < e && j != ci) {
-                        const float4 p = a.pos[j];
-                        const uint32_t pt = __float_as_uint(p.w);
-                        const float dxx = me.x - p.x, dyy = me.y - p.y, dzz = me.z - p.z;
-                        const float d2 = dxx * dxx + dyy * dyy + dzz * dzz;
-                        if (d2 <= a.R2 && ((env[pt >> 5] >> (pt & 31)) & 1u)) {
-                            // the difference vector neighbour - center, its length
-                            // by the reference's expression (sqrt of a float)
-                            const float u = p.x - me.x, v = p.y - me.y, w = p.z - me.z;
-                            const float dd = __builtin_sqrtf(u * u + v * v + w * w);
-                            key = make_uint4(__float_as_uint(dd), ordf(u), ordf(v), ordf(w));
-                            take = true;
-                        }
-                    }
-                    const uint64_t m = __builtin_amdgcn_ballot_w64(take);
-                    const uint32_t add = (uint32_t)__builtin_popcountll(m);
-                    if (add == 0)
-                        continue;
-                    if (cnt + add > (uint32_t)kNpcCap) {
-                        // keep the K + 1 smallest
-                        wave_sort(buf, cnt, lane);
-                        cnt = cnt < K + 1 ? cnt : K + 1;
-                    }
-                    if (take) {
-                        const uint32_t below =
-                            (uint32_t)__builtin_popcountll(m & ((1ull << lane) - 1ull));
-                        buf[cnt + below] = key;
-                    }
-                    cnt += add;
-                    __builtin_amdgcn_wave_barrier();
-                }
+                for (uint32_t j0 = b; j0 < e; j0 += kWave)
+                    f(j0 + lane, e);
             }
         }
-        if (cnt <= K)
+    };
+    // all neighbours within R, any type, self excluded by index
+    uint32_t ntot = 0;
+    sweep([&](uint32_t j, uint32_t e) {
+        if (j < e) {
+            const float4 p = a.pos[j];
+            const float dxx = me.x - p.x, dyy = me.y - p.y, dzz = me.z - p.z;
+            const float d2 = dxx * dxx + dyy * dyy + dzz * dzz;
+            ntot += (d2 <= a.R2 && j != ci) ? 1u : 0u;
+        }
+    });
+    for (int o = kWave / 2; o > 0; o >>= 1)
+        ntot += __shfl_xor(ntot, o, kWave);
+    const uint32_t K = a.K;
+    if (ntot <= K)
+        return;
+    // the env neighbour's key, when candidate j is one
+    auto env_key = [&](uint32_t j, const uint32_t *env, uint4 &key, int &bucket) {
+        const float4 p = a.pos[j];
+        const uint32_t pt = __float_as_uint(p.w);
+        const float dxx = me.x - p.x, dyy = me.y - p.y, dzz = me.z - p.z;
+        const float d2 = dxx * dxx + dyy * dyy + dzz * dzz;
+        if (!(d2 <= a.R2) || j == ci || !((env[pt >> 5] >> (pt & 31)) & 1u))
+            return false;
+        // the difference vector neighbour - center and its length by the
+        // reference's expression (sqrt of a float)
+        const float u = p.x - me.x, v = p.y - me.y, w = p.z - me.z;
+        const float dd = __builtin_sqrtf(u * u + v * v + w * w);
+        key = make_uint4(__float_as_uint(dd), ordf(u), ordf(v), ordf(w));
+        const int bk = (int)(dd * a.inv_bucket);
+        bucket = bk < kNpcBuckets - 1 ? bk : kNpcBuckets - 1;
+        return true;
+    };
+    for (int t = 0; t < a.ntasks; ++t) {
+        if (!((a.trg[t * 8 + (mt >> 5)] >> (mt & 31)) & 1u))
             continue;
-        wave_sort(buf, cnt, lane);
-        const uint4 k = buf[K];
-        if (lane == 0 && __uint_as_float(k.x) <= a.maxlen) {
+        const uint32_t *env = a.env + t * 8;
+        // pass 1: env neighbours per distance bucket
+        for (int b = lane; b < kNpcBuckets; b += kWave)
+            hist[b] = 0;
+        __builtin_amdgcn_wave_barrier();
+        uint32_t nenv = 0;
+        sweep([&](uint32_t j, uint32_t e) {
+            uint4 key;
+            int bk;
+            if (j < e && env_key(j, env, key, bk)) {
+                __hip_atomic_fetch_add(&hist[bk], 1u, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_WORKGROUP);
+                ++nenv;
+            }
+        });
+        for (int o = kWave / 2; o > 0; o >>= 1)
+            nenv += __shfl_xor(nenv, o, kWave);
+        __builtin_amdgcn_wave_barrier();
+        if (nenv <= K)
+            continue;
+        // the bucket of the K-th: lane l sums buckets [l * per, (l + 1) * per)
+        constexpr int per = kNpcBuckets / kWave;
+        uint32_t mine = 0;
+        for (int q = 0; q < per; ++q)
+            mine += hist[lane * per + q];
+        uint32_t incl = mine;
+        for (int o = 1; o < kWave; o <<= 1) {
+            const uint32_t v = __shfl_up(incl, o, kWave);
+            if (lane >= o)
+                incl += v;
+        }
+        const uint32_t excl = incl - mine;
+        // the lane whose range holds the K-th finds the bucket
+        int bsel = -1;
+        uint32_t below = 0;
+        if (excl <= K && K < incl) {
+            uint32_t c = excl;
+            for (int q = 0; q < per; ++q) {
+                const uint32_t h = hist[lane * per + q];
+                if (K < c + h) {
+                    bsel = lane * per + q;
+                    below = c;
+                    break;
+                }
+                c += h;
+            }
+        }
+        const uint64_t own = __builtin_amdgcn_ballot_w64(bsel >= 0);
+        if (own == 0)  // cannot happen: more than K env neighbours were counted
+            continue;
+        const int src = (int)__builtin_ctzll(own);
+        bsel = __shfl(bsel, src, kWave);
+        below = __shfl(below, src, kWave);
+        const uint32_t inb = hist[bsel];
+        uint4 pick;
+        if (inb <= (uint32_t)kNpcCap) {
+            // pass 2: the selected bucket's neighbours, ranked
+            uint32_t cnt = 0;
+            sweep([&](uint32_t j, uint32_t e) {
+                uint4 key;
+                int bk = -1;
+                const bool take = j < e && env_key(j, env, key, bk) && bk == bsel;
+                const uint64_t m = __builtin_amdgcn_ballot_w64(take);
+                if (take)
+                    buf[cnt + (uint32_t)__builtin_popcountll(m & ((1ull << lane) - 1ull))] = key;
+                cnt += (uint32_t)__builtin_popcountll(m);
+            });
+            __builtin_amdgcn_wave_barrier();
+            const uint32_t want = K - below;
+            int found = -1;
+            for (uint32_t q = lane; q < cnt; q += kWave) {
+                const uint4 kq = buf[q];
+                uint32_t rank = 0;
+                for (uint32_t r = 0; r < cnt; ++r) {
+                    const uint4 kr = buf[r];
+                    rank += (key_less(kr, kq) || (r < q && !key_less(kq, kr))) ? 1u : 0u;
+                }
+                if (rank == want)
+                    found = (int)q;
+            }
+            const uint64_t fm = __builtin_amdgcn_ballot_w64(found >= 0);
+            if (fm == 0)  // cannot happen: the ranks are a permutation of [0, cnt)
+                continue;
+            const int q = __shfl(found, (int)__builtin_ctzll(fm), kWave);
+            pick = buf[q];
+        } else {
+            // a crowded bucket: keep the K + 1 smallest in the buffer
+            uint32_t cnt = 0;
+            sweep([&](uint32_t j, uint32_t e) {
+                uint4 key = make_uint4(0, 0, 0, 0);
+                int bk;
+                const bool take = j < e && env_key(j, env, key, bk);
+                const uint64_t m = __builtin_amdgcn_ballot_w64(take);
+                const uint32_t add = (uint32_t)__builtin_popcountll(m);
+                if (add == 0)
+                    return;
+                if (cnt + add > (uint32_t)kNpcCap) {
+                    wave_sort(buf, cnt, lane);
+                    cnt = cnt < K + 1 ? cnt : K + 1;
+                }
+                if (take)
+                    buf[cnt + (uint32_t)__builtin_popcountll(m & ((1ull << lane) - 1ull))] = key;
+                cnt += add;
+                __builtin_amdgcn_wave_barrier();
+            });
+            wave_sort(buf, cnt, lane);
+            pick = buf[K];
+        }
+        if (lane == 0 && __uint_as_float(pick.x) <= a.maxlen) {
             // npc3d::add: t = u * (1 / width) in float, rounded half away
             // from zero through a double, plus the centre offset
             int ix[3];
-            const uint32_t comp[3] = {k.y, k.z, k.w};
+            const uint32_t comp[3] = {pick.y, pick.z, pick.w};
             for (int q = 0; q < 3; ++q) {
                 const float th = unordf(comp[q]) * a.inv_width;
                 const int disc = (th >= 0.f) ? (int)((double)th + 0.5) : (int)((double)th - 0.5);

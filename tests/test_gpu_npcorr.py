@@ -94,3 +94,28 @@ def test_npcorr_through_the_host_driver(ctx, tmp_path):
     finally:
         cl.free()
     assert np.array_equal(got, O.npcorr(xyz, ty, s2, [((1,), (1,)), ((2,), (1,))], 1.2, 0.1, 1))
+
+
+def test_npcorr_crowded_bucket(ctx):
+    """More neighbours in one distance bucket than the wave's list holds:
+    700 ions stacked on three points at the same distance from a few centers."""
+    rng = np.random.default_rng(11)
+    pts = rng.uniform(0, 5, size=(2000, 3)).astype(np.float32)
+    stack = np.repeat(np.array([[2.5, 2.5, 3.0], [2.5, 3.0, 2.5], [3.0, 2.5, 2.5]],
+                               np.float32), [300, 200, 200], axis=0)
+    centers = np.array([[2.5, 2.5, 2.5], [2.6, 2.5, 2.5]], np.float32)
+    xyz = np.concatenate([centers, stack, pts])
+    ty = np.full(xyz.shape[0], 2, np.uint8)
+    ty[:2] = 1
+    s2 = np.full(xyz.shape[0], 50.0, np.float32)
+    tasks = [((1,), (2,))]
+    for K in (0, 250, 400, 447):
+        _say(f"npcorr crowded K={K}")
+        cloud = ctx.upload(xyz, ty, s2)
+        try:
+            got = ctx.npcorr(cloud, tasks, 1.0, 0.05, K)
+        finally:
+            cloud.free()
+        want = O.npcorr(xyz, ty, s2, tasks, 1.0, 0.05, K)
+        assert np.array_equal(got, want)
+        assert got.sum() == 2
