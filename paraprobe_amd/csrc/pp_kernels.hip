@@ -7,17 +7,21 @@
 //   * cell list: bounding box, x-fastest linear cell keys computed in FP64,
 //     a one-digit counting sort (count with atomics, exclusive scan, scatter)
 //     into cell-sorted float4 (x, y, z, type) plus the sorted ion2surf array;
-//   * per job: considered-center flags, stream compaction, tiles of up to T
-//     centers of one cell;
-//   * query: each workgroup pulls tiles from a global queue; the four
-//     wavefronts split the tile's stencil rows into 64-candidate chunks, each
-//     lane holds one candidate and loops over the T centers held in
-//     registers, evaluating the reference's FP32 squared distance
-//     ((dx*dx + dy*dy) + dz*dz, no contraction) and binning in-ball pairs
-//     into LDS histograms with integer atomics.  With MKNN / 1NN / CNTNB the
-//     histograms are per center (16-bit pairs when a bound allows), and after
-//     the tile one wavefront per center scans them to find the category of
-//     every k-th neighbour.
+//   * per job: considered-center flags, stream compaction one center type
+//     after the other, tiles of up to T centers of one type in one cell;
+//   * query (k_query, three modes): each workgroup pulls tiles from eight
+//     per-XCD tile queues; the four wavefronts split the tile's stencil rows
+//     into 64-candidate chunks, each lane holds one candidate and loops over
+//     the T centers held in registers, evaluating the reference's FP32
+//     squared distance ((dx*dx + dy*dy) + dz*dz, no contraction) and binning
+//     in-ball pairs with integer LDS atomics: mode 0 into pair counts per
+//     (center type, env type, category) (RDF / RIPK); mode 1 into full-range
+//     per-center histograms (16-bit pairs when a bound allows; CNTNB, and the
+//     rerun of mode 2); mode 2 into both, the per-center histograms over each
+//     env type's kNN range only.  After a tile one wavefront per center scans
+//     its histograms and writes the category of every requested k-th
+//     neighbour to a per-launch buffer (plain stores), which k_kreduce counts
+//     per (task, k, category).
 #include "pp_launch.h"
 
 #pragma clang fp contract(off)
