@@ -29,6 +29,12 @@ sys.path.insert(0, ROOT)
 METRIC = "pair-distance evals/s at 20M ions; served jobs/s + p99 latency vs plain loop"
 VALU_PEAK_OPS = 78.6e12     # MI355X FP32 VALU element-ops/s (157.3 TFLOP/s counting FMA as 2)
 OPS_PER_EVAL = 9            # 3 sub + 3 mul + 2 add + 1 compare, no FMA
+# HBM bytes of the query kernel per step of the 20M-ion headline job, from two
+# rocprofv3 --pmc runs of this script (FETCH_SIZE x 2 and WRITE_SIZE x 1, the
+# factors a 1 GiB device copy reads): 129.7 GB read + 62.1 GB written
+TRAFFIC_HEADLINE = 1.918e11
+TRAFFIC_SOURCE = "profiles/k_query_frustum20M_rdf_mknn100.md"
+
 
 
 def host_cpu():
@@ -547,7 +553,9 @@ def main():
             "peak": VALU_PEAK_OPS / 1e12,
             "unit": "Tops/s (FP32 VALU element-ops, 9 per candidate pair)",
             "frac": achieved / (VALU_PEAK_OPS / 1e12),
-            "traffic": None,
+            "traffic": TRAFFIC_HEADLINE if tip.n == 20_000_000 else None,
+            "traffic_unit": "bytes per step (HBM, all query launches)",
+            "traffic_source": TRAFFIC_SOURCE if tip.n == 20_000_000 else None,
             "kernel": last.kernel,
             "kernel_ms": q_ms,
             "candidate_evals": cand,
