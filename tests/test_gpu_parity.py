@@ -10,7 +10,7 @@ import pytest
 
 from oracle import oracle as O
 from paraprobe_amd import FAM_1NN, FAM_CNTNB, FAM_MKNN, FAM_RDF, FAM_RIPK, Job
-from paraprobe_amd import synth
+from paraprobe_amd import capi, synth
 
 pytestmark = pytest.mark.gpu
 
@@ -249,3 +249,31 @@ def test_resident_cell_list_any_rmax(ctx):
             _compare(ctx, xyz, ty, s2, Job(TASKS_MIX, 0.0, 0.01, R, ALL, kvalues=[1, 4]), cloud=cl)
     finally:
         cl.free()
+
+
+@pytest.mark.parametrize("rmin,rincr,rmax", [(0.0, 0.01, 5.0), (0.0, 0.1, 2.0), (0.3, 0.01, 1.5),
+                                             (0.0, 0.005, 3.0)])
+def test_distances_packed_around_every_bin_edge(ctx, rmin, rincr, rmax):
+    """Isolated pairs whose distances lie within 40 float steps of every bin
+    edge (the ideal edge positions start + c / tmp): the category of each
+    pair, near-edge path included, must equal the literal expression's."""
+    _say(f"bin edges {rmin}/{rincr}/{rmax}")
+    nb = capi.hist_geometry(rmin, rincr, rmax)
+    r = np.array([rmin + c * (rmax - rmin) / nb for c in range(1, nb + 1)], np.float32)
+    # k float steps from each edge: positive floats order like their bit patterns
+    bits = r.view(np.int32)[:, None] + np.arange(-40, 41, dtype=np.int32)[None, :]
+    xs = bits.reshape(-1).view(np.float32)
+    xs = xs[(xs > 0) & (xs <= np.float32(rmax))]
+    n = xs.shape[0]
+    side = int(np.ceil(np.sqrt(n)))
+    sp = np.float32(2.0 * (rmax + 1.0))
+    j, k = np.divmod(np.arange(n), side)
+    a = np.stack([np.zeros(n, np.float32), j * sp, k * sp], axis=1).astype(np.float32)
+    b = a.copy()
+    b[:, 0] = xs
+    xyz = np.concatenate([a, b])
+    ty = np.ones(2 * n, np.uint8)
+    s2 = np.full(2 * n, 1e6, np.float32)
+    g, _, st = _compare(ctx, xyz, ty, s2, Job([((1,), (1,))], rmin, rincr, rmax,
+                                               FAM_RDF | FAM_MKNN, kvalues=[1]))
+    assert st["inball"] == 2 * n
