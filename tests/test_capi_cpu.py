@@ -52,3 +52,14 @@ def test_create_without_device_fails_cleanly():
 def test_type_mask():
     m = capi.type_mask([0, 1, 9, 255])
     assert m[0] == 0b11 and m[1] == 0b10 and m[31] == 0x80
+
+
+def test_host_library_exports_every_declared_pph_symbol():
+    from paraprobe_amd import horderdist as H
+
+    lib = H.load()
+    declared = [n for n in _declared_functions() if n.startswith("pph_")]
+    assert declared, "no pph declarations found"
+    for name in declared:
+        assert hasattr(lib, name), name
+    assert set(H.EXPORTED) == set(declared)
