@@ -2065,8 +2065,18 @@ pp_status ion2surf_run(pp_ctx *c, const float *xyz, uint64_t n, const float *tri
     // triangle lists per cell
     PP_HIP(c, hipMemsetAsync(count, 0, 4 * (nc + 1), s));
     PP_HIP(c, launch_tribox(d_tri, ntri, R, g, d_box, count, s));
-    PP_HIP(c, scan_exclusive(count, tcs, nc, ws, s));
     std::vector<uint32_t> h_ics(nc + 1), h_tcs(nc + 1);
+    // the per-cell triangle counts, summed in 64 bits: the 32-bit offsets of
+    // the cell lists must not wrap
+    PP_HIP(c, hipMemcpyAsync(h_tcs.data(), count, 4 * nc, hipMemcpyDeviceToHost, s));
+    PP_HIP(c, hipStreamSynchronize(s));
+    uint64_t total = 0;
+    for (uint64_t k = 0; k < nc; ++k)
+        total += h_tcs[k];
+    if (total > 0xFFFFFFF0ull)
+        return fail(c, PP_ERR_UNSUPPORTED,
+                    "triangles too large for the surface grid (over 2^32 cell entries)");
+    PP_HIP(c, scan_exclusive(count, tcs, nc, ws, s));
     PP_HIP(c, hipMemcpyAsync(h_ics.data(), ics, 4 * (nc + 1), hipMemcpyDeviceToHost, s));
     PP_HIP(c, hipMemcpyAsync(h_tcs.data(), tcs, 4 * (nc + 1), hipMemcpyDeviceToHost, s));
     PP_HIP(c, hipStreamSynchronize(s));
