@@ -203,7 +203,8 @@ int32_t pph_write_descrstat3(const char *path, const int64_t *cnts, int64_t len)
  * type id order) when names is not NULL. */
 int32_t pph_read_rangefile(const char *path, float *lo, float *hi, uint8_t *type, int32_t cap,
                            int32_t *nranges, char *names, int32_t names_stride);
-/* End to end from files: settings XML text, a POS file, an RRNG file (NULL:
+/* End to end from files: settings XML text, a POS file (an EPOS file when its
+ * name ends in "epos"), an RRNG file (NULL:
  * no ranging) and a triangle hull (ntri = 0: none) -> the POS decoded, ranged
  * and distanced on the GPU (pp_cloud_from_pos with R = max(SpatStatRadiusMax,
  * SurfaceCellsCarvingRadius)) -> pph_run_spatstat with the range file's ion

@@ -224,6 +224,15 @@ pp_status pp_pos_decode_device(pp_ctx *ctx, const void *d_pos, uint64_t nbytes,
                                const uint8_t *range_type, int32_t nranges, float *d_xyz,
                                float *d_mq, uint8_t *d_type);
 
+/* The same for EPOS files (solverHdl::load_epos_sequentially, S.cpp:
+ * 6933-7066; epos(epos_be_2_stl_le), src/PARAPROBE_EPOSEndianness.h:86-160:
+ * 44-byte records of nine big-endian floats x, y, z, mass-to-charge, tof,
+ * vdc, vpu, detx, dety and two big-endian ints); x, y, z and the
+ * mass-to-charge are decoded.  nbytes must be a multiple of 44. */
+pp_status pp_epos_decode(pp_ctx *ctx, const void *epos, uint64_t nbytes, const float *range_lo,
+                         const float *range_hi, const uint8_t *range_type, int32_t nranges,
+                         float *xyz, float *mq, uint8_t *type);
+
 /* POS bytes + ranges + triangle hull -> a cloud, entirely on the device:
  * pp_pos_decode, then pp_ion2surf with R (an empty hull gives every ion
  * fl32(R * R), as solver::surface_distancing2 does without a hull,
@@ -233,6 +242,11 @@ pp_status pp_cloud_from_pos(pp_ctx *ctx, const void *pos, uint64_t nbytes, const
                             const float *range_hi, const uint8_t *range_type, int32_t nranges,
                             const float *triangles, uint64_t ntri, float R, float cell_width,
                             pp_cloud **out);
+/* The same from EPOS bytes (44-byte records). */
+pp_status pp_cloud_from_epos(pp_ctx *ctx, const void *epos, uint64_t nbytes,
+                             const float *range_lo, const float *range_hi,
+                             const uint8_t *range_type, int32_t nranges, const float *triangles,
+                             uint64_t ntri, float R, float cell_width, pp_cloud **out);
 
 /* kNN range of MKNN / 1NN jobs without CNTNB: per neighbour type, per-center
  * histograms cover only the distance within which `factor` x k neighbours

@@ -60,6 +60,7 @@ def _load():
         lib.pp_oracle_closest_tri.restype = C.c_float
         lib.pp_oracle_ion2surf.argtypes = [P, C.c_int64, P, C.c_int64, C.c_float, P]
         lib.pp_oracle_pos_decode.argtypes = [P, C.c_int64, P, P, P, C.c_int32, P, P, P]
+        lib.pp_oracle_epos_decode.argtypes = [P, C.c_int64, P, P, P, C.c_int32, P, P, P]
         lib.pp_oracle_npcorr.argtypes = [C.c_int64, P, P, P, C.c_int32, P, P, C.c_float,
                                          C.c_float, C.c_int64, P]
         lib.pp_oracle_npcorr.restype = C.c_int
@@ -176,12 +177,12 @@ def ion2surf(xyz, triangles, R) -> np.ndarray:
     return out
 
 
-def pos_decode(pos: bytes, ranges=None):
-    """POS bytes -> (xyz, mq, types) by the reference's byte reversal and
-    first-matching closed range."""
+def pos_decode(pos: bytes, ranges=None, epos: bool = False):
+    """POS (or EPOS) bytes -> (xyz, mq, types) by the reference's byte
+    reversal and first-matching closed range."""
     lib = _load()
     buf = np.frombuffer(pos, dtype=np.uint8)
-    n = buf.shape[0] // 16
+    n = buf.shape[0] // (44 if epos else 16)
     if ranges is None:
         lo = hi = np.zeros(0, np.float32)
         rt = np.zeros(0, np.uint8)
@@ -192,8 +193,9 @@ def pos_decode(pos: bytes, ranges=None):
     xyz = np.empty((n, 3), np.float32)
     mq = np.empty(n, np.float32)
     ty = np.empty(n, np.uint8)
-    lib.pp_oracle_pos_decode(buf.ctypes.data, n, lo.ctypes.data, hi.ctypes.data, rt.ctypes.data,
-                             lo.shape[0], xyz.ctypes.data, mq.ctypes.data, ty.ctypes.data)
+    fn = lib.pp_oracle_epos_decode if epos else lib.pp_oracle_pos_decode
+    fn(buf.ctypes.data, n, lo.ctypes.data, hi.ctypes.data, rt.ctypes.data, lo.shape[0],
+       xyz.ctypes.data, mq.ctypes.data, ty.ctypes.data)
     return xyz, mq, ty
 
 

@@ -669,16 +669,16 @@ void pp_oracle_ion2surf(const float *xyz, int64_t n, const float *tri, int64_t n
  * (src/PARAPROBE_PeriodicTable.cpp:65-91): the first range in the given order
  * with !(mq < lo) && !(mq > hi) gives the type, else 0.
  */
-void pp_oracle_pos_decode(const uint8_t *rec, int64_t n, const float *lo, const float *hi,
-                          const uint8_t *rtype, int32_t nranges, float *xyz, float *mq,
-                          uint8_t *type)
+static void orc_records_decode(const uint8_t *rec, int64_t n, int64_t stride, const float *lo,
+                               const float *hi, const uint8_t *rtype, int32_t nranges,
+                               float *xyz, float *mq, uint8_t *type)
 {
     for (int64_t i = 0; i < n; ++i) {
         float v[4];
         for (int k = 0; k < 4; ++k) {
             uint8_t b[4];
             for (int q = 0; q < 4; ++q)
-                b[q] = rec[16 * i + 4 * k + 3 - q];
+                b[q] = rec[stride * i + 4 * k + 3 - q];
             memcpy(&v[k], b, 4);
         }
         xyz[3 * i] = v[0];
@@ -791,4 +791,20 @@ int pp_oracle_npcorr(int64_t n, const float *xyz, const uint8_t *type, const flo
     }
     free(buf);
     return 0;
+}
+
+void pp_oracle_pos_decode(const uint8_t *rec, int64_t n, const float *lo, const float *hi,
+                          const uint8_t *rtype, int32_t nranges, float *xyz, float *mq,
+                          uint8_t *type)
+{
+    orc_records_decode(rec, n, 16, lo, hi, rtype, nranges, xyz, mq, type);
+}
+
+/* EPOS (src/PARAPROBE_EPOSEndianness.h:86-160): 44-byte records whose first
+ * four big-endian floats are x, y, z and the mass-to-charge. */
+void pp_oracle_epos_decode(const uint8_t *rec, int64_t n, const float *lo, const float *hi,
+                           const uint8_t *rtype, int32_t nranges, float *xyz, float *mq,
+                           uint8_t *type)
+{
+    orc_records_decode(rec, n, 44, lo, hi, rtype, nranges, xyz, mq, type);
 }

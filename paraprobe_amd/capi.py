@@ -65,6 +65,8 @@ EXPORTED = [
     "pp_cloud_from_pos",
     "pp_npcorr_geometry",
     "pp_npcorr_run",
+    "pp_epos_decode",
+    "pp_cloud_from_epos",
 ]
 
 
@@ -163,6 +165,9 @@ def load_library() -> C.CDLL:
     lib.pp_npcorr_geometry.restype = C.c_int64
     lib.pp_npcorr_run.argtypes = [P, P, C.c_int32, P, P, C.c_float, C.c_float, C.c_int64, P,
                                   C.POINTER(C.c_int64)]
+    lib.pp_epos_decode.argtypes = [P, P, C.c_uint64, P, P, P, C.c_int32, P, P, P]
+    lib.pp_cloud_from_epos.argtypes = [P, P, C.c_uint64, P, P, P, C.c_int32, P, C.c_uint64,
+                                       C.c_float, C.c_float, C.POINTER(P)]
     lib.pp_cloud_from_pos.argtypes = [P, P, C.c_uint64, P, P, P, C.c_int32, P, C.c_uint64,
                                       C.c_float, C.c_float, C.POINTER(P)]
     for name in EXPORTED:
@@ -418,35 +423,36 @@ class SpatStat:
                                            out.ctypes.data, C.byref(nb)))
         return out
 
-    def pos_decode(self, pos: bytes, ranges=None, with_mq: bool = True):
-        """POS bytes (16 per ion, big-endian x, y, z, mq) -> (xyz, mq, types);
-        ranges = (lo, hi, type) arrays in the reference's order, or None."""
+    def pos_decode(self, pos: bytes, ranges=None, with_mq: bool = True, epos: bool = False):
+        """POS bytes (16 per ion, big-endian x, y, z, mq) or EPOS bytes (44 per
+        ion) -> (xyz, mq, types); ranges = (lo, hi, type) arrays in the
+        reference's order, or None."""
         buf = np.frombuffer(pos, dtype=np.uint8)
-        n = buf.shape[0] // 16
+        n = buf.shape[0] // (44 if epos else 16)
         lo, hi, rt = _ranges(ranges)
         xyz = np.empty((n, 3), np.float32)
         mq = np.empty(n, np.float32) if with_mq else None
         ty = np.empty(n, np.uint8)
-        self._check(self.lib.pp_pos_decode(self.ctx, buf.ctypes.data, buf.shape[0],
-                                           lo.ctypes.data, hi.ctypes.data, rt.ctypes.data,
-                                           lo.shape[0], xyz.ctypes.data,
-                                           None if mq is None else mq.ctypes.data,
-                                           ty.ctypes.data))
+        fn = self.lib.pp_epos_decode if epos else self.lib.pp_pos_decode
+        self._check(fn(self.ctx, buf.ctypes.data, buf.shape[0], lo.ctypes.data, hi.ctypes.data,
+                       rt.ctypes.data, lo.shape[0], xyz.ctypes.data,
+                       None if mq is None else mq.ctypes.data, ty.ctypes.data))
         return xyz, mq, ty
 
     def cloud_from_pos(self, pos: bytes, ranges, triangles, R: float,
-                       cell_width: float = 0.0) -> Cloud:
-        """POS bytes + ranges + hull -> a cloud, decoded and distanced on the GPU."""
+                       cell_width: float = 0.0, epos: bool = False) -> Cloud:
+        """POS (or EPOS) bytes + ranges + hull -> a cloud, decoded and
+        distanced on the GPU."""
         buf = np.frombuffer(pos, dtype=np.uint8)
         lo, hi, rt = _ranges(ranges)
         tri = np.ascontiguousarray(triangles if triangles is not None else
                                    np.zeros((0, 9), np.float32), dtype=np.float32).reshape(-1, 9)
         h = C.c_void_p()
-        self._check(self.lib.pp_cloud_from_pos(self.ctx, buf.ctypes.data, buf.shape[0],
-                                               lo.ctypes.data, hi.ctypes.data, rt.ctypes.data,
-                                               lo.shape[0], tri.ctypes.data, tri.shape[0],
-                                               C.c_float(R), C.c_float(cell_width), C.byref(h)))
-        return Cloud(self, h, buf.shape[0] // 16)
+        fn = self.lib.pp_cloud_from_epos if epos else self.lib.pp_cloud_from_pos
+        self._check(fn(self.ctx, buf.ctypes.data, buf.shape[0], lo.ctypes.data, hi.ctypes.data,
+                       rt.ctypes.data, lo.shape[0], tri.ctypes.data, tri.shape[0], C.c_float(R),
+                       C.c_float(cell_width), C.byref(h)))
+        return Cloud(self, h, buf.shape[0] // (44 if epos else 16))
 
     def pos_decode_device(self, d_pos: int, nbytes: int, ranges, d_xyz: int, d_mq: int,
                           d_type: int):

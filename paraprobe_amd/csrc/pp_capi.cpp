@@ -2128,17 +2128,20 @@ pp_status pp_ion2surf_device(pp_ctx *c, const float *d_xyz, uint64_t n, const fl
 
 namespace {
 
-// POS decoding and ranging; dev: pos and the outputs are device memory.
+// POS (words = 4) or EPOS (words = 11) decoding and ranging; dev: pos and
+// the outputs are device memory.
 pp_status pos_decode_run(pp_ctx *c, const void *pos, uint64_t nbytes, const float *lo,
                          const float *hi, const uint8_t *rt, int32_t nranges, float *xyz,
-                         float *mq, uint8_t *type, bool dev)
+                         float *mq, uint8_t *type, bool dev, int words = 4)
 {
     if (!c)
         return PP_ERR_INVALID_ARG;
-    if (nbytes % 16 != 0)
+    const uint64_t rb = 4ull * (uint64_t)words;
+    if (nbytes % rb != 0)
         return fail(c, PP_ERR_INVALID_ARG,
-                    "POS size is not an integer multiple of the 16-byte record");
-    const uint64_t n = nbytes / 16;
+                    words == 4 ? "POS size is not an integer multiple of the 16-byte record"
+                               : "EPOS size is not an integer multiple of the 44-byte record");
+    const uint64_t n = nbytes / rb;
     if (n > 0xFFFFFFFDull)
         return fail(c, PP_ERR_INVALID_ARG, "more than 2^32 - 3 ions");
     if (nranges < 0 || nranges > kMaxRanges)
@@ -2147,8 +2150,8 @@ pp_status pos_decode_run(pp_ctx *c, const void *pos, uint64_t nbytes, const floa
         return fail(c, PP_ERR_INVALID_ARG, "NULL range arrays");
     if (n > 0 && (!pos || !xyz || !type))
         return fail(c, PP_ERR_INVALID_ARG, "NULL input or output arrays");
-    if (dev && ((uintptr_t)pos & 15u))
-        return fail(c, PP_ERR_INVALID_ARG, "device POS buffer must be 16-byte aligned");
+    if (dev && ((uintptr_t)pos & (words == 4 ? 15u : 3u)))
+        return fail(c, PP_ERR_INVALID_ARG, "device record buffer misaligned");
     if (n == 0)
         return PP_OK;
     PP_HIP(c, hipSetDevice(c->device));
@@ -2157,7 +2160,7 @@ pp_status pos_decode_run(pp_ctx *c, const void *pos, uint64_t nbytes, const floa
     m0.take<float>(2 * (size_t)nranges + 1);
     m0.take<uint8_t>((size_t)nranges + 1);
     if (!dev) {
-        m0.take<uint32_t>(4 * n);
+        m0.take<uint32_t>((size_t)words * n);
         m0.take<float>(3 * n);
         m0.take<float>(n);
         m0.take<uint8_t>(n);
@@ -2178,7 +2181,7 @@ pp_status pos_decode_run(pp_ctx *c, const void *pos, uint64_t nbytes, const floa
     float *d_xyz = xyz, *d_mq = mq;
     uint8_t *d_type = type;
     if (!dev) {
-        uint32_t *p = m.take<uint32_t>(4 * n);
+        uint32_t *p = m.take<uint32_t>((size_t)words * n);
         d_xyz = m.take<float>(3 * n);
         float *q = m.take<float>(n);
         d_type = m.take<uint8_t>(n);
@@ -2186,8 +2189,8 @@ pp_status pos_decode_run(pp_ctx *c, const void *pos, uint64_t nbytes, const floa
         PP_HIP(c, hipMemcpyAsync(p, pos, nbytes, hipMemcpyHostToDevice, s));
         d_pos = p;
     }
-    PP_HIP(c, launch_posdecode(d_pos, n, d_rng, d_rng + nranges, d_rt, nranges, d_xyz, d_mq,
-                               d_type, s));
+    PP_HIP(c, launch_posdecode(d_pos, n, words, d_rng, d_rng + nranges, d_rt, nranges, d_xyz,
+                               d_mq, d_type, s));
     if (!dev) {
         PP_HIP(c, hipMemcpyAsync(xyz, d_xyz, 12 * n, hipMemcpyDeviceToHost, s));
         if (mq)
@@ -2214,23 +2217,24 @@ pp_status pp_pos_decode_device(pp_ctx *c, const void *d_pos, uint64_t nbytes, co
     return pos_decode_run(c, d_pos, nbytes, lo, hi, rt, nranges, d_xyz, d_mq, d_type, true);
 }
 
-pp_status pp_cloud_from_pos(pp_ctx *c, const void *pos, uint64_t nbytes, const float *lo,
-                            const float *hi, const uint8_t *rt, int32_t nranges,
-                            const float *tri, uint64_t ntri, float R, float cell_width,
-                            pp_cloud **out)
+namespace {
+
+pp_status cloud_from_records(pp_ctx *c, const void *pos, uint64_t nbytes, int words,
+                             const float *lo, const float *hi, const uint8_t *rt,
+                             int32_t nranges, const float *tri, uint64_t ntri, float R,
+                             float cell_width, pp_cloud **out)
 {
     if (!c || !out || (nbytes > 0 && !pos) || (ntri > 0 && !tri))
         return fail(c, PP_ERR_INVALID_ARG, "NULL argument");
-    if (nbytes % 16 != 0)
-        return fail(c, PP_ERR_INVALID_ARG,
-                    "POS size is not an integer multiple of the 16-byte record");
+    if (nbytes % (4ull * words) != 0)
+        return fail(c, PP_ERR_INVALID_ARG, "file size is not an integer multiple of the record");
     if (ntri >= 0xFFFFFFF0ull)
         return fail(c, PP_ERR_INVALID_ARG, "more than 2^32 - 16 triangles");
-    const uint64_t n = nbytes / 16;
+    const uint64_t n = nbytes / (4ull * words);
     PP_HIP(c, hipSetDevice(c->device));
     hipStream_t s = c->main.stream;
     Carver m0(nullptr);
-    m0.take<uint32_t>(4 * n + 4);
+    m0.take<uint32_t>((size_t)words * n + 4);
     m0.take<float>(3 * n + 1);
     m0.take<uint8_t>(n + 1);
     m0.take<float>(n + 1);
@@ -2239,7 +2243,7 @@ pp_status pp_cloud_from_pos(pp_ctx *c, const void *pos, uint64_t nbytes, const f
     PP_HIP(c, hipMalloc(&mem, m0.off + 256));
     std::unique_ptr<void, void (*)(void *)> own(mem, [](void *p) { (void)hipFree(p); });
     Carver m(mem);
-    uint32_t *d_pos = m.take<uint32_t>(4 * n + 4);
+    uint32_t *d_pos = m.take<uint32_t>((size_t)words * n + 4);
     float *d_xyz = m.take<float>(3 * n + 1);
     uint8_t *d_type = m.take<uint8_t>(n + 1);
     float *d_s2 = m.take<float>(n + 1);
@@ -2249,12 +2253,39 @@ pp_status pp_cloud_from_pos(pp_ctx *c, const void *pos, uint64_t nbytes, const f
     if (ntri > 0)
         PP_HIP(c, hipMemcpyAsync(d_tri, tri, 36 * ntri, hipMemcpyHostToDevice, s));
     pp_status st = pos_decode_run(c, d_pos, nbytes, lo, hi, rt, nranges, d_xyz, nullptr, d_type,
-                                  true);
+                                  true, words);
     if (st != PP_OK)
         return st;
     if ((st = ion2surf_run(c, d_xyz, n, d_tri, ntri, R, d_s2, true)) != PP_OK)
         return st;
     return upload_common(c, d_xyz, d_type, d_s2, n, cell_width, hipMemcpyDeviceToDevice, out);
+}
+
+} // namespace
+
+pp_status pp_cloud_from_pos(pp_ctx *c, const void *pos, uint64_t nbytes, const float *lo,
+                            const float *hi, const uint8_t *rt, int32_t nranges,
+                            const float *tri, uint64_t ntri, float R, float cell_width,
+                            pp_cloud **out)
+{
+    return cloud_from_records(c, pos, nbytes, 4, lo, hi, rt, nranges, tri, ntri, R, cell_width,
+                              out);
+}
+
+pp_status pp_cloud_from_epos(pp_ctx *c, const void *epos, uint64_t nbytes, const float *lo,
+                             const float *hi, const uint8_t *rt, int32_t nranges,
+                             const float *tri, uint64_t ntri, float R, float cell_width,
+                             pp_cloud **out)
+{
+    return cloud_from_records(c, epos, nbytes, 11, lo, hi, rt, nranges, tri, ntri, R, cell_width,
+                              out);
+}
+
+pp_status pp_epos_decode(pp_ctx *c, const void *epos, uint64_t nbytes, const float *lo,
+                         const float *hi, const uint8_t *rt, int32_t nranges, float *xyz,
+                         float *mq, uint8_t *type)
+{
+    return pos_decode_run(c, epos, nbytes, lo, hi, rt, nranges, xyz, mq, type, false, 11);
 }
 
 int64_t pp_npcorr_geometry(float rmax, float rincr)

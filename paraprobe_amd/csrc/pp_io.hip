@@ -10,7 +10,8 @@
 // ascending, Unknown's dummy range [0, 0.001] first), whose closed interval
 // holds mq (`!(mq < lo) && !(mq > hi)`, so a NaN mq matches the first range);
 // no match gives type 0 (UNKNOWNTYPE).  One thread per record; the ranges sit
-// in LDS.  HBM-bound: 16 bytes read and 17 written per ion.
+// in LDS.  HBM-bound: 16 bytes read (POS; 44 bytes of EPOS records, of which
+// 16 are used) and 17 written per ion.
 #include "pp_launch.h"
 
 namespace pp {
@@ -29,6 +30,10 @@ __device__ inline float be_float(uint32_t w) { return __uint_as_float(__builtin_
 
 } // namespace
 
+// W: 32-bit words per record, 4 for POS (x, y, z, mq) and 11 for EPOS (x, y,
+// z, mq, tof, vdc, vpu, detx, dety and two ints, src/PARAPROBE_EPOSEndianness.h:
+// 86-160); the first four words are the ones decoded here.
+template <int W>
 __global__ __launch_bounds__(256) void k_posdecode(const uint32_t *__restrict__ rec, uint64_t n,
                                                    const float *__restrict__ lo,
                                                    const float *__restrict__ hi,
@@ -49,7 +54,13 @@ __global__ __launch_bounds__(256) void k_posdecode(const uint32_t *__restrict__ 
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n)
         return;
-    const uint4 w = reinterpret_cast<const uint4 *>(rec)[i];
+    uint4 w;
+    if (W == 4) {
+        w = reinterpret_cast<const uint4 *>(rec)[i];
+    } else {
+        const uint32_t *r = rec + (uint64_t)W * i;
+        w = make_uint4(r[0], r[1], r[2], r[3]);
+    }
     const float x = be_float(w.x), y = be_float(w.y), z = be_float(w.z), m = be_float(w.w);
     uint8_t t = 0;
     for (int r = 0; r < nranges; ++r)
@@ -65,15 +76,21 @@ __global__ __launch_bounds__(256) void k_posdecode(const uint32_t *__restrict__ 
     type[i] = t;
 }
 
-hipError_t launch_posdecode(const uint32_t *rec, uint64_t n, const float *lo, const float *hi,
-                            const uint8_t *rtype, int nranges, float *xyz, float *mq,
-                            uint8_t *type, hipStream_t s)
+hipError_t launch_posdecode(const uint32_t *rec, uint64_t n, int words, const float *lo,
+                            const float *hi, const uint8_t *rtype, int nranges, float *xyz,
+                            float *mq, uint8_t *type, hipStream_t s)
 {
     if (n == 0)
         return hipSuccess;
     const size_t lds = (size_t)nranges * 9 + 16;
-    hipLaunchKernelGGL(k_posdecode, dim3(blocks_for(n, 256)), dim3(256), lds, s, rec, n, lo, hi,
-                       rtype, nranges, xyz, mq, type);
+    if (words == 4)
+        hipLaunchKernelGGL(k_posdecode<4>, dim3(blocks_for(n, 256)), dim3(256), lds, s, rec, n,
+                           lo, hi, rtype, nranges, xyz, mq, type);
+    else if (words == 11)
+        hipLaunchKernelGGL(k_posdecode<11>, dim3(blocks_for(n, 256)), dim3(256), lds, s, rec, n,
+                           lo, hi, rtype, nranges, xyz, mq, type);
+    else
+        return hipErrorInvalidValue;
     return hipGetLastError();
 }
 

@@ -112,12 +112,13 @@ hipError_t launch_surftile(const float4 *ions, const SurfTile *tiles, uint32_t n
                            const uint32_t *tcellstart, const uint32_t *list, const float *tri,
                            const float *box, float R, float delta, float *out, hipStream_t s);
 
-// POS records (16 bytes each, 16-byte aligned) -> xyz, mq (may be null) and
+// POS records (words = 4: 16 bytes each, 16-byte aligned) or EPOS records
+// (words = 11: 44 bytes each, 4-byte aligned) -> xyz, mq (may be null) and
 // ranged types (pp_io.hip); at most kMaxRanges ranges.
 constexpr int kMaxRanges = 4096;
-hipError_t launch_posdecode(const uint32_t *rec, uint64_t n, const float *lo, const float *hi,
-                            const uint8_t *rtype, int nranges, float *xyz, float *mq,
-                            uint8_t *type, hipStream_t s);
+hipError_t launch_posdecode(const uint32_t *rec, uint64_t n, int words, const float *lo,
+                            const float *hi, const uint8_t *rtype, int nranges, float *xyz,
+                            float *mq, uint8_t *type, hipStream_t s);
 
 // NPCORR (pp_npcorr.hip): one wavefront per center over a cell list at least
 // R/2 wide; per-task counts of the K-th env neighbour's binned difference vector.

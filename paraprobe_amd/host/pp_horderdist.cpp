@@ -961,7 +961,12 @@ int32_t pph_run_pos(pp_ctx *ctx, const char *xml, const char *pos_path, const ch
         return PP_ERR_INVALID_ARG;
     }
     std::vector<char> bytes((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
-    if (bytes.size() % 16 != 0) {
+    // a name ending in "epos" holds 44-byte EPOS records (load_epos_sequentially,
+    // S.cpp:6933-7066), anything else 16-byte POS records
+    const std::string pp(pos_path);
+    const bool epos = pp.size() >= 4 && pp.compare(pp.size() - 4, 4, "epos") == 0;
+    const size_t rec = epos ? 44 : 16;
+    if (bytes.size() % rec != 0) {
         std::cerr << "ERROR::Size of file " << pos_path
                   << " is not an integer multiple of expected datatype" << std::endl;
         return PP_ERR_INVALID_ARG;
@@ -984,8 +989,9 @@ int32_t pph_run_pos(pp_ctx *ctx, const char *xml, const char *pos_path, const ch
     pse.flat_ranges(lo, hi, rt);
     const float R = std::max(s.rmax, s.carving_radius);
     pp_cloud *cl = nullptr;
-    pp_status st = pp_cloud_from_pos(ctx, bytes.data(), bytes.size(), lo.data(), hi.data(),
-                                     rt.data(), (int32_t)lo.size(), triangles, ntri, R, 0.f, &cl);
+    pp_status st = (epos ? pp_cloud_from_epos : pp_cloud_from_pos)(
+        ctx, bytes.data(), bytes.size(), lo.data(), hi.data(), rt.data(), (int32_t)lo.size(),
+        triangles, ntri, R, 0.f, &cl);
     if (st != PP_OK)
         return st;
     std::vector<const char *> names;

@@ -157,3 +157,66 @@ def test_run_pos_end_to_end(ctx, tmp_path):
               [int(float(rows[8 + nb][1]))]
         np.testing.assert_array_equal(got, ref["rdf"][i])
     assert len(os.listdir(out)) == 2
+
+
+def _epos_of(v):
+    """EPOS records for rows v = (x, y, z, mq): the other five floats and the
+    two ints filled with recognisable values."""
+    n = v.shape[0]
+    f = np.zeros((n, 9), np.float32)
+    f[:, :4] = v
+    f[:, 4:] = np.arange(5, dtype=np.float32) + 100.0
+    rec = np.zeros(n, dtype=[("f", ">f4", 9), ("i", ">i4", 2)])
+    rec["f"] = f
+    rec["i"] = np.stack([np.arange(n), -np.arange(n)], axis=1)
+    return rec.tobytes()
+
+
+def test_epos_decode_and_cloud(ctx, tmp_path):
+    _, ranges = H.read_rangefile(RRNG)
+    pos, v = _records(300_001, 6)
+    ep = _epos_of(v)
+    assert len(ep) == 44 * v.shape[0]
+    _say("epos decode")
+    xyz, mq, ty = ctx.pos_decode(ep, ranges, epos=True)
+    oxyz, omq, oty = O.pos_decode(ep, ranges, epos=True)
+    assert np.array_equal(xyz.view(np.uint32), oxyz.view(np.uint32))
+    assert np.array_equal(mq.view(np.uint32), omq.view(np.uint32))
+    assert np.array_equal(ty, oty)
+    pxyz, _, pty = ctx.pos_decode(pos, ranges)
+    assert np.array_equal(xyz.view(np.uint32), pxyz.view(np.uint32)) and np.array_equal(ty, pty)
+    with pytest.raises(Exception):
+        ctx.pos_decode(ep[:-4], ranges, epos=True)
+    # the cloud and the file-driven run from EPOS equal those from POS
+    t, rec, want_types = _tip_pos(9)
+    v2 = np.frombuffer(rec, ">f4").reshape(-1, 4).astype(np.float32)
+    ep2 = _epos_of(v2)
+    tri = synth.hull("cylinder", 10.0, 10.0, 40.0, 64, 32)
+    a = ctx.cloud_from_pos(ep2, ranges, tri, 2.0, epos=True)
+    b = ctx.cloud_from_pos(rec, ranges, tri, 2.0)
+    try:
+        job = Job([((1,), (4,)), ((4,), (1, 2))], 0.0, 0.02, 2.0, FAM_RDF | FAM_MKNN,
+                  kvalues=[1, 7])
+        ra, rb = ctx.run(a, job), ctx.run(b, job)
+        for k, val in ra.arrays().items():
+            np.testing.assert_array_equal(val, getattr(rb, k))
+    finally:
+        a.free()
+        b.free()
+    xml = """<Parameters>
+  <AnalysisSpatDistrType>1</AnalysisSpatDistrType>
+  <DescrStatTasksCode>Al-Al;Sc-Al,Fe</DescrStatTasksCode>
+  <SpatStatRadiusMin>0.0</SpatStatRadiusMin>
+  <SpatStatRadiusIncr>0.05</SpatStatRadiusIncr>
+  <SpatStatRadiusMax>1.5</SpatStatRadiusMax>
+</Parameters>"""
+    for name, data in (("tip.pos", rec), ("tip.epos", ep2)):
+        (tmp_path / name).write_bytes(data)
+        d = tmp_path / ("out_" + name.replace(".", "_"))
+        d.mkdir()
+        _say(f"run_pos {name}")
+        assert H.run_pos(ctx, xml, str(tmp_path / name), RRNG, tri, 4, 150_000, 12_000.0,
+                         str(d)) == 0
+    for f in sorted(p.name for p in (tmp_path / "out_tip_pos").iterdir()):
+        assert (tmp_path / "out_tip_pos" / f).read_bytes() == \
+            (tmp_path / "out_tip_epos" / f).read_bytes()

@@ -113,3 +113,17 @@ def test_reference_settings_fixture():
     assert radii == [0.0, f32(0.05), f32(2.5)]
     assert mknn == ""                         # SpatStatMKNNCode is read only with MKNN
     assert H.label_randomization(xml) == 0
+
+
+def test_epos_decode_oracle_against_struct_encoding():
+    rng = np.random.default_rng(2)
+    rows = rng.normal(0, 30, size=(300, 9)).astype(np.float32)
+    ints = rng.integers(-2**31, 2**31 - 1, size=(300, 2))
+    data = b"".join(struct.pack(">9f2i", *[float(v) for v in r], *[int(v) for v in q])
+                    for r, q in zip(rows, ints))
+    _, ranges = H.read_rangefile(RRNG)
+    xyz, mq, ty = O.pos_decode(data, ranges, epos=True)
+    assert np.array_equal(xyz.view(np.uint32), rows[:, :3].copy().view(np.uint32))
+    assert np.array_equal(mq.view(np.uint32), rows[:, 3].copy().view(np.uint32))
+    _, _, want = O.pos_decode(_pos_bytes(rows[:, :4].tolist()), ranges)
+    assert np.array_equal(ty, want)
