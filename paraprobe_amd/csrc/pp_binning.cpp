@@ -134,6 +134,15 @@ bool EdgeTable::build(const HistGeometry &g, std::string *why)
             *why = "bin edges too far from their ideal positions for the nearest-edge guess";
         return false;
     }
+    // the kernel turns the floor-form index into an integer with a
+    // saturating unsigned conversion (negative -> 0, no upper clamp): an
+    // in-ball index y stays below ncat as long as its rounding (a few 2^-24
+    // of R tmp + |1 - start tmp|) stays far below 1
+    if (std::fabs(g.end) * g.tmp + std::fabs(1.0 - g.start * g.tmp) + 2.0 >= 1048576.0) {
+        if (why)
+            *why = "too many bins for the float index coordinate";
+        return false;
+    }
     idx_scale = (float)g.tmp;
     idx_off = (float)(1.5 - g.start * g.tmp);
     idx_off1 = (float)(1.0 - g.start * g.tmp);

@@ -661,8 +661,8 @@ __device__ inline uint32_t next_tile(const QArgs &a, uint32_t myx, int &qx, uint
     return a.end_tile;
 }
 
-// Category of an in-range squared distance: floor of the index coordinate
-// where it is clear of every integer by amb; when any active lane is closer
+// Category of an in-range squared distance (d2 <= fl(R * R)): floor of the
+// index coordinate where it is clear of every integer by amb; when any active lane is closer
 // than that to a bin edge (rare), the wave settles every lane with the exact
 // edge table instead.  Branch-free apart from that wave-uniform test, and no
 // LDS read on the common path.
@@ -676,7 +676,9 @@ __device__ inline int category_of(float d2, float isc, float ioff1, float amb, f
         const int q = (int)__builtin_amdgcn_fmed3f(y + 0.5f, 1.0f, Cm1f);
         return q - (__float_as_uint(d2) < s_edges[q] ? 1 : 0);
     }
-    return (int)__builtin_amdgcn_fmed3f(y, 0.0f, Cm1f);
+    // floor for y >= 0, 0 for y < 0 (v_cvt_u32_f32 saturates); y < ncat for
+    // every in-ball distance (the edge table's build checks the bound)
+    return (int)__float2uint_rz(y);
 }
 
 // The persistent query kernel.  MODE 0: pair counts per (center type, env
