@@ -123,7 +123,8 @@ hipError_t launch_posdecode(const uint32_t *rec, uint64_t n, int words, const fl
 // NPCORR (pp_npcorr.hip): one wavefront per center over a cell list at least
 // R/2 wide; per-task counts of the K-th env neighbour's binned difference vector.
 constexpr int kNpcCap = 512;      // LDS entries per wavefront; K < kNpcCap - 64
-constexpr int kNpcBuckets = 512;  // distance buckets per wavefront
+constexpr int kNpcBuckets = 256;  // distance buckets per task
+constexpr int kNpcGroup = 4;      // tasks counted in one sweep
 struct NpcArgs {
     const float4 *pos;          // cell-sorted (x, y, z, type bits)
     const float *s2;            // cell-sorted ion2surf

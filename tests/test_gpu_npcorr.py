@@ -119,3 +119,40 @@ def test_npcorr_crowded_bucket(ctx):
         want = O.npcorr(xyz, ty, s2, tasks, 1.0, 0.05, K)
         assert np.array_equal(got, want)
         assert got.sum() == 2
+
+
+def test_npcorr_bucket_edges_and_many_tasks(ctx):
+    """Neighbours at the distances that separate the kernel's distance buckets
+    (k R / 256, along the axes and along random directions), so that the
+    shell each pick is searched in is cut exactly at its candidates; and six
+    tasks per center, more than one counting sweep serves."""
+    rng = np.random.default_rng(21)
+    R = 1.0
+    centers = rng.uniform(2.0, 4.0, size=(6, 3)).astype(np.float32)
+    parts = [centers]
+    for c in centers:
+        d = (np.arange(1, 256, dtype=np.float64) * (R / 256.0))
+        axis = np.zeros((d.size, 3))
+        axis[np.arange(d.size), np.arange(d.size) % 3] = np.where(np.arange(d.size) % 2, d, -d)
+        u = rng.normal(size=(d.size, 3))
+        u /= np.linalg.norm(u, axis=1, keepdims=True)
+        parts.append((c + axis).astype(np.float32))
+        parts.append((c + u * d[:, None]).astype(np.float32))
+    parts.append(rng.uniform(0, 6, size=(3000, 3)).astype(np.float32))
+    xyz = np.concatenate(parts)
+    ty = rng.choice([1, 2, 3], size=xyz.shape[0], p=[0.5, 0.3, 0.2]).astype(np.uint8)
+    ty[:6] = 1
+    s2 = np.full(xyz.shape[0], 50.0, np.float32)
+    s2[6:] = rng.uniform(0, 2, size=xyz.shape[0] - 6).astype(np.float32)
+    tasks = [((1,), (1,)), ((1,), (2,)), ((1,), (3,)), ((1,), (1, 2)), ((1, 2), (2, 3)),
+             ((1,), (1, 2, 3))]
+    for K in (0, 5, 60):
+        _say(f"npcorr bucket edges K={K}")
+        cloud = ctx.upload(xyz, ty, s2)
+        try:
+            got = ctx.npcorr(cloud, tasks, R, 0.02, K)
+        finally:
+            cloud.free()
+        want = O.npcorr(xyz, ty, s2, tasks, R, 0.02, K)
+        assert np.array_equal(got, want)
+        assert got.sum() > 0
