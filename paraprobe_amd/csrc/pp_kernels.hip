@@ -890,42 +890,50 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : (T == 8 && MODE
             const bool rdf = (J.families & (FAM_RDF | FAM_RIPK)) != 0;
             const bool count_in = J.count_in != 0;
             const gfvec4 *__restrict__ pos = (const gfvec4 *)J.pos;
+            const uint32_t *gedges = J.edges;  // the hybrid mode's exact edge table
             uint32_t *s_acc = PC ? s_H : s_P;
             const int nch = (int)rfl(s_rowchunk[nrows]);
             unsigned long long my_in = 0;
-            // software pipeline: the loads of chunk q + kWaves are in flight
-            // while chunk q is evaluated against the T centers
+            // each wavefront takes a contiguous block of the tile's chunks and
+            // walks it with a wave-uniform cursor (row r, first index jb of the
+            // chunk, row end re); two chunks alternate, so that the load of the
+            // next is in flight while one is evaluated against the T centers
+            const int qa = (int)rfl((uint32_t)(wid * nch / kWaves));
+            const int qb = (int)rfl((uint32_t)((wid + 1) * nch / kWaves));
             int r = 0;
-            auto locate = [&](int q, uint32_t &jj, uint32_t &rs) -> bool {
-                while ((int)rfl(s_rowchunk[r + 1]) <= q)
+            uint32_t jb = 0, re = 0;
+            if (qa < qb) {
+                while ((int)rfl(s_rowchunk[r + 1]) <= qa)
                     ++r;
-                rs = rfl(s_rows[2 * r]);
-                const uint32_t re = rfl(s_rows[2 * r + 1]);
-                jj = rs + (uint32_t)(q - (int)rfl(s_rowchunk[r])) * kWave + lane;
-                return jj < re;
-            };
-            float4 pn = make_float4(0.f, 0.f, 0.f, 0.f);
-            bool okn = false;
-            if (wid < nch) {
-                uint32_t jj, rs;
-                okn = locate(wid, jj, rs);
-                pn = load_g4(pos, okn ? jj : rs);
+                jb = rfl(s_rows[2 * r]) + (uint32_t)(qa - (int)rfl(s_rowchunk[r])) * kWave;
+                re = rfl(s_rows[2 * r + 1]);
             }
-            for (int q = wid; q < nch; q += kWaves) {
-                float4 p = pn;
+            // the chunk at the cursor (lanes past the row end load its first ion)
+            auto fetch = [&](bool &ok) {
+                const uint32_t j = jb + (uint32_t)lane;
+                ok = j < re;
+                return load_g4(pos, ok ? j : jb);
+            };
+            // the next chunk (there is one: rows without chunks are skipped)
+            auto advance = [&]() {
+                jb += kWave;
+                if (jb >= re) {
+                    do {
+                        ++r;
+                    } while (rfl(s_rowchunk[r + 1]) == rfl(s_rowchunk[r]));
+                    jb = rfl(s_rows[2 * r]);
+                    re = rfl(s_rows[2 * r + 1]);
+                }
+            };
+            auto body = [&](float4 p, const bool ok) {
                 // keep every use of this chunk's data below this point, so the
                 // wait for its load is not hoisted into the prefetch of the next
                 asm volatile("" : "+v"(p.x), "+v"(p.y), "+v"(p.z), "+v"(p.w));
-                const bool ok = okn;
-                const int qn = q + kWaves;
-                if (qn < nch) {
-                    uint32_t jj, rs;
-                    okn = locate(qn, jj, rs);
-                    pn = load_g4(pos, okn ? jj : rs);
-                }
                 const float px = ok ? p.x : __builtin_nanf("");
                 const uint32_t ptype = __float_as_uint(p.w) & 0xFFu;
-                const int woff = ok ? s_envoff[ptype] : -1;
+                // (a lane past the row end has a NaN x, so every distance
+                // test fails and its type's table entries go unused)
+                const int woff = s_envoff[ptype];
                 const bool envok = woff >= 0;
                 // this candidate's pair-count row (only dereferenced when envok)
                 uint32_t *acc_row = (PC ? s_H : s_P) + woff;
@@ -994,13 +1002,12 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : (T == 8 && MODE
                     // hybrid mode: the direct-mode pair counts, and the pairs
                     // within the kNN range of their type also into the
                     // center's own histogram
-                    const int2 ki = ok ? s_kinfo[ptype] : make_int2(-1, (int)0xBF800000u);
+                    const int2 ki = s_kinfo[ptype];
                     const int kof = ki.x;
                     const float kthr = __int_as_float(ki.y);  // -1: no kNN for this type
                     uint32_t *krow = s_H + kof;  // only dereferenced when d2 <= kthr
                     const bool rdf_env = rdf && envok;
                     const float bthr = rdf_env ? R2 : kthr;
-                    const uint32_t *gedges = J.edges;
 #pragma unroll
                     for (int c = 0; c < T; ++c) {
                         if (c < cnt) {  // uniform: skips a partial tile's absent centers
@@ -1026,6 +1033,25 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : (T == 8 && MODE
                         }
                     }
                 }
+            };
+            // the loads are unconditional (past the block's last chunk the
+            // cursor stays and its chunk is loaded again, unused), so that no
+            // branch joins paths with different loads in flight, which would
+            // make the wait for one chunk wait for the other too
+            bool oka = false, okb = false;
+            float4 pa = make_float4(0.f, 0.f, 0.f, 0.f), pb = pa;
+            if (qa < qb)
+                pa = fetch(oka);
+            for (int q = qa; q < qb; q += 2) {
+                if (q + 1 < qb)
+                    advance();
+                pb = fetch(okb);
+                body(pa, oka);
+                if (q + 2 < qb)
+                    advance();
+                pa = fetch(oka);
+                if (q + 1 < qb)
+                    body(pb, okb);
             }
             inball += my_in;
         }
