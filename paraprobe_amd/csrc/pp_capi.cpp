@@ -10,6 +10,7 @@
 #include "../../include/pp_spatstat.h"
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
@@ -534,17 +535,11 @@ pp_status build_celllist(pp_ctx *c, Slot &s, pp_cloud *cl, float w, CellList &L)
     return build_on_grid(c, s, cl->xyz, cl->type, cl->s2, n, g, w, L);
 }
 
-// The typed candidate list of `mask` on the grid of `base` (built once per
-// cloud, base and type set; the caller holds no lock).
-pp_status get_sublist(pp_ctx *c, Slot &s, pp_cloud *cl, const CellList *base,
-                      const uint32_t mask[8], const CellList *&out)
+// The ions of the types in `mask`, cell-sorted on the grid of `base`, into
+// out (its own allocation, out.mem).
+pp_status build_typed_list(pp_ctx *c, Slot &s, const pp_cloud *cl, const CellList *base,
+                           const uint32_t mask[8], CellList &out)
 {
-    std::lock_guard<std::mutex> lk(cl->sub_mu);
-    for (const auto &sb : cl->subs)
-        if (sb.base == base && std::memcmp(sb.mask, mask, 32) == 0) {
-            out = &sb.cl;
-            return PP_OK;
-        }
     const uint64_t n = cl->n;
     Carver t0(nullptr);
     t0.take<uint32_t>(8);
@@ -583,16 +578,33 @@ pp_status get_sublist(pp_ctx *c, Slot &s, pp_cloud *cl, const CellList *base,
         (void)hipFree(tmp);
         return fail(c, PP_ERR_HIP, std::string("gather: ") + hipGetErrorString(e));
     }
+    st = build_on_grid(c, s, sx, sty, ss2, m, base->g, base->w, out, head);
+    (void)hipStreamSynchronize(s.stream);
+    (void)hipFree(tmp);
+    if (st != PP_OK && out.mem) {
+        (void)hipFree(out.mem);
+        out.mem = nullptr;
+    }
+    return st;
+}
+
+// The typed candidate list of `mask` on the grid of `base` (built once per
+// cloud, base and type set; the caller holds no lock).
+pp_status get_sublist(pp_ctx *c, Slot &s, pp_cloud *cl, const CellList *base,
+                      const uint32_t mask[8], const CellList *&out)
+{
+    std::lock_guard<std::mutex> lk(cl->sub_mu);
+    for (const auto &sb : cl->subs)
+        if (sb.base == base && std::memcmp(sb.mask, mask, 32) == 0) {
+            out = &sb.cl;
+            return PP_OK;
+        }
     cl->subs.emplace_back();
     pp_cloud::SubList &sb = cl->subs.back();
     std::memcpy(sb.mask, mask, 32);
     sb.base = base;
-    st = build_on_grid(c, s, sx, sty, ss2, m, base->g, base->w, sb.cl, head);
-    (void)hipStreamSynchronize(s.stream);
-    (void)hipFree(tmp);
+    const pp_status st = build_typed_list(c, s, cl, base, mask, sb.cl);
     if (st != PP_OK) {
-        if (sb.cl.mem)
-            (void)hipFree(sb.cl.mem);
         cl->subs.pop_back();
         return st;
     }
@@ -755,6 +767,11 @@ void fill_jobdev(JobRun &r)
     d.nx = r.cl->g.nx;
     d.ny = r.cl->g.ny;
     d.nz = r.cl->g.nz;
+    for (int a = 0; a < 3; ++a)
+        d.glo[a] = r.cl->g.lo[a];
+    d.gw = r.cl->g.w;
+    d.ginv = r.cl->g.inv_w;
+    d.reach = std::sqrt((double)p.R2) * (1.0 + std::ldexp(1.0, -12)) + 1e-4 * r.cl->g.w;
     d.nrows = (int)r.rows.size();
     d.rows = (const Row *)(r.tables + r.o_rows);
     d.centers = r.centers;
@@ -2341,21 +2358,69 @@ pp_status pp_npcorr_run(pp_ctx *c, const pp_cloud *cl, int32_t ntasks, const uin
     });
     if (st != PP_OK)
         return st;
+    // each task's candidates: the ions of its neighbour types on the same
+    // grid (one list per distinct type set)
+    std::vector<CellList> envl;
+    std::vector<int> lidx(ntasks);
+    struct FreeLists {
+        std::vector<CellList> &v;
+        ~FreeLists()
+        {
+            for (CellList &E : v)
+                if (E.mem)
+                    (void)hipFree(E.mem);
+        }
+    } free_lists{envl};
+    std::vector<std::array<uint32_t, 8>> masks;
+    for (int32_t t = 0; t < ntasks; ++t) {
+        std::array<uint32_t, 8> mk{};
+        std::memcpy(mk.data(), env_mask + 32 * (size_t)t, 32);
+        int k = 0;
+        while (k < (int)masks.size() && masks[k] != mk)
+            ++k;
+        if (k == (int)masks.size()) {
+            masks.push_back(mk);
+            envl.emplace_back();
+            st = build_typed_list(c, sl, cl, &L, mk.data(), envl.back());
+            if (st != PP_OK)
+                return st;
+        }
+        lidx[t] = k;
+    }
     void *mem = nullptr;
     const size_t cbytes = 4 * (size_t)ntasks * nxyz;
-    PP_HIP(c, hipMalloc(&mem, cbytes + 2 * 32 * (size_t)ntasks + 512));
+    Carver m0(nullptr);
+    m0.take<uint32_t>((size_t)ntasks * nxyz);
+    m0.take<uint32_t>(8 * (size_t)ntasks);
+    m0.take<uint32_t>(8 * (size_t)ntasks);
+    m0.take<const float4 *>((size_t)ntasks);
+    m0.take<const uint32_t *>((size_t)ntasks);
+    PP_HIP(c, hipMalloc(&mem, m0.off + 256));
     std::unique_ptr<void, void (*)(void *)> own(mem, [](void *p) { (void)hipFree(p); });
     Carver m(mem);
     uint32_t *d_counts = m.take<uint32_t>((size_t)ntasks * nxyz);
     uint32_t *d_trg = m.take<uint32_t>(8 * (size_t)ntasks);
     uint32_t *d_env = m.take<uint32_t>(8 * (size_t)ntasks);
+    const float4 **d_epos = m.take<const float4 *>((size_t)ntasks);
+    const uint32_t **d_ecs = m.take<const uint32_t *>((size_t)ntasks);
+    std::vector<const float4 *> h_epos(ntasks);
+    std::vector<const uint32_t *> h_ecs(ntasks);
+    for (int32_t t = 0; t < ntasks; ++t) {
+        h_epos[t] = envl[lidx[t]].pos;
+        h_ecs[t] = envl[lidx[t]].cellstart;
+    }
     PP_HIP(c, hipMemsetAsync(d_counts, 0, cbytes, s));
     PP_HIP(c, hipMemcpyAsync(d_trg, trg_mask, 32 * (size_t)ntasks, hipMemcpyHostToDevice, s));
     PP_HIP(c, hipMemcpyAsync(d_env, env_mask, 32 * (size_t)ntasks, hipMemcpyHostToDevice, s));
+    PP_HIP(c, hipMemcpyAsync(d_epos, h_epos.data(), sizeof(void *) * (size_t)ntasks,
+                             hipMemcpyHostToDevice, s));
+    PP_HIP(c, hipMemcpyAsync(d_ecs, h_ecs.data(), sizeof(void *) * (size_t)ntasks,
+                             hipMemcpyHostToDevice, s));
     NpcArgs a{};
     a.pos = L.pos;
     a.s2 = L.s2;
-    a.cellstart = L.cellstart;
+    a.epos = d_epos;
+    a.ecs = d_ecs;
     a.g = L.g;
     a.n = cl->n;
     a.R2 = rmax * rmax;

@@ -53,6 +53,8 @@ struct JobDev {
     const uint32_t *cellstart;
     const float4 *cpos;       // centers' cell-sorted ions (the full list)
     int nx, ny, nz;
+    double glo[3], gw, ginv;  // the grid: cell c spans [glo + c * gw, glo + (c + 1) * gw)
+    double reach;             // R widened by the rounding margins of the in-ball test
     int nrows;
     const Row *rows;
     const uint32_t *centers;  // considered centers, sorted-array indices

@@ -818,17 +818,51 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : (T == 8 && MODE
             s_meta[2 * tid] = (uint32_t)ts | (es << 8);
             s_cen[tid] = make_float4(px, py, pz, 0.f);
         }
+        __syncthreads();
         for (int r = tid; r < nrows; r += kBlock) {
             const Row rw = J.rows[r];
             uint32_t rs = 0, re = 0;
             const int y = cy + rw.dy, z = cz + rw.dz;
             if (y >= 0 && y < J.ny && z >= 0 && z < J.nz) {
-                const int x0 = cx - rw.dxm < 0 ? 0 : cx - rw.dxm;
-                const int x1 = cx + rw.dxm >= J.nx ? J.nx - 1 : cx + rw.dxm;
-                const uint32_t k0 =
-                    ((uint32_t)z * (uint32_t)J.ny + (uint32_t)y) * (uint32_t)J.nx + (uint32_t)x0;
-                rs = J.cellstart[k0];
-                re = J.cellstart[k0 + (uint32_t)(x1 - x0) + 1u];
+                int x0 = cx - rw.dxm < 0 ? 0 : cx - rw.dxm;
+                int x1 = cx + rw.dxm >= J.nx ? J.nx - 1 : cx + rw.dxm;
+                // the stencil row cut to the tile's centers: their bounding
+                // box's distance in y and z from the row's cells bounds the x
+                // extent of their balls (exact cell faces lo + c * w in FP64,
+                // reach = R with a margin far above the FP32 rounding of the
+                // in-ball test)
+                float b0x = s_cen[0].x, b1x = b0x, b0y = s_cen[0].y, b1y = b0y;
+                float b0z = s_cen[0].z, b1z = b0z;
+                for (int c = 1; c < cnt; ++c) {
+                    const float4 q = s_cen[c];
+                    b0x = fminf(b0x, q.x);
+                    b1x = fmaxf(b1x, q.x);
+                    b0y = fminf(b0y, q.y);
+                    b1y = fmaxf(b1y, q.y);
+                    b0z = fminf(b0z, q.z);
+                    b1z = fmaxf(b1z, q.z);
+                }
+                const double ylo = J.glo[1] + y * J.gw, zlo = J.glo[2] + z * J.gw;
+                const double hy = fmax(0.0, fmax(ylo - (double)b1y, (double)b0y - (ylo + J.gw)));
+                const double hz = fmax(0.0, fmax(zlo - (double)b1z, (double)b0z - (zlo + J.gw)));
+                const double h2 = hy * hy + hz * hz, lim = J.reach * J.reach;
+                if (h2 > lim) {
+                    x1 = x0 - 1;
+                } else {
+                    const double ex = sqrt(lim - h2);
+                    const double a = floor(((double)b0x - ex - J.glo[0]) * J.ginv);
+                    const double b = floor(((double)b1x + ex - J.glo[0]) * J.ginv);
+                    if (a > (double)x0)
+                        x0 = (int)a;
+                    if (b < (double)x1)
+                        x1 = (int)b;
+                }
+                if (x0 <= x1) {
+                    const uint32_t k0 = ((uint32_t)z * (uint32_t)J.ny + (uint32_t)y) *
+                                            (uint32_t)J.nx + (uint32_t)x0;
+                    rs = J.cellstart[k0];
+                    re = J.cellstart[k0 + (uint32_t)(x1 - x0) + 1u];
+                }
             }
             s_rows[2 * r] = rs;
             s_rows[2 * r + 1] = re;

@@ -124,16 +124,16 @@ hipError_t launch_posdecode(const uint32_t *rec, uint64_t n, int words, const fl
 // R/2 wide; per-task counts of the K-th env neighbour's binned difference vector.
 constexpr int kNpcCap = 512;      // LDS entries per wavefront; K < kNpcCap - 64
 constexpr int kNpcBuckets = 256;  // distance buckets per task
-constexpr int kNpcGroup = 4;      // tasks counted in one sweep
 struct NpcArgs {
-    const float4 *pos;          // cell-sorted (x, y, z, type bits)
+    const float4 *pos;          // cell-sorted (x, y, z, type bits): the centers
     const float *s2;            // cell-sorted ion2surf
-    const uint32_t *cellstart;  // ncells + 1
     GridGeom g;
     uint64_t n;
     float R2;                   // fl32(R * R)
     int ntasks;
     const uint32_t *trg, *env;  // [ntasks][8] 256-bit type masks
+    const float4 *const *epos;  // [ntasks] the ions of the task's env types, cell-sorted on g
+    const uint32_t *const *ecs; // [ntasks] their cell offsets (ncells + 1), or null: none
     uint32_t K;                 // SpatStatKNNOrder (0-based)
     float maxlen;               // npc3d fvec_maxlen (SpatStatRadiusMax)
     float inv_width;            // fl32(1 / SpatStatRadiusIncr)

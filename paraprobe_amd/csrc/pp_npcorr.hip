@@ -4,8 +4,9 @@
 // Replaces the NPCORR branch of the center loop (src/PARAPROBE_SolverHdl.cpp:
 // 3899-3960) and npc3d::add (src/PARAPROBE_NPointCorrelations3D.cpp:165-207):
 // for every considered center (ion2surf >= fl32(R*R)) and every task whose
-// central types hold its type, when more than K ions of any type lie within
-// R (self excluded), the env-type neighbours within R are ordered by
+// central types hold its type, when more than K ions of the task's
+// neighbour (env) types lie within R (self excluded; more than K of any type
+// then hold as well), those neighbours are ordered by
 // d = sqrtf(fl((u*u + v*v) + w*w)), (u, v, w) = neighbour - center; the
 // K-th (0-based) one's difference vector is binned into a cube of
 // (2h + 1)^3 cells of width `width` centred on the center, each component
@@ -15,19 +16,19 @@
 // to that order; here ties are broken by the smaller (u, v, w),
 // lexicographically, numerically.
 //
-// One wavefront per center over the 5 x 5 x 5 cells of a cell list at least
-// R/2 wide.  One sweep over them counts all neighbours within R and, for up
-// to kNpcGroup of the center's tasks at once, the env neighbours per
-// distance bucket (kNpcBuckets buckets of width R / kNpcBuckets over
-// d = sqrtf(d2), 16-bit counters in LDS); each task then finds the bucket
-// that holds its K-th neighbour, gathers that bucket's neighbours in one more
-// sweep (16 bytes each: d bits, then u, v, w as order-preserving integers)
-// and ranks them; buckets are monotone in d, so every neighbour of a lower
-// bucket precedes them.  When the bucket holds more than kNpcCap neighbours
-// (shells of a perfect lattice), or a task more than 65535 (the counters
-// could carry), the wave falls back to a buffer of candidates that is sorted
-// (bitonic) and cut to the K + 1 smallest whenever it fills up.  K must be
-// below kNpcCap - 64.
+// One wavefront per center.  Each task reads its own candidate list: the
+// ions of its neighbour types, cell-sorted on a grid at least R/2 wide, so
+// that the 5 x 5 x 5 cells around a center hold its ball; the stencil rows
+// are cut to the part a sphere of the needed radius can reach.  A first sweep
+// counts the candidates within R per distance bucket (kNpcBuckets buckets of
+// width R / kNpcBuckets over d = sqrtf(d2)); the bucket that holds the K-th
+// is found by a scan, its candidates are gathered in one more sweep, limited
+// to the bucket's spherical shell (16 bytes each: d bits, then u, v, w as
+// order-preserving integers), and ranked; buckets are monotone in d, so every
+// candidate of a lower bucket precedes them.  When the bucket holds more
+// than kNpcCap candidates (shells of a perfect lattice), the wave falls back
+// to a buffer of candidates that is sorted (bitonic) and cut to the K + 1
+// smallest whenever it fills up.  K must be below kNpcCap - 64.
 #include "pp_launch.h"
 
 #pragma clang fp contract(off)
@@ -98,10 +99,10 @@ struct Rows {
 __global__ __launch_bounds__(256) void k_npcorr(NpcArgs a)
 {
     __shared__ uint4 s_buf[kWaves][kNpcCap];
-    // per wavefront and task of a group: kNpcBuckets 16-bit counters, two per word
-    __shared__ uint32_t s_hist[kWaves][kNpcGroup][kNpcBuckets / 2];
+    __shared__ uint32_t s_hist[kWaves][kNpcBuckets];
     const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
     uint4 *buf = s_buf[wid];
+    uint32_t *hist = s_hist[wid];
     const uint64_t ci = (uint64_t)blockIdx.x * kWaves + wid;
     if (ci >= a.n)
         return;
@@ -110,23 +111,19 @@ __global__ __launch_bounds__(256) void k_npcorr(NpcArgs a)
     // considered: ion2surf >= fl32(R * R) (S.cpp:3846-3860)
     if (!(a.s2[ci] >= a.R2))
         return;
-    bool any = false;
-    for (int t = 0; t < a.ntasks; ++t)
-        any |= ((a.trg[t * 8 + (mt >> 5)] >> (mt & 31)) & 1u) != 0;
-    if (!any)
-        return;
     const int cx = cell_of(me.x, a.g.lo[0], a.g.inv_w, a.g.nx);
     const int cy = cell_of(me.y, a.g.lo[1], a.g.inv_w, a.g.ny);
     const int cz = cell_of(me.z, a.g.lo[2], a.g.inv_w, a.g.nz);
-    // The candidates of a spherical shell ri <= d <= ro around the center:
-    // lane r < 25 takes the stencil row (dy, dz) = (r % 5 - 2, r / 5 - 2) and
-    // keeps the x-cells where the shell can meet the row -- one run, or two
-    // when the inner sphere cuts the row -- as candidate index ranges.  The
-    // bounds are exact in double on the cell boundaries lo + c * w (the same
-    // expression that bins the ions) and widened by 1e-4 w, far more than the
-    // rounding of the FP32 distance tests that decide membership.
+    // The candidates of a spherical shell ri <= d <= ro around the center in
+    // a list with cell offsets cs: lane r < 25 takes the stencil row (dy, dz)
+    // = (r % 5 - 2, r / 5 - 2) and keeps the x-cells where the shell can meet
+    // the row -- one run, or two when the inner sphere cuts the row -- as
+    // candidate index ranges.  The bounds are exact in double on the cell
+    // boundaries lo + c * w (the same expression that bins the ions) and
+    // widened by 1e-4 w, far more than the rounding of the FP32 distance
+    // tests that decide membership.
     const double px = me.x, py = me.y, pz = me.z, cw = a.g.w, margin = 1e-4 * cw;
-    auto rows_for = [&](double ri, double ro) {
+    auto rows_for = [&](const uint32_t *cs, double ri, double ro) {
         Rows rw{0u, 0u, 0u, 0u};
         const int y = cy + lane % 5 - 2, z = cz + lane / 5 - 2;
         if (lane >= 25 || y < 0 || y >= a.g.ny || z < 0 || z >= a.g.nz)
@@ -150,45 +147,69 @@ __global__ __launch_bounds__(256) void k_npcorr(NpcArgs a)
         const int la = cell(px - xo), lb = cell(px - xi), ra = cell(px + xi), rb = cell(px + xo);
         const uint64_t row = ((uint64_t)z * a.g.ny + y) * a.g.nx;
         if (xi2 <= 0.0 || lb + 1 >= ra) {
-            rw.b1 = a.cellstart[row + la];
-            rw.e1 = a.cellstart[row + rb + 1];
+            rw.b1 = cs[row + la];
+            rw.e1 = cs[row + rb + 1];
         } else {
-            rw.b1 = a.cellstart[row + la];
-            rw.e1 = a.cellstart[row + lb + 1];
-            rw.b2 = a.cellstart[row + ra];
-            rw.e2 = a.cellstart[row + rb + 1];
+            rw.b1 = cs[row + la];
+            rw.e1 = cs[row + lb + 1];
+            rw.b2 = cs[row + ra];
+            rw.e2 = cs[row + rb + 1];
         }
         return rw;
     };
-    // visits every candidate of the ranges: f(j, e) per lane, j < e valid
+    // Visits every candidate of the ranges: f(j, valid) per lane (j is only
+    // meaningful when valid).  The ranges go to lanes (lane l < 25: the first
+    // range of row l, lane 25 + l: its second) with an exclusive prefix of
+    // their lengths.  Long ranges are walked one by one, 64 candidates a
+    // step; short ones (sparse type lists) as one concatenation, each lane
+    // finding its range by a binary search over the prefix.
     auto sweep = [&](const Rows &rw, auto &&f) {
-        for (int r = 0; r < 25; ++r) {
-            const uint32_t b1 = __builtin_amdgcn_readlane(rw.b1, r);
-            const uint32_t e1 = __builtin_amdgcn_readlane(rw.e1, r);
-            const uint32_t b2 = __builtin_amdgcn_readlane(rw.b2, r);
-            const uint32_t e2 = __builtin_amdgcn_readlane(rw.e2, r);
-            for (uint32_t j0 = b1; j0 < e1; j0 += kWave)
-                f(j0 + lane, e1);
-            for (uint32_t j0 = b2; j0 < e2; j0 += kWave)
-                f(j0 + lane, e2);
+        const int src = lane >= 25 ? lane - 25 : lane;
+        const uint32_t sb2 = __shfl(rw.b2, src, kWave), se2 = __shfl(rw.e2, src, kWave);
+        const uint32_t b = lane < 25 ? rw.b1 : (lane < 50 ? sb2 : 0u);
+        const uint32_t len = lane < 25 ? rw.e1 - rw.b1 : (lane < 50 ? se2 - sb2 : 0u);
+        uint32_t incl = len;
+        for (int o = 1; o < kWave; o <<= 1) {
+            const uint32_t v = __shfl_up(incl, o, kWave);
+            if (lane >= o)
+                incl += v;
+        }
+        const uint32_t excl = incl - len;
+        const uint32_t total = __shfl(incl, kWave - 1, kWave);
+        const uint32_t nne = (uint32_t)__builtin_popcountll(__builtin_amdgcn_ballot_w64(len > 0));
+        if (total >= 32u * nne) {
+            for (int r = 0; r < 50; ++r) {
+                const uint32_t rb = __builtin_amdgcn_readlane(b, r);
+                const uint32_t re = rb + __builtin_amdgcn_readlane(len, r);
+                for (uint32_t j0 = rb; j0 < re; j0 += kWave)
+                    f(j0 + lane, j0 + lane < re);
+            }
+        } else {
+            for (uint32_t s0 = 0; s0 < total; s0 += kWave) {
+                const uint32_t sl = s0 + lane;
+                int k = 0;
+                for (int st = kWave / 2; st > 0; st >>= 1) {
+                    const uint32_t v = __shfl(excl, k + st, kWave);
+                    k = v <= sl ? k + st : k;
+                }
+                const uint32_t j = __shfl(b, k, kWave) + (sl - __shfl(excl, k, kWave));
+                f(j, sl < total);
+            }
         }
     };
-    // the ball of every neighbour within R (d2 <= fl32(R * R))
-    const Rows ball = rows_for(0.0, (double)a.maxlen * (1.0 + 0x1p-12));
-    const uint32_t K = a.K;
-    // a neighbour within R of candidate j (self excluded): its type, its key
-    // (d bits, then u, v, w order-preserving) and its distance bucket
-    auto neighbour = [&](uint32_t j, uint32_t &pt, float &u, float &v, float &w) {
-        const float4 p = a.pos[j];
-        pt = __float_as_uint(p.w);
-        const float dxx = me.x - p.x, dyy = me.y - p.y, dzz = me.z - p.z;
+    // a neighbour within R of the center in list p (the center itself
+    // included when its type is listed): its difference vector
+    auto neighbour = [&](const float4 *p, uint32_t j, float &u, float &v, float &w) {
+        const float4 q = p[j];
+        const float dxx = me.x - q.x, dyy = me.y - q.y, dzz = me.z - q.z;
         const float d2 = dxx * dxx + dyy * dyy + dzz * dzz;
         // the difference vector neighbour - center
-        u = p.x - me.x;
-        v = p.y - me.y;
-        w = p.z - me.z;
-        return d2 <= a.R2 && j != ci;
+        u = q.x - me.x;
+        v = q.y - me.y;
+        w = q.z - me.z;
+        return d2 <= a.R2;
     };
+    // its key (d bits, then u, v, w order-preserving) and distance bucket
     auto key_of = [&](float u, float v, float w, int &bucket) {
         // its length by the reference's expression (sqrt of a float)
         const float dd = __builtin_sqrtf(u * u + v * v + w * w);
@@ -196,194 +217,175 @@ __global__ __launch_bounds__(256) void k_npcorr(NpcArgs a)
         bucket = bk < kNpcBuckets - 1 ? bk : kNpcBuckets - 1;
         return make_uint4(__float_as_uint(dd), ordf(u), ordf(v), ordf(w));
     };
-    uint32_t ntot = 0;
-    bool first = true;
-    // the center's tasks, kNpcGroup at a time: one sweep counts the env
-    // neighbours of each per distance bucket (and, the first time, all
-    // neighbours); then each task finds its K-th
-    for (int t0 = 0; t0 < a.ntasks;) {
-        int tl[kNpcGroup];
-        int L = 0;
-        while (t0 < a.ntasks && L < kNpcGroup) {
-            if ((a.trg[t0 * 8 + (mt >> 5)] >> (mt & 31)) & 1u)
-                tl[L++] = t0;
-            ++t0;
-        }
-        if (L == 0)
-            break;
-        for (int q = lane; q < L * (kNpcBuckets / 2); q += kWave)
-            s_hist[wid][q / (kNpcBuckets / 2)][q % (kNpcBuckets / 2)] = 0;
+    const double reach = (double)a.maxlen * (1.0 + 0x1p-12);
+    for (int t = 0; t < a.ntasks; ++t) {
+        if (!((a.trg[t * 8 + (mt >> 5)] >> (mt & 31)) & 1u))
+            continue;
+        const float4 *ep = a.epos[t];
+        const uint32_t *ecs = a.ecs[t];
+        if (!ecs)  // no ion of the task's neighbour types
+            continue;
+        // the center is among its own candidates when its type is listed:
+        // its key (d = 0, u = v = w = +0) is the smallest, so the K-th
+        // neighbour is the (K + 1)-th candidate (a coincident ion has the
+        // same key, so which of them stands for the center is immaterial)
+        const bool self = ((a.env[t * 8 + (mt >> 5)] >> (mt & 31)) & 1u) != 0;
+        const uint32_t Ke = a.K + (self ? 1u : 0u);
+        // count the candidates within R per distance bucket
+        for (int q = lane; q < kNpcBuckets; q += kWave)
+            hist[q] = 0;
         __builtin_amdgcn_wave_barrier();
-        uint32_t nenv[kNpcGroup] = {0, 0, 0, 0};
-        sweep(ball, [&](uint32_t j, uint32_t e) {
-            if (j >= e)
+        const Rows ball = rows_for(ecs, 0.0, reach);
+        uint32_t nenv = 0;
+        sweep(ball, [&](uint32_t j, bool ok) {
+            if (!ok)
                 return;
-            uint32_t pt;
             float u, v, w;
-            if (!neighbour(j, pt, u, v, w))
-                return;
-            ntot += first ? 1u : 0u;
-            bool hit[kNpcGroup];
-            bool anyhit = false;
-            for (int l = 0; l < kNpcGroup; ++l) {
-                hit[l] = l < L && ((a.env[tl[l] * 8 + (pt >> 5)] >> (pt & 31)) & 1u);
-                anyhit |= hit[l];
-            }
-            if (!anyhit)
+            if (!neighbour(ep, j, u, v, w))
                 return;
             int bk;
             (void)key_of(u, v, w, bk);
-            for (int l = 0; l < kNpcGroup; ++l)
-                if (hit[l]) {
-                    __hip_atomic_fetch_add(&s_hist[wid][l][bk >> 1], 1u << (16 * (bk & 1)),
-                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                    ++nenv[l];
-                }
+            __hip_atomic_fetch_add(&hist[bk], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            ++nenv;
         });
-        if (first) {
-            for (int o = kWave / 2; o > 0; o >>= 1)
-                ntot += __shfl_xor(ntot, o, kWave);
-            first = false;
-        }
-        for (int l = 0; l < kNpcGroup; ++l)
-            for (int o = kWave / 2; o > 0; o >>= 1)
-                nenv[l] += __shfl_xor(nenv[l], o, kWave);
+        for (int o = kWave / 2; o > 0; o >>= 1)
+            nenv += __shfl_xor(nenv, o, kWave);
         __builtin_amdgcn_wave_barrier();
-        if (ntot <= K)
-            return;
-        for (int l = 0; l < L; ++l) {
-            const int t = tl[l];
-            const uint32_t *env = a.env + t * 8;
-            if (nenv[l] <= K)
+        // more than K neighbours of the task's types (S.cpp:3940-3951; more
+        // than K of any type then holds as well)
+        if (nenv <= Ke)
+            continue;
+        uint4 pick;
+        int bsel = -1;
+        uint32_t below = 0, inb = 0;
+        {
+            // the bucket of the Ke-th: lane l sums buckets [l * per, (l + 1) * per)
+            constexpr int per = kNpcBuckets / kWave;
+            uint32_t mine = 0;
+            for (int q = 0; q < per; ++q)
+                mine += hist[lane * per + q];
+            uint32_t incl = mine;
+            for (int o = 1; o < kWave; o <<= 1) {
+                const uint32_t vv = __shfl_up(incl, o, kWave);
+                if (lane >= o)
+                    incl += vv;
+            }
+            const uint32_t excl = incl - mine;
+            if (excl <= Ke && Ke < incl) {
+                uint32_t c = excl;
+                for (int q = 0; q < per; ++q) {
+                    const uint32_t h = hist[lane * per + q];
+                    if (Ke < c + h) {
+                        bsel = lane * per + q;
+                        below = c;
+                        break;
+                    }
+                    c += h;
+                }
+            }
+            const uint64_t own = __builtin_amdgcn_ballot_w64(bsel >= 0);
+            if (own == 0)  // cannot happen: more than Ke candidates were counted
                 continue;
-            uint4 pick;
-            bool crowded = nenv[l] > 0xFFFFu;  // 16-bit counters could carry
-            int bsel = -1;
-            uint32_t below = 0, inb = 0;
-            if (!crowded) {
-                // the bucket of the K-th: lane l sums buckets [l * per, (l + 1) * per)
-                constexpr int per = kNpcBuckets / kWave;
-                auto cnt_of = [&](int b) {
-                    return (s_hist[wid][l][b >> 1] >> (16 * (b & 1))) & 0xFFFFu;
-                };
-                uint32_t mine = 0;
-                for (int q = 0; q < per; ++q)
-                    mine += cnt_of(lane * per + q);
-                uint32_t incl = mine;
-                for (int o = 1; o < kWave; o <<= 1) {
-                    const uint32_t vv = __shfl_up(incl, o, kWave);
-                    if (lane >= o)
-                        incl += vv;
-                }
-                const uint32_t excl = incl - mine;
-                if (excl <= K && K < incl) {
-                    uint32_t c = excl;
-                    for (int q = 0; q < per; ++q) {
-                        const uint32_t h = cnt_of(lane * per + q);
-                        if (K < c + h) {
-                            bsel = lane * per + q;
-                            below = c;
-                            break;
-                        }
-                        c += h;
-                    }
-                }
-                const uint64_t own = __builtin_amdgcn_ballot_w64(bsel >= 0);
-                if (own == 0)  // cannot happen: more than K env neighbours were counted
-                    continue;
-                const int src = (int)__builtin_ctzll(own);
-                bsel = __shfl(bsel, src, kWave);
-                below = __shfl(below, src, kWave);
-                inb = cnt_of(bsel);
-                crowded = inb > (uint32_t)kNpcCap;
-            }
-            if (!crowded) {
-                // the selected bucket's neighbours, ranked; they lie in the
-                // shell of the bucket's distances
-                const double bw = 1.0 / (double)a.inv_bucket;
-                const Rows shell = rows_for(bsel * bw * (1.0 - 0x1p-12),
-                                            bsel == kNpcBuckets - 1
-                                                ? (double)a.maxlen * (1.0 + 0x1p-12)
-                                                : (bsel + 1) * bw * (1.0 + 0x1p-12));
-                uint32_t cnt = 0;
-                sweep(shell, [&](uint32_t j, uint32_t e) {
-                    uint32_t pt = 0;
-                    float u = 0.f, v = 0.f, w = 0.f;
-                    int bk = -1;
-                    uint4 key = make_uint4(0, 0, 0, 0);
-                    bool take = j < e && neighbour(j, pt, u, v, w) &&
-                                ((env[pt >> 5] >> (pt & 31)) & 1u);
-                    if (take) {
-                        key = key_of(u, v, w, bk);
-                        take = bk == bsel;
-                    }
-                    const uint64_t m = __builtin_amdgcn_ballot_w64(take);
-                    if (take)
-                        buf[cnt + (uint32_t)__builtin_popcountll(m & ((1ull << lane) - 1ull))] = key;
-                    cnt += (uint32_t)__builtin_popcountll(m);
-                });
-                __builtin_amdgcn_wave_barrier();
-                const uint32_t want = K - below;
-                int found = -1;
-                for (uint32_t q = lane; q < cnt; q += kWave) {
-                    const uint4 kq = buf[q];
-                    uint32_t rank = 0;
-                    for (uint32_t r = 0; r < cnt; ++r) {
-                        const uint4 kr = buf[r];
-                        rank += (key_less(kr, kq) || (r < q && !key_less(kq, kr))) ? 1u : 0u;
-                    }
-                    if (rank == want)
-                        found = (int)q;
-                }
-                const uint64_t fm = __builtin_amdgcn_ballot_w64(found >= 0);
-                if (fm == 0)  // cannot happen: the ranks are a permutation of [0, cnt)
-                    continue;
-                const int q = __shfl(found, (int)__builtin_ctzll(fm), kWave);
-                pick = buf[q];
-            } else {
-                // a crowded bucket: keep the K + 1 smallest in the buffer
-                uint32_t cnt = 0;
-                sweep(ball, [&](uint32_t j, uint32_t e) {
-                    uint32_t pt = 0;
-                    float u = 0.f, v = 0.f, w = 0.f;
-                    int bk;
-                    uint4 key = make_uint4(0, 0, 0, 0);
-                    const bool take = j < e && neighbour(j, pt, u, v, w) &&
-                                      ((env[pt >> 5] >> (pt & 31)) & 1u);
-                    if (take)
-                        key = key_of(u, v, w, bk);
-                    const uint64_t m = __builtin_amdgcn_ballot_w64(take);
-                    const uint32_t add = (uint32_t)__builtin_popcountll(m);
-                    if (add == 0)
-                        return;
-                    if (cnt + add > (uint32_t)kNpcCap) {
-                        wave_sort(buf, cnt, lane);
-                        cnt = cnt < K + 1 ? cnt : K + 1;
-                    }
-                    if (take)
-                        buf[cnt + (uint32_t)__builtin_popcountll(m & ((1ull << lane) - 1ull))] = key;
-                    cnt += add;
-                    __builtin_amdgcn_wave_barrier();
-                });
-                wave_sort(buf, cnt, lane);
-                pick = buf[K];
-            }
-            if (lane == 0 && __uint_as_float(pick.x) <= a.maxlen) {
-                // npc3d::add: t = u * (1 / width) in float, rounded half away
-                // from zero through a double, plus the centre offset
-                int ix[3];
-                const uint32_t comp[3] = {pick.y, pick.z, pick.w};
-                for (int q = 0; q < 3; ++q) {
-                    const float th = unordf(comp[q]) * a.inv_width;
-                    const int disc = (th >= 0.f) ? (int)((double)th + 0.5) : (int)((double)th - 0.5);
-                    ix[q] = disc + a.half;
-                }
-                const uint64_t bin = (uint64_t)ix[0] + (uint64_t)ix[1] * a.edge +
-                                     (uint64_t)ix[2] * a.edge * a.edge;
-                atomicAdd(&a.counts[(uint64_t)t * a.nxyz + bin], 1u);
-            }
-            __builtin_amdgcn_wave_barrier();
+            const int src = (int)__builtin_ctzll(own);
+            bsel = __shfl(bsel, src, kWave);
+            below = __shfl(below, src, kWave);
+            inb = hist[bsel];
         }
+        if (inb <= (uint32_t)kNpcCap) {
+            // the selected bucket's candidates, ranked; they lie in the shell
+            // of the bucket's distances
+            const double bw = 1.0 / (double)a.inv_bucket;
+            const Rows shell = rows_for(ecs, bsel * bw * (1.0 - 0x1p-12),
+                                        bsel == kNpcBuckets - 1 ? reach
+                                                                : (bsel + 1) * bw * (1.0 + 0x1p-12));
+            uint32_t cnt = 0;
+            sweep(shell, [&](uint32_t j, bool ok) {
+                float u = 0.f, v = 0.f, w = 0.f;
+                int bk = -1;
+                uint4 key = make_uint4(0, 0, 0, 0);
+                bool take = ok && neighbour(ep, j, u, v, w);
+                if (take) {
+                    key = key_of(u, v, w, bk);
+                    take = bk == bsel;
+                }
+                const uint64_t m = __builtin_amdgcn_ballot_w64(take);
+                if (take)
+                    buf[cnt + (uint32_t)__builtin_popcountll(m & ((1ull << lane) - 1ull))] = key;
+                cnt += (uint32_t)__builtin_popcountll(m);
+            });
+            __builtin_amdgcn_wave_barrier();
+            const uint32_t want = Ke - below;
+            int found = -1;
+            for (uint32_t q = lane; q < cnt; q += kWave) {
+                const uint4 kq = buf[q];
+                uint32_t rank = 0;
+                for (uint32_t r = 0; r < cnt; ++r) {
+                    const uint4 kr = buf[r];
+                    rank += (key_less(kr, kq) || (r < q && !key_less(kq, kr))) ? 1u : 0u;
+                }
+                if (rank == want)
+                    found = (int)q;
+            }
+            const uint64_t fm = __builtin_amdgcn_ballot_w64(found >= 0);
+            if (fm == 0)  // cannot happen: the ranks are a permutation of [0, cnt)
+                continue;
+            const int q = __shfl(found, (int)__builtin_ctzll(fm), kWave);
+            pick = buf[q];
+        } else {
+            // a crowded bucket: keep the K + 1 smallest in the buffer; one
+            // zero key (the center's own) is left out when the center's type
+            // is listed
+            const uint32_t K = a.K;
+            bool dropped = !self;
+            uint32_t cnt = 0;
+            sweep(ball, [&](uint32_t j, bool ok) {
+                float u = 0.f, v = 0.f, w = 0.f;
+                int bk;
+                uint4 key = make_uint4(0, 0, 0, 0);
+                bool take = ok && neighbour(ep, j, u, v, w);
+                if (take)
+                    key = key_of(u, v, w, bk);
+                if (!dropped) {
+                    const uint64_t zm = __builtin_amdgcn_ballot_w64(
+                        take && key.x == 0u && key.y == 0x80000000u && key.z == 0x80000000u &&
+                        key.w == 0x80000000u);
+                    if (zm) {
+                        if (lane == (int)__builtin_ctzll(zm))
+                            take = false;
+                        dropped = true;
+                    }
+                }
+                const uint64_t m = __builtin_amdgcn_ballot_w64(take);
+                const uint32_t add = (uint32_t)__builtin_popcountll(m);
+                if (add == 0)
+                    return;
+                if (cnt + add > (uint32_t)kNpcCap) {
+                    wave_sort(buf, cnt, lane);
+                    cnt = cnt < K + 1 ? cnt : K + 1;
+                }
+                if (take)
+                    buf[cnt + (uint32_t)__builtin_popcountll(m & ((1ull << lane) - 1ull))] = key;
+                cnt += add;
+                __builtin_amdgcn_wave_barrier();
+            });
+            wave_sort(buf, cnt, lane);
+            pick = buf[K];
+        }
+        if (lane == 0 && __uint_as_float(pick.x) <= a.maxlen) {
+            // npc3d::add: t = u * (1 / width) in float, rounded half away
+            // from zero through a double, plus the centre offset
+            int ix[3];
+            const uint32_t comp[3] = {pick.y, pick.z, pick.w};
+            for (int q = 0; q < 3; ++q) {
+                const float th = unordf(comp[q]) * a.inv_width;
+                const int disc = (th >= 0.f) ? (int)((double)th + 0.5) : (int)((double)th - 0.5);
+                ix[q] = disc + a.half;
+            }
+            const uint64_t bin = (uint64_t)ix[0] + (uint64_t)ix[1] * a.edge +
+                                 (uint64_t)ix[2] * a.edge * a.edge;
+            atomicAdd(&a.counts[(uint64_t)t * a.nxyz + bin], 1u);
+        }
+        __builtin_amdgcn_wave_barrier();
     }
 }
 
