@@ -2395,6 +2395,7 @@ pp_status pp_npcorr_run(pp_ctx *c, const pp_cloud *cl, int32_t ntasks, const uin
     m0.take<uint32_t>(8 * (size_t)ntasks);
     m0.take<const float4 *>((size_t)ntasks);
     m0.take<const uint32_t *>((size_t)ntasks);
+    m0.take<float>((size_t)ntasks);
     PP_HIP(c, hipMalloc(&mem, m0.off + 256));
     std::unique_ptr<void, void (*)(void *)> own(mem, [](void *p) { (void)hipFree(p); });
     Carver m(mem);
@@ -2403,11 +2404,26 @@ pp_status pp_npcorr_run(pp_ctx *c, const pp_cloud *cl, int32_t ntasks, const uin
     uint32_t *d_env = m.take<uint32_t>(8 * (size_t)ntasks);
     const float4 **d_epos = m.take<const float4 *>((size_t)ntasks);
     const uint32_t **d_ecs = m.take<const uint32_t *>((size_t)ntasks);
+    float *d_r1 = m.take<float>((size_t)ntasks);
     std::vector<const float4 *> h_epos(ntasks);
     std::vector<const uint32_t *> h_ecs(ntasks);
+    // the first counting radius: 1.4 times the radius that holds K + 2
+    // neighbour-type ions at the cloud's mean density (R when that is not
+    // well below R)
+    std::vector<float> h_r1(ntasks);
+    const double dens = cl->density();
     for (int32_t t = 0; t < ntasks; ++t) {
         h_epos[t] = envl[lidx[t]].pos;
         h_ecs[t] = envl[lidx[t]].cellstart;
+        double m_env = 0;
+        for (int ty = 0; ty < 256; ++ty)
+            if ((env_mask[32 * (size_t)t + ty / 8] >> (ty % 8)) & 1u)
+                m_env += (double)cl->type_count[ty];
+        const double rho = dens * m_env / (double)cl->n;
+        const double r = rho > 0 ? 1.4 * std::cbrt(3.0 * (double)(knn_order + 2) /
+                                                   (4.0 * 3.14159265358979 * rho))
+                                 : (double)rmax;
+        h_r1[t] = r < 0.8 * (double)rmax ? (float)r : rmax;
     }
     PP_HIP(c, hipMemsetAsync(d_counts, 0, cbytes, s));
     PP_HIP(c, hipMemcpyAsync(d_trg, trg_mask, 32 * (size_t)ntasks, hipMemcpyHostToDevice, s));
@@ -2416,11 +2432,13 @@ pp_status pp_npcorr_run(pp_ctx *c, const pp_cloud *cl, int32_t ntasks, const uin
                              hipMemcpyHostToDevice, s));
     PP_HIP(c, hipMemcpyAsync(d_ecs, h_ecs.data(), sizeof(void *) * (size_t)ntasks,
                              hipMemcpyHostToDevice, s));
+    PP_HIP(c, hipMemcpyAsync(d_r1, h_r1.data(), 4 * (size_t)ntasks, hipMemcpyHostToDevice, s));
     NpcArgs a{};
     a.pos = L.pos;
     a.s2 = L.s2;
     a.epos = d_epos;
     a.ecs = d_ecs;
+    a.r1 = d_r1;
     a.g = L.g;
     a.n = cl->n;
     a.R2 = rmax * rmax;

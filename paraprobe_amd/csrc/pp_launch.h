@@ -134,6 +134,7 @@ struct NpcArgs {
     const uint32_t *trg, *env;  // [ntasks][8] 256-bit type masks
     const float4 *const *epos;  // [ntasks] the ions of the task's env types, cell-sorted on g
     const uint32_t *const *ecs; // [ntasks] their cell offsets (ncells + 1), or null: none
+    const float *r1;            // [ntasks] first counting radius (>= maxlen: R at once)
     uint32_t K;                 // SpatStatKNNOrder (0-based)
     float maxlen;               // npc3d fvec_maxlen (SpatStatRadiusMax)
     float inv_width;            // fl32(1 / SpatStatRadiusIncr)

@@ -20,8 +20,10 @@
 // ions of its neighbour types, cell-sorted on a grid at least R/2 wide, so
 // that the 5 x 5 x 5 cells around a center hold its ball; the stencil rows
 // are cut to the part a sphere of the needed radius can reach.  A first sweep
-// counts the candidates within R per distance bucket (kNpcBuckets buckets of
-// width R / kNpcBuckets over d = sqrtf(d2)); the bucket that holds the K-th
+// counts the candidates per distance bucket (kNpcBuckets buckets over
+// d = sqrtf(d2)) within a radius r1 that holds the K-th at the list's mean
+// density with a margin, or within R when r1 holds too few; the bucket that
+// holds the K-th
 // is found by a scan, its candidates are gathered in one more sweep, limited
 // to the bucket's spherical shell (16 bytes each: d bits, then u, v, w as
 // order-preserving integers), and ranked; buckets are monotone in d, so every
@@ -209,12 +211,16 @@ __global__ __launch_bounds__(256) void k_npcorr(NpcArgs a)
         w = q.z - me.z;
         return d2 <= a.R2;
     };
-    // its key (d bits, then u, v, w order-preserving) and distance bucket
+    // its key (d bits, then u, v, w order-preserving) and distance bucket:
+    // (int)(d * inv_b), clamped to the last bucket when the buckets span R
+    // (full), else -1 beyond them
+    float inv_b = a.inv_bucket;
+    bool full = true;
     auto key_of = [&](float u, float v, float w, int &bucket) {
         // its length by the reference's expression (sqrt of a float)
         const float dd = __builtin_sqrtf(u * u + v * v + w * w);
-        const int bk = (int)(dd * a.inv_bucket);
-        bucket = bk < kNpcBuckets - 1 ? bk : kNpcBuckets - 1;
+        const int bk = (int)(dd * inv_b);
+        bucket = bk < kNpcBuckets - 1 ? bk : (full ? kNpcBuckets - 1 : (bk == kNpcBuckets - 1 ? bk : -1));
         return make_uint4(__float_as_uint(dd), ordf(u), ordf(v), ordf(w));
     };
     const double reach = (double)a.maxlen * (1.0 + 0x1p-12);
@@ -231,26 +237,43 @@ __global__ __launch_bounds__(256) void k_npcorr(NpcArgs a)
         // same key, so which of them stands for the center is immaterial)
         const bool self = ((a.env[t * 8 + (mt >> 5)] >> (mt & 31)) & 1u) != 0;
         const uint32_t Ke = a.K + (self ? 1u : 0u);
-        // count the candidates within R per distance bucket
-        for (int q = lane; q < kNpcBuckets; q += kWave)
-            hist[q] = 0;
-        __builtin_amdgcn_wave_barrier();
-        const Rows ball = rows_for(ecs, 0.0, reach);
+        // count the candidates per distance bucket: first within the radius
+        // r1 that holds the K-th at the list's mean density with a margin
+        // (buckets of width r1 / kNpcBuckets), and only when fewer than
+        // K + 1 lie there, within R (buckets of width R / kNpcBuckets).
+        // Candidates beyond r1 have larger keys than every one within it.
+        const float r1 = a.r1[t];
+        full = !(r1 < a.maxlen);
+        inv_b = full ? a.inv_bucket : (float)kNpcBuckets / r1;
         uint32_t nenv = 0;
-        sweep(ball, [&](uint32_t j, bool ok) {
-            if (!ok)
-                return;
-            float u, v, w;
-            if (!neighbour(ep, j, u, v, w))
-                return;
-            int bk;
-            (void)key_of(u, v, w, bk);
-            __hip_atomic_fetch_add(&hist[bk], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-            ++nenv;
-        });
-        for (int o = kWave / 2; o > 0; o >>= 1)
-            nenv += __shfl_xor(nenv, o, kWave);
-        __builtin_amdgcn_wave_barrier();
+        for (;;) {
+            for (int q = lane; q < kNpcBuckets; q += kWave)
+                hist[q] = 0;
+            __builtin_amdgcn_wave_barrier();
+            const Rows ball = rows_for(ecs, 0.0, full ? reach : (double)r1 * (1.0 + 0x1p-12));
+            nenv = 0;
+            sweep(ball, [&](uint32_t j, bool ok) {
+                if (!ok)
+                    return;
+                float u, v, w;
+                if (!neighbour(ep, j, u, v, w))
+                    return;
+                int bk;
+                (void)key_of(u, v, w, bk);
+                if (bk < 0)
+                    return;
+                __hip_atomic_fetch_add(&hist[bk], 1u, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_WORKGROUP);
+                ++nenv;
+            });
+            for (int o = kWave / 2; o > 0; o >>= 1)
+                nenv += __shfl_xor(nenv, o, kWave);
+            __builtin_amdgcn_wave_barrier();
+            if (nenv > Ke || full)
+                break;
+            full = true;
+            inv_b = a.inv_bucket;
+        }
         // more than K neighbours of the task's types (S.cpp:3940-3951; more
         // than K of any type then holds as well)
         if (nenv <= Ke)
@@ -294,10 +317,11 @@ __global__ __launch_bounds__(256) void k_npcorr(NpcArgs a)
         if (inb <= (uint32_t)kNpcCap) {
             // the selected bucket's candidates, ranked; they lie in the shell
             // of the bucket's distances
-            const double bw = 1.0 / (double)a.inv_bucket;
+            const double bw = 1.0 / (double)inv_b;
             const Rows shell = rows_for(ecs, bsel * bw * (1.0 - 0x1p-12),
-                                        bsel == kNpcBuckets - 1 ? reach
-                                                                : (bsel + 1) * bw * (1.0 + 0x1p-12));
+                                        (bsel == kNpcBuckets - 1 && full)
+                                            ? reach
+                                            : (bsel + 1) * bw * (1.0 + 0x1p-12));
             uint32_t cnt = 0;
             sweep(shell, [&](uint32_t j, bool ok) {
                 float u = 0.f, v = 0.f, w = 0.f;
@@ -338,6 +362,7 @@ __global__ __launch_bounds__(256) void k_npcorr(NpcArgs a)
             const uint32_t K = a.K;
             bool dropped = !self;
             uint32_t cnt = 0;
+            const Rows ball = rows_for(ecs, 0.0, reach);
             sweep(ball, [&](uint32_t j, bool ok) {
                 float u = 0.f, v = 0.f, w = 0.f;
                 int bk;

@@ -156,3 +156,27 @@ def test_npcorr_bucket_edges_and_many_tasks(ctx):
         want = O.npcorr(xyz, ty, s2, tasks, R, 0.02, K)
         assert np.array_equal(got, want)
         assert got.sum() > 0
+
+
+def test_npcorr_sparse_region_of_a_dense_cloud(ctx):
+    """Most ions in a dense blob, the centers in a sparse background: the
+    first counting radius (set from the cloud's mean density) holds too few
+    neighbours there, so the count is redone over R."""
+    rng = np.random.default_rng(5)
+    blob = rng.normal(loc=(2.0, 2.0, 2.0), scale=0.3, size=(20000, 3)).astype(np.float32)
+    back = rng.uniform(0, 12, size=(3000, 3)).astype(np.float32)
+    xyz = np.concatenate([blob, back])
+    ty = rng.choice([1, 2], size=xyz.shape[0], p=[0.8, 0.2]).astype(np.uint8)
+    s2 = np.full(xyz.shape[0], 100.0, np.float32)
+    s2[:blob.shape[0]] = 0.0  # the blob's ions are neighbours only
+    tasks = [((1,), (1,)), ((2,), (1, 2)), ((1, 2), (2,))]
+    for K in (2, 20):
+        _say(f"npcorr sparse region K={K}")
+        cloud = ctx.upload(xyz, ty, s2)
+        try:
+            got = ctx.npcorr(cloud, tasks, 2.5, 0.1, K)
+        finally:
+            cloud.free()
+        want = O.npcorr(xyz, ty, s2, tasks, 2.5, 0.1, K)
+        assert np.array_equal(got, want)
+        assert got.sum() > 0
