@@ -674,7 +674,8 @@ int pp_cpu_ion2surf(int64_t n, const float *xyz, int64_t ntri, const float *tri,
 // neighbours within R as positions (the _p3dm1 range queries, the other slabs
 // probed when the ball crosses the own slab), per task the env ones as
 // (difference vector, sqrt length), nth_element for the K-th, binned.  Equal
-// lengths are ordered by the smaller (u, v, w) so that the result is
+// lengths are ordered by the squared length, then the smaller (u, v, w), so
+// that the result is
 // deterministic (the reference leaves them to nth_element over its KD-tree
 // order).  counts is [ntasks][edge^3]; timings as for pp_cpu_spatstat.
 int pp_cpu_npcorr(int64_t n, const float *xyz, const uint8_t *type, const float *ion2surf,
@@ -692,11 +693,13 @@ int pp_cpu_npcorr(int64_t n, const float *xyz, const uint8_t *type, const float 
     const double t1 = omp_get_wtime();
     std::memset(counts, 0, sizeof(uint32_t) * (size_t)(ntasks * nxyz));
     struct NP {
-        float u, v, w, d;
+        float u, v, w, d2, d;
     };
     auto less = [](const NP &a, const NP &b) {
         if (a.d != b.d)
             return a.d < b.d;
+        if (a.d2 != b.d2)
+            return a.d2 < b.d2;
         if (a.u != b.u)
             return a.u < b.u;
         if (a.v != b.v)
@@ -740,7 +743,8 @@ int pp_cpu_npcorr(int64_t n, const float *xyz, const uint8_t *type, const float 
                             e.u = q.x - me.x;
                             e.v = q.y - me.y;
                             e.w = q.z - me.z;
-                            e.d = std::sqrt(e.u * e.u + e.v * e.v + e.w * e.w);
+                            e.d2 = e.u * e.u + e.v * e.v + e.w * e.w;
+                            e.d = std::sqrt(e.d2);
                             tmp.push_back(e);
                         }
                     if (tmp.size() <= (size_t)K)

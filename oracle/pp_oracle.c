@@ -706,14 +706,15 @@ static void orc_records_decode(const uint8_t *rec, int64_t n, int64_t stride, co
  * For every considered center (ion2surf >= fl32(R*R)) and every task whose
  * central types hold its type: when more than K ions of any type lie within R
  * (self excluded), the env-type neighbours within R are ordered by
- * d = sqrtf((u*u + v*v) + w*w), (u, v, w) = neighbour - center, ties by the
- * smaller u, then v, then w (the reference leaves ties to nth_element); the
+ * d = sqrtf((u*u + v*v) + w*w), (u, v, w) = neighbour - center, equal d by
+ * the squared length, then the smaller u, v, w (the reference leaves ties to
+ * nth_element); the
  * K-th one is binned when d <= R: per component t = u * (1.f / width), index
  * (int)(t + 0.5) or (int)(t - 0.5) in double, plus half.  counts is
  * [ntasks][edge^3], x fastest.
  */
 typedef struct {
-    float d, u, v, w;
+    float d, d2, u, v, w;
 } onpc;
 
 static int onpc_cmp(const void *pa, const void *pb)
@@ -721,6 +722,8 @@ static int onpc_cmp(const void *pa, const void *pb)
     const onpc *a = (const onpc *)pa, *b = (const onpc *)pb;
     if (a->d != b->d)
         return a->d < b->d ? -1 : 1;
+    if (a->d2 != b->d2)
+        return a->d2 < b->d2 ? -1 : 1;
     if (a->u != b->u)
         return a->u < b->u ? -1 : 1;
     if (a->v != b->v)
@@ -771,7 +774,8 @@ int pp_oracle_npcorr(int64_t n, const float *xyz, const uint8_t *type, const flo
                 e.u = xyz[3 * j] - p[0];
                 e.v = xyz[3 * j + 1] - p[1];
                 e.w = xyz[3 * j + 2] - p[2];
-                e.d = sqrtf(e.u * e.u + e.v * e.v + e.w * e.w);
+                e.d2 = e.u * e.u + e.v * e.v + e.w * e.w;
+                e.d = sqrtf(e.d2);
                 buf[m++] = e;
             }
             if (m <= K)

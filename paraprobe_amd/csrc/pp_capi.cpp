@@ -2448,7 +2448,7 @@ pp_status pp_npcorr_run(pp_ctx *c, const pp_cloud *cl, int32_t ntasks, const uin
     a.K = (uint32_t)knn_order;
     a.maxlen = rmax;
     a.inv_width = 1.f / rincr;
-    a.inv_bucket = (float)kNpcBuckets / rmax;
+    a.inv_bucket = (float)kNpcBuckets / a.R2;
     a.half = (int)((edge - 1) / 2);
     a.edge = (uint64_t)edge;
     a.nxyz = nxyz;

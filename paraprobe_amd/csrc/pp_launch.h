@@ -138,7 +138,7 @@ struct NpcArgs {
     uint32_t K;                 // SpatStatKNNOrder (0-based)
     float maxlen;               // npc3d fvec_maxlen (SpatStatRadiusMax)
     float inv_width;            // fl32(1 / SpatStatRadiusIncr)
-    float inv_bucket;           // kNpcBuckets / R: distance bucket of d = d * inv_bucket
+    float inv_bucket;           // kNpcBuckets / R2: distance bucket of d2 = d2 * inv_bucket
     int half;                   // halfedgelen
     uint64_t edge, nxyz;        // edgelen, edgelen^3
     uint32_t *counts;           // [ntasks][nxyz]

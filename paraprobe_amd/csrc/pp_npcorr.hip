@@ -13,20 +13,21 @@
 // rounded half away from zero in double ((int)(t + 0.5) / (int)(t - 0.5) of
 // t = fl(u * fl(1 / width))), when d <= maxlen.  The reference picks the K-th
 // with nth_element over its KD-tree order, so equal distances leave the pick
-// to that order; here ties are broken by the smaller (u, v, w),
+// to that order; here equal d are ordered by the squared length d2 (of which
+// d is a monotone function), then by the smaller (u, v, w),
 // lexicographically, numerically.
 //
 // One wavefront per center.  Each task reads its own candidate list: the
 // ions of its neighbour types, cell-sorted on a grid at least R/2 wide, so
 // that the 5 x 5 x 5 cells around a center hold its ball; the stencil rows
 // are cut to the part a sphere of the needed radius can reach.  A first sweep
-// counts the candidates per distance bucket (kNpcBuckets buckets over
-// d = sqrtf(d2)) within a radius r1 that holds the K-th at the list's mean
+// counts the candidates per distance bucket (kNpcBuckets buckets, uniform
+// in d2) within a radius r1 that holds the K-th at the list's mean
 // density with a margin, or within R when r1 holds too few; the bucket that
 // holds the K-th
 // is found by a scan, its candidates are gathered in one more sweep, limited
-// to the bucket's spherical shell (16 bytes each: d bits, then u, v, w as
-// order-preserving integers), and ranked; buckets are monotone in d, so every
+// to the bucket's spherical shell (16 bytes each: d2 bits, then u, v, w as
+// order-preserving integers), and ranked; buckets are monotone in the key, so every
 // candidate of a lower bucket precedes them.  When the bucket holds more
 // than kNpcCap candidates (shells of a perfect lattice), the wave falls back
 // to a buffer of candidates that is sorted (bitonic) and cut to the K + 1
@@ -159,13 +160,17 @@ __global__ __launch_bounds__(256) void k_npcorr(NpcArgs a)
         }
         return rw;
     };
-    // Visits every candidate of the ranges: f(j, valid) per lane (j is only
-    // meaningful when valid).  The ranges go to lanes (lane l < 25: the first
-    // range of row l, lane 25 + l: its second) with an exclusive prefix of
-    // their lengths.  Long ranges are walked one by one, 64 candidates a
-    // step; short ones (sparse type lists) as one concatenation, each lane
-    // finding its range by a binary search over the prefix.
-    auto sweep = [&](const Rows &rw, auto &&f) {
+    // Visits every candidate of the ranges in list p: f(q, valid) per lane
+    // with the candidate's record q (only meaningful when valid).  The ranges
+    // go to lanes (lane l < 25: the first range of row l, lane 25 + l: its
+    // second) with an exclusive prefix of their lengths.  Long ranges are
+    // walked one by one, 64 candidates a step; short ones (sparse type lists)
+    // as one concatenation, each lane finding its range by a binary search
+    // over the prefix.  The records of the next step are loaded while f runs
+    // on the current ones; the loads are unconditional (the last step loads
+    // its own records again), so that no branch joins paths with different
+    // loads in flight.
+    auto sweep = [&](const Rows &rw, const float4 *p, auto &&f) {
         const int src = lane >= 25 ? lane - 25 : lane;
         const uint32_t sb2 = __shfl(rw.b2, src, kWave), se2 = __shfl(rw.e2, src, kWave);
         const uint32_t b = lane < 25 ? rw.b1 : (lane < 50 ? sb2 : 0u);
@@ -178,50 +183,92 @@ __global__ __launch_bounds__(256) void k_npcorr(NpcArgs a)
         }
         const uint32_t excl = incl - len;
         const uint32_t total = __shfl(incl, kWave - 1, kWave);
+        if (total == 0)
+            return;
         const uint32_t nne = (uint32_t)__builtin_popcountll(__builtin_amdgcn_ballot_w64(len > 0));
         if (total >= 32u * nne) {
-            for (int r = 0; r < 50; ++r) {
-                const uint32_t rb = __builtin_amdgcn_readlane(b, r);
-                const uint32_t re = rb + __builtin_amdgcn_readlane(len, r);
-                for (uint32_t j0 = rb; j0 < re; j0 += kWave)
-                    f(j0 + lane, j0 + lane < re);
+            // cursor: step start cj in the range ending at ce (wave-uniform)
+            int nr = 0;
+            uint32_t cj = 0, ce = 0;
+            auto next_range = [&]() {
+                while (nr < 50) {
+                    const uint32_t rb = __builtin_amdgcn_readlane(b, nr);
+                    const uint32_t rl = __builtin_amdgcn_readlane(len, nr);
+                    ++nr;
+                    if (rl) {
+                        cj = rb;
+                        ce = rb + rl;
+                        return true;
+                    }
+                }
+                return false;
+            };
+            bool has = next_range();
+            bool cok = cj + lane < ce;
+            float4 cur = p[cok ? cj + lane : cj];
+            while (has) {
+                const float4 q = cur;
+                const bool ok = cok;
+                const uint32_t last = cj;
+                cj += kWave;
+                if (cj >= ce)
+                    has = next_range();
+                cok = has && cj + lane < ce;
+                cur = p[cok ? cj + lane : (has ? cj : last)];
+                f(q, ok);
             }
         } else {
-            for (uint32_t s0 = 0; s0 < total; s0 += kWave) {
-                const uint32_t sl = s0 + lane;
+            auto at = [&](uint32_t sl) {
                 int k = 0;
                 for (int st = kWave / 2; st > 0; st >>= 1) {
                     const uint32_t v = __shfl(excl, k + st, kWave);
                     k = v <= sl ? k + st : k;
                 }
-                const uint32_t j = __shfl(b, k, kWave) + (sl - __shfl(excl, k, kWave));
-                f(j, sl < total);
+                return __shfl(b, k, kWave) + (sl - __shfl(excl, k, kWave));
+            };
+            const uint32_t first = __shfl(b, (int)__builtin_ctzll(__builtin_amdgcn_ballot_w64(len > 0)),
+                                          kWave);
+            // (the searches run on every lane: a lane shuffle reads nothing
+            // from lanes that are switched off)
+            bool cok = (uint32_t)lane < total;
+            const uint32_t j0 = at((uint32_t)lane);
+            float4 cur = p[cok ? j0 : first];
+            for (uint32_t s0 = 0; s0 < total; s0 += kWave) {
+                const float4 q = cur;
+                const bool ok = cok;
+                const uint32_t sn = s0 + kWave + (uint32_t)lane;
+                cok = sn < total;
+                const uint32_t jn = at(sn);
+                cur = p[cok ? jn : first];
+                f(q, ok);
             }
         }
     };
-    // a neighbour within R of the center in list p (the center itself
-    // included when its type is listed): its difference vector
-    auto neighbour = [&](const float4 *p, uint32_t j, float &u, float &v, float &w) {
-        const float4 q = p[j];
+    // a neighbour within R of the center (the center itself included when
+    // its type is listed): its difference vector and squared length (the
+    // same value as fl((u*u + v*v) + w*w), the squares of negated terms)
+    auto neighbour = [&](const float4 &q, float &u, float &v, float &w, float &d2) {
         const float dxx = me.x - q.x, dyy = me.y - q.y, dzz = me.z - q.z;
-        const float d2 = dxx * dxx + dyy * dyy + dzz * dzz;
+        d2 = dxx * dxx + dyy * dyy + dzz * dzz;
         // the difference vector neighbour - center
         u = q.x - me.x;
         v = q.y - me.y;
         w = q.z - me.z;
         return d2 <= a.R2;
     };
-    // its key (d bits, then u, v, w order-preserving) and distance bucket:
-    // (int)(d * inv_b), clamped to the last bucket when the buckets span R
-    // (full), else -1 beyond them
-    float inv_b = a.inv_bucket;
+    // its key: d2 bits, then u, v, w order-preserving (the order by d =
+    // sqrtf(d2), a monotone function, with equal d ordered by d2 first)
+    auto key_of = [&](float u, float v, float w, float d2) {
+        return make_uint4(__float_as_uint(d2), ordf(u), ordf(v), ordf(w));
+    };
+    // its bucket (int)(d2 * inv_b2), monotone in the key: clamped to the
+    // last bucket when the buckets span R^2 (full), else -1 beyond them
+    float inv_b2 = a.inv_bucket;
     bool full = true;
-    auto key_of = [&](float u, float v, float w, int &bucket) {
-        // its length by the reference's expression (sqrt of a float)
-        const float dd = __builtin_sqrtf(u * u + v * v + w * w);
-        const int bk = (int)(dd * inv_b);
-        bucket = bk < kNpcBuckets - 1 ? bk : (full ? kNpcBuckets - 1 : (bk == kNpcBuckets - 1 ? bk : -1));
-        return make_uint4(__float_as_uint(dd), ordf(u), ordf(v), ordf(w));
+    auto bucket_of = [&](float d2) {
+        const int bk = (int)(d2 * inv_b2);
+        return bk < kNpcBuckets - 1 ? bk
+                                    : (full ? kNpcBuckets - 1 : (bk == kNpcBuckets - 1 ? bk : -1));
     };
     const double reach = (double)a.maxlen * (1.0 + 0x1p-12);
     for (int t = 0; t < a.ntasks; ++t) {
@@ -244,7 +291,7 @@ __global__ __launch_bounds__(256) void k_npcorr(NpcArgs a)
         // Candidates beyond r1 have larger keys than every one within it.
         const float r1 = a.r1[t];
         full = !(r1 < a.maxlen);
-        inv_b = full ? a.inv_bucket : (float)kNpcBuckets / r1;
+        inv_b2 = full ? a.inv_bucket : (float)kNpcBuckets / (r1 * r1);
         uint32_t nenv = 0;
         for (;;) {
             for (int q = lane; q < kNpcBuckets; q += kWave)
@@ -252,14 +299,13 @@ __global__ __launch_bounds__(256) void k_npcorr(NpcArgs a)
             __builtin_amdgcn_wave_barrier();
             const Rows ball = rows_for(ecs, 0.0, full ? reach : (double)r1 * (1.0 + 0x1p-12));
             nenv = 0;
-            sweep(ball, [&](uint32_t j, bool ok) {
+            sweep(ball, ep, [&](const float4 &q, bool ok) {
                 if (!ok)
                     return;
-                float u, v, w;
-                if (!neighbour(ep, j, u, v, w))
+                float u, v, w, d2;
+                if (!neighbour(q, u, v, w, d2))
                     return;
-                int bk;
-                (void)key_of(u, v, w, bk);
+                const int bk = bucket_of(d2);
                 if (bk < 0)
                     return;
                 __hip_atomic_fetch_add(&hist[bk], 1u, __ATOMIC_RELAXED,
@@ -272,7 +318,7 @@ __global__ __launch_bounds__(256) void k_npcorr(NpcArgs a)
             if (nenv > Ke || full)
                 break;
             full = true;
-            inv_b = a.inv_bucket;
+            inv_b2 = a.inv_bucket;
         }
         // more than K neighbours of the task's types (S.cpp:3940-3951; more
         // than K of any type then holds as well)
@@ -317,21 +363,16 @@ __global__ __launch_bounds__(256) void k_npcorr(NpcArgs a)
         if (inb <= (uint32_t)kNpcCap) {
             // the selected bucket's candidates, ranked; they lie in the shell
             // of the bucket's distances
-            const double bw = 1.0 / (double)inv_b;
-            const Rows shell = rows_for(ecs, bsel * bw * (1.0 - 0x1p-12),
+            const double bw = 1.0 / (double)inv_b2;
+            const Rows shell = rows_for(ecs, sqrt(bsel * bw) * (1.0 - 0x1p-12),
                                         (bsel == kNpcBuckets - 1 && full)
                                             ? reach
-                                            : (bsel + 1) * bw * (1.0 + 0x1p-12));
+                                            : sqrt((bsel + 1) * bw) * (1.0 + 0x1p-12));
             uint32_t cnt = 0;
-            sweep(shell, [&](uint32_t j, bool ok) {
-                float u = 0.f, v = 0.f, w = 0.f;
-                int bk = -1;
-                uint4 key = make_uint4(0, 0, 0, 0);
-                bool take = ok && neighbour(ep, j, u, v, w);
-                if (take) {
-                    key = key_of(u, v, w, bk);
-                    take = bk == bsel;
-                }
+            sweep(shell, ep, [&](const float4 &q, bool ok) {
+                float u = 0.f, v = 0.f, w = 0.f, d2 = 0.f;
+                bool take = ok && neighbour(q, u, v, w, d2) && bucket_of(d2) == bsel;
+                const uint4 key = key_of(u, v, w, d2);
                 const uint64_t m = __builtin_amdgcn_ballot_w64(take);
                 if (take)
                     buf[cnt + (uint32_t)__builtin_popcountll(m & ((1ull << lane) - 1ull))] = key;
@@ -363,13 +404,10 @@ __global__ __launch_bounds__(256) void k_npcorr(NpcArgs a)
             bool dropped = !self;
             uint32_t cnt = 0;
             const Rows ball = rows_for(ecs, 0.0, reach);
-            sweep(ball, [&](uint32_t j, bool ok) {
-                float u = 0.f, v = 0.f, w = 0.f;
-                int bk;
-                uint4 key = make_uint4(0, 0, 0, 0);
-                bool take = ok && neighbour(ep, j, u, v, w);
-                if (take)
-                    key = key_of(u, v, w, bk);
+            sweep(ball, ep, [&](const float4 &q, bool ok) {
+                float u = 0.f, v = 0.f, w = 0.f, d2 = 0.f;
+                bool take = ok && neighbour(q, u, v, w, d2);
+                const uint4 key = key_of(u, v, w, d2);
                 if (!dropped) {
                     const uint64_t zm = __builtin_amdgcn_ballot_w64(
                         take && key.x == 0u && key.y == 0x80000000u && key.z == 0x80000000u &&
@@ -396,7 +434,8 @@ __global__ __launch_bounds__(256) void k_npcorr(NpcArgs a)
             wave_sort(buf, cnt, lane);
             pick = buf[K];
         }
-        if (lane == 0 && __uint_as_float(pick.x) <= a.maxlen) {
+        // binned when d = sqrtf(d2) <= maxlen (npc3d::add)
+        if (lane == 0 && __builtin_sqrtf(__uint_as_float(pick.x)) <= a.maxlen) {
             // npc3d::add: t = u * (1 / width) in float, rounded half away
             // from zero through a double, plus the centre offset
             int ix[3];

@@ -180,3 +180,21 @@ def test_npcorr_sparse_region_of_a_dense_cloud(ctx):
         want = O.npcorr(xyz, ty, s2, tasks, 2.5, 0.1, K)
         assert np.array_equal(got, want)
         assert got.sum() > 0
+
+
+def test_npcorr_equal_distance_different_square(ctx):
+    """Two neighbours with equal float length sqrtf(d2) and squared lengths
+    one ulp apart: the smaller d2 is ranked first, as in the oracle."""
+    f = np.float32
+    xyz = np.array([[0, 0, 0], [f("0.380425"), f("0.5418678"), 0],
+                    [f("-0.380425"), f("0.54186785"), 0]], np.float32)
+    ty = np.array([1, 2, 2], np.uint8)
+    s2 = np.array([100, 0, 0], np.float32)
+    for K in (0, 1):
+        cloud = ctx.upload(xyz, ty, s2)
+        try:
+            got = ctx.npcorr(cloud, [((1,), (2,))], 2.0, 0.25, K)
+        finally:
+            cloud.free()
+        assert np.array_equal(got, O.npcorr(xyz, ty, s2, [((1,), (2,))], 2.0, 0.25, K))
+        assert got.sum() == 1

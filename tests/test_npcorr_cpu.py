@@ -67,10 +67,25 @@ def test_k_conditions_and_exclusions():
 
 
 def test_ties_take_the_smallest_vector():
-    # six neighbours at the same distance: (-1, 0, 0) is the smallest (u, v, w)
+    # six neighbours at the same distance and squared distance: (-1, 0, 0) is
+    # the smallest (u, v, w)
     neigh = [[1, 0, 0], [0, 1, 0], [0, 0, 1], [-1, 0, 0], [0, -1, 0], [0, 0, -1]]
     assert _bin(_one_center(neigh, 0)) == (-4, 0, 0)
     assert _bin(_one_center(neigh, 1)) == (0, -4, 0)
+
+
+def test_equal_distances_take_the_smaller_square_first():
+    # two neighbours whose float lengths sqrtf(d2) are equal while their
+    # squared lengths differ by one ulp: the smaller d2 comes first although
+    # the other has the smaller u
+    f = np.float32
+    a = [f("0.380425"), f("0.5418678"), f(0)]
+    b = [f("-0.380425"), f("0.54186785"), f(0)]
+    d2a = f(f(a[0] * a[0]) + f(a[1] * a[1]))
+    d2b = f(f(b[0] * b[0]) + f(b[1] * b[1]))
+    assert d2a < d2b and np.sqrt(d2a) == np.sqrt(d2b)
+    assert _bin(_one_center([a, b], 0)) == (2, 2, 0)
+    assert _bin(_one_center([a, b], 1)) == (-2, 2, 0)
 
 
 def test_total_counts_against_kdtree():
