@@ -185,23 +185,20 @@ __global__ __launch_bounds__(256) void k_npcorr(NpcArgs a)
         const uint32_t total = __shfl(incl, kWave - 1, kWave);
         if (total == 0)
             return;
-        const uint32_t nne = (uint32_t)__builtin_popcountll(__builtin_amdgcn_ballot_w64(len > 0));
+        uint64_t nonempty = __builtin_amdgcn_ballot_w64(len > 0);
+        const uint32_t nne = (uint32_t)__builtin_popcountll(nonempty);
         if (total >= 32u * nne) {
-            // cursor: step start cj in the range ending at ce (wave-uniform)
-            int nr = 0;
+            // cursor: step start cj in the range ending at ce (wave-uniform);
+            // the ranges left are the set bits of `nonempty`
             uint32_t cj = 0, ce = 0;
             auto next_range = [&]() {
-                while (nr < 50) {
-                    const uint32_t rb = __builtin_amdgcn_readlane(b, nr);
-                    const uint32_t rl = __builtin_amdgcn_readlane(len, nr);
-                    ++nr;
-                    if (rl) {
-                        cj = rb;
-                        ce = rb + rl;
-                        return true;
-                    }
-                }
-                return false;
+                if (!nonempty)
+                    return false;
+                const int nr = (int)__builtin_ctzll(nonempty);
+                nonempty &= nonempty - 1;
+                cj = __builtin_amdgcn_readlane(b, nr);
+                ce = cj + __builtin_amdgcn_readlane(len, nr);
+                return true;
             };
             bool has = next_range();
             bool cok = cj + lane < ce;
@@ -226,8 +223,7 @@ __global__ __launch_bounds__(256) void k_npcorr(NpcArgs a)
                 }
                 return __shfl(b, k, kWave) + (sl - __shfl(excl, k, kWave));
             };
-            const uint32_t first = __shfl(b, (int)__builtin_ctzll(__builtin_amdgcn_ballot_w64(len > 0)),
-                                          kWave);
+            const uint32_t first = __shfl(b, (int)__builtin_ctzll(nonempty), kWave);
             // (the searches run on every lane: a lane shuffle reads nothing
             // from lanes that are switched off)
             bool cok = (uint32_t)lane < total;
