@@ -102,7 +102,7 @@ struct Rows {
 __global__ __launch_bounds__(256) void k_npcorr(NpcArgs a)
 {
     __shared__ uint4 s_buf[kWaves][kNpcCap];
-    __shared__ uint32_t s_hist[kWaves][kNpcBuckets];
+    __shared__ uint32_t s_hist[kWaves][kNpcBuckets + 1];
     const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
     uint4 *buf = s_buf[wid];
     uint32_t *hist = s_hist[wid];
@@ -257,14 +257,15 @@ __global__ __launch_bounds__(256) void k_npcorr(NpcArgs a)
     auto key_of = [&](float u, float v, float w, float d2) {
         return make_uint4(__float_as_uint(d2), ordf(u), ordf(v), ordf(w));
     };
-    // its bucket (int)(d2 * inv_b2), monotone in the key: clamped to the
-    // last bucket when the buckets span R^2 (full), else -1 beyond them
+    // its bucket (int)(d2 * inv_b2), monotone in the key, capped at bmax:
+    // the last bucket when the buckets span R^2 (full), else the slot past
+    // them, which collects what lies beyond and is never selected
     float inv_b2 = a.inv_bucket;
     bool full = true;
+    uint32_t bmax = kNpcBuckets - 1;
     auto bucket_of = [&](float d2) {
-        const int bk = (int)(d2 * inv_b2);
-        return bk < kNpcBuckets - 1 ? bk
-                                    : (full ? kNpcBuckets - 1 : (bk == kNpcBuckets - 1 ? bk : -1));
+        const uint32_t bk = __float2uint_rz(d2 * inv_b2);
+        return bk < bmax ? bk : bmax;
     };
     const double reach = (double)a.maxlen * (1.0 + 0x1p-12);
     for (int t = 0; t < a.ntasks; ++t) {
@@ -282,39 +283,45 @@ __global__ __launch_bounds__(256) void k_npcorr(NpcArgs a)
         const uint32_t Ke = a.K + (self ? 1u : 0u);
         // count the candidates per distance bucket: first within the radius
         // r1 that holds the K-th at the list's mean density with a margin
-        // (buckets of width r1 / kNpcBuckets), and only when fewer than
-        // K + 1 lie there, within R (buckets of width R / kNpcBuckets).
+        // (buckets of r1^2 / kNpcBuckets in d2), and only when fewer than
+        // K + 1 lie there, within R (buckets of R^2 / kNpcBuckets).
         // Candidates beyond r1 have larger keys than every one within it.
         const float r1 = a.r1[t];
         full = !(r1 < a.maxlen);
         inv_b2 = full ? a.inv_bucket : (float)kNpcBuckets / (r1 * r1);
-        uint32_t nenv = 0;
+        bmax = full ? kNpcBuckets - 1 : kNpcBuckets;
+        // lane l sums buckets [l * per, (l + 1) * per): mine, and the
+        // inclusive scan of those sums over the lanes; nenv = candidates
+        // in the buckets
+        constexpr int per = kNpcBuckets / kWave;
+        uint32_t mine = 0, incl = 0, nenv = 0;
         for (;;) {
-            for (int q = lane; q < kNpcBuckets; q += kWave)
+            for (int q = lane; q <= kNpcBuckets; q += kWave)
                 hist[q] = 0;
             __builtin_amdgcn_wave_barrier();
             const Rows ball = rows_for(ecs, 0.0, full ? reach : (double)r1 * (1.0 + 0x1p-12));
-            nenv = 0;
             sweep(ball, ep, [&](const float4 &q, bool ok) {
-                if (!ok)
-                    return;
                 float u, v, w, d2;
-                if (!neighbour(q, u, v, w, d2))
-                    return;
-                const int bk = bucket_of(d2);
-                if (bk < 0)
-                    return;
-                __hip_atomic_fetch_add(&hist[bk], 1u, __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_WORKGROUP);
-                ++nenv;
+                if (ok && neighbour(q, u, v, w, d2))
+                    __hip_atomic_fetch_add(&hist[bucket_of(d2)], 1u, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP);
             });
-            for (int o = kWave / 2; o > 0; o >>= 1)
-                nenv += __shfl_xor(nenv, o, kWave);
             __builtin_amdgcn_wave_barrier();
+            mine = 0;
+            for (int q = 0; q < per; ++q)
+                mine += hist[lane * per + q];
+            incl = mine;
+            for (int o = 1; o < kWave; o <<= 1) {
+                const uint32_t vv = __shfl_up(incl, o, kWave);
+                if (lane >= o)
+                    incl += vv;
+            }
+            nenv = __shfl(incl, kWave - 1, kWave);
             if (nenv > Ke || full)
                 break;
             full = true;
             inv_b2 = a.inv_bucket;
+            bmax = kNpcBuckets - 1;
         }
         // more than K neighbours of the task's types (S.cpp:3940-3951; more
         // than K of any type then holds as well)
@@ -324,17 +331,7 @@ __global__ __launch_bounds__(256) void k_npcorr(NpcArgs a)
         int bsel = -1;
         uint32_t below = 0, inb = 0;
         {
-            // the bucket of the Ke-th: lane l sums buckets [l * per, (l + 1) * per)
-            constexpr int per = kNpcBuckets / kWave;
-            uint32_t mine = 0;
-            for (int q = 0; q < per; ++q)
-                mine += hist[lane * per + q];
-            uint32_t incl = mine;
-            for (int o = 1; o < kWave; o <<= 1) {
-                const uint32_t vv = __shfl_up(incl, o, kWave);
-                if (lane >= o)
-                    incl += vv;
-            }
+            // the bucket of the Ke-th
             const uint32_t excl = incl - mine;
             if (excl <= Ke && Ke < incl) {
                 uint32_t c = excl;
@@ -367,7 +364,7 @@ __global__ __launch_bounds__(256) void k_npcorr(NpcArgs a)
             uint32_t cnt = 0;
             sweep(shell, ep, [&](const float4 &q, bool ok) {
                 float u = 0.f, v = 0.f, w = 0.f, d2 = 0.f;
-                bool take = ok && neighbour(q, u, v, w, d2) && bucket_of(d2) == bsel;
+                bool take = ok && neighbour(q, u, v, w, d2) && bucket_of(d2) == (uint32_t)bsel;
                 const uint4 key = key_of(u, v, w, d2);
                 const uint64_t m = __builtin_amdgcn_ballot_w64(take);
                 if (take)
