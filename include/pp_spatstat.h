@@ -191,9 +191,9 @@ pp_status pp_ion2surf_device(pp_ctx *ctx, const float *d_xyz, uint64_t n,
  * of every considered center of a central type that has more than knn_order
  * neighbours of any type within rmax, binned to the nearest cell (halves away
  * from zero) when its length is <= rmax; counts is [ntasks][edge^3] uint32
- * (npc3d::cnts), x fastest.  Equal distances are ordered by the smaller
- * (u, v, w) (the reference leaves them to nth_element over its KD-tree
- * order).  knn_order must be below 448.  *centers_binned (may be NULL)
+ * (npc3d::cnts), x fastest.  Equal distances are ordered by the squared
+ * distance, then the smaller (u, v, w) (the reference leaves them to
+ * nth_element over its KD-tree order).  knn_order must be below 448.  *centers_binned (may be NULL)
  * receives the number of counts added. */
 int64_t pp_npcorr_geometry(float rmax, float rincr);
 pp_status pp_npcorr_run(pp_ctx *ctx, const pp_cloud *cloud, int32_t ntasks,
