@@ -484,26 +484,38 @@ pp_status build_on_grid(pp_ctx *c, Slot &s, const float *xyz, const uint8_t *typ
     L.s2 = m.take<float>(n);
     L.g = g;
     L.w = w;
+    // each cell's ions in octant order (tiles of consecutive centers then
+    // span less than a cell, so their stencil rows cut shorter), unless the
+    // octant offsets would be large next to the cloud
+    const int sub =
+        (nc << 3) <= std::max<uint64_t>(4 * n, (uint64_t)1 << 22) && (nc << 3) < 0xFFFFFFF0ull ? 3
+                                                                                              : 0;
+    const uint64_t nk = nc << sub;
     // temporaries after the first extra_scratch_bytes of the scratch arena
     Carver t0(nullptr);
     t0.off = extra_scratch_bytes;
-    t0.take<uint32_t>(nc);
+    t0.take<uint32_t>(nk);
     t0.take<uint32_t>(n);
     t0.take<uint32_t>(n);
-    t0.take<uint32_t>(scan_workspace_words(nc));
+    t0.take<uint32_t>(scan_workspace_words(nk));
+    if (sub)
+        t0.take<uint32_t>(nk + 1);
     pp_status st = ensure(c, s, s.scratch, t0.off + 256);
     if (st != PP_OK)
         return st;
     Carver t(s.scratch.p);
     t.off = extra_scratch_bytes;
-    uint32_t *count = t.take<uint32_t>(nc);
+    uint32_t *count = t.take<uint32_t>(nk);
     uint32_t *key = t.take<uint32_t>(n);
     uint32_t *rank = t.take<uint32_t>(n);
-    uint32_t *ws = t.take<uint32_t>(scan_workspace_words(nc));
-    PP_HIP(c, hipMemsetAsync(count, 0, nc * sizeof(uint32_t), s.stream));
-    PP_HIP(c, launch_cellkeys(xyz, n, g, count, key, rank, s.stream));
-    PP_HIP(c, scan_exclusive(count, L.cellstart, nc, ws, s.stream));
-    PP_HIP(c, launch_scatter(xyz, type, s2, n, key, rank, L.cellstart, L.pos, L.s2, s.stream));
+    uint32_t *ws = t.take<uint32_t>(scan_workspace_words(nk));
+    uint32_t *start = sub ? t.take<uint32_t>(nk + 1) : L.cellstart;
+    PP_HIP(c, hipMemsetAsync(count, 0, nk * sizeof(uint32_t), s.stream));
+    PP_HIP(c, launch_cellkeys(xyz, n, g, count, key, rank, s.stream, sub));
+    PP_HIP(c, scan_exclusive(count, start, nk, ws, s.stream));
+    PP_HIP(c, launch_scatter(xyz, type, s2, n, key, rank, start, L.pos, L.s2, s.stream));
+    if (sub)
+        PP_HIP(c, launch_stride(start, L.cellstart, nc + 1, sub, s.stream));
     return PP_OK;
 }
 

@@ -242,29 +242,62 @@ __device__ inline int cell_coord(float v, double lo, double inv, int dim)
     return c >= dim ? dim - 1 : c;
 }
 
+// The half of its cell an ion lies in along one axis (0 lower, 1 upper),
+// from the same FP64 coordinate as cell_coord.
+__device__ inline uint32_t cell_half(float v, double lo, double inv, int c)
+{
+    const double t = ((double)v - lo) * inv - (double)c;
+    return t >= 0.5 ? 1u : 0u;
+}
+
+// key = cell (x fastest), or with sub = 3 the cell times 8 plus the octant of
+// the cell the ion lies in, so that a sort by key orders each cell's ions by
+// octant (the cell offsets are every 8th key offset)
 __global__ __launch_bounds__(256) void k_cellkeys(const float *__restrict__ xyz, uint64_t n,
                                                   GridGeom g, uint32_t *__restrict__ count,
                                                   uint32_t *__restrict__ key,
-                                                  uint32_t *__restrict__ rank)
+                                                  uint32_t *__restrict__ rank, int sub)
 {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n)
         return;
-    const int cx = cell_coord(xyz[3 * i], g.lo[0], g.inv_w, g.nx);
-    const int cy = cell_coord(xyz[3 * i + 1], g.lo[1], g.inv_w, g.ny);
-    const int cz = cell_coord(xyz[3 * i + 2], g.lo[2], g.inv_w, g.nz);
-    const uint32_t k = ((uint32_t)cz * (uint32_t)g.ny + (uint32_t)cy) * (uint32_t)g.nx + (uint32_t)cx;
+    const float x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
+    const int cx = cell_coord(x, g.lo[0], g.inv_w, g.nx);
+    const int cy = cell_coord(y, g.lo[1], g.inv_w, g.ny);
+    const int cz = cell_coord(z, g.lo[2], g.inv_w, g.nz);
+    uint32_t k = ((uint32_t)cz * (uint32_t)g.ny + (uint32_t)cy) * (uint32_t)g.nx + (uint32_t)cx;
+    if (sub)
+        k = (k << 3) | cell_half(x, g.lo[0], g.inv_w, cx) |
+            (cell_half(y, g.lo[1], g.inv_w, cy) << 1) | (cell_half(z, g.lo[2], g.inv_w, cz) << 2);
     key[i] = k;
     rank[i] = atomicAdd(&count[k], 1u);
 }
 
+__global__ __launch_bounds__(256) void k_stride(const uint32_t *__restrict__ in,
+                                                uint32_t *__restrict__ out, uint64_t nout,
+                                                int shift)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nout)
+        out[i] = in[i << shift];
+}
+
+hipError_t launch_stride(const uint32_t *in, uint32_t *out, uint64_t nout, int shift,
+                         hipStream_t s)
+{
+    if (nout == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_stride, dim3(grid_for(nout, 256)), dim3(256), 0, s, in, out, nout, shift);
+    return hipGetLastError();
+}
+
 hipError_t launch_cellkeys(const float *xyz, uint64_t n, GridGeom g, uint32_t *count,
-                           uint32_t *key, uint32_t *rank, hipStream_t s)
+                           uint32_t *key, uint32_t *rank, hipStream_t s, int sub)
 {
     if (n == 0)
         return hipSuccess;
     hipLaunchKernelGGL(k_cellkeys, dim3(grid_for(n, 256)), dim3(256), 0, s, xyz, n, g, count,
-                       key, rank);
+                       key, rank, sub);
     return hipGetLastError();
 }
 

@@ -24,10 +24,14 @@ hipError_t scan_exclusive(const uint32_t *in, uint32_t *out, uint64_t n, uint32_
 hipError_t launch_bbox(const float *xyz, uint64_t n, uint32_t *out6, hipStream_t s);
 float ord_to_float(uint32_t u);
 
-// Cell-list build: key/rank per ion, then scatter into cell-sorted float4
-// (x, y, z, type) and the sorted ion2surf array.
+// Cell-list build: key/rank per ion (sub = 3: keys of cell octants, count
+// holds 8 entries per cell), then scatter into cell-sorted float4
+// (x, y, z, type) and the sorted ion2surf array; launch_stride takes every
+// (1 << shift)-th offset (the cell offsets of octant keys).
 hipError_t launch_cellkeys(const float *xyz, uint64_t n, GridGeom g, uint32_t *count,
-                           uint32_t *key, uint32_t *rank, hipStream_t s);
+                           uint32_t *key, uint32_t *rank, hipStream_t s, int sub = 0);
+hipError_t launch_stride(const uint32_t *in, uint32_t *out, uint64_t nout, int shift,
+                         hipStream_t s);
 hipError_t launch_scatter(const float *xyz, const uint8_t *type, const float *s2, uint64_t n,
                           const uint32_t *key, const uint32_t *rank, const uint32_t *cellstart,
                           float4 *pos, float *s2sorted, hipStream_t s);
