@@ -4,14 +4,17 @@
 // (src/PARAPROBE_KDTree.cpp:423-655) and the OpenMP center loop of
 // horderdist::compute_generic_spatstat3 (src/PARAPROBE_SolverHdl.cpp:3770-4176)
 // with a uniform cell list and one persistent query kernel:
-//   * cell list: bounding box, x-fastest linear cell keys computed in FP64,
-//     a one-digit counting sort (count with atomics, exclusive scan, scatter)
-//     into cell-sorted float4 (x, y, z, type) plus the sorted ion2surf array;
+//   * cell list: bounding box, x-fastest linear cell keys computed in FP64
+//     (times 8 plus the ion's octant of its cell where those offsets stay
+//     small), a one-digit counting sort (count with atomics, exclusive scan,
+//     scatter) into cell-sorted float4 (x, y, z, type) plus the sorted
+//     ion2surf array;
 //   * per job: considered-center flags, stream compaction one center type
 //     after the other, tiles of up to T centers of one type in one cell;
 //   * query (k_query, three modes): each workgroup pulls tiles from eight
-//     per-XCD tile queues; the four wavefronts split the tile's stencil rows
-//     into 64-candidate chunks, each lane holds one candidate and loops over
+//     per-XCD tile queues; the tile's stencil rows are cut to the bounding
+//     box of its centers; the four wavefronts split the rows into
+//     64-candidate chunks, each lane holds one candidate and loops over
 //     the T centers held in registers, evaluating the reference's FP32
 //     squared distance ((dx*dx + dy*dy) + dz*dz, no contraction) and binning
 //     in-ball pairs with integer LDS atomics: mode 0 into pair counts per

@@ -277,3 +277,23 @@ def test_distances_packed_around_every_bin_edge(ctx, rmin, rincr, rmax):
     g, _, st = _compare(ctx, xyz, ty, s2, Job([((1,), (1,))], rmin, rincr, rmax,
                                                FAM_RDF | FAM_MKNN, kvalues=[1]))
     assert st["inball"] == 2 * n
+
+
+def test_sparse_cloud_in_a_large_box(ctx):
+    """Clusters far apart in a large box: the grid is capped near 1M cells,
+    where the cell list keeps each cell's ions unordered (no octant order),
+    next to a dense cloud whose cells are octant-ordered; both match the
+    oracle."""
+    rng = np.random.default_rng(17)
+    parts = [rng.normal(loc=c, scale=0.6, size=(400, 3))
+             for c in rng.uniform(0, 400, size=(6, 3))]
+    xyz = np.concatenate(parts).astype(np.float32)
+    ty = rng.choice([1, 2], size=xyz.shape[0], p=[0.7, 0.3]).astype(np.uint8)
+    s2 = np.full(xyz.shape[0], 50.0, np.float32)
+    job = Job([((1,), (1, 2)), ((2,), (1,))], 0.0, 0.05, 1.0, FAM_RDF | FAM_MKNN | FAM_1NN,
+              kvalues=[1, 2, 5])
+    _say("sparse cloud in a 400 nm box")
+    _compare(ctx, xyz, ty, s2, job)
+    xyz, ty, s2 = _cube(8000, 5, L=6.0)
+    _say("dense cube")
+    _compare(ctx, xyz, ty, s2, job)
