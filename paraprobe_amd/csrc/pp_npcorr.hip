@@ -114,7 +114,6 @@ __global__ __launch_bounds__(256) void k_npcorr(NpcArgs a)
     // considered: ion2surf >= fl32(R * R) (S.cpp:3846-3860)
     if (!(a.s2[ci] >= a.R2))
         return;
-    const int cx = cell_of(me.x, a.g.lo[0], a.g.inv_w, a.g.nx);
     const int cy = cell_of(me.y, a.g.lo[1], a.g.inv_w, a.g.ny);
     const int cz = cell_of(me.z, a.g.lo[2], a.g.inv_w, a.g.nz);
     // The candidates of a spherical shell ri <= d <= ro around the center in
