@@ -57,14 +57,19 @@ static inline unsigned grid_for(uint64_t n, unsigned block)
     return (unsigned)(g ? g : 1);
 }
 
+// Inclusive prefix sum over the 64 lanes with DPP moves (no LDS round
+// trips): shifts by 1, 2, 4, 8 inside each row of 16 lanes (a lane without a
+// source adds 0), then row_bcast:15 adds row 0's total to row 1 and row 2's
+// to row 3, and row_bcast:31 adds the total of rows 0-1 to rows 2 and 3.
 __device__ inline uint32_t wave_incl_scan(uint32_t v, int lane)
 {
-#pragma unroll
-    for (int d = 1; d < kWave; d <<= 1) {
-        uint32_t t = __shfl_up(v, d, kWave);
-        if (lane >= d)
-            v += t;
-    }
+    (void)lane;
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xF, 0xF, false);  // row_shr:1
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xF, 0xF, false);  // row_shr:2
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xF, 0xF, false);  // row_shr:4
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xF, 0xF, false);  // row_shr:8
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xA, 0xF, false);  // row_bcast:15
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xC, 0xF, false);  // row_bcast:31
     return v;
 }
 
@@ -1229,7 +1234,7 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : (T == 8 && MODE
                                         kb[(uint32_t)kid * (uint32_t)TR + (uint32_t)row] = (uint16_t)cat;
                                 }
                             }
-                            carry += __shfl(incl, kWave - 1, kWave);
+                            carry += (uint32_t)__builtin_amdgcn_readlane((int)incl, kWave - 1);
                             if (carry >= need)
                                 break;
                         }
