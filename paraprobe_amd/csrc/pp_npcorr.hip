@@ -51,6 +51,26 @@ __device__ inline float unordf(uint32_t o)
     return __uint_as_float((o & 0x80000000u) ? (o & 0x7FFFFFFFu) : ~o);
 }
 
+// Inclusive prefix sum over the 64 lanes with DPP moves: shifts by 1, 2, 4, 8
+// inside each row of 16 lanes (a lane without a source adds 0), then
+// row_bcast:15 and row_bcast:31 carry the row totals forward.
+__device__ inline uint32_t wave_scan(uint32_t v)
+{
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xF, 0xF, false);  // row_shr:1
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xF, 0xF, false);  // row_shr:2
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xF, 0xF, false);  // row_shr:4
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xF, 0xF, false);  // row_shr:8
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xA, 0xF, false);  // row_bcast:15
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xC, 0xF, false);  // row_bcast:31
+    return v;
+}
+
+// a value of one lane, the lane index being the same in the whole wave
+__device__ inline uint32_t lane_value(uint32_t v, int l)
+{
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, l);
+}
+
 __device__ inline bool key_less(const uint4 &a, const uint4 &b)
 {
     if (a.x != b.x)
@@ -174,14 +194,9 @@ __global__ __launch_bounds__(256) void k_npcorr(NpcArgs a)
         const uint32_t sb2 = __shfl(rw.b2, src, kWave), se2 = __shfl(rw.e2, src, kWave);
         const uint32_t b = lane < 25 ? rw.b1 : (lane < 50 ? sb2 : 0u);
         const uint32_t len = lane < 25 ? rw.e1 - rw.b1 : (lane < 50 ? se2 - sb2 : 0u);
-        uint32_t incl = len;
-        for (int o = 1; o < kWave; o <<= 1) {
-            const uint32_t v = __shfl_up(incl, o, kWave);
-            if (lane >= o)
-                incl += v;
-        }
+        const uint32_t incl = wave_scan(len);
         const uint32_t excl = incl - len;
-        const uint32_t total = __shfl(incl, kWave - 1, kWave);
+        const uint32_t total = lane_value(incl, kWave - 1);
         if (total == 0)
             return;
         uint64_t nonempty = __builtin_amdgcn_ballot_w64(len > 0);
@@ -222,7 +237,7 @@ __global__ __launch_bounds__(256) void k_npcorr(NpcArgs a)
                 }
                 return __shfl(b, k, kWave) + (sl - __shfl(excl, k, kWave));
             };
-            const uint32_t first = __shfl(b, (int)__builtin_ctzll(nonempty), kWave);
+            const uint32_t first = lane_value(b, (int)__builtin_ctzll(nonempty));
             // (the searches run on every lane: a lane shuffle reads nothing
             // from lanes that are switched off)
             bool cok = (uint32_t)lane < total;
@@ -309,13 +324,8 @@ __global__ __launch_bounds__(256) void k_npcorr(NpcArgs a)
             mine = 0;
             for (int q = 0; q < per; ++q)
                 mine += hist[lane * per + q];
-            incl = mine;
-            for (int o = 1; o < kWave; o <<= 1) {
-                const uint32_t vv = __shfl_up(incl, o, kWave);
-                if (lane >= o)
-                    incl += vv;
-            }
-            nenv = __shfl(incl, kWave - 1, kWave);
+            incl = wave_scan(mine);
+            nenv = lane_value(incl, kWave - 1);
             if (nenv > Ke || full)
                 break;
             full = true;
@@ -348,8 +358,8 @@ __global__ __launch_bounds__(256) void k_npcorr(NpcArgs a)
             if (own == 0)  // cannot happen: more than Ke candidates were counted
                 continue;
             const int src = (int)__builtin_ctzll(own);
-            bsel = __shfl(bsel, src, kWave);
-            below = __shfl(below, src, kWave);
+            bsel = (int)lane_value((uint32_t)bsel, src);
+            below = lane_value(below, src);
             inb = hist[bsel];
         }
         if (inb <= (uint32_t)kNpcCap) {
@@ -386,7 +396,7 @@ __global__ __launch_bounds__(256) void k_npcorr(NpcArgs a)
             const uint64_t fm = __builtin_amdgcn_ballot_w64(found >= 0);
             if (fm == 0)  // cannot happen: the ranks are a permutation of [0, cnt)
                 continue;
-            const int q = __shfl(found, (int)__builtin_ctzll(fm), kWave);
+            const int q = (int)lane_value((uint32_t)found, (int)__builtin_ctzll(fm));
             pick = buf[q];
         } else {
             // a crowded bucket: keep the K + 1 smallest in the buffer; one
