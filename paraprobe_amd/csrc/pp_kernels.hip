@@ -1010,6 +1010,12 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : (T == 8 && MODE
                 // test fails and its type's table entries go unused)
                 const int woff = s_envoff[ptype];
                 const bool envok = woff >= 0;
+                // the tile's center count as an opaque scalar, so that each
+                // "c < ncen" below is one scalar compare and branch here (the
+                // compiler keeps hoisted compares as lane masks, which cost
+                // two vector instructions per center and chunk to branch on)
+                int ncen = cnt;
+                asm volatile("" : "+s"(ncen));
                 // this candidate's pair-count row (only dereferenced when envok)
                 uint32_t *acc_row = (PC ? s_H : s_P) + woff;
                 if (PC) {
@@ -1056,7 +1062,7 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : (T == 8 && MODE
                     // of resident wavefronts, higher
 #pragma unroll
                     for (int c = 0; c < T; ++c) {
-                        if (c < cnt) {  // uniform: skips a partial tile's absent centers
+                        if (c == 0 || c < ncen) {  // uniform: skips a partial tile's absent centers
                             const float dx = cxr[c] - px;
                             const float dy = cyr[c] - p.y;
                             const float dz = czr[c] - p.z;
@@ -1085,7 +1091,7 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : (T == 8 && MODE
                     const float bthr = rdf_env ? R2 : kthr;
 #pragma unroll
                     for (int c = 0; c < T; ++c) {
-                        if (c < cnt) {  // uniform: skips a partial tile's absent centers
+                        if (c == 0 || c < ncen) {  // uniform: skips a partial tile's absent centers
                             const float dx = cxr[c] - px;
                             const float dy = cyr[c] - p.y;
                             const float dz = czr[c] - p.z;
