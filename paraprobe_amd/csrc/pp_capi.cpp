@@ -793,7 +793,11 @@ void fill_jobdev(JobRun &r)
     d.idx_scale = p.et.idx_scale;
     d.idx_off = p.et.idx_off;
     d.idx_off1 = p.et.idx_off1;
-    d.amb = p.et.amb_margin(p.hg, p.R);
+    {
+        const float amb = p.et.amb_margin(p.hg, p.R);  // at most 0.5
+        d.idx_off1a = (float)(1.0 - p.hg.start * p.hg.tmp + (double)amb);
+        d.amb2 = 2.0f * amb;
+    }
     d.cat0 = p.cat0;
     d.cat_sentinel = p.cat_sentinel;
     d.E = p.E;

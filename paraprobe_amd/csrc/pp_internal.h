@@ -65,8 +65,11 @@ struct JobDev {
     const uint32_t *edges;    // C entries, edges[0] unused
     float idx_scale, idx_off;
     // floor form: category = floor(fmaf(sqrt(d2), idx_scale, idx_off1))
-    // wherever that argument lies at least amb from an integer
-    float idx_off1, amb;
+    // wherever that argument lies at least amb from an integer; the kernel
+    // tests the shifted coordinate fmaf(sqrt(d2), idx_scale, idx_off1a) with
+    // idx_off1a = fl(1 - start tmp + amb): clear when its fraction is at
+    // least amb2 = 2 amb, and then its floor is the category
+    float idx_off1, idx_off1a, amb2;
     int cat0;                 // category of d = 0 (self pair)
     int cat_sentinel;         // category of float(R + 1e-6)
     // types

@@ -702,24 +702,31 @@ __device__ inline uint32_t next_tile(const QArgs &a, uint32_t myx, int &qx, uint
     return a.end_tile;
 }
 
-// Category of an in-range squared distance (d2 <= fl(R * R)): floor of the
-// index coordinate where it is clear of every integer by amb; when any active lane is closer
-// than that to a bin edge (rare), the wave settles every lane with the exact
-// edge table instead.  Branch-free apart from that wave-uniform test, and no
-// LDS read on the common path.
-__device__ inline int category_of(float d2, float isc, float ioff1, float amb, float Cm1f,
-                                  const uint32_t *s_edges)
+// Category of an in-range squared distance (d2 <= fl(R * R)).  The index
+// coordinate y = (d - start) tmp + 1 has category floor(y) wherever y lies at
+// least max_offset from every integer, and fmaf(sqrt(d2), isc, ioff1) is
+// within y_err of it (amb = max_offset + y_err).  The shifted coordinate
+// ya = fmaf(sqrt(d2), isc, ioffa), ioffa = fl(1 - start tmp + amb), is within
+// y_err of y + amb; where its fraction is at least 2 amb, i.e. ya lies in
+// [m + 2 amb, m + 1), y lies in [m + max_offset, m + 1 - max_offset) and the
+// category is m = floor(ya).  When any active lane is nearer an edge (rare),
+// the wave settles every lane with the exact edge table from the nearest
+// ideal edge of the unshifted coordinate.  No LDS read on the common path.
+__device__ inline int category_of(float d2, float isc, float ioff1, float ioffa, float amb2,
+                                  float Cm1f, const uint32_t *s_edges)
 {
-    const float y = __builtin_fmaf(__builtin_amdgcn_sqrtf(d2), isc, ioff1);
-    const float f = __builtin_amdgcn_fractf(y);
-    const bool near = __builtin_fabsf(f - 0.5f) > 0.5f - amb;
+    const float s = __builtin_amdgcn_sqrtf(d2);
+    const float ya = __builtin_fmaf(s, isc, ioffa);
+    const bool near = __builtin_amdgcn_fractf(ya) < amb2;
     if (__builtin_expect(__builtin_amdgcn_ballot_w64(near) != 0, 0)) {
+        const float y = __builtin_fmaf(s, isc, ioff1);
         const int q = (int)__builtin_amdgcn_fmed3f(y + 0.5f, 1.0f, Cm1f);
         return q - (__float_as_uint(d2) < s_edges[q] ? 1 : 0);
     }
-    // floor for y >= 0, 0 for y < 0 (v_cvt_u32_f32 saturates); y < ncat for
-    // every in-ball distance (the edge table's build checks the bound)
-    return (int)__float2uint_rz(y);
+    // floor for ya >= 0, 0 for ya < 0 (v_cvt_u32_f32 saturates: y < 1 is
+    // the lowest category); ya < y + 0.5 + y_err < ncat for every in-ball
+    // distance (d <= R = end puts y at most at ncat - 1)
+    return (int)__float2uint_rz(ya);
 }
 
 // The persistent query kernel.  MODE 0: pair counts per (center type, env
@@ -960,7 +967,7 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : (T == 8 && MODE
             }
             const float R2 = J.R2;
             const float isc = J.idx_scale, ioff = J.idx_off;
-            const float ioff1 = J.idx_off1, amb = J.amb;
+            const float ioff1 = J.idx_off1, ioffa = J.idx_off1a, amb2 = J.amb2;
             const float Cm1f = (float)(C - 1);
             const bool rdf = (J.families & (FAM_RDF | FAM_RIPK)) != 0;
             const bool count_in = J.count_in != 0;
@@ -1072,7 +1079,7 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : (T == 8 && MODE
                                 my_in += (unsigned long long)__builtin_popcountll(
                                     __builtin_amdgcn_ballot_w64(in));
                             if (in && envok) {
-                                const int cat = category_of(d2, isc, ioff1, amb, Cm1f, s_edges);
+                                const int cat = category_of(d2, isc, ioff1, ioffa, amb2, Cm1f, s_edges);
                                 __hip_atomic_fetch_add(acc_row + cat, 1u,
                                                        __ATOMIC_RELAXED,
                                                        __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -1100,7 +1107,7 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : (T == 8 && MODE
                                 my_in += (unsigned long long)__builtin_popcountll(
                                     __builtin_amdgcn_ballot_w64(d2 <= R2));
                             if (d2 <= bthr) {
-                                const int cat = category_of(d2, isc, ioff1, amb, Cm1f, gedges);
+                                const int cat = category_of(d2, isc, ioff1, ioffa, amb2, Cm1f, gedges);
                                 if (rdf_env)
                                     __hip_atomic_fetch_add(acc_row + cat, 1u,
                                                            __ATOMIC_RELAXED,
