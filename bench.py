@@ -116,6 +116,7 @@ def batch_leg(ctx, args):
                     "uniform or clustered frustum tips, one type pair each, Rmax U(1,5) nm, "
                     "RDF + MKNN k=1..kmax (kmax in 1,10,50), bin 0.01 nm",
         "autotune": bool(args.autotune),
+        "graph_launch": bool(args.graph),
         "plain_jobs_per_s": len(jobs) / plain_s,
         "batched_jobs_per_s": len(jobs) / batch_s,
         "speedup": plain_s / batch_s,
@@ -442,6 +443,8 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=400)
     ap.add_argument("--serve-slots", type=int, default=1)
     ap.add_argument("--serve-batch", type=int, default=32)
+    ap.add_argument("--graph", type=int, default=1,
+                    help="1: batch launch groups captured into HIP graphs; 0: eager launches")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (every box has one MI355X); "
@@ -466,6 +469,7 @@ def main():
             raise SystemExit("bench: hipDeviceSynchronize failed")
 
     with SpatStat(0) as ctx:
+        ctx.set_graph_launch(bool(args.graph))
         cloud = ctx.upload(tip.xyz, tip.types, tip.ion2surf_sqr)
         for i in range(args.warmup):
             r = ctx.run(cloud, job)

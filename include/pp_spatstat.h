@@ -255,6 +255,16 @@ pp_status pp_cloud_from_epos(pp_ctx *ctx, const void *epos, uint64_t nbytes,
  * depend on the factor.  0 = full-range per-center histograms always. */
 pp_status pp_set_knn_range(pp_ctx *ctx, float factor);
 
+/* Launch form of the query kernel (no counterpart in the reference, whose
+ * OpenMP loop has no launches): 1 (default) = each launch group of a run of
+ * several jobs (a pp_batch_run batch, a serving batch) -- the persistent
+ * query launches of its chunks and their k-th neighbour reductions -- is
+ * captured into one HIP graph and launched as a unit; 0 = the same launches
+ * issued one by one.  Single-job runs always launch one by one (a graph's
+ * instantiation costs more than it saves there).  Results are identical
+ * either way. */
+pp_status pp_set_graph_launch(pp_ctx *ctx, int32_t on);
+
 /* ---- serving: a request loop over resident clouds (continuous batching) ----
  * Requests are queued by pp_serve_submit and admitted by their device-memory
  * footprint against hbm_budget_bytes; each of `slots` workers takes every

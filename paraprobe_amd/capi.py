@@ -49,6 +49,7 @@ EXPORTED = [
     "pp_batch_run",
     "pp_set_tuning",
     "pp_set_knn_range",
+    "pp_set_graph_launch",
     "pp_cloud_add_cell_list",
     "pp_serve_create",
     "pp_serve_submit",
@@ -148,6 +149,7 @@ def load_library() -> C.CDLL:
     lib.pp_batch_run.argtypes = [P, C.c_int32, C.POINTER(P), C.POINTER(_Job), C.POINTER(_Out)]
     lib.pp_set_tuning.argtypes = [P, C.c_int32, C.c_int32]
     lib.pp_set_knn_range.argtypes = [P, C.c_float]
+    lib.pp_set_graph_launch.argtypes = [P, C.c_int32]
     lib.pp_cloud_add_cell_list.argtypes = [P, P, C.c_float]
     lib.pp_serve_create.argtypes = [P, C.POINTER(_ServeOpts), C.POINTER(P)]
     lib.pp_serve_submit.argtypes = [P, P, C.POINTER(_Job), C.POINTER(_Out), C.POINTER(C.c_uint64)]
@@ -387,6 +389,12 @@ class SpatStat:
         """kNN range factor (expected neighbours per k) of MKNN / 1NN jobs; 0
         keeps full-range per-center histograms."""
         self._check(self.lib.pp_set_knn_range(self.ctx, factor))
+
+    def set_graph_launch(self, on: bool = True):
+        """Capture each launch group of a multi-job run (batch, serving
+        batch) into a HIP graph (default) or issue its launches one by one;
+        single-job runs always launch one by one.  Results are identical."""
+        self._check(self.lib.pp_set_graph_launch(self.ctx, 1 if on else 0))
 
     def ion2surf(self, xyz: np.ndarray, triangles: np.ndarray, R: float) -> np.ndarray:
         """Squared distance of every ion to the triangle hull, capped at

@@ -84,6 +84,7 @@ struct pp_ctx {
     std::string err;
     int tune_T = 0, tune_r = 0;
     double knn_factor = 3.0;  // kNN range: expected neighbours / k (0: full range)
+    int graph_launch = 1;     // batch launch groups captured into a HIP graph
     std::map<long, std::pair<int, int>> tuned;  // autotune bucket -> (T, cells per Rmax)
     Slot main;
 };
@@ -1065,12 +1066,29 @@ struct Keep {
     std::deque<std::vector<JobDev>> devs;
     std::deque<std::vector<uint32_t>> words;
     std::deque<std::vector<const Tile *>> tiles;
+    bool graph = false;               // capture each group's launches into a graph
+    hipStream_t stream = nullptr;     // the stream the graphs were launched on
+    std::vector<hipGraphExec_t> execs;
+    Keep() = default;
+    Keep(const Keep &) = delete;
+    Keep &operator=(const Keep &) = delete;
+    ~Keep()
+    {
+        // the launched graphs finish before their executables go
+        if (!execs.empty()) {
+            (void)hipStreamSynchronize(stream);
+            for (hipGraphExec_t e : execs)
+                (void)hipGraphExecDestroy(e);
+        }
+    }
 };
 
 // One query launch group: the jobs `ids` (one kernel mode) with their device
 // descriptors `base`, as launch chunks, each followed by the reduction of its
 // k-th neighbour categories.  May halve T (rebuilding the group's tiles) to
-// fit the LDS; returns the T used.
+// fit the LDS; returns the T used.  With keep.graph the group's kernel
+// launches (queue clears, query chunks, reductions; the descriptors are
+// copied before) are captured into one HIP graph, which is launched once.
 pp_status launch_group(pp_ctx *c, Slot &sl, std::vector<JobRun> &runs, const std::vector<int> &ids,
                        const std::vector<JobDev> &base, int mode, int &T, Keep &keep)
 {
@@ -1203,7 +1221,10 @@ pp_status launch_group(pp_ctx *c, Slot &sl, std::vector<JobRun> &runs, const std
                      "maxC=%d maxE=%d maxTrg=%d kmax=%d rows=%d HK=%d\n",
                      mode, T, c16, lds, nb, nj, total, nch, maxC, maxE, maxTrg, maxK, maxRows,
                      maxHK);
-    for (size_t k = 0; k < nch; ++k) {
+    if (keep.graph)
+        PP_HIP(c, hipStreamBeginCapture(sl.stream, hipStreamCaptureModeRelaxed));
+    hipError_t le = hipSuccess;
+    for (size_t k = 0; k < nch && le == hipSuccess; ++k) {
         const KChunk &ch = chunks[k];
         QueryLaunch ql{};
         ql.jobs = d_jobs + k * nj;
@@ -1231,12 +1252,31 @@ pp_status launch_group(pp_ctx *c, Slot &sl, std::vector<JobRun> &runs, const std
                       ? std::max<uint32_t>(1u, std::min<uint32_t>(
                                                    4u, (ch.g1 - ch.g0) / ((uint32_t)ql.grid * 16u)))
                       : 1u;
-        PP_HIP(c, launch_query(ql, sl.stream));
+        le = launch_query(ql, sl.stream);
         const uint32_t units = huoff[k * (nj + 1) + nj];
-        if (units > 0)
-            PP_HIP(c, launch_kreduce(d_jobs + k * nj, nj, d_uoff + k * (nj + 1), units, T, tpu,
-                                     red_lds, sl.stream));
+        if (le == hipSuccess && units > 0)
+            le = launch_kreduce(d_jobs + k * nj, nj, d_uoff + k * (nj + 1), units, T, tpu,
+                                red_lds, sl.stream);
     }
+    if (keep.graph) {
+        // the capture always ends, so the stream is usable after an error
+        hipGraph_t gr = nullptr;
+        const hipError_t ee = hipStreamEndCapture(sl.stream, &gr);
+        if (le == hipSuccess)
+            le = ee;
+        if (le == hipSuccess) {
+            hipGraphExec_t ex = nullptr;
+            le = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
+            if (le == hipSuccess) {
+                keep.stream = sl.stream;
+                keep.execs.push_back(ex);
+                le = hipGraphLaunch(ex, sl.stream);
+            }
+        }
+        if (gr)
+            (void)hipGraphDestroy(gr);
+    }
+    PP_HIP(c, le);
     for (int q = 0; q < nj; ++q) {
         runs[ids[q]].T_used = T;
         runs[ids[q]].c16_used = c16;
@@ -1277,13 +1317,14 @@ pp_status run_jobs(pp_ctx *c, Slot &sl, int njobs, const pp_cloud *const *clouds
     if (!outs)
         return fail(c, PP_ERR_INVALID_ARG, "outputs are NULL");
     std::vector<JobRun> runs(njobs);
-    int tuneT, tuneR;
+    int tuneT, tuneR, graph_launch;
     double knn_factor;
     {
         std::lock_guard<std::mutex> lk(c->mu);
         tuneT = c->tune_T;
         tuneR = c->tune_r;
         knn_factor = c->knn_factor;
+        graph_launch = c->graph_launch;
     }
     int T = 0;
     for (int i = 0; i < njobs; ++i) {
@@ -1428,6 +1469,9 @@ pp_status run_jobs(pp_ctx *c, Slot &sl, int njobs, const pp_cloud *const *clouds
             PP_HIP(c, hipMemsetAsync(r.flags, 0, 4 * r.cloud->n, sl.stream));
     }
     Keep keep;
+    // graphs for runs of several jobs only: for a single job the graph's
+    // instantiation (~0.1 ms) costs more than its launches
+    keep.graph = graph_launch != 0 && njobs > 1;
     int Tm[3] = {T, T, T};
     PP_HIP(c, hipEventRecord(sl.ev[3], sl.stream));
     for (int mode : {0, 2, 1}) {
@@ -2478,6 +2522,17 @@ pp_status pp_npcorr_run(pp_ctx *c, const pp_cloud *cl, int32_t ntasks, const uin
             tot += counts[i];
         *centers_binned = tot;
     }
+    return PP_OK;
+}
+
+pp_status pp_set_graph_launch(pp_ctx *c, int32_t on)
+{
+    if (!c)
+        return PP_ERR_INVALID_ARG;
+    if (on != 0 && on != 1)
+        return fail(c, PP_ERR_INVALID_ARG, "graph launch flag must be 0 or 1");
+    std::lock_guard<std::mutex> lk(c->mu);
+    c->graph_launch = on;
     return PP_OK;
 }
 

@@ -1363,10 +1363,19 @@ hipError_t query_max_blocks_per_cu(int T, int c16, int mode, size_t lds, int *nb
     PP_DISPATCH(occ_t, lds, nb);
 }
 
+// clears the tile queue counters (a kernel, not a memset node: the query
+// launches may be captured into a HIP graph)
+__global__ __launch_bounds__(64) void k_queue_clear(uint32_t *__restrict__ q)
+{
+    for (int i = threadIdx.x; i < kQueueWords; i += 64)
+        q[i] = 0u;
+}
+
 hipError_t launch_query(const QueryLaunch &q, hipStream_t s)
 {
     const int T = q.T, c16 = q.c16, mode = q.mode;
-    hipError_t e = hipMemsetAsync(q.queue, 0, sizeof(uint32_t) * kQueueWords, s);
+    hipLaunchKernelGGL(k_queue_clear, dim3(1), dim3(64), 0, s, q.queue);
+    hipError_t e = hipGetLastError();
     if (e != hipSuccess)
         return e;
     if (q.end_tile <= q.first_tile)
