@@ -812,6 +812,7 @@ void fill_jobdev(JobRun &r)
     d.kmax = p.kmax;
     d.nk = p.nku;
     d.kidx_of_k = (const int32_t *)(r.tables + r.o_kidx);
+    d.k_dense = p.kmax > 0 && p.nku == p.kmax ? 1 : 0;
     d.P = r.P;
     d.K = r.K;
     d.N1 = r.N1;

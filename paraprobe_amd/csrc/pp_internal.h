@@ -84,6 +84,7 @@ struct JobDev {
     uint32_t families;
     int kmax, nk;
     const int32_t *kidx_of_k; // [kmax + 1] index into the descending k list, or -1
+    int k_dense;              // every k of 1..kmax requested: kidx_of_k[k] = k - 1
     // outputs (device)
     unsigned long long *P;    // [ntrg][E][C] pair counts per center type / env type / category
     uint32_t *K;              // [ntasks][nk][C] MKNN

@@ -1181,6 +1181,7 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : (T == 8 && MODE
         }
         if (want_k || want_1 || want_cn) {
             const int kmax = want_k ? J.kmax : 0;
+            const bool kdense = J.k_dense != 0;
             const uint32_t need = want_k ? (uint32_t)kmax : 1u;
             const int lmax = J.lmax;
             // this tile's block of k-th neighbour rows in the chunk buffer
@@ -1241,10 +1242,18 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : (T == 8 && MODE
                                 kb[(uint32_t)J.nk * (uint32_t)TR + (uint32_t)row] = (uint16_t)cat;
                             if (want_k) {
                                 const uint32_t hi = cum < (uint32_t)kmax ? cum : (uint32_t)kmax;
-                                for (uint32_t k = prev + 1u; k <= hi; ++k) {
-                                    const int kid = s_kidx[k];
-                                    if (kid >= 0)
-                                        kb[(uint32_t)kid * (uint32_t)TR + (uint32_t)row] = (uint16_t)cat;
+                                if (kdense) {
+                                    // every k requested: plane k - 1, no table read
+                                    uint32_t o = prev * (uint32_t)TR + (uint32_t)row;
+                                    for (uint32_t k = prev + 1u; k <= hi; ++k, o += (uint32_t)TR)
+                                        kb[o] = (uint16_t)cat;
+                                } else {
+                                    for (uint32_t k = prev + 1u; k <= hi; ++k) {
+                                        const int kid = s_kidx[k];
+                                        if (kid >= 0)
+                                            kb[(uint32_t)kid * (uint32_t)TR + (uint32_t)row] =
+                                                (uint16_t)cat;
+                                    }
                                 }
                             }
                             carry += (uint32_t)__builtin_amdgcn_readlane((int)incl, kWave - 1);
