@@ -73,6 +73,19 @@ __device__ inline uint32_t wave_incl_scan(uint32_t v, int lane)
     return v;
 }
 
+// Inclusive running maximum over the 64 lanes of values >= -1, with the same
+// DPP moves as wave_incl_scan (a lane without a source takes -1)
+__device__ inline int wave_max_scan(int v)
+{
+    v = __builtin_elementwise_max(v, __builtin_amdgcn_update_dpp(-1, v, 0x111, 0xF, 0xF, false));
+    v = __builtin_elementwise_max(v, __builtin_amdgcn_update_dpp(-1, v, 0x112, 0xF, 0xF, false));
+    v = __builtin_elementwise_max(v, __builtin_amdgcn_update_dpp(-1, v, 0x114, 0xF, 0xF, false));
+    v = __builtin_elementwise_max(v, __builtin_amdgcn_update_dpp(-1, v, 0x118, 0xF, 0xF, false));
+    v = __builtin_elementwise_max(v, __builtin_amdgcn_update_dpp(-1, v, 0x142, 0xA, 0xF, false));
+    v = __builtin_elementwise_max(v, __builtin_amdgcn_update_dpp(-1, v, 0x143, 0xC, 0xF, false));
+    return v;
+}
+
 // ---------------------------------------------------------------------------
 // exclusive scan (reduce-then-scan, 2048 items per workgroup)
 
@@ -578,7 +591,7 @@ hipError_t launch_bump(const uint32_t *cpos, uint64_t n, const uint32_t *toff, u
 // query kernel
 
 struct Lay {
-    int edges, envoff, trgslot, kinfo, ckoff, kidx, rows, rowchunk, cen, meta, P, H, total;
+    int edges, envoff, trgslot, kinfo, ckoff, kidx, win, rows, rowchunk, cen, meta, P, H, total;
 };
 
 __host__ __device__ inline int al4(int x) { return (x + 3) & ~3; }
@@ -602,6 +615,8 @@ __host__ __device__ inline Lay make_layout(int T, int c16, int mode, int maxC, i
     o += mode == 2 ? kMaxEnv : 0;
     L.kidx = o;
     o += mode ? al4(maxKmax + 2) : 0;
+    L.win = o;  // per wavefront: the k window of the k-th category extraction
+    o += mode ? kWaves * kWave : 0;
     L.rows = o;
     o += al4(2 * maxRows);
     L.rowchunk = o;
@@ -751,6 +766,7 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : (T == 8 && MODE
     int2 *s_kinfo = (int2 *)(lds + L.kinfo);
     int32_t *s_ckoff = (int32_t *)(lds + L.ckoff);
     int32_t *s_kidx = (int32_t *)(lds + L.kidx);
+    int32_t *s_win = (int32_t *)(lds + L.win);
     uint32_t *s_rows = lds + L.rows;
     uint32_t *s_rowchunk = lds + L.rowchunk;
     float4 *s_cen = (float4 *)(lds + L.cen);
@@ -1227,8 +1243,36 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : (T == 8 && MODE
                     const int ck = HY ? (int)(rfl(J.task_ck[tk]) & 0x7FFFFFFFu) : C;
                     if ((want_k || want_1) && kb) {
                         // categories of the k-th nearest neighbours: a prefix
-                        // scan over the center's category counts, lane = category
+                        // scan over the center's category counts (lane =
+                        // category) gives each category the first k it holds;
+                        // the MKNN orders are then settled 64 at a time (lane =
+                        // k): each category marks its first k in the wave's
+                        // window row, and a running maximum over the window
+                        // carries every category to the k after it, up to the
+                        // next mark.  A window is written once every category
+                        // that starts in it has been counted.
                         uint32_t carry = 0;
+                        uint32_t kw = 0;  // the window holds k = kw + 1 .. kw + 64
+                        int wcur = -1;    // category of k = kw (-1: none yet)
+                        int32_t *wrow = s_win + wid * kWave;
+                        if (want_k)
+                            wrow[lane] = -1;
+                        // writes the window's orders (k <= lim: the scanned
+                        // category, above: the sentinel) and moves it on
+                        auto flush = [&](uint32_t lim) {
+                            int m = wrow[lane];
+                            wrow[lane] = -1;
+                            m = __builtin_elementwise_max(wave_max_scan(m), wcur);
+                            wcur = __builtin_amdgcn_readlane(m, kWave - 1);
+                            const uint32_t k = kw + 1u + (uint32_t)lane;
+                            if (k <= (uint32_t)kmax) {
+                                const int kid = kdense ? (int)k - 1 : s_kidx[k];
+                                if (kid >= 0)
+                                    kb[(uint32_t)kid * (uint32_t)TR + (uint32_t)row] =
+                                        (uint16_t)(k <= lim ? m : J.cat_sentinel);
+                            }
+                            kw += (uint32_t)kWave;
+                        };
                         for (int cb = 0; cb < ck; cb += kWave) {
                             const int cat = cb + lane;
                             uint32_t v = 0;
@@ -1240,23 +1284,22 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : (T == 8 && MODE
                             const uint32_t cum = carry + incl, prev = cum - v;
                             if (want_1 && prev < 1u && cum >= 1u)
                                 kb[(uint32_t)J.nk * (uint32_t)TR + (uint32_t)row] = (uint16_t)cat;
+                            const uint32_t cnew =
+                                carry + (uint32_t)__builtin_amdgcn_readlane((int)incl, kWave - 1);
                             if (want_k) {
-                                const uint32_t hi = cum < (uint32_t)kmax ? cum : (uint32_t)kmax;
-                                if (kdense) {
-                                    // every k requested: plane k - 1, no table read
-                                    uint32_t o = prev * (uint32_t)TR + (uint32_t)row;
-                                    for (uint32_t k = prev + 1u; k <= hi; ++k, o += (uint32_t)TR)
-                                        kb[o] = (uint16_t)cat;
-                                } else {
-                                    for (uint32_t k = prev + 1u; k <= hi; ++k) {
-                                        const int kid = s_kidx[k];
-                                        if (kid >= 0)
-                                            kb[(uint32_t)kid * (uint32_t)TR + (uint32_t)row] =
-                                                (uint16_t)cat;
-                                    }
+                                const uint32_t first = prev + 1u;  // this category's first k
+                                while (kw < (uint32_t)kmax) {
+                                    if (v > 0u && first > kw && first <= kw + (uint32_t)kWave)
+                                        wrow[first - kw - 1u] = cat;
+                                    const uint32_t wend = kw + (uint32_t)kWave < (uint32_t)kmax
+                                                              ? kw + (uint32_t)kWave
+                                                              : (uint32_t)kmax;
+                                    if (cnew < wend)
+                                        break;  // later chunks may still start in the window
+                                    flush(wend);
                                 }
                             }
-                            carry += (uint32_t)__builtin_amdgcn_readlane((int)incl, kWave - 1);
+                            carry = cnew;
                             if (carry >= need)
                                 break;
                         }
@@ -1268,13 +1311,9 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : (T == 8 && MODE
                         // sentinel category (S.cpp:4093-4095, 4035-4036)
                         if (want_1 && carry == 0u && lane == 0)
                             kb[(uint32_t)J.nk * (uint32_t)TR + (uint32_t)row] = (uint16_t)J.cat_sentinel;
-                        if (want_k) {
-                            for (int k = (int)carry + 1 + lane; k <= kmax; k += kWave) {
-                                const int kid = s_kidx[k];
-                                if (kid >= 0)
-                                    kb[(uint32_t)kid * (uint32_t)TR + (uint32_t)row] = (uint16_t)J.cat_sentinel;
-                            }
-                        }
+                        if (want_k)
+                            while (kw < (uint32_t)kmax)
+                                flush(carry);
                     }
                     if (PC && want_cn) {
                         for (int e = 0; e < E; ++e) {
