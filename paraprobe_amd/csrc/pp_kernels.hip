@@ -1273,13 +1273,27 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : (T == 8 && MODE
                             }
                             kw += (uint32_t)kWave;
                         };
+                        // a task of one neighbour type (the usual case) reads
+                        // one histogram row, located once per task
+                        const bool one_env = __builtin_popcount(envset) == 1;
+                        const int e1 = one_env ? __builtin_ctz(envset) : 0;
+                        const int off1 =
+                            hoff + (PC ? e1 * C : (int)rfl((uint32_t)s_ckoff[e1]));
                         for (int cb = 0; cb < ck; cb += kWave) {
                             const int cat = cb + lane;
                             uint32_t v = 0;
-                            if (cat < ck)
-                                for (int e = 0; e < E; ++e)
-                                    if ((envset >> e) & 1u)
-                                        v += hv(e, cat);
+                            if (cat < ck) {
+                                if (one_env) {
+                                    const uint32_t w = s_H[off1 + cat];
+                                    v = C16 ? ((w >> sh) & 0xFFFFu) : w;
+                                    if (e1 == es && cat == cat0)
+                                        v -= 1u;
+                                } else {
+                                    for (int e = 0; e < E; ++e)
+                                        if ((envset >> e) & 1u)
+                                            v += hv(e, cat);
+                                }
+                            }
                             const uint32_t incl = wave_incl_scan(v, lane);
                             const uint32_t cum = carry + incl, prev = cum - v;
                             if (want_1 && prev < 1u && cum >= 1u)
