@@ -151,11 +151,16 @@ bool EdgeTable::build(const HistGeometry &g, std::string *why)
 
 float EdgeTable::amb_margin(const HistGeometry &g, double R) const
 {
-    // |y - ((d - start) tmp + 1)| <= |sqrt error| * scale + d |scale - tmp|
-    //   + |offset error| + fma rounding, each below 2^-22 relative to its own
-    // magnitude (2 ulp sqrt, half-ulp roundings); 5e-7 covers their sum
-    const double ymax = std::fabs(R) * g.tmp + std::fabs(1.0 - g.start * g.tmp) + ncat_of(g) + 1.0;
-    const double m = max_offset + 5e-7 * ymax + 1e-6;
+    // |y - ((d - start) tmp + 1)| for d <= R, with A = R tmp and
+    // B = |1 - start tmp|: the square root's error (v_sqrt_f32, at most 2 ulp,
+    // 2^-22 relative) scaled, 2.4e-7 A; the float scale's rounding (2^-24
+    // relative), 6e-8 A; the fma's rounding of a result below A + B + 1 (the
+    // shifted coordinate adds at most 0.5), 6e-8 (A + B + 1); the float
+    // offset's rounding, 6e-8 (B + 1).  The margin is max_offset plus 1.5
+    // times their sum.
+    const double A = std::fabs(R) * g.tmp, B = std::fabs(1.0 - g.start * g.tmp);
+    const double yerr = 2.4e-7 * A + 6e-8 * A + 6e-8 * (A + B + 1.0) + 6e-8 * (B + 1.0);
+    const double m = max_offset + 1.5 * yerr + 1e-6;
     return (float)std::min(m, 0.5);
 }
 
