@@ -1222,7 +1222,8 @@ pp_status launch_group(pp_ctx *c, Slot &sl, std::vector<JobRun> &runs, const std
                      "maxC=%d maxE=%d maxTrg=%d kmax=%d rows=%d HK=%d\n",
                      mode, T, c16, lds, nb, nj, total, nch, maxC, maxE, maxTrg, maxK, maxRows,
                      maxHK);
-    if (keep.graph)
+    const bool capture = keep.graph && nch > 0;  // (no empty graphs)
+    if (capture)
         PP_HIP(c, hipStreamBeginCapture(sl.stream, hipStreamCaptureModeRelaxed));
     hipError_t le = hipSuccess;
     for (size_t k = 0; k < nch && le == hipSuccess; ++k) {
@@ -1259,7 +1260,7 @@ pp_status launch_group(pp_ctx *c, Slot &sl, std::vector<JobRun> &runs, const std
             le = launch_kreduce(d_jobs + k * nj, nj, d_uoff + k * (nj + 1), units, T, tpu,
                                 red_lds, sl.stream);
     }
-    if (keep.graph) {
+    if (capture) {
         // the capture always ends, so the stream is usable after an error
         hipGraph_t gr = nullptr;
         const hipError_t ee = hipStreamEndCapture(sl.stream, &gr);

@@ -71,3 +71,23 @@ def test_graph_launch_equals_eager_launch(ctx):
         ctx.set_graph_launch(True)
         for c in clouds:
             c.free()
+
+
+def test_graph_batch_with_empty_clouds(ctx):
+    _say("test_graph_batch_with_empty_clouds: batches of empty clouds, alone and mixed")
+    xyz, ty, s2 = _cloud(3000, 41, 6.0)
+    empty = ctx.upload(np.zeros((0, 3), np.float32), np.zeros(0, np.uint8),
+                       np.zeros(0, np.float32))
+    full = ctx.upload(xyz, ty, s2)
+    job = Job([((1,), (2,))], 0.0, 0.02, 1.0, FAM_RDF | FAM_MKNN | FAM_1NN, kvalues=[1, 3])
+    try:
+        ctx.set_graph_launch(True)
+        outs = ctx.run_batch([empty, empty], [job, job])
+        for o in outs:
+            assert o.rdf.sum() == 0 and o.centers_considered == 0
+        mixed = ctx.run_batch([empty, full, empty], [job, job, job])
+        _same(ctx.run(full, job), mixed[1])
+        assert mixed[0].rdf.sum() == 0 and mixed[2].rdf.sum() == 0
+    finally:
+        empty.free()
+        full.free()
