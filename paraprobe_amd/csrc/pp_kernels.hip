@@ -733,15 +733,17 @@ __device__ inline int category_of(float d2, float isc, float ioff1, float ioffa,
     const float s = __builtin_amdgcn_sqrtf(d2);
     const float ya = __builtin_fmaf(s, isc, ioffa);
     const bool near = __builtin_amdgcn_fractf(ya) < amb2;
+    // floor for ya >= 0, 0 for ya < 0 (v_cvt_u32_f32 saturates: y < 1 is
+    // the lowest category); ya < y + 0.5 + y_err < ncat for every in-ball
+    // distance (d <= R = end puts y at most at ncat - 1).  Taken before the
+    // test, so the common path runs straight through.
+    int cat = (int)__float2uint_rz(ya);
     if (__builtin_expect(__builtin_amdgcn_ballot_w64(near) != 0, 0)) {
         const float y = __builtin_fmaf(s, isc, ioff1);
         const int q = (int)__builtin_amdgcn_fmed3f(y + 0.5f, 1.0f, Cm1f);
-        return q - (__float_as_uint(d2) < s_edges[q] ? 1 : 0);
+        cat = q - (__float_as_uint(d2) < s_edges[q] ? 1 : 0);
     }
-    // floor for ya >= 0, 0 for ya < 0 (v_cvt_u32_f32 saturates: y < 1 is
-    // the lowest category); ya < y + 0.5 + y_err < ncat for every in-ball
-    // distance (d <= R = end puts y at most at ncat - 1)
-    return (int)__float2uint_rz(ya);
+    return cat;
 }
 
 // The persistent query kernel.  MODE 0: pair counts per (center type, env
