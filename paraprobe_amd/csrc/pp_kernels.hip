@@ -1070,7 +1070,7 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : (T == 8 && MODE
     #pragma unroll
                     for (int c = 0; c < T; ++c) {
                         const bool in = d2[c] <= R2;
-                        if (count_in)
+                        if (__builtin_expect(count_in, 0))
                             my_in += (unsigned long long)__builtin_popcountll(
                                 __builtin_amdgcn_ballot_w64(in));
                         if (in && envok) {
@@ -1093,7 +1093,7 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : (T == 8 && MODE
                             const float dz = czr[c] - p.z;
                             const float d2 = dx * dx + dy * dy + dz * dz;
                             const bool in = d2 <= R2;
-                            if (count_in)
+                            if (__builtin_expect(count_in, 0))
                                 my_in += (unsigned long long)__builtin_popcountll(
                                     __builtin_amdgcn_ballot_w64(in));
                             if (in && envok) {
@@ -1121,7 +1121,7 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : (T == 8 && MODE
                             const float dy = cyr[c] - p.y;
                             const float dz = czr[c] - p.z;
                             const float d2 = dx * dx + dy * dy + dz * dz;
-                            if (count_in)
+                            if (__builtin_expect(count_in, 0))
                                 my_in += (unsigned long long)__builtin_popcountll(
                                     __builtin_amdgcn_ballot_w64(d2 <= R2));
                             if (d2 <= bthr) {
