@@ -31,8 +31,8 @@ VALU_PEAK_OPS = 78.6e12     # MI355X FP32 VALU element-ops/s (157.3 TFLOP/s coun
 OPS_PER_EVAL = 9            # 3 sub + 3 mul + 2 add + 1 compare, no FMA
 # HBM bytes of the query kernel per step of the 20M-ion headline job, from two
 # rocprofv3 --pmc runs of this script (FETCH_SIZE x 2 and WRITE_SIZE x 1, the
-# factors a 1 GiB device copy reads): 133.1 GB read + 13.9 GB written
-TRAFFIC_HEADLINE = 1.470e11
+# factors a 1 GiB device copy reads): 141.0 GB read + 15.7 GB written
+TRAFFIC_HEADLINE = 1.567e11
 TRAFFIC_SOURCE = "profiles/k_query_frustum20M_rdf_mknn100.md"
 
 
