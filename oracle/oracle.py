@@ -56,6 +56,9 @@ def _load():
             C.c_int64, P, P, P, P, P, C.c_float, C.c_float, C.c_float, C.c_int64, C.c_int32, P,
         ]
         lib.pp_oracle_binedge_report.restype = C.c_int
+        lib.pp_oracle_category_sweep.argtypes = [C.c_float, C.c_float, C.c_float, C.c_float,
+                                                 C.c_float, P, P, C.c_int64]
+        lib.pp_oracle_category_sweep.restype = C.c_int64
         lib.pp_oracle_closest_tri.argtypes = [P, P]
         lib.pp_oracle_closest_tri.restype = C.c_float
         lib.pp_oracle_ion2surf.argtypes = [P, C.c_int64, P, C.c_int64, C.c_float, P]
@@ -84,6 +87,19 @@ def categories_d2(rmin, rincr, rmax, d2: np.ndarray) -> np.ndarray:
     out = np.empty(d2.shape[0], dtype=np.int32)
     _load().pp_oracle_categories_d2(rmin, rincr, rmax, d2.ctypes.data, d2.shape[0], out.ctypes.data)
     return out
+
+
+def category_sweep(rmin, rincr, rmax, d_lo, d_hi, cap=1 << 20):
+    """Every float d in [d_lo, d_hi] through the literal histogram::add:
+    (first float of each new category, that category), in order; raises if
+    the category ever decreases."""
+    first = np.zeros(cap, np.uint32)
+    cat = np.zeros(cap, np.int32)
+    n = _load().pp_oracle_category_sweep(rmin, rincr, rmax, d_lo, d_hi, first.ctypes.data,
+                                         cat.ctypes.data, cap)
+    if n < 0 or n > cap:
+        raise ValueError("category sweep: not monotone, or too many changes")
+    return first[:n].view(np.float32), cat[:n]
 
 
 def _mask(types):

@@ -121,6 +121,93 @@ void pp_oracle_categories_d2(float rmin, float rincr, float rmax, const float *d
         out[i] = (int32_t)ohist_cat(&h, sqrtf(d2[i]));
 }
 
+/* Every float d in [d_lo, d_hi] (non-negative), in bit-pattern order, through
+ * the literal histogram::add: records each change of category (first[i] = bit
+ * pattern of the first float of the new category, cat[i] = that category; at
+ * most cap entries) and returns the number of changes, or -1 when the
+ * category ever decreases. */
+int64_t pp_oracle_category_sweep(float rmin, float rincr, float rmax, float d_lo, float d_hi,
+                                 uint32_t *first, int32_t *cat, int64_t cap)
+{
+    ohist h;
+    ohist_init(&h, rmin, rincr, rmax);
+    if (!h.valid || !(d_lo >= 0.f) || !(d_hi >= d_lo))
+        return -1;
+    uint32_t u0, u1;
+    memcpy(&u0, &d_lo, 4);
+    memcpy(&u1, &d_hi, 4);
+    enum { NCH = 256, LCAP = 4096 };
+    /* chunks in parallel: each keeps its own changes, merged in order */
+    static uint32_t lf[NCH][LCAP];
+    static int32_t lc[NCH][LCAP];
+    int64_t ln[NCH];
+    uint32_t c_first[NCH], c_last[NCH];
+    int bad = 0;
+    const uint64_t span = (uint64_t)u1 - u0 + 1;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 1) reduction(| : bad)
+#endif
+    for (int k = 0; k < NCH; ++k) {
+        const uint64_t a = u0 + span * (uint64_t)k / NCH, b = u0 + span * (uint64_t)(k + 1) / NCH;
+        ln[k] = 0;
+        c_first[k] = c_last[k] = 0;
+        if (a >= b)
+            continue;
+        float x;
+        uint32_t ua = (uint32_t)a;
+        memcpy(&x, &ua, 4);
+        uint32_t prev = ohist_cat(&h, x);
+        c_first[k] = prev;
+        for (uint64_t u = a + 1; u < b; ++u) {
+            const uint32_t uu = (uint32_t)u;
+            memcpy(&x, &uu, 4);
+            const uint32_t c = ohist_cat(&h, x);
+            if (c != prev) {
+                if (c < prev)
+                    bad = 1;
+                if (ln[k] < LCAP) {
+                    lf[k][ln[k]] = uu;
+                    lc[k][ln[k]] = (int32_t)c;
+                }
+                ++ln[k];
+                prev = c;
+            }
+        }
+        c_last[k] = prev;
+    }
+    if (bad)
+        return -1;
+    int64_t n = 0;
+    int have = 0;
+    uint32_t prev = 0;
+    for (int k = 0; k < NCH; ++k) {
+        const uint64_t a = u0 + span * (uint64_t)k / NCH, b = u0 + span * (uint64_t)(k + 1) / NCH;
+        if (a >= b)
+            continue;
+        if (have && c_first[k] != prev) {
+            if (c_first[k] < prev)
+                return -1;
+            if (n < cap) {
+                first[n] = (uint32_t)a;
+                cat[n] = (int32_t)c_first[k];
+            }
+            ++n;
+        }
+        if (ln[k] > LCAP)
+            return -1;
+        for (int64_t i = 0; i < ln[k]; ++i) {
+            if (n < cap) {
+                first[n] = lf[k][i];
+                cat[n] = lc[k][i];
+            }
+            ++n;
+        }
+        prev = c_last[k];
+        have = 1;
+    }
+    return n;
+}
+
 static inline int has_type(const uint8_t *mask32, uint8_t t)
 {
     return (mask32[t >> 3] >> (t & 7)) & 1;
@@ -130,6 +217,43 @@ static int cmp_float(const void *a, const void *b)
 {
     float fa = *(const float *)a, fb = *(const float *)b;
     return (fa > fb) - (fa < fb);
+}
+
+/* Moves the kneed smallest of a[0..m) to the front, in ascending order
+ * (kneed <= m): Hoare partitions around a median-of-three pivot until the
+ * kneed-th position is settled (what std::nth_element guarantees, S.cpp:4088),
+ * then a sort of the front.  Values are distances, never NaN. */
+static void smallest_sorted(float *a, int64_t m, int64_t kneed)
+{
+    int64_t lo = 0, hi = m - 1;
+    const int64_t k = kneed - 1;
+    while (hi > lo) {
+        const int64_t mid = lo + (hi - lo) / 2;
+        float x = a[lo], y = a[mid], z = a[hi];
+        const float piv = (x < y) ? ((y < z) ? y : (x < z ? z : x)) : ((x < z) ? x : (y < z ? z : y));
+        int64_t i = lo, j = hi;
+        while (i <= j) {
+            while (a[i] < piv)
+                ++i;
+            while (a[j] > piv)
+                --j;
+            if (i <= j) {
+                const float t = a[i];
+                a[i] = a[j];
+                a[j] = t;
+                ++i;
+                --j;
+            }
+        }
+        /* a[lo..j] <= piv <= a[i..hi], and a[j+1..i-1] == piv */
+        if (k <= j)
+            hi = j;
+        else if (k >= i)
+            lo = i;
+        else
+            break;
+    }
+    qsort(a, (size_t)kneed, sizeof(float), cmp_float);
 }
 
 typedef struct {
@@ -397,9 +521,18 @@ int pp_oracle_spatstat(int64_t n, const float *xyz, const uint8_t *type, const f
                             }
                             tmp[m++] = nb[q].d;
                         }
-                    /* k-th order statistic: a full sort gives the same value as
-                     * nth_element (S.cpp:4088) */
-                    if (m > 1)
+                    /* k-th order statistic: the kneed smallest in order give
+                     * the same values as nth_element (S.cpp:4088); a full sort
+                     * when most of the list is needed */
+                    int64_t kneed = 0;
+                    for (int kth = 0; kth < nk; ++kth)
+                        if (kminus1_desc[kth] + 1 > kneed)
+                            kneed = kminus1_desc[kth] + 1;
+                    if (kneed > m)
+                        kneed = m;
+                    if (m > 1 && 4 * kneed < m)
+                        smallest_sorted(tmp, m, kneed);
+                    else if (m > 1)
                         qsort(tmp, (size_t)m, sizeof(float), cmp_float);
                     for (int kth = 0; kth < nk; ++kth) {
                         int64_t k = kminus1_desc[kth];
@@ -498,6 +631,10 @@ int pp_oracle_binedge_report(int64_t n, const float *xyz, const uint8_t *type, c
                         const float dx = xi - xyz[3 * j], dy = yi - xyz[3 * j + 1],
                                     dz = zi - xyz[3 * j + 2];
                         const float d2 = dx * dx + dy * dy + dz * dz;
+                        /* the FP32 expression is within a few ulp of the exact
+                         * value: far outside the ball neither test keeps it */
+                        if (d2 > R2 * 1.001f)
+                            continue;
                         const double ex = (double)xi - (double)xyz[3 * j];
                         const double ey = (double)yi - (double)xyz[3 * j + 1];
                         const double ez = (double)zi - (double)xyz[3 * j + 2];

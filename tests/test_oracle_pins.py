@@ -60,23 +60,26 @@ def test_category_closed_forms():
     assert F.category(0.0, 0.01, 2.0, float(sent)) == 201
 
 
-def test_category_monotone_and_edges_dense():
-    # exhaustive over every float d in [0, 2.02): category is monotone and
-    # changes exactly at the bin edges start + b / tmp (to within one ulp)
-    lo = np.float32(0.0).view(np.uint32)
-    hi = np.float32(2.02).view(np.uint32)
-    u = np.arange(lo, hi, 97, dtype=np.uint32)  # stride keeps it fast; edges checked below
-    d = u.view(np.float32)
-    cats = F.categories_d2(0.0, 0.01, 2.0, (d * d).astype(np.float32))
-    assert np.all(np.diff(cats) >= 0)
-    for b in range(1, 200):
-        e = np.float32(b / 100.0)
-        below = np.nextafter(e, np.float32(0), dtype=np.float32)
-        above = np.nextafter(e, np.float32(3), dtype=np.float32)
-        cb = F.category(0.0, 0.01, 2.0, float(below))
-        ca = F.category(0.0, 0.01, 2.0, float(above))
-        assert cb in (b, b + 1) and ca in (b + 1, b + 2)
-        assert ca - cb >= 1 - 1 and cb <= b + 1
+@pytest.mark.parametrize("rincr,nbins", [(0.01, 200), (0.1, 19)])
+def test_category_sweep_every_float(rincr, nbins):
+    # every float d in [0, 2.02] through histogram::add: the category starts at
+    # 1 (bin 0), rises by exactly one at each change, and changes only at the
+    # ideal edges start + b / tmp (tmp = nbins / (end - start), so 2/19-wide
+    # bins for the truncated 2.0 / 0.1 geometry), to within one float ulp; the
+    # last change is to cnts_highest at d == Rmax exactly.
+    first, cat = F.category_sweep(0.0, rincr, 2.0, 0.0, 2.02)
+    assert F.category(0.0, rincr, 2.0, 0.0) == 1
+    assert len(first) == nbins
+    np.testing.assert_array_equal(cat, np.arange(2, nbins + 2))
+    tmp = nbins / 2.0
+    for b, t in enumerate(first[:-1].astype(np.float64), start=1):
+        ulp = float(np.spacing(np.float32(t)))
+        assert abs(t - b / tmp) <= ulp, (b, t)
+    assert first[-1] == np.float32(2.0)
+    # the float just below each change is still in the category before it
+    below = np.nextafter(first, np.float32(0), dtype=np.float32)
+    for c, d in zip(cat, below):
+        assert F.category(0.0, rincr, 2.0, float(d)) == c - 1
 
 
 # --------------------------------------------------------------------------
@@ -244,3 +247,112 @@ def test_exclusion_threshold_is_inclusive():
     s2[::2] = np.nextafter(R2, np.float32(0), dtype=np.float32)
     h, st = F.spatstat(xyz, ty, s2, [((1, 2, 3), (1,))], 0.0, 0.1, float(R), F.FAM_RDF)
     assert st["considered"] == xyz.shape[0] // 2
+
+
+# --------------------------------------------------------------------------
+# FP64 bin-edge report (pp_oracle_binedge_report): the exact real distance of
+# the two stored points, binned in double, against the FP32 pipeline
+
+def _one_pair(nbor, rmin=0.0, rincr=0.125, rmax=2.0):
+    xyz = np.array([(0.0, 0.0, 0.0), nbor], dtype=np.float32)
+    ty = np.array([1, 1], np.uint8)
+    s2 = np.array([1e6, 0.0], np.float32)  # only ion 0 is a center
+    return F.binedge_report(xyz, ty, s2, (1,), (1,), rmin, rincr, rmax)
+
+
+def test_binedge_fp32_rounds_onto_edge():
+    # d = (1 - 2^-24, 2^-12, 2^-12 (1 - 2^-24)): the float squares and sums
+    # round to exactly 1.0 (1 - 2^-23, then + 2^-24, then + 2^-24 - 2^-47),
+    # so sqrtf gives 1.0 = the edge of bin 8 (bins of 1/8: tmp = 8), while the
+    # exact d^2 = 1 - 2^-48 + 2^-72 < 1 puts the real distance in bin 7
+    t = 2.0 ** -12
+    p = (1.0 - 2.0 ** -24, t, t * (1.0 - 2.0 ** -24))
+    assert np.float32(p[2]) == p[2]
+    r = _one_pair(p)
+    assert r == {"pairs": 1, "bin_differs_from_fp64": 1, "within_one_ulp_of_edge": 1,
+                 "ball_membership_flips": 0}
+    # the same pair 1/4 of a bin further out: no edge nearby, same bin either way
+    r = _one_pair((1.0 + 2.0 ** -5, 0.0, 0.0))
+    assert r == {"pairs": 1, "bin_differs_from_fp64": 0, "within_one_ulp_of_edge": 0,
+                 "ball_membership_flips": 0}
+
+
+def test_binedge_ball_flip_beyond_R():
+    # d = (2 - 2^-23, 2^-11, 2^-11): the float sum is 4 - 2^-21 (the square's
+    # 2^-46 rounded off), + 2^-22, + 2^-22 = 4.0 = fl(R^2), so the FP32 test
+    # keeps the pair, while the exact d^2 = 4 + 2^-46 > R^2 puts it outside
+    p = (2.0 - 2.0 ** -23, 2.0 ** -11, 2.0 ** -11)
+    r = _one_pair(p)
+    assert r["ball_membership_flips"] == 1
+    assert r["pairs"] == 1                      # examined: inside by FP32
+    assert r["bin_differs_from_fp64"] == 0      # d == Rmax and d > Rmax are both highest
+    assert r["within_one_ulp_of_edge"] == 1     # at the last edge, Rmax itself
+    # (2, 2^-11, 2^-11) flips too: 4 + 2^-22 is a tie that rounds to even, 4.0
+    assert _one_pair((2.0, 2.0 ** -11, 2.0 ** -11))["ball_membership_flips"] == 1
+    # one float ulp beyond Rmax on an axis, d^2 = 4 + 2^-20 is exact: outside either way
+    r = _one_pair((2.0 + 2.0 ** -22, 0.0, 0.0))
+    assert r == {"pairs": 0, "bin_differs_from_fp64": 0, "within_one_ulp_of_edge": 0,
+                 "ball_membership_flips": 0}
+
+
+def test_binedge_edges_follow_tmp_not_width():
+    # 2.0 / 0.1 -> 19 bins of 2/19: the ideal edges are multiples of 2/19
+    r = _one_pair((np.float32(10.0 / 19.0), 0.0, 0.0), 0.0, 0.1, 2.0)
+    assert r["within_one_ulp_of_edge"] == 1
+    r = _one_pair((0.5, 0.0, 0.0), 0.0, 0.1, 2.0)   # 0.026 from the nearest edge 10/19
+    assert r["within_one_ulp_of_edge"] == 0
+
+
+def test_binedge_fcc_shells_on_edges():
+    # fcc lattice, a = 0.4 nm, one center at the origin: the shells within
+    # R = 1 are d^2 = 0.08 m for m = 1..12 with 12, 6, 24, 12, 24, 8, 48, 6,
+    # 36, 24, 24, 24 sites (248 in all).  With bins of 0.01 only the shells
+    # m = 2 (d = 0.4, the six (+-0.4, 0, 0)) and m = 8 (d = 0.8, the six
+    # (+-0.8, 0, 0)) sit on ideal edges, and their float coordinates put
+    # them within one ulp of the edge: 12 near-edge pairs.  Their FP32 bins
+    # equal the exact ones (sqrtf(fl(x^2)) = x), so nothing differs.
+    g = np.arange(-6, 7)
+    ijk = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    ijk = ijk[(ijk.sum(axis=1) % 2) == 0]
+    xyz = (ijk * 0.2).astype(np.float32)
+    order = np.argsort((ijk ** 2).sum(axis=1), kind="stable")
+    xyz = xyz[order]                              # the origin first
+    ty = np.ones(xyz.shape[0], np.uint8)
+    s2 = np.zeros(xyz.shape[0], np.float32)
+    s2[0] = 1e6
+    r = F.binedge_report(xyz, ty, s2, (1,), (1,), 0.0, 0.01, 1.0)
+    assert r == {"pairs": 248, "bin_differs_from_fp64": 0, "within_one_ulp_of_edge": 12,
+                 "ball_membership_flips": 0}
+
+
+def test_binedge_pairs_equal_oracle_inball_count():
+    # examined pairs = in-ball env-type pairs of the considered centers, which
+    # is the RDF total of the same one-task job; center_stride samples centers
+    xyz, ty = _tip(4000, 21)
+    rng = np.random.default_rng(22)
+    s2 = rng.uniform(0, 3, size=xyz.shape[0]).astype(np.float32)
+    h, _ = F.spatstat(xyz, ty, s2, [((1, 2), (2, 3))], 0.0, 0.01, 1.0, F.FAM_RDF)
+    r = F.binedge_report(xyz, ty, s2, (1, 2), (2, 3), 0.0, 0.01, 1.0)
+    assert r["pairs"] == h["rdf"][0].sum()
+    keep = np.zeros(xyz.shape[0], bool)
+    keep[::7] = True
+    h7, _ = F.spatstat(xyz, ty, np.where(keep, s2, 0).astype(np.float32), [((1, 2), (2, 3))],
+                       0.0, 0.01, 1.0, F.FAM_RDF)
+    r7 = F.binedge_report(xyz, ty, s2, (1, 2), (2, 3), 0.0, 0.01, 1.0, center_stride=7)
+    assert r7["pairs"] == h7["rdf"][0].sum()
+
+
+def test_mknn_partial_selection_equals_full_sort():
+    # the oracle settles the kmax smallest by partition when they are a small
+    # part of the neighbour list, and sorts the whole list otherwise; the k-th
+    # values must not depend on which (duplicates included)
+    xyz, ty = _tip(3000, 31, L=5.0)
+    xyz[::9] = xyz[1::9]                     # coincident pairs: ties at d = 0 and beyond
+    s2 = np.full(xyz.shape[0], 1e6, np.float32)
+    tasks = [((1,), (1,)), ((2, 3), (1, 2))]
+    a, _ = F.spatstat(xyz, ty, s2, tasks, 0.0, 0.01, 1.5, F.FAM_MKNN, kvalues=[1, 3, 10])
+    b, _ = F.spatstat(xyz, ty, s2, tasks, 0.0, 0.01, 1.5, F.FAM_MKNN,
+                      kvalues=[1, 3, 10, 400])
+    # descending k: a holds k = 10, 3, 1; b holds k = 400, 10, 3, 1
+    np.testing.assert_array_equal(a["mknn"], b["mknn"][:, 1:])
+    assert b["mknn"][:, 0, -1].sum() > 0     # k = 400 exceeds some lists: the sentinel
