@@ -1,10 +1,13 @@
 """Benchmark of the gfx950 RDF / kNN spatial-statistics path.
 
 One step = one pass of the hot path over the resident 20M-ion clustered
-frustum tip: cell-list build, center/tile preparation, the query kernel for
-all 9 ordered type pairs (RDF + MKNN k=1..100, Rmax 5 nm, bin 0.01 nm), and
-the int64 histograms back on the host.  Inputs are in HBM before the timed
-region starts.
+frustum tip: center/tile preparation, the query kernel for all 9 ordered type
+pairs (RDF + MKNN k=1..100, Rmax 5 nm, bin 0.01 nm), and the int64 histograms
+back on the host.  Inputs are in HBM before the timed region starts, and so
+is the cloud's cell list: the first (warmup) job builds it and every step
+reuses it, as the reference's KD-trees are built once by
+init_spatialindexing before compute_generic_spatstat3.  The cell-list build
+alone is reported as ms_index_build (a fresh copy of the cloud, timed).
 
 Prints ONE JSON line.  `value` is reference-equivalent pair-distance
 evaluations per second: the sum over considered centers of the neighbours
@@ -27,8 +30,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "pair-distance evals/s at 20M ions; served jobs/s + p99 latency vs plain loop"
-VALU_PEAK_OPS = 78.6e12     # MI355X FP32 VALU element-ops/s (157.3 TFLOP/s counting FMA as 2)
-OPS_PER_EVAL = 9            # 3 sub + 3 mul + 2 add + 1 compare, no FMA
+FP32_PEAK_FLOPS = 157.3e12  # MI355X published FP32 vector peak (an FMA counts as 2 FLOP)
+VALU_PEAK_OPS = 78.6e12     # the same peak as FP32 element-ops (one FMA = one op)
+PLAIN_PEAK_OPS = 39.3e12    # without packed FP32: one wave-64 VALU instruction per CU per clock
+FLOP_PER_EVAL = 8           # 3 sub + 3 mul + 2 add (no FMA: fp-contract off)
+OPS_PER_EVAL = 9            # the same plus the in-ball compare
 # HBM bytes of the query kernel per step of the 20M-ion headline job, from two
 # rocprofv3 --pmc runs of this script (FETCH_SIZE x 2 and WRITE_SIZE x 1, the
 # factors a 1 GiB device copy reads): 141.0 GB read + 15.7 GB written
@@ -60,6 +66,86 @@ def headline_job():
     return Job(ALL_PAIRS, 0.0, 0.01, 5.0, FAM_RDF | FAM_MKNN, kvalues=list(range(1, 101)))
 
 
+def roofline(cand, kernel_ms, kernel, traffic=None, traffic_source=None):
+    """The query kernel against MI355X's published FP32 peak: 8 FLOP per
+    candidate pair evaluation over the kernel's own time (HIP events around
+    its launches)."""
+    s = kernel_ms / 1e3
+    achieved = cand * FLOP_PER_EVAL / s / 1e12
+    ops = cand * OPS_PER_EVAL / s / 1e12
+    return {
+        "bound": "valu",
+        "achieved": achieved,
+        "peak": FP32_PEAK_FLOPS / 1e12,
+        "unit": "TFLOP/s (FP32 VALU, 8 FLOP per candidate pair evaluation)",
+        "frac": achieved / (FP32_PEAK_FLOPS / 1e12),
+        "elementops_achieved": ops,
+        "frac_elementops": ops / (VALU_PEAK_OPS / 1e12),
+        "frac_practical": ops / (PLAIN_PEAK_OPS / 1e12),
+        "practical_note": "9 element-ops per evaluation (the 8 FLOP and the in-ball compare) "
+                          "against 78.6 T element-ops/s; packed FP32 (v_pk_*_f32) is disabled "
+                          "in this build, so plain FP32 issues one wave-64 instruction per CU "
+                          "per clock: 39.3 T element-ops/s is the practical ceiling",
+        "traffic": traffic,
+        "traffic_unit": "bytes per step (HBM, all query launches)",
+        "traffic_source": traffic_source,
+        "kernel": kernel,
+        "kernel_ms": kernel_ms,
+        "candidate_evals": cand,
+    }
+
+
+def index_build_ms(ctx, tip, job, reps=3):
+    """The cell-list build on its own: the first job on a freshly uploaded
+    copy of the cloud builds it (later jobs reuse it); median of reps."""
+    out = []
+    for _ in range(reps):
+        c = ctx.upload(tip.xyz, tip.types, tip.ion2surf_sqr)
+        try:
+            out.append(ctx.run(c, job).ms_index)
+        finally:
+            c.free()
+    return float(np.median(out))
+
+
+def binedge(tip, job, stride, cores):
+    """The FP64 bin-edge report (oracle) on every stride-th ion as a center
+    (the ions are in random order), all of the job's pairs at once: how many
+    in-ball pairs lie within one float ulp of an ideal bin edge, how many
+    FP32-pipeline bins differ from the exact distance's, how many pairs the
+    FP32 and exact in-ball tests disagree on."""
+    from oracle import oracle as O
+
+    trg = sorted({t for ts, _ in job.tasks for t in ts})
+    env = sorted({e for _, es in job.tasks for e in es})
+    t0 = time.perf_counter()
+    r = O.binedge_report(tip.xyz, tip.types, tip.ion2surf_sqr, trg, env, job.rmin, job.rincr,
+                         job.rmax, center_stride=stride, nthreads=cores)
+    r["sample"] = (f"every {stride}th ion as a center ({100.0 / stride:g}% of the centers), "
+                   f"all {len(job.tasks)} type pairs; {time.perf_counter() - t0:.1f} s on "
+                   f"{cores} host cores")
+    print(f"# binedge: {r['pairs']} pairs, {r['within_one_ulp_of_edge']} within one ulp of an "
+          f"edge, {r['bin_differs_from_fp64']} bins differ from FP64, "
+          f"{r['ball_membership_flips']} ball flips", file=sys.stderr, flush=True)
+    return r
+
+
+def slab(tip, zwin, R):
+    """The ions of z in [zwin - R, zwin + R) with the surface distance kept
+    only for z in zwin (the others cannot be centers): every neighbour of a
+    window center is in the slab, so its histograms equal the whole tip's
+    for those centers."""
+    from paraprobe_amd import synth
+
+    z = tip.xyz[:, 2]
+    keep = (z >= zwin[0] - R - 0.01) & (z < zwin[1] + R + 0.01)
+    s2 = tip.ion2surf_sqr[keep].copy()
+    zk = z[keep]
+    s2[(zk < zwin[0]) | (zk >= zwin[1])] = 0.0
+    return synth.Tip(np.ascontiguousarray(tip.xyz[keep]), np.ascontiguousarray(tip.types[keep]),
+                     s2, f"slab z in [{zwin[0]}, {zwin[1]}) +- {R} nm")
+
+
 def cpu_baseline(tip, job, zwin, cores):
     """The reference's algorithm on the host cores, on the centers of a z
     window of the same tip (the other centers excluded through ion2surf)."""
@@ -77,6 +163,93 @@ def cpu_baseline(tip, job, zwin, cores):
 
 def _same(a, b) -> bool:
     return all(np.array_equal(v, getattr(b, k)) for k, v in a.arrays().items())
+
+
+class Phase:
+    """A named ROCTx range around one phase of the benchmark, so that
+    `rocprofv3 --kernel-trace --marker-trace` traces show where each phase
+    starts and ends (`bench.py --busy DIR` turns them into GPU-busy
+    fractions).  A no-op when the ROCTx library is absent."""
+
+    _lib = None
+    _tried = False
+
+    def __init__(self, name: str):
+        self.name = name.encode()
+
+    @classmethod
+    def lib(cls):
+        if not cls._tried:
+            cls._tried = True
+            try:
+                cls._lib = ctypes.CDLL("librocprofiler-sdk-roctx.so.1")
+                cls._lib.roctxRangePushA.argtypes = [ctypes.c_char_p]
+            except OSError:
+                cls._lib = None
+        return cls._lib
+
+    def __enter__(self):
+        if self.lib() is not None:
+            self._lib.roctxRangePushA(self.name)
+        return self
+
+    def __exit__(self, *exc):
+        if self._lib is not None:
+            self._lib.roctxRangePop()
+        return False
+
+
+def busy_report(trace_dir: str) -> str:
+    """GPU-busy fraction of every named phase of a rocprofv3 trace
+    (`--kernel-trace --marker-trace --output-format csv`): the union of the
+    kernel intervals inside the phase over its wall time, with the kernels
+    that take the most time.  Returns a markdown table."""
+    import csv
+    import glob
+
+    def rows(pattern):
+        out = []
+        for f in sorted(glob.glob(os.path.join(trace_dir, "**", pattern), recursive=True)):
+            with open(f, newline="") as fh:
+                out.extend(csv.DictReader(fh))
+        return out
+
+    kern = [(int(r["Start_Timestamp"]), int(r["End_Timestamp"]), r["Kernel_Name"])
+            for r in rows("*kernel_trace.csv")]
+    kern.sort()
+    phases = []
+    for r in rows("*marker_api_trace.csv"):
+        name = r.get("Function") or r.get("Message") or ""
+        s, e = int(r["Start_Timestamp"]), int(r["End_Timestamp"])
+        if e > s:
+            phases.append((s, e, name))
+    phases.sort()
+    lines = ["| phase | wall ms | kernel-busy ms | busy fraction | kernels | top kernels (ms) |",
+             "|---|---|---|---|---|---|"]
+    for s, e, name in phases:
+        busy, cur_s, cur_e, n = 0, None, None, 0
+        per = {}
+        for ks, ke, kn in kern:
+            if ke <= s or ks >= e:
+                continue
+            ks, ke = max(ks, s), min(ke, e)
+            n += 1
+            short = kn.split("(")[0].replace("void ", "")
+            per[short] = per.get(short, 0) + (ke - ks)
+            if cur_e is None or ks > cur_e:
+                if cur_e is not None:
+                    busy += cur_e - cur_s
+                cur_s, cur_e = ks, ke
+            else:
+                cur_e = max(cur_e, ke)
+        if cur_e is not None:
+            busy += cur_e - cur_s
+        top = sorted(per.items(), key=lambda kv: -kv[1])[:4]
+        tops = ", ".join(f"{k} {v / 1e6:.1f}" for k, v in top)
+        wall = (e - s) / 1e6
+        lines.append(f"| {name} | {wall:.1f} | {busy / 1e6:.1f} | "
+                     f"{busy / 1e6 / wall if wall > 0 else 0:.3f} | {n} | {tops} |")
+    return "\n".join(lines)
 
 
 def batch_leg(ctx, args):
@@ -99,12 +272,14 @@ def batch_leg(ctx, args):
     warm_s = time.perf_counter() - t
     print(f"# batch: {len(jobs)} clouds, {nions} ions, generated in {gen_s:.1f} s, "
           f"warmup {warm_s:.1f} s", file=sys.stderr, flush=True)
-    t = time.perf_counter()
-    plain = [ctx.run(c, j) for c, j in zip(clouds, jobs)]
-    plain_s = time.perf_counter() - t
-    t = time.perf_counter()
-    batched = ctx.run_batch(clouds, jobs)
-    batch_s = time.perf_counter() - t
+    with Phase("batch_plain_loop"):
+        t = time.perf_counter()
+        plain = [ctx.run(c, j) for c, j in zip(clouds, jobs)]
+        plain_s = time.perf_counter() - t
+    with Phase("batch_batched"):
+        t = time.perf_counter()
+        batched = ctx.run_batch(clouds, jobs)
+        batch_s = time.perf_counter() - t
     parity = all(_same(a, b) for a, b in zip(plain, batched))
     ctx.set_tuning(0, 0)
     for c in clouds:
@@ -125,8 +300,16 @@ def batch_leg(ctx, args):
 
 
 def serve_leg(ctx, args):
-    """8 resident tips and a Poisson request stream: the continuous-batching
-    server against the plain per-request loop."""
+    """8 resident tips and a Poisson stream of 10,000 requests: the
+    continuous-batching server against the plain per-request loop.
+
+    The plain loop runs back to back over a sample (the first
+    `--serve-plain-sample` requests): its service times give the plain
+    capacity and, through a first-come first-served replay of the same
+    arrivals, its p50/p99; the server runs the same sample submitted at once
+    (capacity), then the whole stream as an open loop of Poisson arrivals at
+    0.7 x the plain capacity (p50/p99 over all requests, and over the sample
+    for a like-for-like comparison)."""
     import threading
 
     from paraprobe_amd import workloads
@@ -143,9 +326,10 @@ def serve_leg(ctx, args):
     nions = sum(tp.n for tp in tips)
     del tips
     M = args.serve_requests
+    S = min(args.serve_plain_sample, M)
     stream = workloads.serve_stream(10_000)[:M]
     print(f"# serve: 8 resident tips ({nions} ions) ready in {time.perf_counter() - t:.1f} s; "
-          f"{M} requests of the 10,000-request stream", file=sys.stderr, flush=True)
+          f"{M} requests, plain loop on the first {S}", file=sys.stderr, flush=True)
     # build every (cloud, cell width, neighbour type) candidate list once, with
     # jobs whose central type holds no ion (so no query work is done)
     from paraprobe_amd import FAM_RDF, Job
@@ -156,59 +340,79 @@ def serve_leg(ctx, args):
                 ctx.run(c, Job([((7,), (e,))], 0.0, 0.01, R, FAM_RDF))
     # plain loop, back to back: per-request service times
     plain, svc = [], []
-    t0 = time.perf_counter()
-    for r in stream:
-        a = time.perf_counter()
-        plain.append(ctx.run(clouds[r.cloud], r.job))
-        svc.append(time.perf_counter() - a)
-    plain_s = time.perf_counter() - t0
-    plain_rate = M / plain_s
+    with Phase("serve_plain_loop"):
+        t0 = time.perf_counter()
+        for r in stream[:S]:
+            a = time.perf_counter()
+            plain.append(ctx.run(clouds[r.cloud], r.job))
+            svc.append(time.perf_counter() - a)
+        plain_s = time.perf_counter() - t0
+    plain_rate = S / plain_s
+    print(f"# serve: plain loop {plain_rate:.1f} jobs/s on {S} requests", file=sys.stderr,
+          flush=True)
 
-    def served(rate):
-        """Runs the stream through the server; rate None submits everything at
-        once (capacity), otherwise Poisson arrivals at `rate` per second."""
-        gaps = np.random.default_rng(99).exponential(1.0 / rate, size=M) if rate else np.zeros(M)
+    def served(reqs, rate, keep):
+        """Runs reqs through the server; rate None submits everything at once
+        (capacity), otherwise Poisson arrivals at `rate` per second.  Returns
+        (completions per second, latencies in ms, results of the first `keep`
+        requests, statuses)."""
+        m = len(reqs)
+        gaps = np.random.default_rng(99).exponential(1.0 / rate, size=m) if rate else np.zeros(m)
         arrive = np.cumsum(gaps)
-        done, lat, results = {}, {}, {}
+        done, results, status = {}, {}, {}
         with Server(ctx, slots=args.serve_slots, max_batch_jobs=args.serve_batch) as srv:
             start = time.perf_counter()
             tick = {}
             stop = threading.Event()
 
             def poller():
-                while not stop.is_set() or len(done) < M:
+                while not stop.is_set() or len(done) < m:
                     for tk, st, res in srv.poll():
                         done[tk] = time.perf_counter()
-                        results[tk] = (st, res)
-                    if len(done) >= M:
+                        status[tk] = st
+                        if tick[tk][0] < keep:
+                            results[tk] = res
+                    if len(done) >= m:
                         break
                     time.sleep(0.0002)
 
             th = threading.Thread(target=poller)
             th.start()
-            for i, r in enumerate(stream):
+            for i, r in enumerate(reqs):
                 now = time.perf_counter() - start
                 if arrive[i] > now:
                     time.sleep(arrive[i] - now)
                 tk = srv.submit(clouds[r.cloud], r.job)
                 tick[tk] = (i, start + arrive[i])
+                if rate and (i + 1) % 1000 == 0:
+                    print(f"# serve: {i + 1} of {m} requests submitted, {len(done)} done",
+                          file=sys.stderr, flush=True)
             stop.set()
             th.join()
             end = max(done.values())
-        out = [None] * M
+        lat = np.zeros(m)
+        out = [None] * keep
+        ok = True
         for tk, (i, a) in tick.items():
             lat[i] = done[tk] - a
-            st, res = results[tk]
-            out[i] = res if st == 0 else None
-        return M / (end - start), np.array([lat[i] for i in range(M)]) * 1e3, out
+            ok &= status[tk] == 0
+            if i < keep:
+                out[i] = results[tk] if status[tk] == 0 else None
+        return m / (end - start), lat * 1e3, out, ok
 
-    cap_rate, _, res_cap = served(None)
-    lam = 0.7 * cap_rate
-    rate, lat_ms, res_open = served(lam)
-    parity = all(o is not None and _same(p, o) for p, o in zip(plain, res_cap)) and \
+    with Phase("serve_capacity"):
+        cap_rate, _, res_cap, ok_cap = served(stream[:S], None, S)
+    lam = 0.7 * plain_rate
+    print(f"# serve: capacity {cap_rate:.1f} jobs/s; open loop of {M} requests at "
+          f"{lam:.1f}/s", file=sys.stderr, flush=True)
+    with Phase("serve_open_loop"):
+        rate, lat_ms, res_open, ok_open = served(stream, lam, S)
+    parity = ok_cap and ok_open and \
+        all(o is not None and _same(p, o) for p, o in zip(plain, res_cap)) and \
         all(o is not None and _same(p, o) for p, o in zip(plain, res_open))
-    # the plain loop under the same arrivals: FIFO, one request at a time
-    arrive = np.cumsum(np.random.default_rng(99).exponential(1.0 / lam, size=M))
+    # the plain loop under the same arrivals (the first S of them):
+    # first come first served, one request at a time
+    arrive = np.cumsum(np.random.default_rng(99).exponential(1.0 / lam, size=M))[:S]
     fin, plat = 0.0, []
     for a, s_ in zip(arrive, svc):
         fin = max(fin, a) + s_
@@ -216,26 +420,109 @@ def serve_leg(ctx, args):
     plat = np.array(plat) * 1e3
     for c in clouds:
         c.free()
+    p50, p99 = float(np.percentile(lat_ms, 50)), float(np.percentile(lat_ms, 99))
+    s50, s99 = float(np.percentile(lat_ms[:S], 50)), float(np.percentile(lat_ms[:S], 99))
     print(f"# serve: plain {plain_rate:.1f} jobs/s, served capacity {cap_rate:.1f} jobs/s, "
-          f"open loop at {lam:.1f}/s: p50 {np.percentile(lat_ms, 50):.1f} ms "
-          f"p99 {np.percentile(lat_ms, 99):.1f} ms, parity {parity}", file=sys.stderr, flush=True)
+          f"open loop at {lam:.1f}/s: p50 {p50:.1f} ms p99 {p99:.1f} ms (first {S}: "
+          f"{s50:.1f} / {s99:.1f}; plain {np.percentile(plat, 50):.1f} / "
+          f"{np.percentile(plat, 99):.1f}), parity {parity}", file=sys.stderr, flush=True)
     return {
-        "workload": f"8 resident clustered frustum tips (1M-50M ions, {nions} total), first {M} "
-                    "requests of a 10,000-request stream: one type pair, Rmax U(1,5) nm, RDF or "
+        "workload": f"8 resident clustered frustum tips (1M-50M ions, {nions} total), a "
+                    f"{M}-request Poisson stream: one type pair, Rmax U(1,5) nm, RDF or "
                     "MKNN k=1..kmax (kmax U(1,100)), bin 0.01 nm",
+        "requests": M,
+        "plain_sample": S,
         "slots": args.serve_slots,
         "max_batch_jobs": args.serve_batch,
-        "served_capacity_jobs_per_s": cap_rate,
         "plain_jobs_per_s": plain_rate,
+        "served_capacity_jobs_per_s": cap_rate,
         "speedup": cap_rate / plain_rate,
         "poisson_rate_per_s": lam,
         "served_jobs_per_s": rate,
-        "served_p50_ms": float(np.percentile(lat_ms, 50)),
-        "served_p99_ms": float(np.percentile(lat_ms, 99)),
-        "plain_p50_ms_same_arrivals": float(np.percentile(plat, 50)),
-        "plain_p99_ms_same_arrivals": float(np.percentile(plat, 99)),
+        "served_p50_ms": p50,
+        "served_p99_ms": p99,
+        "served_p50_ms_sample": s50,
+        "served_p99_ms_sample": s99,
+        "plain_p50_ms_same_arrivals_sample": float(np.percentile(plat, 50)),
+        "plain_p99_ms_same_arrivals_sample": float(np.percentile(plat, 99)),
         "parity_served_vs_plain": parity,
+        "parity_scope": f"the first {S} requests, capacity and open-loop runs; "
+                        f"all {M} completed with PP_OK" if parity else "failed",
     }
+
+
+def large_leg(ctx, args, sync):
+    """The large configuration: 100M ions in a frustum r 60 -> 30 nm (22
+    ions/nm^3, 300,000 clusters), all 9 pair RDFs at Rmax 3 nm, bin 0.005 nm
+    (600 bins) plus MKNN k = 1..20, timed like the headline; the reference's
+    algorithm on the host cores for the centers of a 1 nm z-window (on the
+    slab that holds their neighbours, the GPU compared on the same slab);
+    the FP64 bin-edge report on 1% of the centers."""
+    from paraprobe_amd import FAM_MKNN, FAM_RDF, Job, synth
+
+    t = time.perf_counter()
+    tip = synth.frustum_100m(n=args.large_n)
+    print(f"# large: generated {tip.n} ions in {time.perf_counter() - t:.1f} s",
+          file=sys.stderr, flush=True)
+    job = Job(synth.ALL_PAIRS, 0.0, 0.005, 3.0, FAM_RDF | FAM_MKNN, kvalues=list(range(1, 21)))
+    cloud = ctx.upload(tip.xyz, tip.types, tip.ion2surf_sqr)
+    try:
+        first = ctx.run(cloud, job)
+        sync()
+        with Phase("large_steps"):
+            t0 = time.perf_counter()
+            res = [ctx.run(cloud, job) for _ in range(args.large_steps)]
+            sync()
+            dt = time.perf_counter() - t0
+    finally:
+        cloud.free()
+    for r in res[1:]:
+        if not _same(r, res[0]):
+            raise SystemExit("bench: large results differ between steps")
+    last = res[-1]
+    ms_step = dt * 1e3 / args.large_steps
+    kms = float(np.mean([r.ms_kernel for r in res]))
+    print(f"# large: {ms_step:.1f} ms per job (query kernel {kms:.1f} ms), "
+          f"{last.pairs_inball} in-ball pairs", file=sys.stderr, flush=True)
+    out = {
+        "workload": "frustum-100M-clustered: 100,000,000 ions, r 60->30 nm, h 689 nm "
+                    "(22 ions/nm^3), types A/B/C 0.95/0.04/0.01, 300,000 clusters r~U(1,2) nm "
+                    "B=40%; 9 type pairs RDF + MKNN k=1..20, Rmax 3 nm, bin 0.005 nm, int64",
+        "n_ions": tip.n,
+        "steps": args.large_steps,
+        "ms_per_step": ms_step,
+        "value": last.pairs_inball / (ms_step / 1e3),
+        "unit": "pair-distance evals/s",
+        "centers_considered": last.centers_considered,
+        "pairs_inball": last.pairs_inball,
+        "ms_index_build": first.ms_index,
+        "ms_query": float(np.mean([r.ms_query for r in res])),
+        "roofline": roofline(last.pairs_candidate, kms, last.kernel),
+    }
+    mid = float(np.median(tip.xyz[:, 2]))
+    zwin = (round(mid, 1) - 0.5, round(mid, 1) + 0.5)
+    sl = slab(tip, zwin, 3.0)
+    cw = ctx.upload(sl.xyz, sl.types, sl.ion2surf_sqr)
+    try:
+        gwin = ctx.run(cw, job)
+    finally:
+        cw.free()
+    model, cores = host_cpu()
+    if not args.no_cpu:
+        hist, st, _, _ = cpu_baseline(sl, job, zwin, cores)
+        out["cpu_baseline"] = {
+            "value": st["inball"] / st["compute_s"],
+            "unit": "pair-distance evals/s",
+            "cores": cores,
+            "kind": "port",
+            "sample": f"centers with z in [{zwin[0]:.1f}, {zwin[1]:.1f}) nm ({st['considered']} "
+                      f"centers, {st['inball']} in-ball pairs) on the slab of their neighbours "
+                      f"({sl.n} ions); compute {st['compute_s']:.1f} s timed, z-slab + KD-tree "
+                      f"build {st['index_s']:.1f} s not; {model}",
+            "parity_with_gpu": all(np.array_equal(getattr(gwin, k), v) for k, v in hist.items()),
+        }
+        out["binedge"] = binedge(tip, job, 100, cores)
+    return out
 
 
 def surf_leg(ctx, tip, hip, args):
@@ -434,18 +721,29 @@ def main():
     ap.add_argument("--calib", action="store_true",
                     help="also run a 1 GiB device copy (HBM counter calibration)")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-large", action="store_true")
+    ap.add_argument("--large-n", type=int, default=100_000_000)
+    ap.add_argument("--large-steps", type=int, default=3)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-surf", action="store_true")
     ap.add_argument("--no-pos", action="store_true")
     ap.add_argument("--no-npcorr", action="store_true")
     ap.add_argument("--batch-jobs", type=int, default=256)
     ap.add_argument("--autotune", type=int, default=1)
-    ap.add_argument("--serve-requests", type=int, default=400)
-    ap.add_argument("--serve-slots", type=int, default=1)
+    ap.add_argument("--serve-requests", type=int, default=10_000)
+    ap.add_argument("--serve-plain-sample", type=int, default=1000,
+                    help="requests of the stream the plain loop is timed on")
+    ap.add_argument("--serve-slots", type=int, default=2)
     ap.add_argument("--serve-batch", type=int, default=32)
     ap.add_argument("--graph", type=int, default=1,
                     help="1: batch launch groups captured into HIP graphs; 0: eager launches")
+    ap.add_argument("--busy", metavar="TRACE_DIR", default=None,
+                    help="print the GPU-busy fraction of each phase of a rocprofv3 "
+                         "--kernel-trace --marker-trace csv trace and exit (no GPU)")
     args = ap.parse_args()
+    if args.busy:
+        print(busy_report(args.busy))
+        return
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (every box has one MI355X); "
               "use --gpus 1", file=sys.stderr)
@@ -476,19 +774,22 @@ def main():
             print(f"# warmup {i}: {r.ms_total:.1f} ms (query {r.ms_query:.1f} ms)",
                   file=sys.stderr, flush=True)
         sync()
-        t0 = time.perf_counter()
-        res = []
-        for i in range(args.steps):
-            res.append(ctx.run(cloud, job))
-            print(f"# step {i}: {res[-1].ms_total:.1f} ms (query {res[-1].ms_query:.1f} ms)",
-                  file=sys.stderr, flush=True)
-        sync()
-        dt = time.perf_counter() - t0
+        with Phase("headline_steps"):
+            t0 = time.perf_counter()
+            res = []
+            for i in range(args.steps):
+                res.append(ctx.run(cloud, job))
+                print(f"# step {i}: {res[-1].ms_total:.1f} ms (query {res[-1].ms_query:.1f} ms)",
+                      file=sys.stderr, flush=True)
+            sync()
+            dt = time.perf_counter() - t0
         last = res[-1]
         for r in res[1:]:
             for k, v in r.arrays().items():
                 if not np.array_equal(v, getattr(res[0], k)):
                     raise SystemExit("bench: results differ between steps")
+        ms_index_build = index_build_ms(ctx, tip, job)
+        print(f"# cell-list build {ms_index_build:.2f} ms", file=sys.stderr, flush=True)
         # GPU result of the CPU sample window, for the parity check
         zwin = (125.0 - args.cpu_window / 2, 125.0 + args.cpu_window / 2)
         s2w = tip.ion2surf_sqr.copy()
@@ -499,6 +800,8 @@ def main():
         cw.free()
         cloud.free()
         extra = {}
+        if not args.no_large:
+            extra["large"] = large_leg(ctx, args, sync)
         if not args.no_surf:
             extra["ion2surf"] = surf_leg(ctx, tip, hip, args)
         if not args.no_pos:
@@ -526,8 +829,8 @@ def main():
     inball = last.pairs_inball
     value = inball / (ms_step / 1e3)
     q_ms = float(np.mean([r.ms_query for r in res]))
+    k_ms = float(np.mean([r.ms_kernel for r in res]))
     cand = last.pairs_candidate
-    achieved = cand * OPS_PER_EVAL / (q_ms / 1e3) / 1e12
     line = {
         "metric": METRIC,
         "value": value,
@@ -551,25 +854,19 @@ def main():
             "bin_nm": 0.01,
             "kmax": 100,
         },
-        "roofline": {
-            "bound": "valu",
-            "achieved": achieved,
-            "peak": VALU_PEAK_OPS / 1e12,
-            "unit": "Tops/s (FP32 VALU element-ops, 9 per candidate pair)",
-            "frac": achieved / (VALU_PEAK_OPS / 1e12),
-            "traffic": TRAFFIC_HEADLINE if tip.n == 20_000_000 else None,
-            "traffic_unit": "bytes per step (HBM, all query launches)",
-            "traffic_source": TRAFFIC_SOURCE if tip.n == 20_000_000 else None,
-            "kernel": last.kernel,
-            "kernel_ms": q_ms,
-            "candidate_evals": cand,
-        },
+        "roofline": roofline(cand, k_ms, last.kernel,
+                             TRAFFIC_HEADLINE if tip.n == 20_000_000 else None,
+                             TRAFFIC_SOURCE if tip.n == 20_000_000 else None),
         "centers_considered": last.centers_considered,
         "pairs_inball": inball,
-        "candidate_evals_per_s": cand / (q_ms / 1e3),
+        "candidate_evals_per_s": cand / (k_ms / 1e3),
         "ms_index": float(np.mean([r.ms_index for r in res])),
+        "ms_index_note": "each step reuses the cloud's cell list (built by the first job)",
+        "ms_index_build": ms_index_build,
         "ms_prep": float(np.mean([r.ms_prep for r in res])),
         "ms_query": q_ms,
+        "ms_query_note": "query kernel launches, their k-th neighbour reductions and the "
+                         "full-range rerun; roofline.kernel_ms is the query kernel alone",
     }
     if not args.no_cpu:
         model, cores = host_cpu()
@@ -586,6 +883,7 @@ def main():
                       f"span), z-slab + KD-tree build {st['index_s']:.1f} s not timed; {model}",
             "parity_with_gpu": parity,
         }
+        line["binedge"] = binedge(tip, job, 100, cores)
     line.update(extra)
     print(json.dumps(line), flush=True)
 

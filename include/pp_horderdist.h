@@ -115,6 +115,9 @@ public:
     /* NPCORR counts [task][edge^3] (x fastest) of the last run, edge = npcorr_edge */
     std::vector<uint32_t> npcorr;
     int64_t npcorr_edge = 0;
+    /* when set (a results file made by create_results_h5), each run also
+     * writes its NPCORR cube there, as report_npc3d_histvalues_hdf5 does */
+    std::string results_h5;
 
 private:
     pp_ctx *ctx_;
@@ -122,6 +125,13 @@ private:
     const PeriodicTable &pse_;
     std::vector<TypeSpecDescrStat> spatstat_tasks;
 };
+
+/* HDF5 output through libpp_h5.so (include/pp_h5.h), found next to this
+ * library at run time: whether it loads, and creating the results file with
+ * the /DescrSpatStats groups (h5Hdl::create_paraprobe_results_file,
+ * src/PARAPROBE_HDF5.cpp:369-408). */
+bool hdf5_output_available();
+bool create_results_h5(const std::string &fn, bool npcorr);
 
 /* rndlabeler (S.cpp:2809-3000): label randomisation of the whole tip.  The
  * generator is a std::mt19937 seeded with Settings::RndDescrStatsPRNGSeed

@@ -98,6 +98,8 @@ typedef struct pp_hist_out {
                                  *   only (with an exact full-range rerun of the centers
                                  *   whose k-th neighbour lies beyond it) */
     int64_t centers_rerun;      /* centers of that full-range rerun */
+    double ms_kernel;           /* the query kernel launches alone (HIP events around each;
+                                 * 0 when they ran inside a HIP graph) */
 } pp_hist_out;
 
 /* Context bound to one device; one per host thread (calls on a context are
@@ -109,6 +111,12 @@ const char *pp_last_error_string(const pp_ctx *ctx);
 /* Histogram geometry of histogram(rmin, rincr, rmax)
  * (src/PARAPROBE_Histogram.h:54-89), for sizing outputs. */
 pp_status pp_hist_geometry(float rmin, float rincr, float rmax, int64_t *nbins);
+
+/* Whether the GPU path bins histogram(rmin, rincr, rmax) exactly (PP_OK), or
+ * refuses it (PP_ERR_UNSUPPORTED: bins finer than the float index coordinate
+ * resolves; PP_ERR_INVALID_ARG: settings Settings::checkUserInput rejects).
+ * Host only; a job with such settings fails the same way. */
+pp_status pp_hist_supported(float rmin, float rincr, float rmax);
 
 /* Uploads a point cloud (ions: x,y,z AoS float, uint8 type id, squared
  * distance to the tip surface -- the reference's ion2surf -- as float).

@@ -44,6 +44,9 @@ HIP_FLAGS = [
 LIB_SPATSTAT = os.path.join(PKG, "libpp_spatstat.so")
 LIB_SYNTH = os.path.join(PKG, "libpp_synth.so")
 LIB_HORDERDIST = os.path.join(PKG, "libpp_horderdist.so")
+LIB_H5 = os.path.join(PKG, "libpp_h5.so")
+# the HDF5 C library of this image; the HDF5 writer is built only where it exists
+HDF5_PREFIX = os.environ.get("PP_HDF5_PREFIX", "/opt/conda")
 
 
 def _run(cmd: list[str]) -> None:
@@ -96,11 +99,29 @@ def build_horderdist() -> str:
     return LIB_HORDERDIST
 
 
+def build_h5() -> str:
+    """libpp_h5.so: the optional HDF5 writer of the /DescrSpatStats datasets
+    (host C++ against the HDF5 C library; nothing else links it)."""
+    inc = os.path.join(HDF5_PREFIX, "include")
+    lib = os.path.join(HDF5_PREFIX, "lib")
+    if not os.path.exists(os.path.join(inc, "hdf5.h")):
+        return ""
+    out_tmp = LIB_H5 + ".tmp"
+    _run([CXX, "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-I", INCLUDE, "-I", inc,
+          os.path.join(HOST, "h5", "pp_h5.cpp"), "-o", out_tmp, "-L", lib, "-lhdf5",
+          "-Wl,-rpath," + lib])
+    os.replace(out_tmp, LIB_H5)
+    return LIB_H5
+
+
 def build_all() -> list[str]:
     out = [build_synth(), build_spatstat()]
     h = build_horderdist()
     if h:
         out.append(h)
+    h5 = build_h5()
+    if h5:
+        out.append(h5)
     shutil.rmtree(os.path.join(PKG, "_build"), ignore_errors=True)
     return out
 

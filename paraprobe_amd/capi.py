@@ -41,6 +41,7 @@ EXPORTED = [
     "pp_spatstat_destroy",
     "pp_last_error_string",
     "pp_hist_geometry",
+    "pp_hist_supported",
     "pp_cloud_upload",
     "pp_cloud_from_device",
     "pp_cloud_free",
@@ -119,6 +120,7 @@ class _Out(C.Structure):
         ("counters16", C.c_int32),
         ("percenter", C.c_int32),
         ("centers_rerun", C.c_int64),
+        ("ms_kernel", C.c_double),
     ]
 
 
@@ -141,6 +143,7 @@ def load_library() -> C.CDLL:
     lib.pp_last_error_string.argtypes = [P]
     lib.pp_last_error_string.restype = C.c_char_p
     lib.pp_hist_geometry.argtypes = [C.c_float, C.c_float, C.c_float, C.POINTER(C.c_int64)]
+    lib.pp_hist_supported.argtypes = [C.c_float, C.c_float, C.c_float]
     lib.pp_cloud_upload.argtypes = [P, P, P, P, C.c_uint64, C.c_float, C.POINTER(P)]
     lib.pp_cloud_from_device.argtypes = [P, P, P, P, C.c_uint64, C.c_float, C.POINTER(P)]
     lib.pp_cloud_free.argtypes = [P, P]
@@ -242,6 +245,7 @@ class Result:
     kernel: str = ""
     percenter: int = 0      # 0 none, 1 full-range per-center histograms, 2 kNN range only
     centers_rerun: int = 0  # centers re-run over the full range (mode 2)
+    ms_kernel: float = 0.0  # the query kernel launches alone
 
     def arrays(self) -> dict:
         return {k: getattr(self, k) for k in ("rdf", "nn1", "ripk", "mknn", "cntnb")
@@ -306,6 +310,7 @@ class _Prepared:
                 ("direct", "percenter", "hybrid")[o.percenter], o.cells_per_rmax),
             percenter=o.percenter,
             centers_rerun=o.centers_rerun,
+            ms_kernel=o.ms_kernel,
         )
 
 

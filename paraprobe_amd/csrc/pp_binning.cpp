@@ -149,18 +149,24 @@ bool EdgeTable::build(const HistGeometry &g, std::string *why)
     return true;
 }
 
-float EdgeTable::amb_margin(const HistGeometry &g, double R) const
+double EdgeTable::y_err(const HistGeometry &g, double R) const
 {
     // |y - ((d - start) tmp + 1)| for d <= R, with A = R tmp and
     // B = |1 - start tmp|: the square root's error (v_sqrt_f32, at most 2 ulp,
     // 2^-22 relative) scaled, 2.4e-7 A; the float scale's rounding (2^-24
     // relative), 6e-8 A; the fma's rounding of a result below A + B + 1 (the
     // shifted coordinate adds at most 0.5), 6e-8 (A + B + 1); the float
-    // offset's rounding, 6e-8 (B + 1).  The margin is max_offset plus 1.5
-    // times their sum.
+    // offset's rounding, 6e-8 (B + 1)
     const double A = std::fabs(R) * g.tmp, B = std::fabs(1.0 - g.start * g.tmp);
-    const double yerr = 2.4e-7 * A + 6e-8 * A + 6e-8 * (A + B + 1.0) + 6e-8 * (B + 1.0);
-    const double m = max_offset + 1.5 * yerr + 1e-6;
+    return 2.4e-7 * A + 6e-8 * A + 6e-8 * (A + B + 1.0) + 6e-8 * (B + 1.0);
+}
+
+float EdgeTable::amb_margin(const HistGeometry &g, double R) const
+{
+    // max_offset plus 1.5 times the rounding bound.  At 0.5 every lane takes
+    // the near-edge path, which is exact while exact_up_to(g, R) holds (jobs
+    // where it does not are refused before they run), so the cap loses nothing.
+    const double m = max_offset + 1.5 * y_err(g, R) + 1e-6;
     return (float)std::min(m, 0.5);
 }
 

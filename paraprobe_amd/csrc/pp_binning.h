@@ -50,6 +50,12 @@ struct EdgeTable {
     // integer, floor(y) is the category; closer to an integer, the edge table
     // decides.  amb_margin(R) = max_offset + y_err for distances up to R.
     float idx_off1 = 0;
+    // bound on |y - ((d - start) tmp + 1)| for distances up to R
+    double y_err(const HistGeometry &g, double R) const;
+    // the kernel's binning is exact up to R when max_offset + y_err < 0.5 (the
+    // near-edge path rounds y to the nearest index and settles it with the
+    // table); jobs beyond that are refused
+    bool exact_up_to(const HistGeometry &g, double R) const { return max_offset + y_err(g, R) < 0.5; }
     float amb_margin(const HistGeometry &g, double R) const;
     // Builds the table; returns false (with a reason) when the bins are too
     // narrow for a nearest-edge guess to be provably within one edge.

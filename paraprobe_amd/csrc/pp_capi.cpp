@@ -72,6 +72,9 @@ struct Slot {
     DevBuf cnbuf;    // CNTNB counters
     DevBuf kbuf;     // k-th neighbour categories of one launch chunk
     hipEvent_t ev[6] = {};
+    // event pairs around the query-kernel launches of the current call
+    std::vector<hipEvent_t> qev;
+    size_t nq = 0;
 };
 
 } // namespace
@@ -180,6 +183,10 @@ void slot_free(Slot &s)
             (void)hipEventDestroy(e);
             e = nullptr;
         }
+    for (hipEvent_t e : s.qev)
+        (void)hipEventDestroy(e);
+    s.qev.clear();
+    s.nq = 0;
     if (s.stream) {
         (void)hipStreamDestroy(s.stream);
         s.stream = nullptr;
@@ -254,6 +261,9 @@ pp_status plan_job(pp_ctx *c, const pp_job *j, JobPlan &p)
     std::string why;
     if (!p.et.build(p.hg, &why))
         return fail(c, PP_ERR_UNSUPPORTED, why);
+    if (!p.et.exact_up_to(p.hg, j->rmax))
+        return fail(c, PP_ERR_UNSUPPORTED,
+                    "bins too fine for the float index coordinate up to Rmax");
     p.C = (int)p.hg.ncat();
     p.R = j->rmax;
     p.R2 = p.R * p.R;
@@ -337,6 +347,9 @@ pp_status plan_job(pp_ctx *c, const pp_job *j, JobPlan &p)
             if ((p.task_trg[k] >> s) & 1u)
                 p.trg_tasks[(size_t)s * p.lmax + l++] = (uint32_t)k;
     }
+    // k-th neighbour categories travel as uint16, kKbEmpty reserved
+    if (p.nplanes() > 0 && p.C >= (int)kKbEmpty)
+        return fail(c, PP_ERR_UNSUPPORTED, "kNN histograms of more than 65533 bins");
     p.cat0 = (int)p.hg.category(0.0f);
     p.cat_sentinel = (int)p.hg.category((float)((double)p.R + kEps));
     return PP_OK;
@@ -825,7 +838,7 @@ void fill_jobdev(JobRun &r)
     d.trg_tasks = r.tables + r.o_ttasks;
     d.nplanes = p.nplanes();
     d.KB = nullptr;  // set per launch chunk
-    d.KT = nullptr;
+    d.tiles = r.tiles;
     d.tile_base = 0;
     d.kb_tiles = 0;
     d.HK = p.HK;
@@ -1009,7 +1022,7 @@ size_t kb_tile_bytes(const JobPlan &p, int T)
     if (np == 0)
         return 0;
     const size_t TR = (size_t)T * (size_t)p.lmax;
-    return TR * (2 * (size_t)np + 1);
+    return TR * 2 * (size_t)np;
 }
 
 // Splits the launch's tiles (job after job) into chunks whose k-th neighbour
@@ -1194,7 +1207,6 @@ pp_status launch_group(pp_ctx *c, Slot &sl, std::vector<JobRun> &runs, const std
             if (np > 0 && ch.ntl[q] > 0) {
                 const size_t TR = (size_t)T * plans[q]->lmax;
                 d.KB = kc.take<uint16_t>((size_t)ch.ntl[q] * np * TR);
-                d.KT = kc.take<uint8_t>((size_t)ch.ntl[q] * TR);
                 d.tile_base = ch.base[q];
                 d.kb_tiles = ch.ntl[q];
                 units += (uint32_t)np * ((ch.ntl[q] + tpu - 1) / tpu);
@@ -1254,7 +1266,24 @@ pp_status launch_group(pp_ctx *c, Slot &sl, std::vector<JobRun> &runs, const std
                       ? std::max<uint32_t>(1u, std::min<uint32_t>(
                                                    4u, (ch.g1 - ch.g0) / ((uint32_t)ql.grid * 16u)))
                       : 1u;
-        le = launch_query(ql, sl.stream);
+        if (!capture && le == hipSuccess) {
+            // time the query launch alone (not its reduction)
+            while (sl.qev.size() < 2 * (sl.nq + 1)) {
+                hipEvent_t e = nullptr;
+                le = hipEventCreate(&e);
+                if (le != hipSuccess)
+                    break;
+                sl.qev.push_back(e);
+            }
+            if (le == hipSuccess)
+                le = hipEventRecord(sl.qev[2 * sl.nq], sl.stream);
+        }
+        if (le == hipSuccess)
+            le = launch_query(ql, sl.stream);
+        if (!capture && le == hipSuccess) {
+            le = hipEventRecord(sl.qev[2 * sl.nq + 1], sl.stream);
+            ++sl.nq;
+        }
         const uint32_t units = huoff[k * (nj + 1) + nj];
         if (le == hipSuccess && units > 0)
             le = launch_kreduce(d_jobs + k * nj, nj, d_uoff + k * (nj + 1), units, T, tpu,
@@ -1318,6 +1347,7 @@ pp_status run_jobs(pp_ctx *c, Slot &sl, int njobs, const pp_cloud *const *clouds
     auto t_start = std::chrono::steady_clock::now();
     if (!outs)
         return fail(c, PP_ERR_INVALID_ARG, "outputs are NULL");
+    sl.nq = 0;
     std::vector<JobRun> runs(njobs);
     int tuneT, tuneR, graph_launch;
     double knn_factor;
@@ -1555,6 +1585,12 @@ pp_status run_jobs(pp_ctx *c, Slot &sl, int njobs, const pp_cloud *const *clouds
     PP_HIP(c, hipEventElapsedTime(&ms_idx, sl.ev[0], sl.ev[1]));
     PP_HIP(c, hipEventElapsedTime(&ms_prep, sl.ev[1], sl.ev[2]));
     PP_HIP(c, hipEventElapsedTime(&ms_q, sl.ev[3], sl.ev[4]));
+    double ms_kern = 0;
+    for (size_t q = 0; q < sl.nq; ++q) {
+        float ms = 0;
+        PP_HIP(c, hipEventElapsedTime(&ms, sl.qev[2 * q], sl.qev[2 * q + 1]));
+        ms_kern += ms;
+    }
     const double ms_total =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start)
             .count();
@@ -1574,6 +1610,7 @@ pp_status run_jobs(pp_ctx *c, Slot &sl, int njobs, const pp_cloud *const *clouds
         outs[i].counters16 = runs[i].c16_used;
         outs[i].percenter = runs[i].plan.mode;
         outs[i].centers_rerun = runs[i].rerun;
+        outs[i].ms_kernel = ms_kern;
         any_overflow |= overflow;
     }
     if (any_overflow)
@@ -1724,12 +1761,9 @@ double job_work(const pp_cloud *cl, const JobPlan &p)
 }
 
 // Work that one serving batch may gather (about 20 ms of query kernel), so
-// that small requests share launches and large ones do not hold back the
-// requests behind them.
+// that small requests share launches and a large one does not hold back the
+// requests that arrived before it finishes.
 constexpr double kBatchWork = 3.0e10;
-// Batches are formed shortest estimated work first among the admitted
-// requests; one that has waited this long goes first, in arrival order.
-constexpr double kMaxWaitMs = 100.0;
 
 } // namespace
 
@@ -1747,41 +1781,26 @@ struct pp_server {
     uint64_t next_ticket = 1, submitted = 0, completed = 0;
     std::vector<std::pair<uint64_t, int32_t>> done;
 
-    // Moves the next batch out of `pending` (called with mu held): requests
-    // that waited kMaxWaitMs or more first (arrival order), then the smallest
-    // estimated work; each must fit the remaining budget, and the batch stops
-    // at max_batch_jobs or about kBatchWork.  Returns false if nothing fits.
+    // Moves the next batch out of `pending` (called with mu held): the
+    // waiting requests in arrival order, as many as fit the remaining budget,
+    // max_batch_jobs and about kBatchWork of estimated work (a larger request
+    // runs alone).  No request overtakes an earlier one, so the tail latency
+    // stays that of a first-come first-served loop.  Returns false when the
+    // oldest waiting request does not fit the budget yet.
     bool pick(std::vector<Request> &batch, size_t &fp)
     {
-        const auto now = std::chrono::steady_clock::now();
         double work = 0;
         while ((int)batch.size() < opts.max_batch_jobs && !pending.empty()) {
-            int best = -1;
-            bool best_old = false;
-            for (size_t k = 0; k < pending.size(); ++k) {
-                const Request &r = pending[k];
-                if (used + r.footprint > budget)
-                    continue;
-                if (!batch.empty() && work + r.work > kBatchWork)
-                    continue;
-                const bool old =
-                    std::chrono::duration<double, std::milli>(now - r.arrival).count() >=
-                    kMaxWaitMs;
-                if (best < 0 || (old && !best_old) ||
-                    (old == best_old && !old && r.work < pending[best].work)) {
-                    best = (int)k;
-                    best_old = old;
-                }
-                if (old)
-                    break;  // the oldest waiting request: take it now
-            }
-            if (best < 0)
+            Request &r = pending.front();
+            if (used + r.footprint > budget)
                 break;
-            used += pending[best].footprint;
-            fp += pending[best].footprint;
-            work += pending[best].work;
-            batch.push_back(std::move(pending[best]));
-            pending.erase(pending.begin() + best);
+            if (!batch.empty() && work + r.work > kBatchWork)
+                break;
+            used += r.footprint;
+            fp += r.footprint;
+            work += r.work;
+            batch.push_back(std::move(r));
+            pending.pop_front();
         }
         return !batch.empty();
     }
@@ -1864,6 +1883,17 @@ pp_status pp_spatstat_destroy(pp_ctx *c)
 }
 
 const char *pp_last_error_string(const pp_ctx *c) { return c ? c->err.c_str() : "no context"; }
+
+pp_status pp_hist_supported(float rmin, float rincr, float rmax)
+{
+    if (!(rmin >= 0.f) || !(rincr >= kEps) || !(rmax >= kEps) || !(rmin < rmax))
+        return PP_ERR_INVALID_ARG;
+    HistGeometry hg(rmin, rincr, rmax);
+    EdgeTable et;
+    if (!et.build(hg, nullptr) || !et.exact_up_to(hg, rmax))
+        return PP_ERR_UNSUPPORTED;
+    return PP_OK;
+}
 
 pp_status pp_hist_geometry(float rmin, float rincr, float rmax, int64_t *nbins)
 {

@@ -19,6 +19,9 @@ constexpr int kXcds = 8;           // MI355X accelerator complex dies
 // 256-byte line (atomics on one line serialise at ~11 ns each).
 constexpr int kQueueStride = 64;
 constexpr int kQueueWords = kXcds * kQueueStride;
+// k-th neighbour category of a row that the reduction skips (categories are
+// below 65535: jobs with kNN and more bins are refused)
+constexpr uint16_t kKbEmpty = 0xFFFFu;
 
 // Stencil row: cells (cx-dxm .. cx+dxm, cy+dy, cz+dz) are contiguous in the
 // cell-sorted ion array because the cell key is x-fastest.
@@ -101,10 +104,14 @@ struct JobDev {
     const uint32_t *trg_tasks;
     // k-th neighbour categories of the tiles of one launch chunk, reduced into
     // K / N1 afterwards: per tile, nplanes planes of T * lmax uint16 (row
-    // c * lmax + l; plane nk is the 1NN when requested) and T * lmax task ids
-    // (0xFF: empty row).  Tile tl of the chunk is local tile tile_base + tl.
+    // c * lmax + l: center c of the tile, its center type's task
+    // trg_tasks[slot * lmax + l]; plane nk is the 1NN when requested).  Rows
+    // of absent centers (c >= the tile's count) and of missing tasks
+    // (l >= trg_ntask[slot]) are never written and the reduction skips them;
+    // rows of a center handed to the full-range rerun hold kKbEmpty in every
+    // plane.  Tile tl of the chunk is the job's tile tile_base + tl.
     uint16_t *KB;
-    uint8_t *KT;
+    const Tile *tiles;        // the job's tiles (their slot and center count)
     int nplanes;
     uint32_t tile_base;
     uint32_t kb_tiles;        // tiles of this job in the chunk

@@ -591,7 +591,7 @@ hipError_t launch_bump(const uint32_t *cpos, uint64_t n, const uint32_t *toff, u
 // query kernel
 
 struct Lay {
-    int edges, envoff, trgslot, kinfo, ckoff, kidx, win, rows, rowchunk, cen, meta, P, H, total;
+    int edges, envoff, trgslot, kinfo, ckoff, kidx, win, rows, rowpre, cen, meta, P, H, total;
 };
 
 __host__ __device__ inline int al4(int x) { return (x + 3) & ~3; }
@@ -619,7 +619,7 @@ __host__ __device__ inline Lay make_layout(int T, int c16, int mode, int maxC, i
     o += mode ? kWaves * kWave : 0;
     L.rows = o;
     o += al4(2 * maxRows);
-    L.rowchunk = o;
+    L.rowpre = o;
     o += al4(maxRows + 1);
     L.cen = o;
     o += 4 * T;
@@ -770,7 +770,7 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : (T == 8 && MODE
     int32_t *s_kidx = (int32_t *)(lds + L.kidx);
     int32_t *s_win = (int32_t *)(lds + L.win);
     uint32_t *s_rows = lds + L.rows;
-    uint32_t *s_rowchunk = lds + L.rowchunk;
+    uint32_t *s_rowpre = lds + L.rowpre;
     float4 *s_cen = (float4 *)(lds + L.cen);
     uint32_t *s_meta = lds + L.meta;  // [2c] tc | es << 8, [2c+1] unused
     uint32_t *s_P = lds + L.P;
@@ -940,24 +940,22 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : (T == 8 && MODE
         }
         __syncthreads();
         if (wid == 0) {
-            // chunk prefix over rows (64 candidates per chunk)
-            uint32_t carry = 0, ctot = 0;
+            // candidate prefix over the rows: the tile's candidates are the
+            // concatenation of its rows, cut into 64-candidate chunks
+            uint32_t carry = 0;
             for (int r0 = 0; r0 < nrows; r0 += kWave) {
                 const int r = r0 + lane;
                 uint32_t len = 0;
                 if (r < nrows)
                     len = s_rows[2 * r + 1] - s_rows[2 * r];
-                const uint32_t ch = (len + kWave - 1) / kWave;
-                const uint32_t inc = wave_incl_scan(ch, lane);
-                const uint32_t linc = wave_incl_scan(len, lane);
+                const uint32_t inc = wave_incl_scan(len, lane);
                 if (r < nrows)
-                    s_rowchunk[r] = carry + inc - ch;
+                    s_rowpre[r] = carry + inc - len;
                 carry += __shfl(inc, kWave - 1, kWave);
-                ctot += __shfl(linc, kWave - 1, kWave);
             }
             if (lane == 0) {
-                s_rowchunk[nrows] = carry;
-                s_ctot = ctot;
+                s_rowpre[nrows] = carry;
+                s_ctot = carry;
             }
         }
         __syncthreads();
@@ -992,37 +990,58 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : (T == 8 && MODE
             const gfvec4 *__restrict__ pos = (const gfvec4 *)J.pos;
             const uint32_t *gedges = J.edges;  // the hybrid mode's exact edge table
             uint32_t *s_acc = PC ? s_H : s_P;
-            const int nch = (int)rfl(s_rowchunk[nrows]);
+            const uint32_t ctot = rfl(s_ctot);
+            const int nch = (int)((ctot + kWave - 1) / kWave);
             unsigned long long my_in = 0;
-            // each wavefront takes a contiguous block of the tile's chunks and
-            // walks it with a wave-uniform cursor (row r, first index jb of the
-            // chunk, row end re); two chunks alternate, so that the load of the
-            // next is in flight while one is evaluated against the T centers
+            // each wavefront takes a contiguous block of the tile's chunks;
+            // chunk q holds candidates q * 64 .. q * 64 + 63 of the rows'
+            // concatenation, so a chunk may span rows and only the tile's
+            // last chunk is partly empty.  Every lane keeps its own cursor:
+            // row r, index j, row end re (r == nrows: past the last row).
+            // Two chunks alternate, so that the load of the next is in
+            // flight while one is evaluated against the T centers.
             const int qa = (int)rfl((uint32_t)(wid * nch / kWaves));
             const int qb = (int)rfl((uint32_t)((wid + 1) * nch / kWaves));
-            int r = 0;
-            uint32_t jb = 0, re = 0;
+            int r = nrows;
+            uint32_t j = 0, re = 0xFFFFFFFFu;
             if (qa < qb) {
-                while ((int)rfl(s_rowchunk[r + 1]) <= qa)
-                    ++r;
-                jb = rfl(s_rows[2 * r]) + (uint32_t)(qa - (int)rfl(s_rowchunk[r])) * kWave;
-                re = rfl(s_rows[2 * r + 1]);
+                // the row of candidate s0: the first h with rowpre[h] > s0,
+                // less one (that row is non-empty)
+                const uint32_t s0 = (uint32_t)qa * kWave + (uint32_t)lane;
+                if (s0 < ctot) {
+                    int lo = 0, hi = nrows;
+                    while (lo < hi) {
+                        const int mid = (lo + hi) >> 1;
+                        if (s_rowpre[mid] <= s0)
+                            lo = mid + 1;
+                        else
+                            hi = mid;
+                    }
+                    r = lo - 1;
+                    j = s_rows[2 * r] + (s0 - s_rowpre[r]);
+                    re = s_rows[2 * r + 1];
+                }
             }
-            // the chunk at the cursor (lanes past the row end load its first ion)
+            // the chunk at the cursor (lanes past the last row load ion 0)
             auto fetch = [&](bool &ok) {
-                const uint32_t j = jb + (uint32_t)lane;
-                ok = j < re;
-                return load_g4(pos, ok ? j : jb);
+                ok = r < nrows;
+                return load_g4(pos, j);
             };
-            // the next chunk (there is one: rows without chunks are skipped)
+            // the next chunk: 64 candidates on, across row ends (empty rows
+            // are passed over)
             auto advance = [&]() {
-                jb += kWave;
-                if (jb >= re) {
-                    do {
-                        ++r;
-                    } while (rfl(s_rowchunk[r + 1]) == rfl(s_rowchunk[r]));
-                    jb = rfl(s_rows[2 * r]);
-                    re = rfl(s_rows[2 * r + 1]);
+                j += kWave;
+                while (j >= re) {
+                    if (r + 1 >= nrows) {
+                        r = nrows;
+                        j = 0;
+                        re = 0xFFFFFFFFu;
+                        break;
+                    }
+                    const uint32_t ov = j - re;
+                    ++r;
+                    j = s_rows[2 * r] + ov;
+                    re = s_rows[2 * r + 1];
                 }
             };
             auto body = [&](float4 p, const bool ok) {
@@ -1206,14 +1225,7 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : (T == 8 && MODE
             const uint64_t tl = (uint64_t)(g - a.tile_off[job] - J.tile_base);
             const uint64_t TR = (uint64_t)T * (uint64_t)lmax;
             uint16_t *kb = J.KB ? J.KB + tl * (uint64_t)J.nplanes * TR : nullptr;
-            uint8_t *kt = J.KT ? J.KT + tl * TR : nullptr;
-            for (int c = wid; c < T; c += kWaves) {
-                if (c >= cnt) {
-                    if (kt)
-                        for (int l = lane; l < lmax; l += kWave)
-                            kt[c * lmax + l] = 0xFFu;
-                    continue;
-                }
+            for (int c = wid; c < cnt; c += kWaves) {
                 const uint32_t m = rfl(s_meta[2 * c]);
                 const uint32_t tc = m & 0xFFu;
                 const int es = (int)((m >> 8) & 0xFFu);
@@ -1229,9 +1241,6 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : (T == 8 && MODE
                     return w;
                 };
                 const int L = (int)rfl(J.trg_ntask[tc]);
-                if (kt)
-                    for (int l = L + lane; l < lmax; l += kWave)
-                        kt[c * lmax + l] = 0xFFu;
                 // a task whose kNN range is short of the full range must hold
                 // its k neighbours there, else the center is rerun over the
                 // full range (its rows here are then marked empty)
@@ -1239,8 +1248,6 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : (T == 8 && MODE
                 for (int l = 0; l < L; ++l) {
                     const int tk = (int)rfl(J.trg_tasks[tc * lmax + l]);
                     const int row = c * lmax + l;
-                    if (kt && lane == 0)
-                        kt[row] = (uint8_t)tk;
                     const uint32_t envset = rfl(J.task_env[tk]);
                     const int ck = HY ? (int)(rfl(J.task_ck[tk]) & 0x7FFFFFFFu) : C;
                     if ((want_k || want_1) && kb) {
@@ -1349,12 +1356,18 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : (T == 8 && MODE
                         }
                     }
                 }
-                if (HY && short_range && lane == 0) {
-                    // the lane that wrote the rows' task ids empties them again
-                    J.unres[J.centers[t_first + c]] = 1u;
-                    if (kt)
-                        for (int l = 0; l < L; ++l)
-                            kt[c * lmax + l] = 0xFFu;
+                if (HY && short_range) {
+                    // the rerun settles this center: every plane of its rows
+                    // (some already written) is emptied for the reduction
+                    if (lane == 0)
+                        J.unres[J.centers[t_first + c]] = 1u;
+                    if (kb) {
+                        const int np = J.nplanes;
+                        for (int i = lane; i < np * L; i += kWave) {
+                            const int pl = i / L, l = i - pl * L;
+                            kb[(uint32_t)pl * (uint32_t)TR + (uint32_t)(c * lmax + l)] = kKbEmpty;
+                        }
+                    }
                 }
             }
         }
@@ -1481,14 +1494,21 @@ __global__ __launch_bounds__(256) void k_kreduce(const JobDev *__restrict__ jobs
     }
     const uint32_t n = (t1 - t0) * TR;
     const uint16_t *kb = J.KB + ((size_t)t0 * J.nplanes + plane) * TR;
-    const uint8_t *kt = J.KT + (size_t)t0 * TR;
     const size_t pstride = (size_t)J.nplanes * TR;
+    const Tile *tiles = J.tiles + J.tile_base + t0;
+    const uint32_t lmax = (uint32_t)J.lmax;
     for (uint32_t e = threadIdx.x; e < n; e += blockDim.x) {
         const uint32_t tt = e / TR, r = e - tt * TR;
-        const uint32_t tk = kt[(size_t)tt * TR + r];
-        if (tk == 0xFFu)
+        const uint32_t c = r / lmax, l = r - c * lmax;
+        // the row's center and task: absent centers of a partial tile and
+        // tasks beyond its center type's were never written
+        const Tile t = tiles[tt];
+        if (c >= (uint32_t)t.count || l >= J.trg_ntask[t.pad])
             continue;
         const uint32_t cat = kb[(size_t)tt * pstride + r];
+        if (cat == kKbEmpty)
+            continue;
+        const uint32_t tk = J.trg_tasks[t.pad * lmax + l];
         if (use_lds)
             atomicAdd(&h[tk * C + cat], 1u);
         else

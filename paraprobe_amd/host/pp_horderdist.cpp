@@ -3,6 +3,8 @@
 // gfx950 C ABI.
 #include "pp_horderdist.h"
 
+#include <dlfcn.h>
+
 #include <algorithm>
 #include <cctype>
 #include <cmath>
@@ -605,6 +607,53 @@ void RndLabeler::reset_original_types_global()
 }
 
 // ---------------------------------------------------------------------------
+// HDF5 output: libpp_h5.so (next to this library), loaded when first needed;
+// without it the NPCORR counts stay in memory only
+
+namespace {
+
+struct H5Writer {
+    int (*create)(const char *, int32_t) = nullptr;
+    int (*centers)(const char *, uint64_t, float) = nullptr;
+    int (*hist)(const char *, const char *, const char *, int32_t, const uint32_t *,
+                uint64_t) = nullptr;
+    bool ok = false;
+};
+
+const H5Writer &h5_writer()
+{
+    static H5Writer w = [] {
+        H5Writer r;
+        Dl_info info;
+        if (!dladdr((const void *)&h5_writer, &info) || !info.dli_fname)
+            return r;
+        std::string dir = info.dli_fname;
+        const size_t slash = dir.rfind('/');
+        dir = slash == std::string::npos ? "." : dir.substr(0, slash);
+        void *h = dlopen((dir + "/libpp_h5.so").c_str(), RTLD_NOW | RTLD_LOCAL);
+        if (!h)
+            return r;
+        r.create = (int (*)(const char *, int32_t))dlsym(h, "pph5_create_results");
+        r.centers = (int (*)(const char *, uint64_t, float))dlsym(h, "pph5_report_npc3d_bincenters");
+        r.hist = (int (*)(const char *, const char *, const char *, int32_t, const uint32_t *,
+                          uint64_t))dlsym(h, "pph5_report_npc3d_histvalues");
+        r.ok = r.create && r.centers && r.hist;
+        return r;
+    }();
+    return w;
+}
+
+} // namespace
+
+bool hdf5_output_available() { return h5_writer().ok; }
+
+bool create_results_h5(const std::string &fn, bool npcorr)
+{
+    const H5Writer &w = h5_writer();
+    return w.ok && w.create(fn.c_str(), npcorr ? 1 : 0) == 0;
+}
+
+// ---------------------------------------------------------------------------
 // driver
 
 Horderdist::Horderdist(pp_ctx *ctx, const SpatstatSettings &s, const PeriodicTable &pse)
@@ -694,6 +743,27 @@ pp_status Horderdist::compute_generic_spatstat3(const pp_cloud *cloud, const Vol
             std::cerr << "ERROR::n-point correlations on the GPU failed: "
                       << pp_last_error_string(ctx_) << std::endl;
             return st;
+        }
+        if (!results_h5.empty()) {
+            // report_npc3d_bincenters_hdf5 + report_npc3d_histvalues_hdf5
+            // (S.cpp:4270-4288): the bin centers once per file (the
+            // randomised rerun finds them written), one column per task
+            const H5Writer &w = h5_writer();
+            if (w.ok) {
+                (void)w.centers(results_h5.c_str(), (uint64_t)npcorr_edge, s_.rincr);
+                const uint64_t nxyz = (uint64_t)npcorr_edge * npcorr_edge * npcorr_edge;
+                for (int32_t t = 0; t < T; ++t) {
+                    std::string tn, en;
+                    for (const auto &kv : spatstat_tasks[t].trgcandidates)
+                        tn += kv.first;
+                    for (const auto &kv : spatstat_tasks[t].envcandidates)
+                        en += kv.first;
+                    if (w.hist(results_h5.c_str(), tn.c_str(), en.c_str(), randomized ? 1 : 0,
+                               &npcorr[(size_t)t * nxyz], nxyz) != 0)
+                        std::cerr << "WARNING::2-point statistics of " << tn << "_" << en
+                                  << " not written to " << results_h5 << std::endl;
+                }
+            }
         }
         fam &= ~PP_FAM_NPCORR;
         job.families = fam;
@@ -890,6 +960,17 @@ int32_t pph_run_spatstat(pp_ctx *ctx, const pp_cloud *cloud, const char *xml, ui
     m.nions_inside = nions_inside;
     m.volume_inside = volume_inside;
     const std::string dir = outdir ? outdir : "";
+    if (s.do_npcorr) {
+        // the results file of solverHdl::generate_hdf5_resultsfile (S.cpp:
+        // 7440-7453) with the /DescrSpatStats groups, for the NPCORR cubes
+        const std::string fn = (dir.empty() ? std::string() : dir + "/") + "PARAPROBE.SimID." +
+                               std::to_string(simid) + ".Results.h5";
+        if (pph::create_results_h5(fn, true))
+            h.results_h5 = fn;
+        else
+            std::cerr << "WARNING::no HDF5 output (libpp_h5.so missing or " << fn
+                      << " not writable); 2-point statistics stay in memory" << std::endl;
+    }
     pp_status st = h.compute_generic_spatstat3(cloud, m, dir);
     if (st != PP_OK || !s.add_label_rnd)
         return st;

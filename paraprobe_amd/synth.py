@@ -119,6 +119,20 @@ def frustum_20m(seed: int = 2, n: int = 20_000_000, nclusters: int = 50_000) -> 
                cluster_fracs(base))
 
 
+def frustum_100m(seed: int = 4, n: int = 100_000_000, nclusters: int = 300_000,
+                 density: float = 22.0) -> Tip:
+    """The large configuration: 100M ions in a frustum r 60 -> 30 nm whose
+    height (689 nm) gives 22 ions/nm^3, types 0.95/0.04/0.01, 300,000 Poisson
+    clusters r ~ U(1,2) nm with B at 40%.  A smaller n keeps the shape and the
+    density (every length scaled by (n / 100M)^(1/3), the clusters by n)."""
+    base = [0.95, 0.04, 0.01]
+    h0 = 100_000_000 / density / frustum_volume(60.0, 30.0, 1.0)
+    s = (n / 100_000_000) ** (1.0 / 3.0)
+    nc = int(round(nclusters * n / 100_000_000))
+    return tip(n, seed, "frustum", 60.0 * s, 30.0 * s, h0 * s, base, nc, 1.0, 2.0,
+               cluster_fracs(base))
+
+
 def frustum_volume(r0: float, r1: float, h: float) -> float:
     return math.pi * h / 3.0 * (r0 * r0 + r0 * r1 + r1 * r1)
 

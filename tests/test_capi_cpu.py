@@ -63,3 +63,20 @@ def test_host_library_exports_every_declared_pph_symbol():
     for name in declared:
         assert hasattr(lib, name), name
     assert set(H.EXPORTED) == set(declared)
+
+
+def test_hist_supported_refuses_bins_beyond_the_near_edge_bound():
+    lib = capi.load_library()
+    # the measured geometries
+    for g in ((0.0, 0.01, 5.0), (0.0, 0.005, 3.0), (0.0, 0.1, 2.0), (4.8, 0.002, 5.0)):
+        assert lib.pp_hist_supported(*g) == capi.PP_OK, g
+    # over a million bins of 4.9e-6 nm up to 5 nm: the edges' offset from their
+    # ideal index (0.127) plus the float index coordinate's rounding bound
+    # (0.367) stays below half a bin, so the near-edge path is exact
+    assert lib.pp_hist_supported(0.0, 4.9e-6, 5.0) == capi.PP_OK
+    # 4.8e-6 passes the table build and the 2^20 index bound, but 0.130 +
+    # 0.375 reaches half a bin: refused (the margin would otherwise be capped)
+    assert lib.pp_hist_supported(0.0, 4.8e-6, 5.0) == 6  # PP_ERR_UNSUPPORTED
+    # what Settings::checkUserInput rejects
+    assert lib.pp_hist_supported(0.0, 1e-7, 5.0) == 1     # PP_ERR_INVALID_ARG
+    assert lib.pp_hist_supported(2.0, 0.1, 1.0) == 1

@@ -151,3 +151,50 @@ def test_relabelled_cloud_equals_fresh_upload(ctx):
         finally:
             for c in (src, rel, fresh):
                 c.free()
+
+
+def test_npcorr_to_hdf5(ctx, tmp_path):
+    """AnalysisSpatDistrType 5 with label randomisation: the NPCORR cubes of
+    the original and the randomised labels go to PARAPROBE.SimID.<id>.Results.h5
+    (/DescrSpatStats/TwoPointStats: BinNXNYNZ, BinCenterXYZ and one
+    <trg>_<env>_<Orig|Rndm> column per task, report_npc3d_*_hdf5,
+    S.cpp:4506-4575), equal to the oracle's counts."""
+    from paraprobe_amd import h5
+
+    if not h5.available():
+        pytest.skip("libpp_h5.so not built (no HDF5 library)")
+    print("test_npcorr_to_hdf5", flush=True)
+    rng = np.random.default_rng(31)
+    n = 6000
+    xyz = rng.uniform(0, 8, size=(n, 3)).astype(np.float32)
+    ty = rng.choice([1, 2, 3], size=n, p=[0.6, 0.3, 0.1]).astype(np.uint8)
+    s2 = rng.uniform(0, 4, size=n).astype(np.float32)
+    xml = XML.replace("<AnalysisSpatDistrType>1246</AnalysisSpatDistrType>",
+                      "<AnalysisSpatDistrType>5</AnalysisSpatDistrType>")
+    xml = xml.replace("<SpatStatRadiusIncr>0.02</SpatStatRadiusIncr>",
+                      "<SpatStatRadiusIncr>0.1</SpatStatRadiusIncr>")
+    xml = xml.replace("<SpatStatKNNOrder>1</SpatStatKNNOrder>",
+                      "<SpatStatKNNOrder>2</SpatStatKNNOrder>")
+    xml = xml.replace("</Parameters>", "  <SpatStatAdditionalLabelRandomization>1"
+                      "</SpatStatAdditionalLabelRandomization>\n</Parameters>")
+    cl = ctx.upload(xyz, ty, s2)
+    try:
+        st = H.run_spatstat(ctx, cl, xml, 11, TYPES, 5000, 400.0, str(tmp_path))
+    finally:
+        cl.free()
+    assert st == 0
+    fn = str(tmp_path / "PARAPROBE.SimID.11.Results.h5")
+    tasks = [((1,), (1,)), ((2,), (2,)), ((1, 2), (3,)), ((3,), (1, 2))]
+    names = ["Al_Al", "Sc_Sc", "AlSc_Zr", "Zr_AlSc"]
+    # SpatStatKNNOrder 2 in the XML is the 0-based order 1 (Settings.cpp:484-491)
+    ref = O.npcorr(xyz, ty, s2, tasks, 1.2, 0.1, 1)
+    rnd = O.npcorr(xyz, H.shuffle_types(ty), s2, tasks, 1.2, 0.1, 1)
+    edge = ref.shape[1]
+    assert h5.read(fn, "/DescrSpatStats/TwoPointStats/BinNXNYNZ").ravel().tolist() == [edge] * 3
+    assert h5.read(fn, "/DescrSpatStats/TwoPointStats/BinCenterXYZ").shape == (edge ** 3, 3)
+    for t, nm in enumerate(names):
+        got = h5.read(fn, f"/DescrSpatStats/TwoPointStats/{nm}_Orig")
+        np.testing.assert_array_equal(got.ravel(), ref[t].ravel())
+        got = h5.read(fn, f"/DescrSpatStats/TwoPointStats/{nm}_Rndm")
+        np.testing.assert_array_equal(got.ravel(), rnd[t].ravel())
+    assert ref.sum() > 0 and not np.array_equal(ref, rnd)

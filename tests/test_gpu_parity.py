@@ -94,7 +94,8 @@ def test_knn_range_does_not_change_results(ctx, factor):
         ctx.set_knn_range(3.0)
 
 
-@pytest.mark.parametrize("T,r", [(8, 1), (16, 2), (32, 3), (16, 4), (8, 2), (32, 1)])
+@pytest.mark.parametrize("T,r", [(8, 1), (16, 2), (32, 3), (16, 4), (8, 2), (32, 1), (8, 6),
+                                 (8, 8)])
 def test_tuning_does_not_change_results(ctx, T, r):
     _say(f"test_tuning T={T} cells_per_rmax={r}")
     xyz, ty, s2 = _cube(4000, 11)
@@ -251,13 +252,28 @@ def test_resident_cell_list_any_rmax(ctx):
         cl.free()
 
 
-@pytest.mark.parametrize("rmin,rincr,rmax", [(0.0, 0.01, 5.0), (0.0, 0.1, 2.0), (0.3, 0.01, 1.5),
-                                             (0.0, 0.005, 3.0)])
-def test_distances_packed_around_every_bin_edge(ctx, rmin, rincr, rmax):
+EDGE_GEOMS = [(0.0, 0.01, 5.0), (0.0, 0.1, 2.0), (0.3, 0.01, 1.5), (0.0, 0.005, 3.0),
+              (4.8, 0.002, 5.0)]
+# query-kernel modes: RDF alone runs the pair-count mode (0); RDF + MKNN over
+# the full range the per-center mode (1); a kNN range of 0.3 x k expected
+# neighbours the hybrid mode (2), whose short centers re-run in mode 1
+EDGE_MODES = {"paircount": (FAM_RDF, 3.0, 0), "percenter": (FAM_RDF | FAM_MKNN, 0.0, 1),
+              "hybrid": (FAM_RDF | FAM_MKNN, 0.3, 2)}
+
+
+@pytest.mark.parametrize("mode", list(EDGE_MODES))
+@pytest.mark.parametrize("rmin,rincr,rmax", EDGE_GEOMS)
+def test_distances_packed_around_every_bin_edge(ctx, rmin, rincr, rmax, mode):
     """Isolated pairs whose distances lie within 40 float steps of every bin
     edge (the ideal edge positions start + c / tmp): the category of each
-    pair, near-edge path included, must equal the literal expression's."""
-    _say(f"bin edges {rmin}/{rincr}/{rmax}")
+    pair, near-edge path included, must equal the literal expression's, in
+    every kernel mode (the floor form with its near-edge fallback in modes 0
+    and 2, the nearest-edge form in mode 1).  4.8 / 0.002 / 5.0 has the
+    widest margin (start far from 0, 500 bins per nm)."""
+    fams, factor, want = EDGE_MODES[mode]
+    if want == 2 and rmin > 1.0:
+        pytest.skip("the kNN range of these sparse pairs ends below rmin: no hybrid plan")
+    _say(f"bin edges {rmin}/{rincr}/{rmax} mode {mode}")
     nb = capi.hist_geometry(rmin, rincr, rmax)
     r = np.array([rmin + c * (rmax - rmin) / nb for c in range(1, nb + 1)], np.float32)
     # k float steps from each edge: positive floats order like their bit patterns
@@ -274,9 +290,39 @@ def test_distances_packed_around_every_bin_edge(ctx, rmin, rincr, rmax):
     xyz = np.concatenate([a, b])
     ty = np.ones(2 * n, np.uint8)
     s2 = np.full(2 * n, 1e6, np.float32)
-    g, _, st = _compare(ctx, xyz, ty, s2, Job([((1,), (1,))], rmin, rincr, rmax,
-                                               FAM_RDF | FAM_MKNN, kvalues=[1]))
+    ctx.set_knn_range(factor)
+    try:
+        g, _, st = _compare(ctx, xyz, ty, s2, Job([((1,), (1,))], rmin, rincr, rmax, fams,
+                                                   kvalues=[1] if fams & FAM_MKNN else []))
+    finally:
+        ctx.set_knn_range(3.0)
     assert st["inball"] == 2 * n
+    assert g.percenter == want, (g.percenter, want)
+    if want == 2:
+        assert g.centers_rerun > 0
+
+
+def test_headline_parameters_clustered_frustum(ctx):
+    """The headline job's parameters on a 400k-ion clustered frustum of the
+    headline shape: all 9 ordered type pairs, Rmax 5 nm, bin 0.01 nm, RDF +
+    MKNN k = 1..100, bit-exact against the oracle."""
+    _say("headline parameters: 400k-ion clustered frustum, 9 pairs, R 5, k 1..100")
+    tip = synth.scaled_frustum(400_000, 7, True)
+    job = Job(synth.ALL_PAIRS, 0.0, 0.01, 5.0, FAM_RDF | FAM_MKNN, kvalues=list(range(1, 101)))
+    g, _, st = _compare(ctx, tip.xyz, tip.types, tip.ion2surf_sqr, job)
+    assert st["considered"] > 50_000 and g.pairs_inball == st["inball"]
+    _say(f"  kernel mode {g.percenter}, {g.centers_rerun} centers rerun")
+
+
+def test_large_configuration_shape(ctx):
+    """The large configuration's job (9 pair RDFs at Rmax 3 nm, bin 0.005 nm,
+    600 bins, plus MKNN k = 1..20) on its tip shape (frustum r 60 -> 30 nm,
+    22 ions/nm^3, 300,000 clusters per 100M ions) scaled to 300k ions."""
+    _say("large configuration shape: 300k ions, 9 pairs, R 3, bin 0.005, k 1..20")
+    tip = synth.frustum_100m(n=300_000)
+    job = Job(synth.ALL_PAIRS, 0.0, 0.005, 3.0, FAM_RDF | FAM_MKNN, kvalues=list(range(1, 21)))
+    g, _, st = _compare(ctx, tip.xyz, tip.types, tip.ion2surf_sqr, job)
+    assert g.nbins == 600 and st["considered"] > 80_000
 
 
 def test_sparse_cloud_in_a_large_box(ctx):
