@@ -156,6 +156,23 @@ pp_status pp_cloud_footprint(const pp_cloud *cloud, uint64_t *bytes);
  * (S.cpp:3702-4176).  Histograms are written to *out. */
 pp_status pp_spatstat_run(pp_ctx *ctx, const pp_cloud *cloud, const pp_job *job, pp_hist_out *out);
 
+/* pp_spatstat_run without device allocation (no counterpart in the
+ * reference): for callers that cannot allocate device memory -- code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc.
+ * Every temporary of the job (center lists, tiles, tables, counters, the k-th
+ * neighbour categories) is carved from a fixed device scratch of 256 MiB that
+ * the library holds per device until the process ends; a job that needs more
+ * (roughly 32 bytes per ion plus the k-th neighbour categories) returns
+ * PP_ERR_UNSUPPORTED.  The cloud must be resident (uploaded with a cell
+ * width), or have the cell list of this job's width from an earlier
+ * pp_spatstat_run; a job whose neighbour types would get a typed candidate
+ * list uses the full list instead when that typed list does not exist yet
+ * (same results).  Results equal pp_spatstat_run's.  Calls on one device take
+ * turns on the scratch (a lock inside the library); each call clears what it
+ * needs cleared, so nothing carries over between calls. */
+pp_status pp_spatstat_run_noalloc(pp_ctx *ctx, const pp_cloud *cloud, const pp_job *job,
+                                  pp_hist_out *out);
+
 /* Runs njobs jobs (each on its own cloud) in one persistent launch of the
  * query kernel; outs[i] receives job i.  Results equal njobs separate
  * pp_spatstat_run calls. */

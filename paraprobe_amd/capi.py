@@ -47,6 +47,7 @@ EXPORTED = [
     "pp_cloud_free",
     "pp_cloud_footprint",
     "pp_spatstat_run",
+    "pp_spatstat_run_noalloc",
     "pp_batch_run",
     "pp_set_tuning",
     "pp_set_knn_range",
@@ -149,6 +150,7 @@ def load_library() -> C.CDLL:
     lib.pp_cloud_free.argtypes = [P, P]
     lib.pp_cloud_footprint.argtypes = [P, C.POINTER(C.c_uint64)]
     lib.pp_spatstat_run.argtypes = [P, P, C.POINTER(_Job), C.POINTER(_Out)]
+    lib.pp_spatstat_run_noalloc.argtypes = [P, P, C.POINTER(_Job), C.POINTER(_Out)]
     lib.pp_batch_run.argtypes = [P, C.c_int32, C.POINTER(P), C.POINTER(_Job), C.POINTER(_Out)]
     lib.pp_set_tuning.argtypes = [P, C.c_int32, C.c_int32]
     lib.pp_set_knn_range.argtypes = [P, C.c_float]
@@ -500,6 +502,14 @@ class SpatStat:
     def run(self, cloud: Cloud, job: Job, cntnb_cap: int = 65536) -> Result:
         p = _Prepared(job, cntnb_cap)
         self._check(self.lib.pp_spatstat_run(self.ctx, cloud.handle, C.byref(p.cj), C.byref(p.co)))
+        return p.result()
+
+    def run_noalloc(self, cloud: Cloud, job: Job, cntnb_cap: int = 65536) -> Result:
+        """pp_spatstat_run_noalloc: the same job with every device temporary
+        in the library's fixed scratch (no device allocation)."""
+        p = _Prepared(job, cntnb_cap)
+        self._check(self.lib.pp_spatstat_run_noalloc(self.ctx, cloud.handle, C.byref(p.cj),
+                                                     C.byref(p.co)))
         return p.result()
 
     def run_batch(self, clouds: list, jobs: list, cntnb_cap: int = 65536) -> list:

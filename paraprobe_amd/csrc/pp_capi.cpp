@@ -75,7 +75,16 @@ struct Slot {
     // event pairs around the query-kernel launches of the current call
     std::vector<hipEvent_t> qev;
     size_t nq = 0;
+    // a slot whose buffers are carved from a fixed device arena (the
+    // no-allocation entry point): ensure() never allocates there
+    char *fixed = nullptr;
+    size_t fixed_cap = 0, fixed_off = 0;
+    const DevBuf *fixed_last = nullptr;
 };
+
+// room the k-th neighbour buffer of a fixed-arena slot leaves for the launch
+// descriptors of its group
+constexpr size_t kFixedJobbufReserve = size_t(1) << 20;
 
 } // namespace
 
@@ -90,6 +99,7 @@ struct pp_ctx {
     int graph_launch = 1;     // batch launch groups captured into a HIP graph
     std::map<long, std::pair<int, int>> tuned;  // autotune bucket -> (T, cells per Rmax)
     Slot main;
+    Slot noalloc;  // pp_spatstat_run_noalloc: buffers in the fixed device scratch
 };
 
 struct pp_cloud {
@@ -148,6 +158,26 @@ pp_status ensure(pp_ctx *c, Slot &s, DevBuf &b, size_t bytes)
 {
     if (b.cap >= bytes && b.p)
         return PP_OK;
+    if (s.fixed) {
+        // carve from the arena: the buffer carved last grows in place, any
+        // other takes a new region (the arena is reset per call)
+        const size_t start = (&b == s.fixed_last && b.p)
+                                 ? (size_t)((char *)b.p - s.fixed)
+                                 : ((s.fixed_off + 255) & ~(size_t)255);
+        if (start + bytes > s.fixed_cap) {
+            char msg[192];
+            std::snprintf(msg, sizeof(msg),
+                          "the job needs more than the %zu MiB of the no-allocation scratch "
+                          "(at least %zu MiB)",
+                          s.fixed_cap >> 20, (start + bytes + (size_t(1) << 20) - 1) >> 20);
+            return fail(c, PP_ERR_UNSUPPORTED, msg);
+        }
+        b.p = s.fixed + start;
+        b.cap = bytes;
+        s.fixed_off = start + bytes;
+        s.fixed_last = &b;
+        return PP_OK;
+    }
     if (b.p) {
         PP_HIP(c, hipStreamSynchronize(s.stream));
         PP_HIP(c, hipFree(b.p));
@@ -174,7 +204,8 @@ void slot_free(Slot &s)
         (void)hipStreamSynchronize(s.stream);
     for (DevBuf *b : {&s.scratch, &s.jobbuf, &s.cnbuf, &s.kbuf})
         if (b->p) {
-            (void)hipFree(b->p);
+            if (!s.fixed)
+                (void)hipFree(b->p);
             b->p = nullptr;
             b->cap = 0;
         }
@@ -625,6 +656,8 @@ pp_status get_sublist(pp_ctx *c, Slot &s, pp_cloud *cl, const CellList *base,
             out = &sb.cl;
             return PP_OK;
         }
+    if (s.fixed)
+        return PP_OK;  // no new list without allocating: out keeps the full list
     cl->subs.emplace_back();
     pp_cloud::SubList &sb = cl->subs.back();
     std::memcpy(sb.mask, mask, 32);
@@ -975,6 +1008,15 @@ pp_status pick_celllist(pp_ctx *c, Slot &s, pp_cloud *cl, const JobPlan &p,
         }
         return out ? PP_OK : fail(c, PP_ERR_INVALID_ARG, "resident cloud without a cell list");
     }
+    if (s.fixed) {
+        if (!cl->lists.empty() && cl->lists[0].mem && cl->lists[0].w == (float)want) {
+            out = &cl->lists[0];
+            return PP_OK;
+        }
+        return fail(c, PP_ERR_INVALID_ARG,
+                    "the no-allocation entry point needs a resident cloud (uploaded with a "
+                    "cell width) or one whose cell list this job's width already built");
+    }
     if (cl->lists.empty())
         cl->lists.emplace_back();
     CellList &L = cl->lists[0];
@@ -1006,6 +1048,7 @@ pp_status run_jobs(pp_ctx *c, Slot &sl, int njobs, const pp_cloud *const *clouds
                    const pp_job *jobs, pp_hist_out *outs, Shape force = Shape());
 
 constexpr size_t kKbChunkBytes = size_t(1) << 30;  // k-th neighbour buffer of one launch chunk
+constexpr int kNoAllocTimedLaunches = 64;          // query launches the fixed-arena slot times
 constexpr int kReduceRows = 32768;                  // rows per reduction work unit
 
 // One query launch over the global tile range [g0, g1).
@@ -1150,7 +1193,16 @@ pp_status launch_group(pp_ctx *c, Slot &sl, std::vector<JobRun> &runs, const std
     // launch chunks: the k-th neighbour categories of a chunk's tiles must
     // fit the chunk buffer
     std::vector<KChunk> chunks;
-    plan_chunks(plans, tile_off, T, kKbChunkBytes, chunks);
+    size_t kb_budget = kKbChunkBytes;
+    if (sl.fixed) {
+        // what the fixed arena has left after the descriptors' reserve (or
+        // the k-th neighbour buffer already carved, when that is more)
+        const size_t used = (sl.fixed_off + 255) & ~(size_t)255;
+        const size_t room =
+            sl.fixed_cap > used + kFixedJobbufReserve ? sl.fixed_cap - used - kFixedJobbufReserve : 0;
+        kb_budget = std::min(kb_budget, std::max(room, sl.kbuf.cap > 256 ? sl.kbuf.cap - 256 : 0));
+    }
+    plan_chunks(plans, tile_off, T, kb_budget, chunks);
     const size_t nch = chunks.size();
     size_t kb_need = 0;
     for (const KChunk &ch : chunks)
@@ -1266,8 +1318,10 @@ pp_status launch_group(pp_ctx *c, Slot &sl, std::vector<JobRun> &runs, const std
                       ? std::max<uint32_t>(1u, std::min<uint32_t>(
                                                    4u, (ch.g1 - ch.g0) / ((uint32_t)ql.grid * 16u)))
                       : 1u;
-        if (!capture && le == hipSuccess) {
-            // time the query launch alone (not its reduction)
+        const bool timed = !capture && (!sl.fixed || sl.qev.size() >= 2 * (sl.nq + 1));
+        if (timed && le == hipSuccess) {
+            // time the query launch alone (not its reduction); a fixed-arena
+            // slot uses the events made when it was set up
             while (sl.qev.size() < 2 * (sl.nq + 1)) {
                 hipEvent_t e = nullptr;
                 le = hipEventCreate(&e);
@@ -1280,7 +1334,7 @@ pp_status launch_group(pp_ctx *c, Slot &sl, std::vector<JobRun> &runs, const std
         }
         if (le == hipSuccess)
             le = launch_query(ql, sl.stream);
-        if (!capture && le == hipSuccess) {
+        if (timed && le == hipSuccess) {
             le = hipEventRecord(sl.qev[2 * sl.nq + 1], sl.stream);
             ++sl.nq;
         }
@@ -1374,7 +1428,7 @@ pp_status run_jobs(pp_ctx *c, Slot &sl, int njobs, const pp_cloud *const *clouds
                 jT = force.T;
             if (force.r > 0)
                 p.r = force.r;
-        } else if (tuneT < 0) {
+        } else if (tuneT < 0 && !sl.fixed) {
             const long b = tune_bucket(cand_density(clouds[i], p), p);
             bool have;
             {
@@ -1864,9 +1918,20 @@ pp_status pp_spatstat_create(int32_t device, pp_ctx **out)
     c->device = device;
     c->cu_count = prop.multiProcessorCount;
     c->lds_per_block = prop.sharedMemPerBlock;
-    if (slot_init(c.get(), c->main) != PP_OK) {
+    if (slot_init(c.get(), c->main) != PP_OK || slot_init(c.get(), c->noalloc) != PP_OK) {
         slot_free(c->main);
+        slot_free(c->noalloc);
         return PP_ERR_HIP;
+    }
+    // event pairs for the no-allocation entry point's query launches, made now
+    for (int k = 0; k < 2 * kNoAllocTimedLaunches; ++k) {
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) {
+            slot_free(c->main);
+            slot_free(c->noalloc);
+            return PP_ERR_HIP;
+        }
+        c->noalloc.qev.push_back(e);
     }
     *out = c.release();
     return PP_OK;
@@ -1878,6 +1943,7 @@ pp_status pp_spatstat_destroy(pp_ctx *c)
         return PP_ERR_INVALID_ARG;
     (void)hipSetDevice(c->device);
     slot_free(c->main);
+    slot_free(c->noalloc);
     delete c;
     return PP_OK;
 }
@@ -2020,6 +2086,31 @@ pp_status pp_spatstat_run(pp_ctx *c, const pp_cloud *cl, const pp_job *job, pp_h
     if (!c || !cl || !job || !out)
         return fail(c, PP_ERR_INVALID_ARG, "NULL argument");
     return run_jobs(c, c->main, 1, &cl, job, out);
+}
+
+pp_status pp_spatstat_run_noalloc(pp_ctx *c, const pp_cloud *cl, const pp_job *job,
+                                  pp_hist_out *out)
+{
+    if (!c || !cl || !job || !out)
+        return fail(c, PP_ERR_INVALID_ARG, "NULL argument");
+    PP_HIP(c, hipSetDevice(c->device));
+    void *base = nullptr;
+    size_t bytes = 0;
+    PP_HIP(c, noalloc_scratch(c->device, &base, &bytes));
+    // one device copy of the scratch: calls on one device take turns
+    static std::mutex dev_mu[kMaxDevices];
+    std::lock_guard<std::mutex> lk(dev_mu[c->device]);
+    Slot &sl = c->noalloc;
+    for (DevBuf *b : {&sl.scratch, &sl.jobbuf, &sl.cnbuf, &sl.kbuf})
+        *b = DevBuf();
+    sl.fixed = (char *)base;
+    sl.fixed_cap = bytes;
+    sl.fixed_off = 0;
+    sl.fixed_last = nullptr;
+    const pp_status st = run_jobs(c, sl, 1, &cl, job, out);
+    for (DevBuf *b : {&sl.scratch, &sl.jobbuf, &sl.cnbuf, &sl.kbuf})
+        *b = DevBuf();
+    return st;
 }
 
 pp_status pp_batch_run(pp_ctx *c, int32_t njobs, const pp_cloud *const *clouds, const pp_job *jobs,

@@ -149,4 +149,11 @@ struct NpcArgs {
 };
 hipError_t launch_npcorr(const NpcArgs &a, hipStream_t s);
 
+// The fixed scratch of the no-allocation entry point (pp_noalloc.hip): the
+// device address of this device's copy (looked up once per device) and its
+// size.
+constexpr size_t kNoAllocBytes = size_t(256) << 20;
+constexpr int kMaxDevices = 64;
+hipError_t noalloc_scratch(int device, void **ptr, size_t *bytes);
+
 } // namespace pp
