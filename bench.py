@@ -234,7 +234,7 @@ def busy_report(trace_dir: str) -> str:
                 continue
             ks, ke = max(ks, s), min(ke, e)
             n += 1
-            short = kn.split("(")[0].replace("void ", "")
+            short = kn.replace("(anonymous namespace)::", "").split("(")[0].replace("void ", "")
             per[short] = per.get(short, 0) + (ke - ks)
             if cur_e is None or ks > cur_e:
                 if cur_e is not None:

@@ -292,10 +292,12 @@ pp_status pp_set_graph_launch(pp_ctx *ctx, int32_t on);
 
 /* ---- serving: a request loop over resident clouds (continuous batching) ----
  * Requests are queued by pp_serve_submit and admitted by their device-memory
- * footprint against hbm_budget_bytes; each of `slots` workers takes every
- * admitted request waiting at that moment (up to max_batch_jobs) and runs them
- * as one batch on its own HIP stream, so batches formed while others run
- * overlap on the device.  Clouds must be resident (uploaded with a cell width).
+ * footprint against hbm_budget_bytes; each of `slots` workers takes the
+ * admitted requests waiting at that moment, in arrival order (up to
+ * max_batch_jobs and about 20 ms of estimated query work; a larger request
+ * runs alone), and runs them as one batch on its own HIP stream: a batch's
+ * preparation overlaps on the device with the query of the batch before it,
+ * whose query launches it then follows (queries keep arrival order).  Clouds must be resident (uploaded with a cell width).
  * The caller keeps each request's pp_job arrays and pp_hist_out buffers alive
  * until its ticket is reported done. */
 typedef struct pp_serve_opts {

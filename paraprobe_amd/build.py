@@ -69,7 +69,7 @@ def build_spatstat() -> str:
     tmp = os.path.join(PKG, "_build")
     os.makedirs(tmp, exist_ok=True)
     for src in ("pp_kernels.hip", "pp_surfdist.hip", "pp_io.hip", "pp_npcorr.hip",
-                "pp_noalloc.hip", "pp_capi.cpp", "pp_binning.cpp"):
+                "pp_noalloc.hip", "pp_prep.hip", "pp_capi.cpp", "pp_binning.cpp"):
         obj = os.path.join(tmp, src + ".o")
         lang = ["-x", "hip"] if src.endswith(".hip") else []
         _run([HIPCC, *HIP_FLAGS, *lang, "-I", INCLUDE, "-c", os.path.join(CSRC, src), "-o", obj])

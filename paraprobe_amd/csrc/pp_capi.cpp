@@ -124,10 +124,14 @@ struct pp_cloud {
     };
     std::list<SubList> subs;
     std::mutex sub_mu;
+    double occupied = 0;  // nm^3 of the coarse occupancy grid's cells that hold ions
     double density() const
     {
-        // ions per nm^3 inside the tip: the bounding box holds the tip with
-        // roughly two thirds filled (cylinders pi/4, frusta ~0.6)
+        // ions per nm^3 inside the tip: over the cells that hold ions when
+        // measured at upload, else the bounding box taken as two thirds full
+        // (cylinders pi/4, frusta ~0.6)
+        if (occupied > 0)
+            return (double)n / occupied;
         double v = 1.0;
         for (int a = 0; a < 3; ++a)
             v *= std::max(1e-3, (double)hi[a] - (double)lo[a]);
@@ -679,14 +683,13 @@ struct JobRun {
     const CellList *cand = nullptr;  // candidates: cl or a typed list on its grid
     std::vector<Row> rows;
     size_t ncells = 0;
-    // device pointers (carved from the slot's scratch arena)
-    // flags: the hybrid mode's rerun marks; sflags: one slot's selection;
-    // maxcand: [0] largest stencil candidate count, [1] centers, [2] tiles
-    uint32_t *flags = nullptr, *sflags = nullptr, *cpos = nullptr, *centers = nullptr,
-             *ntile = nullptr, *toff = nullptr, *ws = nullptr, *tables = nullptr,
-             *maxcand = nullptr;
-    const uint32_t *tile_src = nullptr;  // center selection of the current tiles
+    // device pointers: the hybrid mode's rerun marks (per sorted ion) and the
+    // tables, carved from the slot's scratch arena; centers and tiles point
+    // into the lists of the run (prep_run)
+    uint32_t *flags = nullptr, *tables = nullptr;
+    const uint32_t *centers = nullptr;
     Tile *tiles = nullptr;
+    int T_req = 8;  // centers per tile of this job's tiles
     unsigned long long *P = nullptr, *stats = nullptr;
     uint32_t *K = nullptr, *N1 = nullptr, *CN = nullptr;
     int cn_cap = 0;
@@ -694,8 +697,8 @@ struct JobRun {
     size_t o_edges = 0, o_envoff = 0, o_trgslot = 0, o_ttrg = 0, o_tenv = 0, o_kidx = 0,
            o_tntask = 0, o_ttasks = 0, o_kinfo = 0, o_ckoff = 0, o_taskck = 0, o_rows = 0,
            tables_words = 0;
-    // read back after prep
-    uint32_t h_counts[4] = {0, 0, 0, 0};  // considered, tiles, maxcand, pad
+    // read back after prep: centers, tiles, largest stencil candidate count
+    uint32_t h_counts[4] = {0, 0, 0, 0};
     uint32_t considered = 0, rerun = 0;   // centers of the job, of its full-range rerun
     // in-ball pairs counted by the kernel; otherwise every candidate is a
     // neighbour type of the RDF, whose pair counts sum to the same total
@@ -705,19 +708,10 @@ struct JobRun {
     JobDev dev{};
 };
 
-void layout_job(JobRun &r, Carver &m)
+// Word offsets of a job's device tables (the size in tables_words).
+void layout_tables(JobRun &r)
 {
     const JobPlan &p = r.plan;
-    const uint64_t n = r.cloud->n;
-    const uint64_t nc = r.ncells;
-    r.flags = m.take<uint32_t>(n);
-    r.sflags = m.take<uint32_t>(n);
-    r.cpos = m.take<uint32_t>(n + 1);
-    r.centers = m.take<uint32_t>(n);
-    r.ntile = m.take<uint32_t>(nc);
-    r.toff = m.take<uint32_t>(nc + 1);
-    r.ws = m.take<uint32_t>(scan_workspace_words(std::max<uint64_t>(n, nc)));
-    r.tiles = m.take<Tile>(n + 1);  // at most one tile per considered center
     size_t o = 0;
     r.o_edges = o;
     o += p.C;
@@ -743,9 +737,13 @@ void layout_job(JobRun &r, Carver &m)
     o += std::max(p.ntasks, 1);
     r.o_rows = (o + 1) & ~(size_t)1;
     o = r.o_rows + 2 * r.rows.size();
-    r.tables_words = o;
-    r.tables = m.take<uint32_t>(o);
-    r.maxcand = m.take<uint32_t>(4);
+    r.tables_words = (o + 63) & ~(size_t)63;
+}
+
+// A job's counters (cleared together with every other job's of the run).
+void layout_outputs(JobRun &r, Carver &m)
+{
+    const JobPlan &p = r.plan;
     r.P = m.take<unsigned long long>((size_t)std::max(p.ntrg * p.E * p.C, 1));
     r.stats = m.take<unsigned long long>(8);  // [4..7]: the full-range rerun's (unused)
     r.K = m.take<uint32_t>((size_t)std::max(p.ntasks * p.nku * p.C, 1));
@@ -753,41 +751,59 @@ void layout_job(JobRun &r, Carver &m)
     // CN is carved later: its size follows from the stencil bound
 }
 
-// Center list and tiles of a job, one target-type slot after the other
-// (every tile holds centers of one type), from the considered ions or from
-// the flags in r.tile_src; reads back centers, tiles and the candidate bound.
-pp_status prep_tiles(pp_ctx *c, Slot &sl, JobRun &r, int T)
+// The lists one preparation of a run builds (sized for every job of the
+// run; a rerun of some of them fits too).
+struct PrepArena {
+    uint64_t E = 0, Ec = 0, nion = 0;
+    int nseg = 0, njobs = 0;
+    uint32_t *flags = nullptr, *cpos = nullptr, *centers = nullptr, *ntile = nullptr,
+             *toff = nullptr, *ws = nullptr;
+    Tile *tiles = nullptr;
+    PrepSeg *segs = nullptr;
+    int32_t *jseg = nullptr;
+    PrepJobStat *js = nullptr;
+};
+
+void size_prep(PrepArena &a, const std::vector<JobRun> &runs)
 {
-    const JobPlan &p = r.plan;
-    const uint64_t n = r.cloud->n;
-    const uint64_t nc = r.ncells;
-    hipStream_t s = sl.stream;
-    const int32_t *d_trgslot = (const int32_t *)(r.tables + r.o_trgslot);
-    const Row *d_rows = (const Row *)(r.tables + r.o_rows);
-    PP_HIP(c, hipMemsetAsync(r.maxcand, 0, 16, s));
-    for (int slot = 0; slot < p.ntrg; ++slot) {
-        PP_HIP(c, launch_flags(r.cl->pos, r.cl->s2, n, d_trgslot, p.R2, slot, r.tile_src,
-                               r.sflags, s));
-        PP_HIP(c, scan_exclusive(r.sflags, r.cpos, n, r.ws, s));
-        PP_HIP(c, launch_compact(r.sflags, r.cpos, n, r.maxcand, r.centers, s));
-        PP_HIP(c, launch_tilecount(r.cl->cellstart, r.cpos, nc, T, r.ntile, s));
-        PP_HIP(c, scan_exclusive(r.ntile, r.toff, nc, r.ws, s));
-        PP_HIP(c, launch_tilewrite(r.cl->cellstart, r.cpos, r.toff, nc, T, slot, r.cl->g, d_rows,
-                                   (int)r.rows.size(), r.tiles, r.maxcand, s));
-        PP_HIP(c, launch_bump(r.cpos, n, r.toff, nc, r.maxcand, s));
+    a.E = a.Ec = a.nion = 0;
+    a.nseg = 0;
+    a.njobs = (int)runs.size();
+    for (const JobRun &r : runs) {
+        const uint64_t k = (uint64_t)r.plan.ntrg;
+        a.E += (uint64_t)r.cloud->n * k;
+        a.Ec += (uint64_t)r.ncells * k;
+        a.nion += r.cloud->n;
+        a.nseg += r.plan.ntrg;
     }
-    PP_HIP(c, hipMemcpyAsync(&r.h_counts[0], r.maxcand + 1, 4, hipMemcpyDeviceToHost, s));
-    PP_HIP(c, hipMemcpyAsync(&r.h_counts[1], r.maxcand + 2, 4, hipMemcpyDeviceToHost, s));
-    PP_HIP(c, hipMemcpyAsync(&r.h_counts[2], r.maxcand, 4, hipMemcpyDeviceToHost, s));
-    return PP_OK;
 }
 
-pp_status prep_job(pp_ctx *c, Slot &sl, JobRun &r, int T)
+void layout_prep(PrepArena &a, Carver &m)
+{
+    a.flags = m.take<uint32_t>(a.E + 1);
+    a.cpos = m.take<uint32_t>(a.E + 1);
+    a.centers = m.take<uint32_t>(a.nion + 1);
+    a.ntile = m.take<uint32_t>(a.Ec + 1);
+    a.toff = m.take<uint32_t>(a.Ec + 1);
+    a.ws = m.take<uint32_t>(prep_workspace_words(a.E, a.Ec));
+    a.tiles = m.take<Tile>(a.nion + 1);  // at most one tile per center
+    a.segs = m.take<PrepSeg>((size_t)std::max(a.nseg, 1));
+    a.jseg = m.take<int32_t>((size_t)a.njobs + 1);
+    a.js = m.take<PrepJobStat>((size_t)std::max(a.njobs, 1));
+}
+
+// Bytes of one job's share of a run's scratch (admission control).
+size_t prep_bytes(uint64_t n, uint64_t nc, int ntrg)
+{
+    const uint64_t k = (uint64_t)std::max(ntrg, 1);
+    return (size_t)(n * k * 8 + n * 4 + n * 16 + nc * k * 8 +
+                    4 * prep_workspace_words(n * k, nc * k) + 4096);
+}
+
+// The host image of a job's device tables.
+void stage_tables(const JobRun &r, uint32_t *h)
 {
     const JobPlan &p = r.plan;
-    hipStream_t s = sl.stream;
-    std::vector<uint32_t> &h = r.h_tables;  // lives until the job's results are back
-    h.assign(r.tables_words, 0u);
     std::memcpy(&h[r.o_edges], p.et.edges.data(), p.C * 4);
     std::memcpy(&h[r.o_envoff], p.envoff, 256 * 4);
     std::memcpy(&h[r.o_trgslot], p.trgslot, 256 * 4);
@@ -807,13 +823,88 @@ pp_status prep_job(pp_ctx *c, Slot &sl, JobRun &r, int T)
         std::memcpy(&h[r.o_taskck], p.task_ck.data(), p.task_ck.size() * 4);
     if (!r.rows.empty())
         std::memcpy(&h[r.o_rows], r.rows.data(), r.rows.size() * sizeof(Row));
-    PP_HIP(c, hipMemcpyAsync(r.tables, h.data(), h.size() * 4, hipMemcpyHostToDevice, s));
-    PP_HIP(c, hipMemsetAsync(r.P, 0, sizeof(unsigned long long) * std::max(p.ntrg * p.E * p.C, 1), s));
-    PP_HIP(c, hipMemsetAsync(r.stats, 0, sizeof(unsigned long long) * 8, s));
-    PP_HIP(c, hipMemsetAsync(r.K, 0, 4 * (size_t)std::max(p.ntasks * p.nku * p.C, 1), s));
-    PP_HIP(c, hipMemsetAsync(r.N1, 0, 4 * (size_t)std::max(p.ntasks * p.C, 1), s));
-    r.tile_src = nullptr;
-    return prep_tiles(c, sl, r, T);
+}
+
+// Center lists and tiles of the jobs `ids` in one pass (pp_prep.hip): per
+// job one segment per center type (every tile holds centers of one type),
+// selected by the considered rule, or, for the full-range rerun, by the
+// job's rerun marks.  Reads back each job's center and tile counts and its
+// stencil candidate bound; the lists stay valid until the next prep_run.
+pp_status prep_run(pp_ctx *c, Slot &sl, std::vector<JobRun> &runs, const std::vector<int> &ids,
+                   PrepArena &a, bool rerun)
+{
+    const int nj = (int)ids.size();
+    if (nj == 0)
+        return PP_OK;
+    std::vector<PrepSeg> segs;
+    std::vector<int32_t> jseg(nj + 1, 0);
+    uint64_t E = 0, Ec = 0;
+    for (int q = 0; q < nj; ++q) {
+        JobRun &r = runs[ids[q]];
+        const JobPlan &p = r.plan;
+        jseg[q] = (int32_t)segs.size();
+        for (int slot = 0; slot < p.ntrg; ++slot) {
+            PrepSeg g{};
+            g.pos = r.cl->pos;
+            g.s2 = r.cl->s2;
+            g.src = rerun ? r.flags : nullptr;
+            g.trgslot = (const int32_t *)(r.tables + r.o_trgslot);
+            g.cellstart = r.cl->cellstart;
+            g.rows = (const Row *)(r.tables + r.o_rows);
+            g.g = r.cl->g;
+            g.ion0 = E;
+            g.cell0 = Ec;
+            g.n = (uint32_t)r.cloud->n;
+            g.nc = (uint32_t)r.ncells;
+            g.nrows = (int)r.rows.size();
+            g.R2 = p.R2;
+            g.slot = slot;
+            g.T = r.T_req;
+            g.job = q;
+            segs.push_back(g);
+            E += r.cloud->n;
+            Ec += r.ncells;
+        }
+    }
+    jseg[nj] = (int32_t)segs.size();
+    if (E > a.E || Ec > a.Ec || (int)segs.size() > a.nseg || nj > a.njobs)
+        return fail(c, PP_ERR_HIP, "internal: preparation larger than its arena");
+    hipStream_t s = sl.stream;
+    if (!segs.empty())
+        PP_HIP(c, hipMemcpyAsync(a.segs, segs.data(), segs.size() * sizeof(PrepSeg),
+                                 hipMemcpyHostToDevice, s));
+    PP_HIP(c, hipMemcpyAsync(a.jseg, jseg.data(), jseg.size() * 4, hipMemcpyHostToDevice, s));
+    PrepLaunch L;
+    L.segs = a.segs;
+    L.nseg = (int)segs.size();
+    L.jseg = a.jseg;
+    L.njobs = nj;
+    L.E = E;
+    L.Ec = Ec;
+    L.flags = a.flags;
+    L.cpos = a.cpos;
+    L.centers = a.centers;
+    L.ntile = a.ntile;
+    L.toff = a.toff;
+    L.ws = a.ws;
+    L.tiles = a.tiles;
+    L.js = a.js;
+    std::vector<PrepJobStat> js(nj);
+    if (L.nseg > 0) {
+        PP_HIP(c, launch_prep(L, s));
+        PP_HIP(c, hipMemcpyAsync(js.data(), a.js, nj * sizeof(PrepJobStat), hipMemcpyDeviceToHost,
+                                 s));
+    }
+    PP_HIP(c, hipStreamSynchronize(s));  // segs and jseg are host memory of this frame
+    for (int q = 0; q < nj; ++q) {
+        JobRun &r = runs[ids[q]];
+        r.h_counts[0] = js[q].centers;
+        r.h_counts[1] = js[q].tiles;
+        r.h_counts[2] = js[q].maxcand;
+        r.centers = a.centers;
+        r.tiles = a.tiles + js[q].tile0;
+    }
+    return PP_OK;
 }
 
 void fill_jobdev(JobRun &r)
@@ -1044,8 +1135,20 @@ struct Shape {
     int r = 0;  // 0: choose
 };
 
+// Runs that share the device through several streams (the serving loop's
+// slots) take turns for their query launches: each waits, on the device, for
+// the query launches of the run before it, so that a run's preparation
+// overlaps the query of the run before while the queries themselves keep
+// arrival order instead of interleaving their launch chunks.
+struct LaunchOrder {
+    std::mutex mu;
+    hipEvent_t ev = nullptr;
+    bool armed = false;
+};
+
 pp_status run_jobs(pp_ctx *c, Slot &sl, int njobs, const pp_cloud *const *clouds,
-                   const pp_job *jobs, pp_hist_out *outs, Shape force = Shape());
+                   const pp_job *jobs, pp_hist_out *outs, Shape force = Shape(),
+                   LaunchOrder *order = nullptr);
 
 constexpr size_t kKbChunkBytes = size_t(1) << 30;  // k-th neighbour buffer of one launch chunk
 constexpr int kNoAllocTimedLaunches = 64;          // query launches the fixed-arena slot times
@@ -1140,10 +1243,10 @@ struct Keep {
     }
 };
 
-// One query launch group: the jobs `ids` (one kernel mode) with their device
-// descriptors `base`, as launch chunks, each followed by the reduction of its
-// k-th neighbour categories.  May halve T (rebuilding the group's tiles) to
-// fit the LDS; returns the T used.  With keep.graph the group's kernel
+// One query launch group: the jobs `ids` (one kernel mode, tiles of T
+// centers) with their device descriptors `base`, as launch chunks, each
+// followed by the reduction of its k-th neighbour categories.  The group's
+// LDS layout must fit (query_lds_bytes).  With keep.graph the group's kernel
 // launches (queue clears, query chunks, reductions; the descriptors are
 // copied before) are captured into one HIP graph, which is launched once.
 pp_status launch_group(pp_ctx *c, Slot &sl, std::vector<JobRun> &runs, const std::vector<int> &ids,
@@ -1167,18 +1270,7 @@ pp_status launch_group(pp_ctx *c, Slot &sl, std::vector<JobRun> &runs, const std
         maxcand = std::max(maxcand, r.h_counts[2]);
     }
     const int c16 = maxcand < 65535u ? 1 : 0;
-    size_t lds = query_lds_bytes(T, c16, mode, maxC, maxE, maxTrg, maxK, maxRows, maxHK);
-    while (lds > c->lds_per_block && T > 8) {
-        // smaller tiles: rebuild the tile lists
-        T /= 2;
-        for (int q = 0; q < nj; ++q) {
-            pp_status st = prep_tiles(c, sl, runs[ids[q]], T);
-            if (st != PP_OK)
-                return st;
-        }
-        PP_HIP(c, hipStreamSynchronize(sl.stream));
-        lds = query_lds_bytes(T, c16, mode, maxC, maxE, maxTrg, maxK, maxRows, maxHK);
-    }
+    const size_t lds = query_lds_bytes(T, c16, mode, maxC, maxE, maxTrg, maxK, maxRows, maxHK);
     if (lds > c->lds_per_block)
         return fail(c, PP_ERR_UNSUPPORTED, "histograms too large for LDS (bins x types x tile)");
     std::vector<uint32_t> tile_off(nj + 1, 0);
@@ -1396,7 +1488,7 @@ pp_status autotune(pp_ctx *c, Slot &sl, const pp_cloud *cl, const pp_job *job, c
 }
 
 pp_status run_jobs(pp_ctx *c, Slot &sl, int njobs, const pp_cloud *const *clouds,
-                   const pp_job *jobs, pp_hist_out *outs, Shape force)
+                   const pp_job *jobs, pp_hist_out *outs, Shape force, LaunchOrder *order)
 {
     auto t_start = std::chrono::steady_clock::now();
     if (!outs)
@@ -1412,7 +1504,6 @@ pp_status run_jobs(pp_ctx *c, Slot &sl, int njobs, const pp_cloud *const *clouds
         knn_factor = c->knn_factor;
         graph_launch = c->graph_launch;
     }
-    int T = 0;
     for (int i = 0; i < njobs; ++i) {
         if (!clouds[i])
             return fail(c, PP_ERR_INVALID_ARG, "cloud is NULL");
@@ -1449,7 +1540,7 @@ pp_status run_jobs(pp_ctx *c, Slot &sl, int njobs, const pp_cloud *const *clouds
             if (tuneR > 0)
                 p.r = tuneR;
         }
-        T = std::max(T, jT);
+        runs[i].T_req = jT;
         plan_hybrid(clouds[i], p, knn_factor);
     }
     PP_HIP(c, hipSetDevice(c->device));
@@ -1502,86 +1593,200 @@ pp_status run_jobs(pp_ctx *c, Slot &sl, int njobs, const pp_cloud *const *clouds
             return fail(c, PP_ERR_UNSUPPORTED, "stencil too large (cell width far below Rmax)");
         runs[i].ncells = runs[i].cl->g.ncells();
     }
+    struct Back {
+        std::vector<unsigned long long> P;
+        std::vector<uint32_t> K, N1, CN;
+        unsigned long long stats[4];
+    };
+    std::vector<Back> back(njobs);
     PP_HIP(c, hipEventRecord(sl.ev[1], sl.stream));
-    // job arenas (after the cell lists: a build may have used the scratch)
+    // centers per tile: the job's request, halved while its LDS layout with
+    // 32-bit per-center counters does not fit
+    for (int i = 0; i < njobs; ++i) {
+        JobRun &r = runs[i];
+        const JobPlan &p = r.plan;
+        auto need = [&](int t) {
+            return query_lds_bytes(t, 0, p.mode, std::max(p.C, 1), std::max(p.E, 1),
+                                   std::max(p.ntrg, 1), p.kmax, (int)r.rows.size(),
+                                   std::max(p.HK, 1));
+        };
+        while (need(r.T_req) > c->lds_per_block && r.T_req > 8)
+            r.T_req /= 2;
+        if (need(r.T_req) > c->lds_per_block)
+            return fail(c, PP_ERR_UNSUPPORTED, "histograms too large for LDS (bins x types x tile)");
+    }
+    // scratch (after the cell lists: a build may have used it): the counters
+    // of every job together (one clear, one read-back), the tables of every
+    // job together (one upload), each job's rerun marks, the run's lists
+    PrepArena pa;
+    size_prep(pa, runs);
+    size_t tables_words = 0;
+    for (int i = 0; i < njobs; ++i) {
+        layout_tables(runs[i]);
+        tables_words += runs[i].tables_words;
+    }
+    auto carve = [&](Carver &m, char *&out0, char *&out1) {
+        out0 = m.base ? m.base + ((m.off + 255) & ~(size_t)255) : nullptr;
+        for (int i = 0; i < njobs; ++i)
+            layout_outputs(runs[i], m);
+        out1 = m.base ? m.base + m.off : nullptr;
+        uint32_t *tab = m.take<uint32_t>(tables_words);
+        for (int i = 0; i < njobs; ++i) {
+            runs[i].tables = tab;
+            tab = tab ? tab + runs[i].tables_words : nullptr;
+        }
+        for (int i = 0; i < njobs; ++i)
+            runs[i].flags = m.take<uint32_t>(runs[i].cloud->n + 1);
+        layout_prep(pa, m);
+    };
+    char *out0 = nullptr, *out1 = nullptr;
     Carver m0(nullptr);
-    for (int i = 0; i < njobs; ++i)
-        layout_job(runs[i], m0);
+    carve(m0, out0, out1);
     pp_status st = ensure(c, sl, sl.scratch, m0.off + 256);
     if (st != PP_OK)
         return st;
     Carver m(sl.scratch.p);
-    for (int i = 0; i < njobs; ++i)
-        layout_job(runs[i], m);
-    for (int i = 0; i < njobs; ++i) {
-        st = prep_job(c, sl, runs[i], T);
+    carve(m, out0, out1);
+    const size_t out_bytes = (size_t)(out1 - out0);
+    {
+        std::vector<uint32_t> h(tables_words, 0u);
+        size_t o = 0;
+        for (int i = 0; i < njobs; ++i) {
+            stage_tables(runs[i], h.data() + o);
+            o += runs[i].tables_words;
+        }
+        if (tables_words)
+            PP_HIP(c, hipMemcpyAsync(runs[0].tables, h.data(), tables_words * 4,
+                                     hipMemcpyHostToDevice, sl.stream));
+        PP_HIP(c, hipMemsetAsync(out0, 0, out_bytes, sl.stream));
+        std::vector<int> all(njobs);
+        for (int i = 0; i < njobs; ++i)
+            all[i] = i;
+        st = prep_run(c, sl, runs, all, pa, false);  // synchronises: h may go
         if (st != PP_OK)
             return st;
     }
-    PP_HIP(c, hipStreamSynchronize(sl.stream));
     PP_HIP(c, hipEventRecord(sl.ev[2], sl.stream));
+    size_t cn_words = 0;
     {
         // CNTNB counters: no neighbour count exceeds the largest stencil candidate count
-        size_t words = 0;
         for (int i = 0; i < njobs; ++i) {
             JobRun &r = runs[i];
             r.cn_cap = (r.plan.families & PP_FAM_CNTNB) ? (int)r.h_counts[2] + 1 : 1;
-            words += ((size_t)std::max(r.plan.ntasks, 1) * (size_t)r.cn_cap + 63) & ~(size_t)63;
+            cn_words += ((size_t)std::max(r.plan.ntasks, 1) * (size_t)r.cn_cap + 63) & ~(size_t)63;
         }
-        st = ensure(c, sl, sl.cnbuf, words * 4 + 256);
+        st = ensure(c, sl, sl.cnbuf, cn_words * 4 + 256);
         if (st != PP_OK)
             return st;
+        PP_HIP(c, hipMemsetAsync(sl.cnbuf.p, 0, cn_words * 4, sl.stream));
         size_t o = 0;
         for (int i = 0; i < njobs; ++i) {
             JobRun &r = runs[i];
             r.CN = (uint32_t *)sl.cnbuf.p + o;
-            const size_t w = (size_t)std::max(r.plan.ntasks, 1) * (size_t)r.cn_cap;
-            o += (w + 63) & ~(size_t)63;
-            PP_HIP(c, hipMemsetAsync(r.CN, 0, w * 4, sl.stream));
+            o += ((size_t)std::max(r.plan.ntasks, 1) * (size_t)r.cn_cap + 63) & ~(size_t)63;
             fill_jobdev(r);
             r.considered = r.h_counts[0];
         }
     }
-    // one launch group per kernel mode (pair counts, hybrid, full-range per
-    // center), then the full-range rerun of the centers whose kNN range fell
-    // short
-    std::vector<int> g[3];
-    std::vector<JobDev> b[3];
-    for (int i = 0; i < njobs; ++i) {
-        JobRun &r = runs[i];
-        g[r.plan.mode].push_back(i);
-        b[r.plan.mode].push_back(r.dev);
-        if (r.plan.mode == 2 && r.plan.trunc_any)
-            PP_HIP(c, hipMemsetAsync(r.flags, 0, 4 * r.cloud->n, sl.stream));
-    }
-    Keep keep;
-    // graphs for runs of several jobs only: for a single job the graph's
-    // instantiation (~0.1 ms) costs more than its launches
-    keep.graph = graph_launch != 0 && njobs > 1;
-    int Tm[3] = {T, T, T};
-    PP_HIP(c, hipEventRecord(sl.ev[3], sl.stream));
-    for (int mode : {0, 2, 1}) {
-        st = launch_group(c, sl, runs, g[mode], b[mode], mode, Tm[mode], keep);
-        if (st != PP_OK)
-            return st;
-    }
-    const int TA = Tm[2];
+    // launch groups of one kernel mode, tile size and counter width (pair
+    // counts, hybrid, full-range per center); a group whose common LDS
+    // layout does not fit runs job by job.  Then the full-range rerun of the
+    // centers whose kNN range fell short.
+    auto lds_of = [&](const std::vector<int> &ids, int mode, int T, int c16) {
+        int maxC = 1, maxE = 1, maxTrg = 1, maxK = 0, maxRows = 1, maxHK = 1;
+        for (int i : ids) {
+            const JobPlan &p = runs[i].plan;
+            maxC = std::max(maxC, p.C);
+            maxE = std::max(maxE, p.E);
+            maxTrg = std::max(maxTrg, p.ntrg);
+            maxK = std::max(maxK, p.kmax);
+            maxRows = std::max(maxRows, (int)runs[i].rows.size());
+            if (mode == 2)
+                maxHK = std::max(maxHK, p.HK);
+        }
+        return query_lds_bytes(T, c16, mode, maxC, maxE, maxTrg, maxK, maxRows, maxHK);
+    };
+    struct Group {
+        int mode, T, c16;
+        std::vector<int> ids;
+        std::vector<JobDev> devs;
+    };
+    auto form_groups = [&](const std::vector<int> &ids, const std::vector<JobDev> &devs,
+                           const std::vector<int> &modes) {
+        std::map<std::array<int, 3>, Group> by;
+        for (size_t k = 0; k < ids.size(); ++k) {
+            const JobRun &r = runs[ids[k]];
+            const int c16 = r.h_counts[2] < 65535u ? 1 : 0;
+            Group &g = by[{modes[k], r.T_req, c16}];
+            g.mode = modes[k];
+            g.T = r.T_req;
+            g.c16 = c16;
+            g.ids.push_back(ids[k]);
+            g.devs.push_back(devs[k]);
+        }
+        std::vector<Group> out;
+        for (int mode : {0, 2, 1})
+            for (auto &kv : by) {
+                Group &g = kv.second;
+                if (g.mode != mode)
+                    continue;
+                if (g.ids.size() > 1 && lds_of(g.ids, g.mode, g.T, g.c16) > c->lds_per_block) {
+                    for (size_t k = 0; k < g.ids.size(); ++k)
+                        out.push_back(Group{g.mode, g.T, g.c16, {g.ids[k]}, {g.devs[k]}});
+                } else {
+                    out.push_back(std::move(g));
+                }
+            }
+        return out;
+    };
+    std::vector<int> hy;  // hybrid jobs with a kNN range short of Rmax: they may rerun
     {
-        std::vector<int> hy;
-        for (int i : g[2])
-            if (runs[i].plan.trunc_any)
+        std::vector<int> ids, modes;
+        std::vector<JobDev> devs;
+        for (int i = 0; i < njobs; ++i) {
+            JobRun &r = runs[i];
+            ids.push_back(i);
+            devs.push_back(r.dev);
+            modes.push_back(r.plan.mode);
+            if (r.plan.mode == 2 && r.plan.trunc_any) {
+                PP_HIP(c, hipMemsetAsync(r.flags, 0, 4 * r.cloud->n, sl.stream));
                 hy.push_back(i);
+            }
+        }
+        std::unique_lock<std::mutex> olk;
+        if (order) {
+            olk = std::unique_lock<std::mutex>(order->mu);
+            if (order->armed)
+                PP_HIP(c, hipStreamWaitEvent(sl.stream, order->ev, 0));
+        }
+        Keep keep;
+        // graphs for runs of several jobs only: for a single job the graph's
+        // instantiation (~0.1 ms) costs more than its launches
+        keep.graph = graph_launch != 0 && njobs > 1;
+        PP_HIP(c, hipEventRecord(sl.ev[3], sl.stream));
+        for (Group &g : form_groups(ids, devs, modes)) {
+            int T = g.T;
+            st = launch_group(c, sl, runs, g.ids, g.devs, g.mode, T, keep);
+            if (st != PP_OK)
+                return st;
+        }
         if (!hy.empty()) {
+            // the rerun's tiles: the flagged centers, with the largest tile
+            // that fits the full-range layout
             for (int i : hy) {
                 JobRun &r = runs[i];
-                r.tile_src = r.flags;
-                st = prep_tiles(c, sl, r, TA);
-                if (st != PP_OK)
-                    return st;
+                const JobPlan &p = r.plan;
+                while (r.T_req > 8 &&
+                       query_lds_bytes(r.T_req, 0, 1, std::max(p.C, 1), std::max(p.E, 1),
+                                       std::max(p.ntrg, 1), p.kmax, (int)r.rows.size(),
+                                       1) > c->lds_per_block)
+                    r.T_req /= 2;
             }
-            PP_HIP(c, hipStreamSynchronize(sl.stream));
-            std::vector<int> gC;
-            std::vector<JobDev> bC;
+            st = prep_run(c, sl, runs, hy, pa, true);
+            if (st != PP_OK)
+                return st;
+            std::vector<int> ids1, modes1;
+            std::vector<JobDev> devs1;
             for (int i : hy) {
                 JobRun &r = runs[i];
                 r.rerun = r.h_counts[0];
@@ -1592,49 +1797,58 @@ pp_status run_jobs(pp_ctx *c, Slot &sl, int njobs, const pp_cloud *const *clouds
                 d.families &= FAM_MKNN | FAM_1NN;
                 d.percenter = 1;
                 d.stats = r.stats + 4;
-                gC.push_back(i);
-                bC.push_back(d);
+                d.centers = r.centers;
+                d.tiles = r.tiles;
+                ids1.push_back(i);
+                devs1.push_back(d);
+                modes1.push_back(1);
             }
-            int TC = TA;
-            const int ta = TA;
-            st = launch_group(c, sl, runs, gC, bC, 1, TC, keep);
-            if (st != PP_OK)
-                return st;
-            for (int i : gC)
-                runs[i].T_used = ta;  // the job's main launch
+            std::vector<int> tmain(njobs);
+            for (int i : ids1)
+                tmain[i] = runs[i].T_used;
+            for (Group &g : form_groups(ids1, devs1, modes1)) {
+                int T = g.T;
+                st = launch_group(c, sl, runs, g.ids, g.devs, g.mode, T, keep);
+                if (st != PP_OK)
+                    return st;
+            }
+            for (int i : ids1)
+                runs[i].T_used = tmain[i];  // the job's main launch
+        }
+        PP_HIP(c, hipEventRecord(sl.ev[4], sl.stream));
+        if (order) {
+            PP_HIP(c, hipEventRecord(order->ev, sl.stream));
+            order->armed = true;
+            olk.unlock();
+        }
+        // read back (the counters of every job in one copy)
+        std::vector<char> hout(out_bytes);
+        std::vector<uint32_t> hcn(cn_words);
+        if (out_bytes)
+            PP_HIP(c, hipMemcpyAsync(hout.data(), out0, out_bytes, hipMemcpyDeviceToHost,
+                                     sl.stream));
+        if (cn_words)
+            PP_HIP(c, hipMemcpyAsync(hcn.data(), sl.cnbuf.p, cn_words * 4, hipMemcpyDeviceToHost,
+                                     sl.stream));
+        PP_HIP(c, hipStreamSynchronize(sl.stream));
+        for (int i = 0; i < njobs; ++i) {
+            JobRun &r = runs[i];
+            const JobPlan &p = r.plan;
+            Back &b = back[i];
+            auto host = [&](const void *dev) { return hout.data() + ((const char *)dev - out0); };
+            b.P.resize((size_t)std::max(p.ntrg * p.E * p.C, 1));
+            std::memcpy(b.P.data(), host(r.P), b.P.size() * 8);
+            std::memcpy(b.stats, host(r.stats), sizeof(b.stats));
+            b.K.resize((size_t)std::max(p.ntasks * p.nku * p.C, 1));
+            std::memcpy(b.K.data(), host(r.K), b.K.size() * 4);
+            b.N1.resize((size_t)std::max(p.ntasks * p.C, 1));
+            std::memcpy(b.N1.data(), host(r.N1), b.N1.size() * 4);
+            if (p.families & PP_FAM_CNTNB) {
+                const uint32_t *src = hcn.data() + (r.CN - (uint32_t *)sl.cnbuf.p);
+                b.CN.assign(src, src + (size_t)p.ntasks * r.cn_cap);
+            }
         }
     }
-    PP_HIP(c, hipEventRecord(sl.ev[4], sl.stream));
-    // read back
-    struct Back {
-        std::vector<unsigned long long> P;
-        std::vector<uint32_t> K, N1, CN;
-        unsigned long long stats[4];
-    };
-    std::vector<Back> back(njobs);
-    for (int i = 0; i < njobs; ++i) {
-        JobRun &r = runs[i];
-        const JobPlan &p = r.plan;
-        Back &b = back[i];
-        b.P.assign((size_t)std::max(p.ntrg * p.E * p.C, 1), 0ull);
-        b.K.assign((size_t)std::max(p.ntasks * p.nku * p.C, 1), 0u);
-        b.N1.assign((size_t)std::max(p.ntasks * p.C, 1), 0u);
-        PP_HIP(c, hipMemcpyAsync(b.P.data(), r.P, b.P.size() * 8, hipMemcpyDeviceToHost, sl.stream));
-        PP_HIP(c, hipMemcpyAsync(b.stats, r.stats, sizeof(b.stats), hipMemcpyDeviceToHost,
-                                 sl.stream));
-        if (p.families & PP_FAM_MKNN)
-            PP_HIP(c, hipMemcpyAsync(b.K.data(), r.K, b.K.size() * 4, hipMemcpyDeviceToHost,
-                                     sl.stream));
-        if (p.families & PP_FAM_1NN)
-            PP_HIP(c, hipMemcpyAsync(b.N1.data(), r.N1, b.N1.size() * 4, hipMemcpyDeviceToHost,
-                                     sl.stream));
-        if (p.families & PP_FAM_CNTNB) {
-            b.CN.assign((size_t)p.ntasks * r.cn_cap, 0u);
-            PP_HIP(c, hipMemcpyAsync(b.CN.data(), r.CN, b.CN.size() * 4, hipMemcpyDeviceToHost,
-                                     sl.stream));
-        }
-    }
-    PP_HIP(c, hipStreamSynchronize(sl.stream));
     float ms_idx = 0, ms_prep = 0, ms_q = 0;
     PP_HIP(c, hipEventElapsedTime(&ms_idx, sl.ev[0], sl.ev[1]));
     PP_HIP(c, hipEventElapsedTime(&ms_prep, sl.ev[1], sl.ev[2]));
@@ -1691,8 +1905,12 @@ size_t job_footprint(const pp_cloud *cl, const JobPlan &p)
     }
     r.ncells = nc;
     r.rows = make_rows(w, p.R);
+    layout_tables(r);
     Carver m(nullptr);
-    layout_job(r, m);
+    layout_outputs(r, m);
+    m.take<uint32_t>(r.tables_words);
+    m.take<uint32_t>(cl->n + 1);  // rerun marks
+    m.off += prep_bytes(cl->n, nc, p.ntrg);
     size_t cn = (p.families & PP_FAM_CNTNB) ? (size_t)p.ntasks * 65536 * 4 : 0;
     // k-th neighbour chunk buffer: at most one tile per considered center
     const size_t kb = std::min(kKbChunkBytes, (size_t)cl->n * kb_tile_bytes(p, 8) + 1024);
@@ -1762,6 +1980,45 @@ pp_status upload_common(pp_ctx *c, const float *xyz, const uint8_t *type, const 
             if (!std::isfinite(cl->lo[a]) || !std::isfinite(cl->hi[a]))
                 return cleanup(fail(c, PP_ERR_INVALID_ARG, "non-finite coordinates"));
         }
+        // the volume the ions occupy, on a grid of cubic cells over the box
+        // (about 2^18, and at least 8 ions per cell on average, so that a
+        // cell inside the tip is never empty by chance): the density rule of
+        // the cell width needs the tip's density, not the box's
+        {
+            double ext[3], vbox = 1.0;
+            for (int a = 0; a < 3; ++a) {
+                ext[a] = std::max(1e-3, (double)cl->hi[a] - (double)cl->lo[a]);
+                vbox *= ext[a];
+            }
+            const double target = std::min(262144.0, std::max(1.0, (double)n / 8.0));
+            const double w = std::cbrt(vbox / target);
+            OccGrid og{};
+            uint64_t ncell = 1;
+            double vcell = 1.0;
+            for (int a = 0; a < 3; ++a) {
+                // cells of about w that tile the box's extent exactly (a thin
+                // box gets one layer as thick as the box)
+                og.d[a] = (int)std::min(4096.0, std::max(1.0, std::round(ext[a] / w)));
+                og.lo[a] = cl->lo[a];
+                og.inv[a] = (double)og.d[a] / ext[a];
+                vcell *= ext[a] / (double)og.d[a];
+                ncell *= (uint64_t)og.d[a];
+            }
+            if (ensure(c, sl, sl.scratch, ncell + 1024) != PP_OK)
+                return cleanup(PP_ERR_OOM_DEVICE);
+            std::vector<uint8_t> hocc(ncell);
+            uint8_t *occ = (uint8_t *)sl.scratch.p;
+            if (hipMemsetAsync(occ, 0, ncell, sl.stream) != hipSuccess ||
+                launch_occupancy(cl->xyz, n, og, occ, sl.stream) != hipSuccess ||
+                hipMemcpyAsync(hocc.data(), occ, ncell, hipMemcpyDeviceToHost, sl.stream) !=
+                    hipSuccess ||
+                hipStreamSynchronize(sl.stream) != hipSuccess)
+                return cleanup(fail(c, PP_ERR_HIP, "occupancy failed"));
+            uint64_t k = 0;
+            for (uint8_t v : hocc)
+                k += v;
+            cl->occupied = (double)k * vcell;
+        }
     }
     if (cell_width > 0) {
         cl->lists.emplace_back();
@@ -1827,6 +2084,7 @@ struct pp_server {
     size_t budget = 0;
     std::vector<std::unique_ptr<Slot>> slots;
     std::vector<std::thread> workers;
+    LaunchOrder order;
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
     std::deque<Request> pending;
@@ -1881,7 +2139,7 @@ struct pp_server {
                 outs.push_back(*r.out);
             }
             const pp_status st = run_jobs(ctx, sl, (int)batch.size(), cls.data(), jobs.data(),
-                                          outs.data());
+                                          outs.data(), Shape(), &order);
             for (size_t k = 0; k < batch.size(); ++k)
                 *batch[k].out = outs[k];
             std::lock_guard<std::mutex> lk(mu);
@@ -2695,6 +2953,11 @@ pp_status pp_serve_create(pp_ctx *c, const pp_serve_opts *opts, pp_server **out)
             return PP_ERR_HIP;
         }
     }
+    if (hipEventCreateWithFlags(&s->order.ev, hipEventDisableTiming) != hipSuccess) {
+        for (auto &sl : s->slots)
+            slot_free(*sl);
+        return PP_ERR_HIP;
+    }
     for (int i = 0; i < opts->slots; ++i)
         s->workers.emplace_back(&pp_server::worker, s.get(), i);
     *out = s.release();
@@ -2778,6 +3041,8 @@ pp_status pp_serve_destroy(pp_server *s)
     (void)hipSetDevice(s->ctx->device);
     for (auto &sl : s->slots)
         slot_free(*sl);
+    if (s->order.ev)
+        (void)hipEventDestroy(s->order.ev);
     delete s;
     return PP_OK;
 }

@@ -9,13 +9,15 @@
 //     small), a one-digit counting sort (count with atomics, exclusive scan,
 //     scatter) into cell-sorted float4 (x, y, z, type) plus the sorted
 //     ion2surf array;
-//   * per job: considered-center flags, stream compaction one center type
-//     after the other, tiles of up to T centers of one type in one cell;
+//   * per run of jobs (pp_prep.hip): considered-center flags, stream
+//     compaction one (job, center type) after the other, tiles of up to T
+//     centers of one type in one cell;
 //   * query (k_query, three modes): each workgroup pulls tiles from eight
 //     per-XCD tile queues; the tile's stencil rows are cut to the bounding
-//     box of its centers; the four wavefronts split the rows into
-//     64-candidate chunks, each lane holds one candidate and loops over
-//     the T centers held in registers, evaluating the reference's FP32
+//     box of its centers; the four wavefronts split the concatenation of
+//     the rows into 64-candidate chunks (a chunk may span rows), each lane
+//     holds one candidate and loops over the T centers held in registers,
+//     evaluating the reference's FP32
 //     squared distance ((dx*dx + dy*dy) + dz*dz, no contraction) and binning
 //     in-ball pairs with integer LDS atomics: mode 0 into pair counts per
 //     (center type, env type, category) (RDF / RIPK); mode 1 into full-range
@@ -419,6 +421,33 @@ __global__ __launch_bounds__(256) void k_typecount(const uint8_t *__restrict__ t
         atomicAdd(&counts[threadIdx.x], h[threadIdx.x]);
 }
 
+// Occupancy of a coarse grid that tiles the cloud's box (cells of
+// 1 / inv[a] along axis a): cell c is marked when an ion lies in it (several
+// ions store the same 1, no atomics needed).
+__global__ __launch_bounds__(256) void k_occupancy(const float *__restrict__ xyz, uint64_t n,
+                                                   OccGrid g, uint8_t *__restrict__ occ)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        int c[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+            c[a] = cell_coord(xyz[3 * i + a], g.lo[a], g.inv[a], g.d[a]);
+        occ[((uint64_t)c[2] * g.d[1] + c[1]) * g.d[0] + c[0]] = 1u;
+    }
+}
+
+hipError_t launch_occupancy(const float *xyz, uint64_t n, OccGrid g, uint8_t *occ, hipStream_t s)
+{
+    if (n == 0)
+        return hipSuccess;
+    unsigned grid = grid_for(n, 256);
+    if (grid > 8192)
+        grid = 8192;
+    hipLaunchKernelGGL(k_occupancy, dim3(grid), dim3(256), 0, s, xyz, n, g, occ);
+    return hipGetLastError();
+}
+
 hipError_t launch_typecount(const uint8_t *type, uint64_t n, uint32_t *counts, hipStream_t s)
 {
     if (n == 0)
@@ -427,163 +456,6 @@ hipError_t launch_typecount(const uint8_t *type, uint64_t n, uint32_t *counts, h
     if (g > 1024)
         g = 1024;
     hipLaunchKernelGGL(k_typecount, dim3(g), dim3(256), 0, s, type, n, counts);
-    return hipGetLastError();
-}
-
-// ---------------------------------------------------------------------------
-// per-job centers and tiles
-
-// Center selection for one target type slot: considered (type a target of
-// some task and ion2surf >= R^2, src/PARAPROBE_SolverHdl.cpp:3846-3860), or
-// flagged in src when given, and of type slot `slot`.  Center lists and tiles
-// are built slot after slot, so every tile holds centers of one type.
-__global__ __launch_bounds__(256) void k_flags(const float4 *__restrict__ pos,
-                                               const float *__restrict__ s2sorted, uint64_t n,
-                                               const int32_t *__restrict__ trgslot, float R2,
-                                               int slot, const uint32_t *__restrict__ src,
-                                               uint32_t *__restrict__ flags)
-{
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n)
-        return;
-    const uint32_t t = __float_as_uint(pos[i].w) & 0xFFu;
-    const int ts = trgslot[t];
-    const bool sel = src ? src[i] != 0u : (ts >= 0 && s2sorted[i] >= R2);
-    flags[i] = (sel && ts == slot) ? 1u : 0u;
-}
-
-hipError_t launch_flags(const float4 *pos, const float *s2sorted, uint64_t n,
-                        const int32_t *trgslot, float R2, int slot, const uint32_t *src,
-                        uint32_t *flags, hipStream_t s)
-{
-    if (n == 0)
-        return hipSuccess;
-    hipLaunchKernelGGL(k_flags, dim3(grid_for(n, 256)), dim3(256), 0, s, pos, s2sorted, n, trgslot,
-                       R2, slot, src, flags);
-    return hipGetLastError();
-}
-
-// centers[base[1] + cpos[i]] = i for the flagged ions (base[1]: centers of
-// the slots before)
-__global__ __launch_bounds__(256) void k_compact(const uint32_t *__restrict__ flags,
-                                                 const uint32_t *__restrict__ cpos, uint64_t n,
-                                                 const uint32_t *__restrict__ base,
-                                                 uint32_t *__restrict__ centers)
-{
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n && flags[i])
-        centers[base[1] + cpos[i]] = (uint32_t)i;
-}
-
-hipError_t launch_compact(const uint32_t *flags, const uint32_t *cpos, uint64_t n,
-                          const uint32_t *base, uint32_t *centers, hipStream_t s)
-{
-    if (n == 0)
-        return hipSuccess;
-    hipLaunchKernelGGL(k_compact, dim3(grid_for(n, 256)), dim3(256), 0, s, flags, cpos, n, base,
-                       centers);
-    return hipGetLastError();
-}
-
-__global__ __launch_bounds__(256) void k_tilecount(const uint32_t *__restrict__ cellstart,
-                                                   const uint32_t *__restrict__ cpos,
-                                                   uint64_t ncells, int T,
-                                                   uint32_t *__restrict__ ntile)
-{
-    const uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncells)
-        return;
-    const uint32_t lo = cpos[cellstart[c]], hi = cpos[cellstart[c + 1]];
-    ntile[c] = (hi - lo + (uint32_t)T - 1u) / (uint32_t)T;
-}
-
-hipError_t launch_tilecount(const uint32_t *cellstart, const uint32_t *cpos, uint64_t ncells,
-                            int T, uint32_t *ntile, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_tilecount, dim3(grid_for(ncells, 256)), dim3(256), 0, s, cellstart, cpos,
-                       ncells, T, ntile);
-    return hipGetLastError();
-}
-
-// row range of stencil row rw around cell (cx, cy, cz): empty when outside
-__device__ inline void row_range(const Row &rw, int cx, int cy, int cz, const GridGeom &g,
-                                 const uint32_t *__restrict__ cellstart, uint32_t &rs, uint32_t &re)
-{
-    rs = re = 0;
-    const int y = cy + rw.dy, z = cz + rw.dz;
-    if (y < 0 || y >= g.ny || z < 0 || z >= g.nz)
-        return;
-    const int x0 = cx - rw.dxm < 0 ? 0 : cx - rw.dxm;
-    const int x1 = cx + rw.dxm >= g.nx ? g.nx - 1 : cx + rw.dxm;
-    const uint64_t k0 = ((uint64_t)z * g.ny + y) * g.nx + x0;
-    rs = cellstart[k0];
-    re = cellstart[k0 + (uint32_t)(x1 - x0) + 1u];
-}
-
-// Tiles of one slot: base[1] centers and base[2] tiles precede it; base[0]
-// is the largest candidate count of any tile's stencil.
-__global__ __launch_bounds__(256) void k_tilewrite(const uint32_t *__restrict__ cellstart,
-                                                   const uint32_t *__restrict__ cpos,
-                                                   const uint32_t *__restrict__ toff,
-                                                   uint64_t ncells, int T, int slot, GridGeom g,
-                                                   const Row *__restrict__ rows, int nrows,
-                                                   Tile *__restrict__ tiles,
-                                                   uint32_t *__restrict__ base)
-{
-    const uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncells)
-        return;
-    const uint32_t lo = cpos[cellstart[c]], hi = cpos[cellstart[c + 1]];
-    if (hi == lo)
-        return;
-    const uint32_t cnt = hi - lo;
-    const uint32_t cb = base[1], t0 = base[2] + toff[c];
-    for (uint32_t m = 0; m * (uint32_t)T < cnt; ++m) {
-        Tile t;
-        t.key = (uint32_t)c;
-        t.first = cb + lo + m * (uint32_t)T;
-        const uint32_t left = cnt - m * (uint32_t)T;
-        t.count = (uint16_t)(left < (uint32_t)T ? left : (uint32_t)T);
-        t.job = 0;
-        t.pad = (uint32_t)slot;
-        tiles[t0 + m] = t;
-    }
-    // candidates of this cell's stencil bound every center's neighbour count
-    const int cx = (int)(c % (uint64_t)g.nx);
-    const int cy = (int)((c / (uint64_t)g.nx) % (uint64_t)g.ny);
-    const int cz = (int)(c / ((uint64_t)g.nx * (uint64_t)g.ny));
-    uint32_t tot = 0;
-    for (int r = 0; r < nrows; ++r) {
-        uint32_t rs, re;
-        row_range(rows[r], cx, cy, cz, g, cellstart, rs, re);
-        tot += re - rs;
-    }
-    atomicMax(&base[0], tot);
-}
-
-hipError_t launch_tilewrite(const uint32_t *cellstart, const uint32_t *cpos, const uint32_t *toff,
-                            uint64_t ncells, int T, int slot, GridGeom g, const Row *rows,
-                            int nrows, Tile *tiles, uint32_t *base, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_tilewrite, dim3(grid_for(ncells, 256)), dim3(256), 0, s, cellstart, cpos,
-                       toff, ncells, T, slot, g, rows, nrows, tiles, base);
-    return hipGetLastError();
-}
-
-// base[1] += centers of the slot just written (cpos[n]), base[2] += its tiles
-__global__ void k_bump(const uint32_t *__restrict__ cpos, uint64_t n,
-                       const uint32_t *__restrict__ toff, uint64_t ncells, uint32_t *base)
-{
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        base[1] += cpos[n];
-        base[2] += toff[ncells];
-    }
-}
-
-hipError_t launch_bump(const uint32_t *cpos, uint64_t n, const uint32_t *toff, uint64_t ncells,
-                       uint32_t *base, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_bump, dim3(1), dim3(64), 0, s, cpos, n, toff, ncells, base);
     return hipGetLastError();
 }
 
@@ -1467,6 +1339,8 @@ hipError_t launch_query(const QueryLaunch &q, hipStream_t s)
 // tiles_per_unit tiles of one job.  The unit's categories are counted per
 // (task, category) in LDS, then added to the job's K / N1 with one global
 // atomic per non-zero count.
+constexpr uint32_t kRedStage = 1024;  // tiles staged in LDS at a time
+
 __global__ __launch_bounds__(256) void k_kreduce(const JobDev *__restrict__ jobs, int njobs,
                                                  const uint32_t *__restrict__ unit_off, int T,
                                                  uint32_t tpu, int lds_words)
@@ -1492,27 +1366,59 @@ __global__ __launch_bounds__(256) void k_kreduce(const JobDev *__restrict__ jobs
             h[i] = 0u;
         __syncthreads();
     }
-    const uint32_t n = (t1 - t0) * TR;
     const uint16_t *kb = J.KB + ((size_t)t0 * J.nplanes + plane) * TR;
     const size_t pstride = (size_t)J.nplanes * TR;
     const Tile *tiles = J.tiles + J.tile_base + t0;
     const uint32_t lmax = (uint32_t)J.lmax;
-    for (uint32_t e = threadIdx.x; e < n; e += blockDim.x) {
-        const uint32_t tt = e / TR, r = e - tt * TR;
-        const uint32_t c = r / lmax, l = r - c * lmax;
-        // the row's center and task: absent centers of a partial tile and
-        // tasks beyond its center type's were never written
-        const Tile t = tiles[tt];
-        if (c >= (uint32_t)t.count || l >= J.trg_ntask[t.pad])
-            continue;
+    const uint32_t ntl = t1 - t0;
+    // the unit's tiles go through LDS kRedStage at a time, each as its center
+    // count, center type slot and that slot's task count, so that a row needs
+    // one global load (its category) besides its task (count <= 32, slot < 32
+    // and task count <= 64 fit 8 bits each)
+    __shared__ uint32_t s_ti[kRedStage];
+    // one row r = c * lmax + l of staged tile st (unit tile s0 + st): absent
+    // centers of a partial tile and tasks beyond its center type's were never
+    // written, rerun rows hold kKbEmpty
+    auto row = [&](uint32_t s0, uint32_t st, uint32_t r, uint32_t c, uint32_t l) {
+        const uint32_t ti = s_ti[st];
+        if (c >= (ti & 0xFFu) || l >= (ti >> 16))
+            return;
+        const uint32_t tt = s0 + st;
         const uint32_t cat = kb[(size_t)tt * pstride + r];
         if (cat == kKbEmpty)
-            continue;
-        const uint32_t tk = J.trg_tasks[t.pad * lmax + l];
+            return;
+        const uint32_t tk = J.trg_tasks[((ti >> 8) & 0xFFu) * lmax + l];
         if (use_lds)
             atomicAdd(&h[tk * C + cat], 1u);
         else
             atomicAdd(&dst[tk * tstride + cat], 1u);
+    };
+    // thread -> (tile of the step, row) when a tile's rows fit the block: the
+    // divisions once, then the block steps over `per` tiles at a time
+    const bool fits = TR <= blockDim.x;
+    const uint32_t per = fits ? blockDim.x / TR : 1u;
+    const uint32_t dt = fits ? threadIdx.x / TR : 0u;
+    const uint32_t r0 = fits ? threadIdx.x - dt * TR : 0u;
+    const uint32_t c0 = r0 / lmax, l0 = r0 - c0 * lmax;
+    for (uint32_t s0 = 0; s0 < ntl; s0 += kRedStage) {
+        const uint32_t sn = ntl - s0 < kRedStage ? ntl - s0 : kRedStage;
+        __syncthreads();  // the previous stage's rows are done with s_ti
+        for (uint32_t i = threadIdx.x; i < sn; i += blockDim.x) {
+            const Tile t = tiles[s0 + i];
+            s_ti[i] = (uint32_t)t.count | (t.pad << 8) | (J.trg_ntask[t.pad] << 16);
+        }
+        __syncthreads();
+        if (fits) {
+            if (dt < per)
+                for (uint32_t st = dt; st < sn; st += per)
+                    row(s0, st, r0, c0, l0);
+        } else {
+            for (uint32_t st = 0; st < sn; ++st)
+                for (uint32_t r = threadIdx.x; r < TR; r += blockDim.x) {
+                    const uint32_t c = r / lmax;
+                    row(s0, st, r, c, r - c * lmax);
+                }
+        }
     }
     if (use_lds) {
         __syncthreads();

@@ -45,25 +45,56 @@ hipError_t launch_gather(const float *xyz, const uint8_t *type, const float *s2,
                          uint8_t *otype, float *os2, hipStream_t s);
 // Per-type ion counts of a cloud (counts has 256 entries, zeroed by the caller).
 hipError_t launch_typecount(const uint8_t *type, uint64_t n, uint32_t *counts, hipStream_t s);
+// A coarse grid that tiles a box exactly: d[a] cells of 1 / inv[a] along axis a.
+struct OccGrid {
+    double lo[3], inv[3];
+    int d[3];
+};
+// Marks occ[cell] = 1 for every cell of g holding an ion (occ cleared by the
+// caller).
+hipError_t launch_occupancy(const float *xyz, uint64_t n, OccGrid g, uint8_t *occ, hipStream_t s);
 
-// Per job and target-type slot: flag the slot's centers (considered, or
-// flagged in src), compact them after the centers of the slots before
-// (base[1]), count and write the slot's tiles after base[2] tiles (tiles
-// carry the slot), track the largest stencil candidate count in base[0], and
-// advance base[1] / base[2].
-hipError_t launch_flags(const float4 *pos, const float *s2sorted, uint64_t n,
-                        const int32_t *trgslot, float R2, int slot, const uint32_t *src,
-                        uint32_t *flags, hipStream_t s);
-hipError_t launch_compact(const uint32_t *flags, const uint32_t *cpos, uint64_t n,
-                          const uint32_t *base, uint32_t *centers, hipStream_t s);
-hipError_t launch_tilecount(const uint32_t *cellstart, const uint32_t *cpos, uint64_t ncells,
-                            int T, uint32_t *ntile, hipStream_t s);
-hipError_t launch_tilewrite(const uint32_t *cellstart, const uint32_t *cpos,
-                            const uint32_t *toff, uint64_t ncells, int T, int slot,
-                            GridGeom g, const Row *rows, int nrows, Tile *tiles,
-                            uint32_t *base, hipStream_t s);
-hipError_t launch_bump(const uint32_t *cpos, uint64_t n, const uint32_t *toff, uint64_t ncells,
-                       uint32_t *base, hipStream_t s);
+// Centers and tiles of a run of jobs in one pass (pp_prep.hip).  One segment
+// per (job, center type slot), a job's segments consecutive; segment q's ions
+// occupy [ion0, ion0 + n) of the run's ion space and its cells
+// [cell0, cell0 + nc) of the cell space.
+struct PrepSeg {
+    const float4 *pos;         // the job's cell-sorted ions (x, y, z, type bits)
+    const float *s2;           // their ion2surf
+    const uint32_t *src;       // a selection per sorted ion (the rerun), or null
+    const int32_t *trgslot;    // [256] target slot of a type id, or -1
+    const uint32_t *cellstart; // nc + 1 cell offsets
+    const Row *rows;           // the job's stencil rows
+    GridGeom g;
+    uint64_t ion0, cell0;
+    uint32_t n, nc;
+    int nrows;
+    float R2;                  // fl32(R * R): considered iff ion2surf >= R2
+    int slot, T, job;
+};
+// Per job: its centers [center0, center0 + centers) of the run's center list,
+// its tiles [tile0, tile0 + tiles) of the run's tile list, and the largest
+// stencil candidate count of a cell with centers.
+struct PrepJobStat {
+    uint32_t center0, centers, tile0, tiles, maxcand, pad[3];
+};
+struct PrepLaunch {
+    const PrepSeg *segs;       // device [nseg]
+    int nseg;
+    const int32_t *jseg;       // device [njobs + 1]: job j's segments [jseg[j], jseg[j + 1])
+    int njobs;
+    uint64_t E, Ec;            // sizes of the ion and cell spaces
+    uint32_t *flags;           // E
+    uint32_t *cpos;            // E + 1: exclusive scan of flags (center positions)
+    uint32_t *centers;         // the run's center list (job-local sorted ion indices)
+    uint32_t *ntile;           // Ec
+    uint32_t *toff;            // Ec + 1
+    uint32_t *ws;              // prep_workspace_words(E, Ec)
+    Tile *tiles;               // the run's tiles (Tile::first indexes centers, Tile::job the job)
+    PrepJobStat *js;           // device [njobs]
+};
+size_t prep_workspace_words(uint64_t E, uint64_t Ec);
+hipError_t launch_prep(const PrepLaunch &p, hipStream_t s);
 
 // Query launch over all jobs of a launch.
 struct QueryLaunch {
