@@ -360,7 +360,8 @@ def serve_leg(ctx, args):
         gaps = np.random.default_rng(99).exponential(1.0 / rate, size=m) if rate else np.zeros(m)
         arrive = np.cumsum(gaps)
         done, results, status = {}, {}, {}
-        with Server(ctx, slots=args.serve_slots, max_batch_jobs=args.serve_batch) as srv:
+        with Server(ctx, slots=args.serve_slots, max_batch_jobs=args.serve_batch,
+                    batch_work=args.serve_batch_work) as srv:
             start = time.perf_counter()
             tick = {}
             stop = threading.Event()
@@ -422,6 +423,21 @@ def serve_leg(ctx, args):
         c.free()
     p50, p99 = float(np.percentile(lat_ms, 50)), float(np.percentile(lat_ms, 99))
     s50, s99 = float(np.percentile(lat_ms[:S], 50)), float(np.percentile(lat_ms[:S], 99))
+    # query time of each request's call (plain) or batch (served), HIP events
+    # around its query launches: separates GPU service time from queueing
+    srun = np.array([o.ms_query for o in res_open if o is not None])
+    prun = np.array([o.ms_query for o in plain])
+    print(f"# serve: query ms p50/p99, first {S}: served {np.percentile(srun, 50):.1f} / "
+          f"{np.percentile(srun, 99):.1f}, plain {np.percentile(prun, 50):.1f} / "
+          f"{np.percentile(prun, 99):.1f}", file=sys.stderr, flush=True)
+    if args.serve_dump:
+        # per request of the sample: arrival, served and plain latencies, query times
+        with open(args.serve_dump, "w") as fh:
+            json.dump({"arrive_s": arrive.tolist(), "served_ms": lat_ms[:S].tolist(),
+                       "plain_ms": plat.tolist(), "plain_svc_ms": (np.array(svc) * 1e3).tolist(),
+                       "served_query_ms": srun.tolist(), "plain_query_ms": prun.tolist(),
+                       "cloud": [r.cloud for r in stream[:S]],
+                       "rmax": [r.job.rmax for r in stream[:S]]}, fh)
     print(f"# serve: plain {plain_rate:.1f} jobs/s, served capacity {cap_rate:.1f} jobs/s, "
           f"open loop at {lam:.1f}/s: p50 {p50:.1f} ms p99 {p99:.1f} ms (first {S}: "
           f"{s50:.1f} / {s99:.1f}; plain {np.percentile(plat, 50):.1f} / "
@@ -434,6 +450,7 @@ def serve_leg(ctx, args):
         "plain_sample": S,
         "slots": args.serve_slots,
         "max_batch_jobs": args.serve_batch,
+        "batch_work": args.serve_batch_work or "library default (5e9 candidate evaluations)",
         "plain_jobs_per_s": plain_rate,
         "served_capacity_jobs_per_s": cap_rate,
         "speedup": cap_rate / plain_rate,
@@ -445,6 +462,8 @@ def serve_leg(ctx, args):
         "served_p99_ms_sample": s99,
         "plain_p50_ms_same_arrivals_sample": float(np.percentile(plat, 50)),
         "plain_p99_ms_same_arrivals_sample": float(np.percentile(plat, 99)),
+        "served_query_p99_ms_sample": float(np.percentile(srun, 99)),
+        "plain_query_p99_ms_sample": float(np.percentile(prun, 99)),
         "parity_served_vs_plain": parity,
         "parity_scope": f"the first {S} requests, capacity and open-loop runs; "
                         f"all {M} completed with PP_OK" if parity else "failed",
@@ -735,6 +754,10 @@ def main():
                     help="requests of the stream the plain loop is timed on")
     ap.add_argument("--serve-slots", type=int, default=2)
     ap.add_argument("--serve-batch", type=int, default=32)
+    ap.add_argument("--serve-batch-work", type=float, default=0.0,
+                    help="estimated candidate evaluations per serving batch (0: the library's)")
+    ap.add_argument("--serve-dump", default=None,
+                    help="write the sample's per-request latencies to this JSON file")
     ap.add_argument("--graph", type=int, default=1,
                     help="1: batch launch groups captured into HIP graphs; 0: eager launches")
     ap.add_argument("--busy", metavar="TRACE_DIR", default=None,

@@ -275,9 +275,10 @@ pp_status pp_cloud_from_epos(pp_ctx *ctx, const void *epos, uint64_t nbytes,
 
 /* kNN range of MKNN / 1NN jobs without CNTNB: per neighbour type, per-center
  * histograms cover only the distance within which `factor` x k neighbours
- * are expected at the cloud's mean density (default 3); centers with fewer
- * than k neighbours there are re-run over the full range, so results do not
- * depend on the factor.  0 = full-range per-center histograms always. */
+ * are expected at the cloud's mean density over its occupied volume
+ * (default 4); centers with fewer than k neighbours there are re-run over
+ * the full range, so results do not depend on the factor.  0 = full-range
+ * per-center histograms always. */
 pp_status pp_set_knn_range(pp_ctx *ctx, float factor);
 
 /* Launch form of the query kernel (no counterpart in the reference, whose
@@ -294,8 +295,8 @@ pp_status pp_set_graph_launch(pp_ctx *ctx, int32_t on);
  * Requests are queued by pp_serve_submit and admitted by their device-memory
  * footprint against hbm_budget_bytes; each of `slots` workers takes the
  * admitted requests waiting at that moment, in arrival order (up to
- * max_batch_jobs and about 20 ms of estimated query work; a larger request
- * runs alone), and runs them as one batch on its own HIP stream: a batch's
+ * max_batch_jobs and batch_work estimated candidate evaluations; a larger
+ * request runs alone), and runs them as one batch on its own HIP stream: a batch's
  * preparation overlaps on the device with the query of the batch before it,
  * whose query launches it then follows (queries keep arrival order).  Clouds must be resident (uploaded with a cell width).
  * The caller keeps each request's pp_job arrays and pp_hist_out buffers alive
@@ -304,6 +305,10 @@ typedef struct pp_serve_opts {
     int32_t slots;              /* concurrent batches (1..8) */
     int32_t max_batch_jobs;     /* requests per batch (1..65535) */
     uint64_t hbm_budget_bytes;  /* admission budget for per-request device memory; 0: half the free HBM */
+    double batch_work;          /* estimated candidate evaluations per batch (a batch's
+                                   requests all complete when its last one does, so this
+                                   bounds the wait a batch adds to its first request);
+                                   0: 5e9, about 3 ms of query kernel */
 } pp_serve_opts;
 
 pp_status pp_serve_create(pp_ctx *ctx, const pp_serve_opts *opts, pp_server **out);

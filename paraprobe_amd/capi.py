@@ -75,7 +75,7 @@ EXPORTED = [
 
 class _ServeOpts(C.Structure):
     _fields_ = [("slots", C.c_int32), ("max_batch_jobs", C.c_int32),
-                ("hbm_budget_bytes", C.c_uint64)]
+                ("hbm_budget_bytes", C.c_uint64), ("batch_work", C.c_double)]
 
 
 class PPError(RuntimeError):
@@ -392,7 +392,7 @@ class SpatStat:
     def set_tuning(self, centers_per_tile: int = 0, cells_per_rmax: int = 0):
         self._check(self.lib.pp_set_tuning(self.ctx, centers_per_tile, cells_per_rmax))
 
-    def set_knn_range(self, factor: float = 3.0):
+    def set_knn_range(self, factor: float = 4.0):
         """kNN range factor (expected neighbours per k) of MKNN / 1NN jobs; 0
         keeps full-range per-center histograms."""
         self._check(self.lib.pp_set_knn_range(self.ctx, factor))
@@ -530,10 +530,10 @@ class Server:
     """Continuous-batching request loop over resident clouds (pp_serve_*)."""
 
     def __init__(self, ctx: SpatStat, slots: int = 4, max_batch_jobs: int = 64,
-                 hbm_budget_bytes: int = 0):
+                 hbm_budget_bytes: int = 0, batch_work: float = 0.0):
         self.ctx = ctx
         self.lib = ctx.lib
-        o = _ServeOpts(slots, max_batch_jobs, hbm_budget_bytes)
+        o = _ServeOpts(slots, max_batch_jobs, hbm_budget_bytes, batch_work)
         h = C.c_void_p()
         ctx._check(self.lib.pp_serve_create(ctx.ctx, C.byref(o), C.byref(h)))
         self.h = h

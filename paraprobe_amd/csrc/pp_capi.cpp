@@ -95,7 +95,7 @@ struct pp_ctx {
     std::mutex mu;  // err, tuning state
     std::string err;
     int tune_T = 0, tune_r = 0;
-    double knn_factor = 3.0;  // kNN range: expected neighbours / k (0: full range)
+    double knn_factor = 4.0;  // kNN range: expected neighbours / k (0: full range)
     int graph_launch = 1;     // batch launch groups captured into a HIP graph
     std::map<long, std::pair<int, int>> tuned;  // autotune bucket -> (T, cells per Rmax)
     Slot main;
@@ -1136,19 +1136,59 @@ struct Shape {
 };
 
 // Runs that share the device through several streams (the serving loop's
-// slots) take turns for their query launches: each waits, on the device, for
-// the query launches of the run before it, so that a run's preparation
-// overlaps the query of the run before while the queries themselves keep
-// arrival order instead of interleaving their launch chunks.
+// slots) take turns for their query launches, in the order of their sequence
+// numbers (the order their batches were formed in): each waits, on the host,
+// for its turn and, on the device, for the query launches of the run before
+// it, so that a run's preparation overlaps the query of the run before while
+// the queries themselves keep arrival order (a small batch whose preparation
+// ends first does not overtake a larger one formed before it).
 struct LaunchOrder {
     std::mutex mu;
+    std::condition_variable cv;
     hipEvent_t ev = nullptr;
     bool armed = false;
+    uint64_t turn = 0;  // sequence number of the run whose query launches go next
+};
+
+// One run's place in a LaunchOrder.  take() blocks until it is this run's
+// turn and holds the order's lock; release() passes the turn on.  The
+// destructor passes on a turn never taken too (a run that failed before its
+// launches), so that later runs never wait for it.
+class OrderTurn {
+  public:
+    OrderTurn(LaunchOrder *o, uint64_t seq) : o_(o), seq_(seq) {}
+    OrderTurn(const OrderTurn &) = delete;
+    OrderTurn &operator=(const OrderTurn &) = delete;
+    ~OrderTurn() { release(); }
+    void take()
+    {
+        if (!o_ || taken_)
+            return;
+        lk_ = std::unique_lock<std::mutex>(o_->mu);
+        o_->cv.wait(lk_, [&] { return o_->turn == seq_; });
+        taken_ = true;
+    }
+    void release()
+    {
+        if (!o_ || released_)
+            return;
+        take();
+        ++o_->turn;
+        released_ = true;
+        lk_.unlock();
+        o_->cv.notify_all();
+    }
+
+  private:
+    LaunchOrder *o_;
+    uint64_t seq_;
+    std::unique_lock<std::mutex> lk_;
+    bool taken_ = false, released_ = false;
 };
 
 pp_status run_jobs(pp_ctx *c, Slot &sl, int njobs, const pp_cloud *const *clouds,
                    const pp_job *jobs, pp_hist_out *outs, Shape force = Shape(),
-                   LaunchOrder *order = nullptr);
+                   LaunchOrder *order = nullptr, uint64_t seq = 0);
 
 constexpr size_t kKbChunkBytes = size_t(1) << 30;  // k-th neighbour buffer of one launch chunk
 constexpr int kNoAllocTimedLaunches = 64;          // query launches the fixed-arena slot times
@@ -1488,9 +1528,11 @@ pp_status autotune(pp_ctx *c, Slot &sl, const pp_cloud *cl, const pp_job *job, c
 }
 
 pp_status run_jobs(pp_ctx *c, Slot &sl, int njobs, const pp_cloud *const *clouds,
-                   const pp_job *jobs, pp_hist_out *outs, Shape force, LaunchOrder *order)
+                   const pp_job *jobs, pp_hist_out *outs, Shape force, LaunchOrder *order,
+                   uint64_t seq)
 {
     auto t_start = std::chrono::steady_clock::now();
+    OrderTurn turn(order, seq);
     if (!outs)
         return fail(c, PP_ERR_INVALID_ARG, "outputs are NULL");
     sl.nq = 0;
@@ -1753,9 +1795,8 @@ pp_status run_jobs(pp_ctx *c, Slot &sl, int njobs, const pp_cloud *const *clouds
                 hy.push_back(i);
             }
         }
-        std::unique_lock<std::mutex> olk;
         if (order) {
-            olk = std::unique_lock<std::mutex>(order->mu);
+            turn.take();
             if (order->armed)
                 PP_HIP(c, hipStreamWaitEvent(sl.stream, order->ev, 0));
         }
@@ -1816,11 +1857,6 @@ pp_status run_jobs(pp_ctx *c, Slot &sl, int njobs, const pp_cloud *const *clouds
                 runs[i].T_used = tmain[i];  // the job's main launch
         }
         PP_HIP(c, hipEventRecord(sl.ev[4], sl.stream));
-        if (order) {
-            PP_HIP(c, hipEventRecord(order->ev, sl.stream));
-            order->armed = true;
-            olk.unlock();
-        }
         // read back (the counters of every job in one copy)
         std::vector<char> hout(out_bytes);
         std::vector<uint32_t> hcn(cn_words);
@@ -1830,6 +1866,15 @@ pp_status run_jobs(pp_ctx *c, Slot &sl, int njobs, const pp_cloud *const *clouds
         if (cn_words)
             PP_HIP(c, hipMemcpyAsync(hcn.data(), sl.cnbuf.p, cn_words * 4, hipMemcpyDeviceToHost,
                                      sl.stream));
+        if (order) {
+            // the next run's queries follow this run's read-back too: a copy
+            // that needs compute units would otherwise wait behind the next
+            // run's persistent query kernel, which holds them all, and this
+            // run would complete only after the next one
+            PP_HIP(c, hipEventRecord(order->ev, sl.stream));
+            order->armed = true;
+            turn.release();
+        }
         PP_HIP(c, hipStreamSynchronize(sl.stream));
         for (int i = 0; i < njobs; ++i) {
             JobRun &r = runs[i];
@@ -2071,10 +2116,10 @@ double job_work(const pp_cloud *cl, const JobPlan &p)
     return centers * cand * (p.percenter ? 1.5 : 1.0);
 }
 
-// Work that one serving batch may gather (about 20 ms of query kernel), so
-// that small requests share launches and a large one does not hold back the
-// requests that arrived before it finishes.
-constexpr double kBatchWork = 3.0e10;
+// Default estimated work that one serving batch may gather (about 3 ms of
+// query kernel): small requests share launches, while the requests of a batch,
+// which all complete when its last one does, wait little for each other.
+constexpr double kBatchWork = 5.0e9;
 
 } // namespace
 
@@ -2091,11 +2136,12 @@ struct pp_server {
     size_t used = 0;
     bool stop = false;
     uint64_t next_ticket = 1, submitted = 0, completed = 0;
+    uint64_t next_seq = 0;  // sequence number of the next batch formed
     std::vector<std::pair<uint64_t, int32_t>> done;
 
     // Moves the next batch out of `pending` (called with mu held): the
     // waiting requests in arrival order, as many as fit the remaining budget,
-    // max_batch_jobs and about kBatchWork of estimated work (a larger request
+    // max_batch_jobs and about opts.batch_work of estimated work (a larger request
     // runs alone).  No request overtakes an earlier one, so the tail latency
     // stays that of a first-come first-served loop.  Returns false when the
     // oldest waiting request does not fit the budget yet.
@@ -2106,7 +2152,7 @@ struct pp_server {
             Request &r = pending.front();
             if (used + r.footprint > budget)
                 break;
-            if (!batch.empty() && work + r.work > kBatchWork)
+            if (!batch.empty() && work + r.work > opts.batch_work)
                 break;
             used += r.footprint;
             fp += r.footprint;
@@ -2124,11 +2170,13 @@ struct pp_server {
         for (;;) {
             std::vector<Request> batch;
             size_t fp = 0;
+            uint64_t seq = 0;
             {
                 std::unique_lock<std::mutex> lk(mu);
                 cv_work.wait(lk, [&] { return (stop && pending.empty()) || pick(batch, fp); });
                 if (batch.empty())
                     return;  // stop requested and nothing left
+                seq = next_seq++;  // under mu: batches are numbered as formed
             }
             std::vector<const pp_cloud *> cls;
             std::vector<pp_job> jobs;
@@ -2139,7 +2187,7 @@ struct pp_server {
                 outs.push_back(*r.out);
             }
             const pp_status st = run_jobs(ctx, sl, (int)batch.size(), cls.data(), jobs.data(),
-                                          outs.data(), Shape(), &order);
+                                          outs.data(), Shape(), &order, seq);
             for (size_t k = 0; k < batch.size(); ++k)
                 *batch[k].out = outs[k];
             std::lock_guard<std::mutex> lk(mu);
@@ -2935,10 +2983,14 @@ pp_status pp_serve_create(pp_ctx *c, const pp_serve_opts *opts, pp_server **out)
     if (opts->slots < 1 || opts->slots > 8 || opts->max_batch_jobs < 1 ||
         opts->max_batch_jobs > 65535)
         return fail(c, PP_ERR_INVALID_ARG, "slots must be 1..8, max_batch_jobs 1..65535");
+    if (!(opts->batch_work >= 0) || !std::isfinite(opts->batch_work))
+        return fail(c, PP_ERR_INVALID_ARG, "batch_work must be finite and >= 0");
     PP_HIP(c, hipSetDevice(c->device));
     std::unique_ptr<pp_server> s(new pp_server());
     s->ctx = c;
     s->opts = *opts;
+    if (s->opts.batch_work == 0)
+        s->opts.batch_work = kBatchWork;
     s->budget = opts->hbm_budget_bytes;
     if (s->budget == 0) {
         size_t fr = 0, tot = 0;

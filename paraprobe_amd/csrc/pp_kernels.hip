@@ -932,6 +932,10 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : (T == 8 && MODE
                 // two vector instructions per center and chunk to branch on)
                 int ncen = cnt;
                 asm volatile("" : "+s"(ncen));
+                // likewise the in-ball counting switch (kept as a lane mask,
+                // it costs two vector instructions per chunk)
+                int cin = (int)rfl(count_in ? 1u : 0u);
+                asm volatile("" : "+s"(cin));
                 // this candidate's pair-count row (only dereferenced when envok)
                 uint32_t *acc_row = (PC ? s_H : s_P) + woff;
                 if (PC) {
@@ -961,7 +965,7 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : (T == 8 && MODE
     #pragma unroll
                     for (int c = 0; c < T; ++c) {
                         const bool in = d2[c] <= R2;
-                        if (__builtin_expect(count_in, 0))
+                        if (__builtin_expect(cin != 0, 0))
                             my_in += (unsigned long long)__builtin_popcountll(
                                 __builtin_amdgcn_ballot_w64(in));
                         if (in && envok) {
@@ -984,7 +988,7 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : (T == 8 && MODE
                             const float dz = czr[c] - p.z;
                             const float d2 = dx * dx + dy * dy + dz * dz;
                             const bool in = d2 <= R2;
-                            if (__builtin_expect(count_in, 0))
+                            if (__builtin_expect(cin != 0, 0))
                                 my_in += (unsigned long long)__builtin_popcountll(
                                     __builtin_amdgcn_ballot_w64(in));
                             if (in && envok) {
@@ -1012,7 +1016,7 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : (T == 8 && MODE
                             const float dy = cyr[c] - p.y;
                             const float dz = czr[c] - p.z;
                             const float d2 = dx * dx + dy * dy + dz * dz;
-                            if (__builtin_expect(count_in, 0))
+                            if (__builtin_expect(cin != 0, 0))
                                 my_in += (unsigned long long)__builtin_popcountll(
                                     __builtin_amdgcn_ballot_w64(d2 <= R2));
                             if (d2 <= bthr) {

@@ -91,7 +91,7 @@ def test_knn_range_does_not_change_results(ctx, factor):
             elif factor == 0.3 and job.families & FAM_MKNN:
                 assert g.percenter == 2 and g.centers_rerun > 0
     finally:
-        ctx.set_knn_range(3.0)
+        ctx.set_knn_range(4.0)
 
 
 @pytest.mark.parametrize("T,r", [(8, 1), (16, 2), (32, 3), (16, 4), (8, 2), (32, 1), (8, 6),
@@ -295,7 +295,7 @@ def test_distances_packed_around_every_bin_edge(ctx, rmin, rincr, rmax, mode):
         g, _, st = _compare(ctx, xyz, ty, s2, Job([((1,), (1,))], rmin, rincr, rmax, fams,
                                                    kvalues=[1] if fams & FAM_MKNN else []))
     finally:
-        ctx.set_knn_range(3.0)
+        ctx.set_knn_range(4.0)
     assert st["inball"] == 2 * n
     assert g.percenter == want, (g.percenter, want)
     if want == 2:

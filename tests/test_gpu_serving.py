@@ -100,6 +100,18 @@ def test_admission_rejects_and_keeps_serving(ctx):
             for _, st, r in res:
                 assert st == 0
                 _same(alone, r)
+        # batches of one request each (batch_work below any request's estimate)
+        # give the same results; a negative batch_work is refused
+        with Server(ctx, slots=2, max_batch_jobs=4, batch_work=1.0) as srv:
+            ts = [srv.submit(cl, job) for _ in range(3)]
+            srv.drain()
+            res = srv.poll()
+            assert sorted(t for t, _, _ in res) == sorted(ts)
+            for _, st, r in res:
+                assert st == 0
+                _same(alone, r)
+        with pytest.raises(PPError):
+            Server(ctx, slots=1, max_batch_jobs=4, batch_work=-1.0)
         # malformed requests are refused at submit; the server keeps going
         with Server(ctx, slots=1, max_batch_jobs=4) as srv:
             with pytest.raises(PPError):
