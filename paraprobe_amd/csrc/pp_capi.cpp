@@ -61,6 +61,8 @@ struct CellList {
     uint32_t *cellstart = nullptr;  // ncells + 1
     float4 *pos = nullptr;          // n
     float *s2 = nullptr;            // n
+    uint2 *xsplit = nullptr;        // ncells x-part starts (xbits > 0)
+    int xbits = 0;                  // x parts per cell: 2^xbits (0: none)
     void *mem = nullptr;
     size_t bytes = 0;
 };
@@ -512,10 +514,21 @@ pp_status build_on_grid(pp_ctx *c, Slot &s, const float *xyz, const uint8_t *typ
                         size_t extra_scratch_bytes = 0)
 {
     const uint64_t nc = g.ncells();
+    // each cell's ions by sub-cell: x eighths (or halves), then y and z
+    // halves (tiles of consecutive centers span a small box, and the stencil
+    // rows leave out the x parts of their end cells beyond the balls),
+    // unless the sub-cell offsets would be large next to the cloud
+    auto fits = [&](int b) {
+        return (nc << b) <= std::max<uint64_t>(4 * n, (uint64_t)1 << 22) &&
+               (nc << b) < 0xFFFFFFF0ull;
+    };
+    const int sub = fits(5) ? 5 : (fits(3) ? 3 : 0);
     Carver m0(nullptr);
     m0.take<uint32_t>(nc + 1);
     m0.take<float4>(n);
     m0.take<float>(n);
+    if (sub)
+        m0.take<uint2>(nc);
     const size_t bytes = m0.off + 256;
     if (L.mem && L.bytes < bytes) {
         PP_HIP(c, hipStreamSynchronize(s.stream));
@@ -531,14 +544,10 @@ pp_status build_on_grid(pp_ctx *c, Slot &s, const float *xyz, const uint8_t *typ
     L.cellstart = m.take<uint32_t>(nc + 1);
     L.pos = m.take<float4>(n);
     L.s2 = m.take<float>(n);
+    L.xsplit = sub ? m.take<uint2>(nc) : nullptr;
+    L.xbits = sub ? sub - 2 : 0;
     L.g = g;
     L.w = w;
-    // each cell's ions in octant order (tiles of consecutive centers then
-    // span less than a cell, so their stencil rows cut shorter), unless the
-    // octant offsets would be large next to the cloud
-    const int sub =
-        (nc << 3) <= std::max<uint64_t>(4 * n, (uint64_t)1 << 22) && (nc << 3) < 0xFFFFFFF0ull ? 3
-                                                                                              : 0;
     const uint64_t nk = nc << sub;
     // temporaries after the first extra_scratch_bytes of the scratch arena
     Carver t0(nullptr);
@@ -563,8 +572,10 @@ pp_status build_on_grid(pp_ctx *c, Slot &s, const float *xyz, const uint8_t *typ
     PP_HIP(c, launch_cellkeys(xyz, n, g, count, key, rank, s.stream, sub));
     PP_HIP(c, scan_exclusive(count, start, nk, ws, s.stream));
     PP_HIP(c, launch_scatter(xyz, type, s2, n, key, rank, start, L.pos, L.s2, s.stream));
-    if (sub)
+    if (sub) {
         PP_HIP(c, launch_stride(start, L.cellstart, nc + 1, sub, s.stream));
+        PP_HIP(c, launch_xsplit(start, nc, sub, L.xsplit, s.stream));
+    }
     return PP_OK;
 }
 
@@ -913,6 +924,8 @@ void fill_jobdev(JobRun &r)
     JobDev &d = r.dev;
     d.pos = r.cand->pos;
     d.cellstart = r.cand->cellstart;
+    d.xsplit = r.cand->xsplit;
+    d.xbits = r.cand->xbits;
     d.cpos = r.cl->pos;
     d.nx = r.cl->g.nx;
     d.ny = r.cl->g.ny;
@@ -1050,11 +1063,10 @@ void compose_outputs(const JobRun &r, const std::vector<unsigned long long> &P,
 // ---------------------------------------------------------------------------
 // kernel shape: centers per tile and cells per Rmax
 
-// Density rule: cells of Rmax / r with r as large as possible (fewer
-// candidates per in-ball pair: 6.3 at r = 1, 3.7 at 2, 2.3 at 4, 1.9 at 5)
-// while a cell still holds ~16 candidates, so that stencil rows fill 64-lane
-// chunks; capped at 4, beyond which the per-tile and per-row work mostly
-// cancels the saving (the autotuner also tries 5).
+// Density rule: cells of Rmax / r, r the nearest integer to the width that
+// holds ~50 candidates, capped at 4 (finer cells cut fewer candidates per
+// in-ball pair than their per-tile and per-row work costs; the autotuner also
+// tries 5).
 // Ions per nm^3 among a job's candidates: the neighbour types alone when they
 // are a minority (the typed candidate list), else every ion.
 double cand_density(const pp_cloud *cl, const JobPlan &p)
@@ -1070,7 +1082,11 @@ double cand_density(const pp_cloud *cl, const JobPlan &p)
 
 int rule_r(double density, double R)
 {
-    const int r = (int)std::floor(R * std::cbrt(std::max(density, 1e-9) / 16.0));
+    // with the stencil rows trimmed to the x parts of their end cells, larger
+    // cells lose little to whole cells and save per-row work (measured: 20M
+    // ions at 27 ions/nm^3 and R = 5 fastest at R/4; 100M ions at 22 ions/nm^3
+    // and R = 3 fastest at R/2)
+    const int r = (int)std::lround(R * std::cbrt(std::max(density, 1e-9) / 50.0));
     return std::min(4, std::max(1, r));
 }
 

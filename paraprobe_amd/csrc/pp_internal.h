@@ -19,6 +19,17 @@ constexpr int kXcds = 8;           // MI355X accelerator complex dies
 // 256-byte line (atomics on one line serialise at ~11 ns each).
 constexpr int kQueueStride = 64;
 constexpr int kQueueWords = kXcds * kQueueStride;
+
+// Where the x parts of a cell start in a cell list sorted by sub-cell key
+// (x part major): byte p - 1 of the 8-byte record is the offset of part p
+// (p = 1 .. 2^xbits - 1) from the cell's first ion; byte 7 = 1 marks a cell
+// of more than 255 ions, whose ions are never trimmed.
+__host__ __device__ inline uint32_t xsplit_at(uint2 rec, int p)
+{
+    const int b = p - 1;
+    return ((b < 4 ? rec.x : rec.y) >> (8 * (b & 3))) & 0xFFu;
+}
+__host__ __device__ inline bool xsplit_whole(uint2 rec) { return (rec.y >> 24) != 0u; }
 // k-th neighbour category of a row that the reduction skips (categories are
 // below 65535: jobs with kNN and more bins are refused)
 constexpr uint16_t kKbEmpty = 0xFFFFu;
@@ -54,6 +65,10 @@ struct JobDev {
     // offsets -- every ion, or only the job's neighbour types on the same grid
     const float4 *pos;
     const uint32_t *cellstart;
+    // x-part starts of every cell of the candidate list (xbits = 0: the
+    // list has no x parts and rows keep whole cells)
+    const uint2 *xsplit;
+    int xbits;
     const float4 *cpos;       // centers' cell-sorted ions (the full list)
     int nx, ny, nz;
     double glo[3], gw, ginv;  // the grid: cell c spans [glo + c * gw, glo + (c + 1) * gw)

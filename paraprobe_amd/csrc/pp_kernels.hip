@@ -273,9 +273,23 @@ __device__ inline uint32_t cell_half(float v, double lo, double inv, int c)
     return t >= 0.5 ? 1u : 0u;
 }
 
-// key = cell (x fastest), or with sub = 3 the cell times 8 plus the octant of
-// the cell the ion lies in, so that a sort by key orders each cell's ions by
-// octant (the cell offsets are every 8th key offset)
+// The part of its cell an ion lies in along x, of 2^bits equal parts, from the
+// same FP64 coordinate as cell_coord (monotone in v; clamped).
+__device__ inline uint32_t cell_part(float v, double lo, double inv, int c, int bits)
+{
+    const double t = (((double)v - lo) * inv - (double)c) * (double)(1 << bits);
+    const double f = floor(t);
+    const double top = (double)((1 << bits) - 1);
+    return (uint32_t)(f < 0.0 ? 0.0 : (f > top ? top : f));
+}
+
+// key = cell (x fastest), or with sub = 3 or 5 the cell times 2^sub plus the
+// sub-cell of the ion: its x part of the cell (sub - 2 bits: halves or
+// eighths) major, then its y and z halves.  A sort by key orders each cell's
+// ions by x part, so the stencil rows can leave out the parts of their end
+// cells beyond the balls, and within a part by quadrant, so that a tile of
+// consecutive centers spans a small box (the cell offsets are every
+// 2^sub-th key offset).
 __global__ __launch_bounds__(256) void k_cellkeys(const float *__restrict__ xyz, uint64_t n,
                                                   GridGeom g, uint32_t *__restrict__ count,
                                                   uint32_t *__restrict__ key,
@@ -290,8 +304,8 @@ __global__ __launch_bounds__(256) void k_cellkeys(const float *__restrict__ xyz,
     const int cz = cell_coord(z, g.lo[2], g.inv_w, g.nz);
     uint32_t k = ((uint32_t)cz * (uint32_t)g.ny + (uint32_t)cy) * (uint32_t)g.nx + (uint32_t)cx;
     if (sub)
-        k = (k << 3) | cell_half(x, g.lo[0], g.inv_w, cx) |
-            (cell_half(y, g.lo[1], g.inv_w, cy) << 1) | (cell_half(z, g.lo[2], g.inv_w, cz) << 2);
+        k = (k << sub) | (cell_part(x, g.lo[0], g.inv_w, cx, sub - 2) << 2) |
+            (cell_half(y, g.lo[1], g.inv_w, cy) << 1) | cell_half(z, g.lo[2], g.inv_w, cz);
     key[i] = k;
     rank[i] = atomicAdd(&count[k], 1u);
 }
@@ -311,6 +325,42 @@ hipError_t launch_stride(const uint32_t *in, uint32_t *out, uint64_t nout, int s
     if (nout == 0)
         return hipSuccess;
     hipLaunchKernelGGL(k_stride, dim3(grid_for(nout, 256)), dim3(256), 0, s, in, out, nout, shift);
+    return hipGetLastError();
+}
+
+// Each cell's x-part starts (XSplit records) from the sub-cell offsets.
+__global__ __launch_bounds__(256) void k_xsplit(const uint32_t *__restrict__ start,
+                                                uint64_t nc, int sub, uint2 *__restrict__ out)
+{
+    const uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc)
+        return;
+    const uint64_t k = c << sub;
+    const uint32_t base = start[k], cnt = start[k + (1u << sub)] - base;
+    uint32_t lo = 0, hi = 0;
+    if (cnt > 255u) {
+        hi = 1u << 24;
+    } else {
+        const int parts = 1 << (sub - 2);
+        for (int p = 1; p < parts; ++p) {
+            const uint32_t off = start[k + ((uint64_t)p << 2)] - base;
+            const int b = p - 1;
+            if (b < 4)
+                lo |= off << (8 * b);
+            else
+                hi |= off << (8 * (b - 4));
+        }
+    }
+    out[c] = make_uint2(lo, hi);
+}
+
+hipError_t launch_xsplit(const uint32_t *start, uint64_t ncells, int sub, uint2 *out,
+                         hipStream_t s)
+{
+    if (ncells == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_xsplit, dim3(grid_for(ncells, 256)), dim3(256), 0, s, start, ncells, sub,
+                       out);
     return hipGetLastError();
 }
 
@@ -784,12 +834,15 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : (T == 8 && MODE
                 const double hy = fmax(0.0, fmax(ylo - (double)b1y, (double)b0y - (ylo + J.gw)));
                 const double hz = fmax(0.0, fmax(zlo - (double)b1z, (double)b0z - (zlo + J.gw)));
                 const double h2 = hy * hy + hz * hz, lim = J.reach * J.reach;
+                // the balls' x extent in cell units: [tlo, thi]
+                double tlo = 0.0, thi = 0.0;
                 if (h2 > lim) {
                     x1 = x0 - 1;
                 } else {
                     const double ex = sqrt(lim - h2);
-                    const double a = floor(((double)b0x - ex - J.glo[0]) * J.ginv);
-                    const double b = floor(((double)b1x + ex - J.glo[0]) * J.ginv);
+                    tlo = ((double)b0x - ex - J.glo[0]) * J.ginv;
+                    thi = ((double)b1x + ex - J.glo[0]) * J.ginv;
+                    const double a = floor(tlo), b = floor(thi);
                     if (a > (double)x0)
                         x0 = (int)a;
                     if (b < (double)x1)
@@ -798,8 +851,33 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : (T == 8 && MODE
                 if (x0 <= x1) {
                     const uint32_t k0 = ((uint32_t)z * (uint32_t)J.ny + (uint32_t)y) *
                                             (uint32_t)J.nx + (uint32_t)x0;
+                    const uint32_t k1 = k0 + (uint32_t)(x1 - x0);
                     rs = J.cellstart[k0];
-                    re = J.cellstart[k0 + (uint32_t)(x1 - x0) + 1u];
+                    re = J.cellstart[k1 + 1u];
+                    if (J.xbits) {
+                        // the end cells' x parts beyond [tlo, thi]: an ion of
+                        // cell x0 with x >= the balls' lower bound has
+                        // t >= tlo, so its part is at least qa (the same
+                        // monotone FP64 expression as the build); likewise at
+                        // most qb in cell x1
+                        const int P = 1 << J.xbits;
+                        if (floor(tlo) == (double)x0) {
+                            const double f = floor((tlo - (double)x0) * (double)P);
+                            const int qa = f < 0.0 ? 0 : (f > (double)(P - 1) ? P - 1 : (int)f);
+                            const uint2 rec = J.xsplit[k0];
+                            if (qa > 0 && !xsplit_whole(rec))
+                                rs += xsplit_at(rec, qa);
+                        }
+                        if (floor(thi) == (double)x1) {
+                            const double f = floor((thi - (double)x1) * (double)P);
+                            const int qb = f < 0.0 ? 0 : (f > (double)(P - 1) ? P - 1 : (int)f);
+                            const uint2 rec = J.xsplit[k1];
+                            if (qb < P - 1 && !xsplit_whole(rec))
+                                re = J.cellstart[k1] + xsplit_at(rec, qb + 1);
+                        }
+                        if (re < rs)
+                            re = rs;
+                    }
                 }
             }
             s_rows[2 * r] = rs;

@@ -24,12 +24,16 @@ hipError_t scan_exclusive(const uint32_t *in, uint32_t *out, uint64_t n, uint32_
 hipError_t launch_bbox(const float *xyz, uint64_t n, uint32_t *out6, hipStream_t s);
 float ord_to_float(uint32_t u);
 
-// Cell-list build: key/rank per ion (sub = 3: keys of cell octants, count
-// holds 8 entries per cell), then scatter into cell-sorted float4
-// (x, y, z, type) and the sorted ion2surf array; launch_stride takes every
-// (1 << shift)-th offset (the cell offsets of octant keys).
+// Cell-list build: key/rank per ion (sub = 3 or 5: keys of sub-cells, count
+// holds 1 << sub entries per cell: the x part of the cell (sub - 2 bits, the
+// major sub-key), then the y and z halves), then scatter into cell-sorted
+// float4 (x, y, z, type) and the sorted ion2surf array; launch_stride takes
+// every (1 << shift)-th offset (the cell offsets of sub-cell keys);
+// launch_xsplit writes each cell's x-part starts (XSplit).
 hipError_t launch_cellkeys(const float *xyz, uint64_t n, GridGeom g, uint32_t *count,
                            uint32_t *key, uint32_t *rank, hipStream_t s, int sub = 0);
+hipError_t launch_xsplit(const uint32_t *start, uint64_t ncells, int sub, uint2 *out,
+                         hipStream_t s);
 hipError_t launch_stride(const uint32_t *in, uint32_t *out, uint64_t nout, int shift,
                          hipStream_t s);
 hipError_t launch_scatter(const float *xyz, const uint8_t *type, const float *s2, uint64_t n,
