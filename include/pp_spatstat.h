@@ -186,6 +186,12 @@ pp_status pp_batch_run(pp_ctx *ctx, int32_t njobs, const pp_cloud *const *clouds
  * bucket. */
 pp_status pp_set_tuning(pp_ctx *ctx, int32_t centers_per_tile, int32_t cells_per_rmax);
 
+/* Cell shape of the cell lists jobs build for themselves (clouds uploaded
+ * without a cell width): cells of Rmax / cells_per_rmax along x and
+ * yz_per_x times finer along y and z (1..4; 0 = built-in choice, 1).
+ * Resident clouds keep their cubic cells.  Results do not depend on it. */
+pp_status pp_set_cell_aspect(pp_ctx *ctx, int32_t yz_per_x);
+
 /* Squared distance of every ion to a triangulated tip surface, the ion2surf
  * input of pp_cloud_upload: replaces solver::surface_distancing2
  * (src/PARAPROBE_SolverHdl.cpp:312-484).  Each ion gets the smallest

@@ -51,6 +51,7 @@ EXPORTED = [
     "pp_batch_run",
     "pp_set_tuning",
     "pp_set_knn_range",
+    "pp_set_cell_aspect",
     "pp_set_graph_launch",
     "pp_cloud_add_cell_list",
     "pp_serve_create",
@@ -154,6 +155,7 @@ def load_library() -> C.CDLL:
     lib.pp_batch_run.argtypes = [P, C.c_int32, C.POINTER(P), C.POINTER(_Job), C.POINTER(_Out)]
     lib.pp_set_tuning.argtypes = [P, C.c_int32, C.c_int32]
     lib.pp_set_knn_range.argtypes = [P, C.c_float]
+    lib.pp_set_cell_aspect.argtypes = [P, C.c_int32]
     lib.pp_set_graph_launch.argtypes = [P, C.c_int32]
     lib.pp_cloud_add_cell_list.argtypes = [P, P, C.c_float]
     lib.pp_serve_create.argtypes = [P, C.POINTER(_ServeOpts), C.POINTER(P)]
@@ -391,6 +393,10 @@ class SpatStat:
 
     def set_tuning(self, centers_per_tile: int = 0, cells_per_rmax: int = 0):
         self._check(self.lib.pp_set_tuning(self.ctx, centers_per_tile, cells_per_rmax))
+
+    def set_cell_aspect(self, yz_per_x: int = 0):
+        """y/z cells per x cell width of the cell lists jobs build (0: built-in)."""
+        self._check(self.lib.pp_set_cell_aspect(self.ctx, yz_per_x))
 
     def set_knn_range(self, factor: float = 4.0):
         """kNN range factor (expected neighbours per k) of MKNN / 1NN jobs; 0

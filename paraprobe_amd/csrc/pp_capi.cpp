@@ -63,6 +63,7 @@ struct CellList {
     float *s2 = nullptr;            // n
     uint2 *xsplit = nullptr;        // ncells x-part starts (xbits > 0)
     int xbits = 0;                  // x parts per cell: 2^xbits (0: none)
+    int ay = 1;                     // requested y/z divisions of the x width
     void *mem = nullptr;
     size_t bytes = 0;
 };
@@ -97,6 +98,7 @@ struct pp_ctx {
     std::mutex mu;  // err, tuning state
     std::string err;
     int tune_T = 0, tune_r = 0;
+    int tune_ay = 0;  // y/z cells per x cell width (0: rule)
     double knn_factor = 4.0;  // kNN range: expected neighbours / k (0: full range)
     int graph_launch = 1;     // batch launch groups captured into a HIP graph
     std::map<long, std::pair<int, int>> tuned;  // autotune bucket -> (T, cells per Rmax)
@@ -252,7 +254,8 @@ struct JobPlan {
     std::vector<int32_t> kth_to_u;  // kdesc entry -> unique index
     int kmax = 0, nku = 0;
     int cat0 = 0, cat_sentinel = 0;
-    int r = 0;          // cells per Rmax of this job's cell list
+    int r = 0;          // cells per Rmax of this job's cell list (along x)
+    int ay = 1;         // y and z cells per x cell width
     // query mode: 0 pair counts only, 1 full-range per-center histograms,
     // 2 hybrid (kNN-range per-center histograms + rerun of short centers)
     int mode = 0;
@@ -396,15 +399,15 @@ pp_status plan_job(pp_ctx *c, const pp_job *j, JobPlan &p)
 // visited iff the closest points of the two cells are within R, with a
 // relative margin of 2^-20 on R^2 that covers the FP32 rounding of the
 // in-ball test (at most ~6e-7 relative).
-std::vector<Row> make_rows(double w, double R)
+std::vector<Row> make_rows(double w, double wyz, double R)
 {
     const double lim = R * R * (1.0 + std::ldexp(1.0, -20));
-    const int dmax = (int)std::floor(std::sqrt(lim) / w) + 1;
+    const int dmax = (int)std::floor(std::sqrt(lim) / wyz) + 1;
     std::vector<Row> rows;
     for (int dz = -dmax; dz <= dmax; ++dz)
         for (int dy = -dmax; dy <= dmax; ++dy) {
             const double my = std::max(0, std::abs(dy) - 1), mz = std::max(0, std::abs(dz) - 1);
-            const double rem = lim - (my * my + mz * mz) * w * w;
+            const double rem = lim - (my * my + mz * mz) * wyz * wyz;
             if (rem < 0)
                 continue;
             const int mx = (int)std::floor(std::sqrt(rem) / w);
@@ -579,7 +582,7 @@ pp_status build_on_grid(pp_ctx *c, Slot &s, const float *xyz, const uint8_t *typ
     return PP_OK;
 }
 
-pp_status build_celllist(pp_ctx *c, Slot &s, pp_cloud *cl, float w, CellList &L)
+pp_status build_celllist(pp_ctx *c, Slot &s, pp_cloud *cl, float w, CellList &L, int ay = 1)
 {
     const uint64_t n = cl->n;
     GridGeom g{};
@@ -589,7 +592,8 @@ pp_status build_celllist(pp_ctx *c, Slot &s, pp_cloud *cl, float w, CellList &L)
         uint64_t tot = 1;
         int d[3];
         for (int a = 0; a < 3; ++a) {
-            d[a] = (int)std::floor(((double)cl->hi[a] - (double)cl->lo[a]) / ww) + 1;
+            d[a] = (int)std::floor(((double)cl->hi[a] - (double)cl->lo[a]) / (a ? ww / ay : ww)) +
+                   1;
             tot *= (uint64_t)d[a];
         }
         if (tot <= std::max<uint64_t>(8 * n + 1024, 1u << 20) && tot < 0xFFFFFFF0ull) {
@@ -602,9 +606,13 @@ pp_status build_celllist(pp_ctx *c, Slot &s, pp_cloud *cl, float w, CellList &L)
     }
     g.w = ww;
     g.inv_w = 1.0 / ww;
+    g.wyz = ww / ay;
+    g.inv_wyz = 1.0 / g.wyz;
     for (int a = 0; a < 3; ++a)
         g.lo[a] = cl->lo[a];
-    return build_on_grid(c, s, cl->xyz, cl->type, cl->s2, n, g, w, L);
+    const pp_status st = build_on_grid(c, s, cl->xyz, cl->type, cl->s2, n, g, w, L);
+    L.ay = ay;
+    return st;
 }
 
 // The ions of the types in `mask`, cell-sorted on the grid of `base`, into
@@ -934,6 +942,7 @@ void fill_jobdev(JobRun &r)
         d.glo[a] = r.cl->g.lo[a];
     d.gw = r.cl->g.w;
     d.ginv = r.cl->g.inv_w;
+    d.gwyz = r.cl->g.wyz;
     d.reach = std::sqrt((double)p.R2) * (1.0 + std::ldexp(1.0, -12)) + 1e-4 * r.cl->g.w;
     d.nrows = (int)r.rows.size();
     d.rows = (const Row *)(r.tables + r.o_rows);
@@ -1116,7 +1125,8 @@ pp_status pick_celllist(pp_ctx *c, Slot &s, pp_cloud *cl, const JobPlan &p,
         return out ? PP_OK : fail(c, PP_ERR_INVALID_ARG, "resident cloud without a cell list");
     }
     if (s.fixed) {
-        if (!cl->lists.empty() && cl->lists[0].mem && cl->lists[0].w == (float)want) {
+        if (!cl->lists.empty() && cl->lists[0].mem && cl->lists[0].w == (float)want &&
+            cl->lists[0].ay == p.ay) {
             out = &cl->lists[0];
             return PP_OK;
         }
@@ -1127,7 +1137,7 @@ pp_status pick_celllist(pp_ctx *c, Slot &s, pp_cloud *cl, const JobPlan &p,
     if (cl->lists.empty())
         cl->lists.emplace_back();
     CellList &L = cl->lists[0];
-    if (!(L.mem && L.w == (float)want)) {
+    if (!(L.mem && L.w == (float)want && L.ay == p.ay)) {
         {
             std::lock_guard<std::mutex> lk(cl->sub_mu);
             for (auto &sb : cl->subs)
@@ -1135,7 +1145,7 @@ pp_status pick_celllist(pp_ctx *c, Slot &s, pp_cloud *cl, const JobPlan &p,
                     (void)hipFree(sb.cl.mem);
             cl->subs.clear();
         }
-        pp_status st = build_celllist(c, s, cl, (float)want, L);
+        pp_status st = build_celllist(c, s, cl, (float)want, L, p.ay);
         if (st != PP_OK)
             return st;
     }
@@ -1553,12 +1563,13 @@ pp_status run_jobs(pp_ctx *c, Slot &sl, int njobs, const pp_cloud *const *clouds
         return fail(c, PP_ERR_INVALID_ARG, "outputs are NULL");
     sl.nq = 0;
     std::vector<JobRun> runs(njobs);
-    int tuneT, tuneR, graph_launch;
+    int tuneT, tuneR, tuneAY, graph_launch;
     double knn_factor;
     {
         std::lock_guard<std::mutex> lk(c->mu);
         tuneT = c->tune_T;
         tuneR = c->tune_r;
+        tuneAY = c->tune_ay;
         knn_factor = c->knn_factor;
         graph_launch = c->graph_launch;
     }
@@ -1599,6 +1610,7 @@ pp_status run_jobs(pp_ctx *c, Slot &sl, int njobs, const pp_cloud *const *clouds
                 p.r = tuneR;
         }
         runs[i].T_req = jT;
+        p.ay = tuneAY > 0 ? tuneAY : 1;
         plan_hybrid(clouds[i], p, knn_factor);
     }
     PP_HIP(c, hipSetDevice(c->device));
@@ -1646,7 +1658,7 @@ pp_status run_jobs(pp_ctx *c, Slot &sl, int njobs, const pp_cloud *const *clouds
                     all_env = false;
             runs[i].count_in = !(all_env && (p.families & (PP_FAM_RDF | PP_FAM_RIPK)));
         }
-        runs[i].rows = make_rows(runs[i].cl->g.w, runs[i].plan.R);
+        runs[i].rows = make_rows(runs[i].cl->g.w, runs[i].cl->g.wyz, runs[i].plan.R);
         if ((int)runs[i].rows.size() > kMaxRows)
             return fail(c, PP_ERR_UNSUPPORTED, "stencil too large (cell width far below Rmax)");
         runs[i].ncells = runs[i].cl->g.ncells();
@@ -1954,18 +1966,18 @@ size_t job_footprint(const pp_cloud *cl, const JobPlan &p)
     r.plan = p;
     r.cloud = cl;
     size_t nc = 1;
-    double w = (double)p.R / std::max(p.r, 1);
+    double w = (double)p.R / std::max(p.r, 1), wyz = w / p.ay;
     if (cl->resident && !cl->lists.empty())
         for (const CellList &L : cl->lists)
-            nc = std::max<size_t>(nc, L.g.ncells()), w = L.g.w;
+            nc = std::max<size_t>(nc, L.g.ncells()), w = L.g.w, wyz = L.g.wyz;
     else {
         double v = 1.0;
         for (int a = 0; a < 3; ++a)
-            v *= std::floor(((double)cl->hi[a] - (double)cl->lo[a]) / w) + 1.0;
+            v *= std::floor(((double)cl->hi[a] - (double)cl->lo[a]) / (a ? w / p.ay : w)) + 1.0;
         nc = (size_t)v;
     }
     r.ncells = nc;
-    r.rows = make_rows(w, p.R);
+    r.rows = make_rows(w, wyz, p.R);
     layout_tables(r);
     Carver m(nullptr);
     layout_outputs(r, m);
@@ -2464,6 +2476,17 @@ pp_status pp_set_tuning(pp_ctx *c, int32_t centers_per_tile, int32_t cells_per_r
     return PP_OK;
 }
 
+pp_status pp_set_cell_aspect(pp_ctx *c, int32_t yz_per_x)
+{
+    if (!c)
+        return PP_ERR_INVALID_ARG;
+    if (yz_per_x < 0 || yz_per_x > 4)
+        return fail(c, PP_ERR_INVALID_ARG, "yz_per_x must be 0..4");
+    std::lock_guard<std::mutex> lk(c->mu);
+    c->tune_ay = yz_per_x;
+    return PP_OK;
+}
+
 namespace {
 
 // ion2surf of n ions against ntri triangles; dev: the three arrays are device
@@ -2557,6 +2580,8 @@ pp_status ion2surf_run(pp_ctx *c, const float *xyz, uint64_t n, const float *tri
     }
     g.w = w;
     g.inv_w = 1.0 / w;
+    g.wyz = w;
+    g.inv_wyz = g.inv_w;
     for (int a = 0; a < 3; ++a)
         g.lo[a] = lo[a];
     const uint64_t nc = g.ncells();

@@ -71,7 +71,9 @@ struct JobDev {
     int xbits;
     const float4 *cpos;       // centers' cell-sorted ions (the full list)
     int nx, ny, nz;
-    double glo[3], gw, ginv;  // the grid: cell c spans [glo + c * gw, glo + (c + 1) * gw)
+    // the grid: cell c spans [glo + c * gw, glo + (c + 1) * gw) along x,
+    // with gwyz for gw along y and z
+    double glo[3], gw, ginv, gwyz;
     double reach;             // R widened by the rounding margins of the in-ball test
     int nrows;
     const Row *rows;

@@ -300,12 +300,12 @@ __global__ __launch_bounds__(256) void k_cellkeys(const float *__restrict__ xyz,
         return;
     const float x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
     const int cx = cell_coord(x, g.lo[0], g.inv_w, g.nx);
-    const int cy = cell_coord(y, g.lo[1], g.inv_w, g.ny);
-    const int cz = cell_coord(z, g.lo[2], g.inv_w, g.nz);
+    const int cy = cell_coord(y, g.lo[1], g.inv_wyz, g.ny);
+    const int cz = cell_coord(z, g.lo[2], g.inv_wyz, g.nz);
     uint32_t k = ((uint32_t)cz * (uint32_t)g.ny + (uint32_t)cy) * (uint32_t)g.nx + (uint32_t)cx;
     if (sub)
         k = (k << sub) | (cell_part(x, g.lo[0], g.inv_w, cx, sub - 2) << 2) |
-            (cell_half(y, g.lo[1], g.inv_w, cy) << 1) | cell_half(z, g.lo[2], g.inv_w, cz);
+            (cell_half(y, g.lo[1], g.inv_wyz, cy) << 1) | cell_half(z, g.lo[2], g.inv_wyz, cz);
     key[i] = k;
     rank[i] = atomicAdd(&count[k], 1u);
 }
@@ -830,9 +830,9 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : (T == 8 && MODE
                     b0z = fminf(b0z, q.z);
                     b1z = fmaxf(b1z, q.z);
                 }
-                const double ylo = J.glo[1] + y * J.gw, zlo = J.glo[2] + z * J.gw;
-                const double hy = fmax(0.0, fmax(ylo - (double)b1y, (double)b0y - (ylo + J.gw)));
-                const double hz = fmax(0.0, fmax(zlo - (double)b1z, (double)b0z - (zlo + J.gw)));
+                const double ylo = J.glo[1] + y * J.gwyz, zlo = J.glo[2] + z * J.gwyz;
+                const double hy = fmax(0.0, fmax(ylo - (double)b1y, (double)b0y - (ylo + J.gwyz)));
+                const double hz = fmax(0.0, fmax(zlo - (double)b1z, (double)b0z - (zlo + J.gwyz)));
                 const double h2 = hy * hy + hz * hz, lim = J.reach * J.reach;
                 // the balls' x extent in cell units: [tlo, thi]
                 double tlo = 0.0, thi = 0.0;

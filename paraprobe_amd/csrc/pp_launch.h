@@ -8,7 +8,8 @@ namespace pp {
 struct GridGeom {
     double lo[3];
     double inv_w;
-    double w;
+    double w;               // cell width along x
+    double wyz, inv_wyz;    // cell width along y and z (w / an integer)
     int nx, ny, nz;
     uint64_t ncells() const { return (uint64_t)nx * ny * nz; }
 };
