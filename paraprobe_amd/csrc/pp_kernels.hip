@@ -5,16 +5,18 @@
 // horderdist::compute_generic_spatstat3 (src/PARAPROBE_SolverHdl.cpp:3770-4176)
 // with a uniform cell list and one persistent query kernel:
 //   * cell list: bounding box, x-fastest linear cell keys computed in FP64
-//     (times 8 plus the ion's octant of its cell where those offsets stay
-//     small), a one-digit counting sort (count with atomics, exclusive scan,
-//     scatter) into cell-sorted float4 (x, y, z, type) plus the sorted
-//     ion2surf array;
+//     (times 32 plus the ion's sub-cell -- its x eighth of the cell, then its
+//     y and z halves -- where those offsets stay small), a one-digit counting
+//     sort (count with atomics, exclusive scan, scatter) into cell-sorted
+//     float4 (x, y, z, type) plus the sorted ion2surf array, and each cell's
+//     x-eighth starts;
 //   * per run of jobs (pp_prep.hip): considered-center flags, stream
 //     compaction one (job, center type) after the other, tiles of up to T
 //     centers of one type in one cell;
 //   * query (k_query, three modes): each workgroup pulls tiles from eight
 //     per-XCD tile queues; the tile's stencil rows are cut to the bounding
-//     box of its centers; the four wavefronts split the concatenation of
+//     box of its centers, their end cells to the x eighths that box's balls
+//     reach; the four wavefronts split the concatenation of
 //     the rows into 64-candidate chunks (a chunk may span rows), each lane
 //     holds one candidate and loops over the T centers held in registers,
 //     evaluating the reference's FP32
