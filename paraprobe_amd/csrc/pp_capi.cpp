@@ -272,6 +272,20 @@ struct JobPlan {
     {
         return ((families & PP_FAM_MKNN) ? nku : 0) + ((families & PP_FAM_1NN) ? 1 : 0);
     }
+    // planes per tile in the buffer: a multiple of 8, so that tile blocks and
+    // groups of planes start on cache lines
+    int npad() const { return (nplanes() + 7) & ~7; }
+    // planes per reduction work unit: as many (8, 4, 2, 1) as the unit's
+    // 16-bit LDS counters allow; 8 when even one plane's do not fit (the
+    // unit then adds straight to global memory)
+    int kb_group() const
+    {
+        const int extra = ntrg * lmax;  // the reduction's task table
+        int g = 8;
+        while (g > 1 && (ntasks * g * C + 1) / 2 + extra > kReduceLdsWords)
+            g /= 2;
+        return (ntasks * g * C + 1) / 2 + extra > kReduceLdsWords ? 8 : g;
+    }
 };
 
 pp_status plan_job(pp_ctx *c, const pp_job *j, JobPlan &p)
@@ -983,6 +997,8 @@ void fill_jobdev(JobRun &r)
     d.trg_ntask = r.tables + r.o_tntask;
     d.trg_tasks = r.tables + r.o_ttasks;
     d.nplanes = p.nplanes();
+    d.npad = p.npad();
+    d.kb_group = p.kb_group();
     d.KB = nullptr;  // set per launch chunk
     d.tiles = r.tiles;
     d.tile_base = 0;
@@ -1216,7 +1232,7 @@ pp_status run_jobs(pp_ctx *c, Slot &sl, int njobs, const pp_cloud *const *clouds
                    const pp_job *jobs, pp_hist_out *outs, Shape force = Shape(),
                    LaunchOrder *order = nullptr, uint64_t seq = 0);
 
-constexpr size_t kKbChunkBytes = size_t(1) << 30;  // k-th neighbour buffer of one launch chunk
+constexpr size_t kKbChunkBytes = size_t(2) << 30;  // k-th neighbour buffer of one launch chunk
 constexpr int kNoAllocTimedLaunches = 64;          // query launches the fixed-arena slot times
 constexpr int kReduceRows = 32768;                  // rows per reduction work unit
 
@@ -1230,7 +1246,7 @@ struct KChunk {
 // Bytes of one tile's block of k-th neighbour categories and task ids.
 size_t kb_tile_bytes(const JobPlan &p, int T)
 {
-    const int np = p.nplanes();
+    const int np = p.npad();
     if (np == 0)
         return 0;
     const size_t TR = (size_t)T * (size_t)p.lmax;
@@ -1400,11 +1416,13 @@ pp_status launch_group(pp_ctx *c, Slot &sl, std::vector<JobRun> &runs, const std
         for (int q = 0; q < nj; ++q)
             if (plans[q]->nplanes() > 0) {
                 maxTR = std::max(maxTR, T * plans[q]->lmax);
-                const int w = plans[q]->ntasks * plans[q]->C;
+                const int w = (plans[q]->ntasks * plans[q]->kb_group() * plans[q]->C + 1) / 2 +
+                              plans[q]->ntrg * plans[q]->lmax;
                 if (w <= kReduceLdsWords)
                     red_lds = std::max(red_lds, w);
             }
-        tpu = (uint32_t)std::max(1, kReduceRows / maxTR);
+        // a unit adds at most tpu * T * lmax rows to one 16-bit counter
+        tpu = (uint32_t)std::max(1, std::min(kReduceRows, 65535) / maxTR);
     }
     for (size_t k = 0; k < nch; ++k) {
         const KChunk &ch = chunks[k];
@@ -1416,10 +1434,11 @@ pp_status launch_group(pp_ctx *c, Slot &sl, std::vector<JobRun> &runs, const std
             huoff[k * (nj + 1) + q] = units;
             if (np > 0 && ch.ntl[q] > 0) {
                 const size_t TR = (size_t)T * plans[q]->lmax;
-                d.KB = kc.take<uint16_t>((size_t)ch.ntl[q] * np * TR);
+                const int G = plans[q]->kb_group();
+                d.KB = kc.take<uint16_t>((size_t)ch.ntl[q] * plans[q]->npad() * TR);
                 d.tile_base = ch.base[q];
                 d.kb_tiles = ch.ntl[q];
-                units += (uint32_t)np * ((ch.ntl[q] + tpu - 1) / tpu);
+                units += (uint32_t)((np + G - 1) / G) * ((ch.ntl[q] + tpu - 1) / tpu);
             }
             hj[k * nj + q] = d;
         }

@@ -120,16 +120,19 @@ struct JobDev {
     const uint32_t *trg_ntask;
     const uint32_t *trg_tasks;
     // k-th neighbour categories of the tiles of one launch chunk, reduced into
-    // K / N1 afterwards: per tile, nplanes planes of T * lmax uint16 (row
-    // c * lmax + l: center c of the tile, its center type's task
-    // trg_tasks[slot * lmax + l]; plane nk is the 1NN when requested).  Rows
-    // of absent centers (c >= the tile's count) and of missing tasks
-    // (l >= trg_ntask[slot]) are never written and the reduction skips them;
-    // rows of a center handed to the full-range rerun hold kKbEmpty in every
-    // plane.  Tile tl of the chunk is the job's tile tile_base + tl.
+    // K / N1 afterwards: per tile, npad (nplanes rounded up to a multiple of
+    // 8) planes of T * lmax uint16 (row c * lmax + l: center c of the tile,
+    // its center type's task trg_tasks[slot * lmax + l]; plane nk is the 1NN
+    // when requested), so that the tiles' blocks and their groups of planes
+    // start on cache lines.  Rows of absent centers (c >= the tile's count),
+    // of missing tasks (l >= trg_ntask[slot]) and of the padding planes are
+    // never written and the reduction skips them; rows of a center handed to
+    // the full-range rerun hold kKbEmpty in every plane.  Tile tl of the
+    // chunk is the job's tile tile_base + tl.
     uint16_t *KB;
     const Tile *tiles;        // the job's tiles (their slot and center count)
-    int nplanes;
+    int nplanes, npad;
+    int kb_group;             // planes per reduction work unit (8, 4, 2 or 1)
     uint32_t tile_base;
     uint32_t kb_tiles;        // tiles of this job in the chunk
     // kNN-range histograms (hybrid mode): per center, env slot e holds the

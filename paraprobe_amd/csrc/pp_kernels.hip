@@ -1180,7 +1180,7 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : (T == 8 && MODE
             // this tile's block of k-th neighbour rows in the chunk buffer
             const uint64_t tl = (uint64_t)(g - a.tile_off[job] - J.tile_base);
             const uint64_t TR = (uint64_t)T * (uint64_t)lmax;
-            uint16_t *kb = J.KB ? J.KB + tl * (uint64_t)J.nplanes * TR : nullptr;
+            uint16_t *kb = J.KB ? J.KB + tl * (uint64_t)J.npad * TR : nullptr;
             for (int c = wid; c < cnt; c += kWaves) {
                 const uint32_t m = rfl(s_meta[2 * c]);
                 const uint32_t tc = m & 0xFFu;
@@ -1419,10 +1419,13 @@ hipError_t launch_query(const QueryLaunch &q, hipStream_t s)
 // ---------------------------------------------------------------------------
 // k-th neighbour histograms of one query chunk
 
-// One work unit: one plane (an MKNN order, or the 1NN) of a block of
-// tiles_per_unit tiles of one job.  The unit's categories are counted per
-// (task, category) in LDS, then added to the job's K / N1 with one global
-// atomic per non-zero count.
+// One work unit: a group of kb_group consecutive planes (MKNN orders, or the
+// 1NN) of a block of tiles_per_unit tiles of one job.  A tile's rows of the
+// group are one contiguous run (planes are padded to a multiple of 8 per
+// tile, so the runs start on cache lines); the unit reads them two rows per
+// thread and counts the categories per (task, plane, category) in 16-bit LDS
+// counters, then adds them to the job's K / N1 with one global atomic per
+// non-zero count.
 constexpr uint32_t kRedStage = 1024;  // tiles staged in LDS at a time
 
 __global__ __launch_bounds__(256) void k_kreduce(const JobDev *__restrict__ jobs, int njobs,
@@ -1435,55 +1438,75 @@ __global__ __launch_bounds__(256) void k_kreduce(const JobDev *__restrict__ jobs
     const JobDev &J = jobs[j];
     const uint32_t lu = u - unit_off[j];
     const uint32_t nblk = (J.kb_tiles + tpu - 1) / tpu;
-    const int plane = (int)(lu / nblk);
-    const uint32_t t0 = (lu - (uint32_t)plane * nblk) * tpu;
+    const int G = J.kb_group;
+    const int grp = (int)(lu / nblk);
+    const uint32_t t0 = (lu - (uint32_t)grp * nblk) * tpu;
     const uint32_t t1 = t0 + tpu < J.kb_tiles ? t0 + tpu : J.kb_tiles;
-    const int C = J.C;
-    const uint32_t TR = (uint32_t)T * (uint32_t)J.lmax;
-    const int words = J.ntasks * C;
-    const bool use_lds = words <= lds_words;
-    const bool is1 = plane == J.nk;  // the 1NN plane follows the MKNN orders
-    uint32_t *dst = is1 ? J.N1 : J.K + (size_t)plane * C;
-    const size_t tstride = is1 ? (size_t)C : (size_t)J.nk * C;
-    if (use_lds) {
-        for (int i = threadIdx.x; i < words; i += blockDim.x)
-            h[i] = 0u;
-        __syncthreads();
-    }
-    const uint16_t *kb = J.KB + ((size_t)t0 * J.nplanes + plane) * TR;
-    const size_t pstride = (size_t)J.nplanes * TR;
-    const Tile *tiles = J.tiles + J.tile_base + t0;
+    const int p0 = grp * G;
+    const int np = J.nplanes, C = J.C;
     const uint32_t lmax = (uint32_t)J.lmax;
+    const uint32_t TR = (uint32_t)T * lmax;  // even: T >= 4
+    const int words16 = J.ntasks * G * C;
+    // the task of row l of each center type, after the counters in LDS
+    // (-1: the type has fewer tasks)
+    const int nw = (words16 + 1) / 2, ntl_task = J.ntrg * (int)lmax;
+    const bool use_lds = nw + ntl_task <= lds_words;
+    int32_t *s_tk = (int32_t *)h + (use_lds ? nw : 0);
+    for (int i = threadIdx.x; i < (use_lds ? nw : 0); i += blockDim.x)
+        h[i] = 0u;
+    const Tile *tiles = J.tiles + J.tile_base + t0;
     const uint32_t ntl = t1 - t0;
+    const uint32_t EP = (uint32_t)G * TR / 2;  // row pairs per tile in the group
+    const size_t tstride = (size_t)J.npad * TR;
+    const uint32_t *kbw = (const uint32_t *)(J.KB + (size_t)t0 * tstride + (size_t)p0 * TR);
+    const size_t twords = tstride / 2;
     // the unit's tiles go through LDS kRedStage at a time, each as its center
-    // count, center type slot and that slot's task count, so that a row needs
-    // one global load (its category) besides its task (count <= 32, slot < 32
-    // and task count <= 64 fit 8 bits each)
+    // count, center type slot and that slot's task count, so that a row pair
+    // needs one global load (its two categories) besides its tasks (count
+    // <= 32, slot < 32 and task count <= 64 fit 8 bits each)
     __shared__ uint32_t s_ti[kRedStage];
-    // one row r = c * lmax + l of staged tile st (unit tile s0 + st): absent
-    // centers of a partial tile and tasks beyond its center type's were never
-    // written, rerun rows hold kKbEmpty
-    auto row = [&](uint32_t s0, uint32_t st, uint32_t r, uint32_t c, uint32_t l) {
-        const uint32_t ti = s_ti[st];
-        if (c >= (ti & 0xFFu) || l >= (ti >> 16))
+    // one category of row r = c * lmax + l of plane pl of a tile (ti: its
+    // staged info)
+    auto count = [&](uint32_t ti, uint32_t pl, uint32_t c, uint32_t l, uint32_t cat) {
+        const int tks = use_lds ? s_tk[((ti >> 8) & 0xFFu) * lmax + l] : 0;
+        if (c >= (ti & 0xFFu) || l >= (ti >> 16) || cat == kKbEmpty)
             return;
-        const uint32_t tt = s0 + st;
-        const uint32_t cat = kb[(size_t)tt * pstride + r];
-        if (cat == kKbEmpty)
-            return;
-        const uint32_t tk = J.trg_tasks[((ti >> 8) & 0xFFu) * lmax + l];
-        if (use_lds)
-            atomicAdd(&h[tk * C + cat], 1u);
-        else
-            atomicAdd(&dst[tk * tstride + cat], 1u);
+        const uint32_t tk =
+            use_lds ? (uint32_t)tks : J.trg_tasks[((ti >> 8) & 0xFFu) * lmax + l];
+        if (use_lds) {
+            const uint32_t i16 = (tk * (uint32_t)G + pl) * (uint32_t)C + cat;
+            atomicAdd(&h[i16 >> 1], 1u << ((i16 & 1u) << 4));
+        } else {
+            const int P = p0 + (int)pl;
+            uint32_t *dst = P == J.nk ? J.N1 + (size_t)tk * C
+                                      : J.K + ((size_t)tk * J.nk + (size_t)P) * C;
+            atomicAdd(&dst[cat], 1u);
+        }
     };
-    // thread -> (tile of the step, row) when a tile's rows fit the block: the
-    // divisions once, then the block steps over `per` tiles at a time
-    const bool fits = TR <= blockDim.x;
-    const uint32_t per = fits ? blockDim.x / TR : 1u;
-    const uint32_t dt = fits ? threadIdx.x / TR : 0u;
-    const uint32_t r0 = fits ? threadIdx.x - dt * TR : 0u;
-    const uint32_t c0 = r0 / lmax, l0 = r0 - c0 * lmax;
+    // row pair e (rows 2e and 2e + 1 of the group's run: plane pl, rows r and
+    // r + 1 of it, TR being even) of staged tile st
+    auto pair = [&](uint32_t st, uint32_t v, uint32_t pl, uint32_t c0, uint32_t l0, uint32_t c1,
+                    uint32_t l1) {
+        const uint32_t ti = s_ti[st];
+        count(ti, pl, c0, l0, v & 0xFFFFu);
+        count(ti, pl, c1, l1, v >> 16);
+    };
+    // thread -> (tile of the step, row pair) when a tile's pairs fit the
+    // block: the divisions once, then the block steps over `per` tiles at a
+    // time, four tiles' loads in flight per thread
+    const bool fits = EP <= blockDim.x;
+    const uint32_t per = fits ? blockDim.x / EP : 1u;
+    const uint32_t dt = fits ? threadIdx.x / EP : 0u;
+    const uint32_t e0 = fits ? threadIdx.x - dt * EP : 0u;
+    const uint32_t pl0 = 2u * e0 / TR, r0 = 2u * e0 - pl0 * TR;
+    const uint32_t c00 = r0 / lmax, l00 = r0 - c00 * lmax;
+    const uint32_t c01 = (r0 + 1u) / lmax, l01 = r0 + 1u - c01 * lmax;
+    const bool live0 = p0 + (int)pl0 < np;  // not a padding plane
+    if (use_lds)
+        for (int i = threadIdx.x; i < ntl_task; i += blockDim.x) {
+            const int slot = i / (int)lmax, l = i - slot * (int)lmax;
+            s_tk[i] = l < (int)J.trg_ntask[slot] ? (int32_t)J.trg_tasks[i] : -1;
+        }
     for (uint32_t s0 = 0; s0 < ntl; s0 += kRedStage) {
         const uint32_t sn = ntl - s0 < kRedStage ? ntl - s0 : kRedStage;
         __syncthreads();  // the previous stage's rows are done with s_ti
@@ -1493,24 +1516,52 @@ __global__ __launch_bounds__(256) void k_kreduce(const JobDev *__restrict__ jobs
         }
         __syncthreads();
         if (fits) {
-            if (dt < per)
-                for (uint32_t st = dt; st < sn; st += per)
-                    row(s0, st, r0, c0, l0);
+            if (dt < per && live0)
+                for (uint32_t st = dt; st < sn; st += 4u * per) {
+                    uint32_t v[4];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const uint32_t sq = st + (uint32_t)q * per;
+                        v[q] = sq < sn ? kbw[(size_t)(s0 + sq) * twords + e0] : 0u;
+                    }
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const uint32_t sq = st + (uint32_t)q * per;
+                        if (sq < sn)
+                            pair(sq, v[q], pl0, c00, l00, c01, l01);
+                    }
+                }
         } else {
             for (uint32_t st = 0; st < sn; ++st)
-                for (uint32_t r = threadIdx.x; r < TR; r += blockDim.x) {
-                    const uint32_t c = r / lmax;
-                    row(s0, st, r, c, r - c * lmax);
+                for (uint32_t e = threadIdx.x; e < EP; e += blockDim.x) {
+                    const uint32_t pl = 2u * e / TR, r = 2u * e - pl * TR;
+                    if (p0 + (int)pl >= np)
+                        continue;
+                    const uint32_t c0 = r / lmax, c1 = (r + 1u) / lmax;
+                    pair(st, kbw[(size_t)(s0 + st) * twords + e], pl, c0, r - c0 * lmax, c1,
+                         r + 1u - c1 * lmax);
                 }
         }
     }
     if (use_lds) {
         __syncthreads();
-        for (int i = threadIdx.x; i < words; i += blockDim.x) {
-            const uint32_t v = h[i];
-            if (v) {
-                const int tk = i / C;
-                atomicAdd(&dst[tk * tstride + (i - tk * C)], v);
+        for (int i = threadIdx.x; i < (words16 + 1) / 2; i += blockDim.x) {
+            const uint32_t w = h[i];
+            if (!w)
+                continue;
+#pragma unroll
+            for (int half = 0; half < 2; ++half) {
+                const uint32_t cnt = (w >> (16 * half)) & 0xFFFFu;
+                const int i16 = 2 * i + half;
+                if (!cnt || i16 >= words16)
+                    continue;
+                const int tk = i16 / (G * C);
+                const int rem = i16 - tk * G * C;
+                const int pl = rem / C, cat = rem - pl * C;
+                const int P = p0 + pl;
+                uint32_t *dst = P == J.nk ? J.N1 + (size_t)tk * C
+                                          : J.K + ((size_t)tk * J.nk + (size_t)P) * C;
+                atomicAdd(&dst[cat], cnt);
             }
         }
     }

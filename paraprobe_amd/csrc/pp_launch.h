@@ -123,13 +123,16 @@ hipError_t query_max_blocks_per_cu(int T, int c16, int mode, size_t lds, int *nb
 hipError_t launch_query(const QueryLaunch &q, hipStream_t s);
 
 // Histograms of the k-th neighbour categories one query chunk wrote
-// (JobDev::KB / KT) into K and N1.  unit_off[njobs + 1] is the prefix of the
-// per-job work units (planes x tile blocks of tiles_per_unit tiles); lds_words
-// is the largest ntasks * C among the jobs that reduce through LDS.
+// (JobDev::KB) into K and N1.  unit_off[njobs + 1] is the prefix of the
+// per-job work units (groups of kb_group planes x blocks of tiles_per_unit
+// tiles); lds_words is the largest (ntasks * kb_group * C + 1) / 2 + ntrg *
+// lmax among the jobs that reduce through LDS (16-bit counters, two per word:
+// a unit adds at most tiles_per_unit * T * lmax <= 65535 rows to one counter;
+// then the task of each center type's rows).
 hipError_t launch_kreduce(const JobDev *jobs, int njobs, const uint32_t *unit_off,
                           uint32_t units, int T, uint32_t tiles_per_unit, int lds_words,
                           hipStream_t s);
-constexpr int kReduceLdsWords = 16384;  // ntasks * C above this: global atomics
+constexpr int kReduceLdsWords = 16384;  // LDS words of a reduction workgroup at most
 
 // Ion-to-surface squared distances (pp_surfdist.hip): triangle boxes and
 // per-cell triangle counts on grid g, the cell lists (cursor starts as a copy
