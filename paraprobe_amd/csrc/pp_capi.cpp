@@ -83,6 +83,10 @@ struct Slot {
     char *fixed = nullptr;
     size_t fixed_cap = 0, fixed_off = 0;
     const DevBuf *fixed_last = nullptr;
+    // instantiated graphs of the launch groups of the slot's last runs, by
+    // position in the run: the next run's capture updates them in place
+    // (same kernels and chunk count) instead of instantiating anew
+    std::vector<hipGraphExec_t> gexec;
 };
 
 // room the k-th neighbour buffer of a fixed-arena slot leaves for the launch
@@ -226,6 +230,10 @@ void slot_free(Slot &s)
         (void)hipEventDestroy(e);
     s.qev.clear();
     s.nq = 0;
+    for (hipGraphExec_t e : s.gexec)
+        if (e)
+            (void)hipGraphExecDestroy(e);
+    s.gexec.clear();
     if (s.stream) {
         (void)hipStreamDestroy(s.stream);
         s.stream = nullptr;
@@ -1310,18 +1318,16 @@ struct Keep {
     std::deque<std::vector<const Tile *>> tiles;
     bool graph = false;               // capture each group's launches into a graph
     hipStream_t stream = nullptr;     // the stream the graphs were launched on
-    std::vector<hipGraphExec_t> execs;
+    size_t ngraphs = 0;               // groups launched as graphs so far in this run
     Keep() = default;
     Keep(const Keep &) = delete;
     Keep &operator=(const Keep &) = delete;
     ~Keep()
     {
-        // the launched graphs finish before their executables go
-        if (!execs.empty()) {
+        // the launched graphs finish before the slot's next run updates
+        // their executables
+        if (ngraphs)
             (void)hipStreamSynchronize(stream);
-            for (hipGraphExec_t e : execs)
-                (void)hipGraphExecDestroy(e);
-        }
     }
 };
 
@@ -1527,12 +1533,30 @@ pp_status launch_group(pp_ctx *c, Slot &sl, std::vector<JobRun> &runs, const std
         if (le == hipSuccess)
             le = ee;
         if (le == hipSuccess) {
-            hipGraphExec_t ex = nullptr;
-            le = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
+            // this group's executable from the slot's last run, updated to
+            // the new capture when the kernels and chunk count match
+            const size_t pos = keep.ngraphs;
+            if (sl.gexec.size() <= pos)
+                sl.gexec.resize(pos + 1, nullptr);
+            hipGraphExec_t &ex = sl.gexec[pos];
+            if (ex) {
+                hipGraphNode_t bad = nullptr;
+                hipGraphExecUpdateResult res = hipGraphExecUpdateError;
+                if (hipGraphExecUpdate(ex, gr, &bad, &res) != hipSuccess ||
+                    res != hipGraphExecUpdateSuccess) {
+                    (void)hipGetLastError();
+                    (void)hipGraphExecDestroy(ex);
+                    ex = nullptr;
+                }
+            }
+            if (!ex)
+                le = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
             if (le == hipSuccess) {
                 keep.stream = sl.stream;
-                keep.execs.push_back(ex);
+                ++keep.ngraphs;
                 le = hipGraphLaunch(ex, sl.stream);
+            } else {
+                ex = nullptr;
             }
         }
         if (gr)
