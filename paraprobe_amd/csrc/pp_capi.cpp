@@ -1251,7 +1251,8 @@ struct KChunk {
     size_t bytes = 0;                 // k-th neighbour buffer bytes
 };
 
-// Bytes of one tile's block of k-th neighbour categories and task ids.
+// Bytes of one tile's block of k-th neighbour categories (planes padded to a
+// multiple of 8).
 size_t kb_tile_bytes(const JobPlan &p, int T)
 {
     const int np = p.npad();
@@ -1267,7 +1268,7 @@ void plan_chunks(const std::vector<const JobPlan *> &plans, const std::vector<ui
                  int T, size_t budget, std::vector<KChunk> &out)
 {
     const int njobs = (int)plans.size();
-    const size_t slack = 512;  // alignment of a job's two arrays in the chunk buffer
+    const size_t slack = 512;  // alignment of a job's array in the chunk buffer
     out.clear();
     KChunk cur;
     cur.base.assign(njobs, 0u);
