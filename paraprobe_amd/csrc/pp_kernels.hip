@@ -1156,19 +1156,26 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : (T == 8 && MODE
 
         // ---------------- per-center extraction ----------------
         const int cat0 = J.cat0;
+        // each center's own d = 0 pair leaves its histogram of its own type
+        // first (in a 16-bit half the count holds that pair, so nothing
+        // borrows from the other half)
+        if (tid < cnt) {
+            const uint32_t es = (s_meta[2 * tid] >> 8) & 0xFFu;
+            if (es != 0xFFu) {
+                const int h0 = (C16 ? (tid >> 1) : tid) * (PC ? EC : HK) +
+                               (PC ? (int)es * C : s_ckoff[es]) + cat0;
+                __hip_atomic_fetch_add(&s_H[h0], (C16 && (tid & 1)) ? 0xFFFF0000u : 0xFFFFFFFFu,
+                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
+        }
+        __syncthreads();
         if (PC && (J.families & (FAM_RDF | FAM_RIPK))) {
             // pair counts per (center type, env type, category); each thread
             // owns distinct (env, category) entries, so no atomics are needed
             for (int idx = tid; idx < EC; idx += kBlock) {
-                const int e = idx / C;
-                const int cat = idx - e * C;
                 for (int c = 0; c < cnt; ++c) {
                     const uint32_t w = s_H[(C16 ? (c >> 1) : c) * EC + idx];
-                    uint32_t v = C16 ? ((c & 1) ? (w >> 16) : (w & 0xFFFFu)) : w;
-                    const uint32_t m = s_meta[2 * c];
-                    if ((int)((m >> 8) & 0xFFu) == e && cat == cat0)
-                        v -= 1u;
-                    s_P[idx] += v;
+                    s_P[idx] += C16 ? ((c & 1) ? (w >> 16) : (w & 0xFFFFu)) : w;
                 }
             }
         }
@@ -1181,20 +1188,21 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : (T == 8 && MODE
             const uint64_t tl = (uint64_t)(g - a.tile_off[job] - J.tile_base);
             const uint64_t TR = (uint64_t)T * (uint64_t)lmax;
             uint16_t *kb = J.KB ? J.KB + tl * (uint64_t)J.npad * TR : nullptr;
+            // the 1NN switch as an opaque scalar, so that its per-category
+            // test is skipped, not masked, when the job has no 1NN
+            int w1 = (int)rfl(want_1 ? 1u : 0u);
+            asm volatile("" : "+s"(w1));
             for (int c = wid; c < cnt; c += kWaves) {
                 const uint32_t m = rfl(s_meta[2 * c]);
                 const uint32_t tc = m & 0xFFu;
-                const int es = (int)((m >> 8) & 0xFFu);
                 const int hoff = (C16 ? (c >> 1) : c) * (PC ? EC : HK);
                 const int sh = (C16 && (c & 1)) ? 16 : 0;
                 // count of env slot e, category cat in this center's histogram
-                // (its own d = 0 pair removed)
+                // (its own d = 0 pair already removed)
                 auto hv = [&](int e, int cat) -> uint32_t {
-                    uint32_t w = s_H[hoff + (PC ? e * C : (int)rfl((uint32_t)s_ckoff[e])) + cat];
-                    w = C16 ? ((w >> sh) & 0xFFFFu) : w;
-                    if (e == es && cat == cat0)
-                        w -= 1u;
-                    return w;
+                    const uint32_t w =
+                        s_H[hoff + (PC ? e * C : (int)rfl((uint32_t)s_ckoff[e])) + cat];
+                    return C16 ? ((w >> sh) & 0xFFFFu) : w;
                 };
                 const int L = (int)rfl(J.trg_ntask[tc]);
                 // a task whose kNN range is short of the full range must hold
@@ -1208,8 +1216,9 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : (T == 8 && MODE
                     const int ck = HY ? (int)(rfl(J.task_ck[tk]) & 0x7FFFFFFFu) : C;
                     if ((want_k || want_1) && kb) {
                         // categories of the k-th nearest neighbours: a prefix
-                        // scan over the center's category counts (lane =
-                        // category) gives each category the first k it holds;
+                        // scan over the center's category counts (two
+                        // categories per lane) gives each category the first k
+                        // it holds;
                         // the MKNN orders are then settled 64 at a time (lane =
                         // k): each category marks its first k in the wave's
                         // window row, and a running maximum over the window
@@ -1244,32 +1253,45 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : (T == 8 && MODE
                         const int e1 = one_env ? __builtin_ctz(envset) : 0;
                         const int off1 =
                             hoff + (PC ? e1 * C : (int)rfl((uint32_t)s_ckoff[e1]));
-                        for (int cb = 0; cb < ck; cb += kWave) {
-                            const int cat = cb + lane;
-                            uint32_t v = 0;
-                            if (cat < ck) {
-                                if (one_env) {
-                                    const uint32_t w = s_H[off1 + cat];
-                                    v = C16 ? ((w >> sh) & 0xFFFFu) : w;
-                                    if (e1 == es && cat == cat0)
-                                        v -= 1u;
-                                } else {
-                                    for (int e = 0; e < E; ++e)
-                                        if ((envset >> e) & 1u)
-                                            v += hv(e, cat);
+                        // 128 categories at a time, two per lane (ca, ca + 1):
+                        // one scan of the pair sums gives both prefixes
+                        for (int cb = 0; cb < ck; cb += 2 * kWave) {
+                            const int ca = cb + 2 * lane;
+                            uint32_t va = 0, vb = 0;
+                            if (one_env) {
+                                if (ca < ck) {
+                                    const uint32_t w = s_H[off1 + ca];
+                                    va = C16 ? ((w >> sh) & 0xFFFFu) : w;
                                 }
+                                if (ca + 1 < ck) {
+                                    const uint32_t w = s_H[off1 + ca + 1];
+                                    vb = C16 ? ((w >> sh) & 0xFFFFu) : w;
+                                }
+                            } else {
+                                for (int e = 0; e < E; ++e)
+                                    if ((envset >> e) & 1u) {
+                                        if (ca < ck)
+                                            va += hv(e, ca);
+                                        if (ca + 1 < ck)
+                                            vb += hv(e, ca + 1);
+                                    }
                             }
-                            const uint32_t incl = wave_incl_scan(v, lane);
-                            const uint32_t cum = carry + incl, prev = cum - v;
-                            if (want_1 && prev < 1u && cum >= 1u)
-                                kb[(uint32_t)J.nk * (uint32_t)TR + (uint32_t)row] = (uint16_t)cat;
+                            const uint32_t incl = wave_incl_scan(va + vb, lane);
+                            const uint32_t cumb = carry + incl, cuma = cumb - vb;
+                            const uint32_t preva = cuma - va;  // prevb = cuma
+                            if (w1 != 0 && preva < 1u && cumb >= 1u)
+                                kb[(uint32_t)J.nk * (uint32_t)TR + (uint32_t)row] =
+                                    (uint16_t)(cuma >= 1u ? ca : ca + 1);
                             const uint32_t cnew =
                                 carry + (uint32_t)__builtin_amdgcn_readlane((int)incl, kWave - 1);
                             if (want_k) {
-                                const uint32_t first = prev + 1u;  // this category's first k
+                                // each category's first k
+                                const uint32_t fa = preva + 1u, fb = cuma + 1u;
                                 while (kw < (uint32_t)kmax) {
-                                    if (v > 0u && first > kw && first <= kw + (uint32_t)kWave)
-                                        wrow[first - kw - 1u] = cat;
+                                    if (va > 0u && fa > kw && fa <= kw + (uint32_t)kWave)
+                                        wrow[fa - kw - 1u] = ca;
+                                    if (vb > 0u && fb > kw && fb <= kw + (uint32_t)kWave)
+                                        wrow[fb - kw - 1u] = ca + 1;
                                     const uint32_t wend = kw + (uint32_t)kWave < (uint32_t)kmax
                                                               ? kw + (uint32_t)kWave
                                                               : (uint32_t)kmax;
