@@ -1168,6 +1168,19 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : (T == 8 && MODE
                                        __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             }
         }
+        // the tasks of the tile's center type (every center of a tile has
+        // it): per task l its id, env set and kNN-range categories, read once
+        // here rather than by each center's wave in series, into the stencil
+        // rows' words (free until the next tile) as far as they hold them
+        const int Lt = (int)J.trg_ntask[ts];
+        const int Ls = Lt < nrows ? Lt : nrows;  // rows + prefix hold 3 * nrows + 1 words
+        uint32_t *s_task = s_rows;  // (the main loop's closing barrier freed them)
+        for (int l = tid; l < Ls; l += kBlock) {
+            const uint32_t tk = J.trg_tasks[ts * J.lmax + l];
+            s_task[3 * l] = tk;
+            s_task[3 * l + 1] = J.task_env[tk];
+            s_task[3 * l + 2] = HY ? J.task_ck[tk] : (uint32_t)C;
+        }
         __syncthreads();
         if (PC && (J.families & (FAM_RDF | FAM_RIPK))) {
             // pair counts per (center type, env type, category); each thread
@@ -1193,8 +1206,6 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : (T == 8 && MODE
             int w1 = (int)rfl(want_1 ? 1u : 0u);
             asm volatile("" : "+s"(w1));
             for (int c = wid; c < cnt; c += kWaves) {
-                const uint32_t m = rfl(s_meta[2 * c]);
-                const uint32_t tc = m & 0xFFu;
                 const int hoff = (C16 ? (c >> 1) : c) * (PC ? EC : HK);
                 const int sh = (C16 && (c & 1)) ? 16 : 0;
                 // count of env slot e, category cat in this center's histogram
@@ -1204,16 +1215,19 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : (T == 8 && MODE
                         s_H[hoff + (PC ? e * C : (int)rfl((uint32_t)s_ckoff[e])) + cat];
                     return C16 ? ((w >> sh) & 0xFFFFu) : w;
                 };
-                const int L = (int)rfl(J.trg_ntask[tc]);
+                const int L = Lt;
                 // a task whose kNN range is short of the full range must hold
                 // its k neighbours there, else the center is rerun over the
                 // full range (its rows here are then marked empty)
                 bool short_range = false;
                 for (int l = 0; l < L; ++l) {
-                    const int tk = (int)rfl(J.trg_tasks[tc * lmax + l]);
+                    const bool st = l < Ls;
+                    const int tk = (int)rfl(st ? s_task[3 * l] : J.trg_tasks[ts * lmax + l]);
                     const int row = c * lmax + l;
-                    const uint32_t envset = rfl(J.task_env[tk]);
-                    const int ck = HY ? (int)(rfl(J.task_ck[tk]) & 0x7FFFFFFFu) : C;
+                    const uint32_t envset = rfl(st ? s_task[3 * l + 1] : J.task_env[tk]);
+                    const uint32_t ckw =
+                        rfl(st ? s_task[3 * l + 2] : (HY ? J.task_ck[tk] : (uint32_t)C));
+                    const int ck = HY ? (int)(ckw & 0x7FFFFFFFu) : C;
                     if ((want_k || want_1) && kb) {
                         // categories of the k-th nearest neighbours: a prefix
                         // scan over the center's category counts (two
@@ -1254,28 +1268,37 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : (T == 8 && MODE
                         const int off1 =
                             hoff + (PC ? e1 * C : (int)rfl((uint32_t)s_ckoff[e1]));
                         // 128 categories at a time, two per lane (ca, ca + 1):
-                        // one scan of the pair sums gives both prefixes
-                        for (int cb = 0; cb < ck; cb += 2 * kWave) {
-                            const int ca = cb + 2 * lane;
-                            uint32_t va = 0, vb = 0;
+                        // one scan of the pair sums gives both prefixes; the
+                        // next 128 are read while these are settled
+                        auto load2 = [&](int c0, uint32_t &a, uint32_t &b) {
+                            a = 0u;
+                            b = 0u;
                             if (one_env) {
-                                if (ca < ck) {
-                                    const uint32_t w = s_H[off1 + ca];
-                                    va = C16 ? ((w >> sh) & 0xFFFFu) : w;
+                                if (c0 < ck) {
+                                    const uint32_t w = s_H[off1 + c0];
+                                    a = C16 ? ((w >> sh) & 0xFFFFu) : w;
                                 }
-                                if (ca + 1 < ck) {
-                                    const uint32_t w = s_H[off1 + ca + 1];
-                                    vb = C16 ? ((w >> sh) & 0xFFFFu) : w;
+                                if (c0 + 1 < ck) {
+                                    const uint32_t w = s_H[off1 + c0 + 1];
+                                    b = C16 ? ((w >> sh) & 0xFFFFu) : w;
                                 }
                             } else {
                                 for (int e = 0; e < E; ++e)
                                     if ((envset >> e) & 1u) {
-                                        if (ca < ck)
-                                            va += hv(e, ca);
-                                        if (ca + 1 < ck)
-                                            vb += hv(e, ca + 1);
+                                        if (c0 < ck)
+                                            a += hv(e, c0);
+                                        if (c0 + 1 < ck)
+                                            b += hv(e, c0 + 1);
                                     }
                             }
+                        };
+                        uint32_t va, vb;
+                        load2(2 * lane, va, vb);
+                        for (int cb = 0; cb < ck; cb += 2 * kWave) {
+                            const int ca = cb + 2 * lane;
+                            uint32_t na = 0u, nb = 0u;
+                            if (cb + 2 * kWave < ck)
+                                load2(ca + 2 * kWave, na, nb);
                             const uint32_t incl = wave_incl_scan(va + vb, lane);
                             const uint32_t cumb = carry + incl, cuma = cumb - vb;
                             const uint32_t preva = cuma - va;  // prevb = cuma
@@ -1303,8 +1326,10 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : (T == 8 && MODE
                             carry = cnew;
                             if (carry >= need)
                                 break;
+                            va = na;
+                            vb = nb;
                         }
-                        if (HY && carry < need && (rfl(J.task_ck[tk]) >> 31)) {
+                        if (HY && carry < need && (ckw >> 31)) {
                             short_range = true;
                             break;
                         }
