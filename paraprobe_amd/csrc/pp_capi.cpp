@@ -280,6 +280,9 @@ struct JobPlan {
     {
         return ((families & PP_FAM_MKNN) ? nku : 0) + ((families & PP_FAM_1NN) ? 1 : 0);
     }
+    // k -> order index entries the query kernel keeps in LDS: none when the
+    // orders are k = 1 .. kmax (the index is k - 1)
+    int kidx_len() const { return kmax > 0 && nku == kmax ? 0 : kmax; }
     // planes per tile in the buffer: a multiple of 8, so that tile blocks and
     // groups of planes start on cache lines
     int npad() const { return (nplanes() + 7) & ~7; }
@@ -1113,6 +1116,16 @@ double cand_density(const pp_cloud *cl, const JobPlan &p)
     return cl->density() * (frac < 0.8 ? frac : 1.0);
 }
 
+// One more than the largest type id among a cloud's ions (the query kernel's
+// per-type tables hold that many entries).
+int type_span(const pp_cloud *cl)
+{
+    int t = 255;
+    while (t > 0 && cl->type_count[t] == 0)
+        --t;
+    return t + 1;
+}
+
 int rule_r(double density, double R)
 {
     // with the stencil rows trimmed to the x parts of their end cells, larger
@@ -1344,7 +1357,7 @@ pp_status launch_group(pp_ctx *c, Slot &sl, std::vector<JobRun> &runs, const std
     const int nj = (int)ids.size();
     if (nj == 0)
         return PP_OK;
-    int maxC = 1, maxE = 1, maxTrg = 1, maxK = 0, maxRows = 1, maxHK = 1;
+    int maxC = 1, maxE = 1, maxTrg = 1, maxK = 0, maxRows = 1, maxHK = 1, ntypes = 1;
     uint32_t maxcand = 0;
     for (int q = 0; q < nj; ++q) {
         const JobRun &r = runs[ids[q]];
@@ -1352,14 +1365,16 @@ pp_status launch_group(pp_ctx *c, Slot &sl, std::vector<JobRun> &runs, const std
         maxC = std::max(maxC, p.C);
         maxE = std::max(maxE, p.E);
         maxTrg = std::max(maxTrg, p.ntrg);
-        maxK = std::max(maxK, p.kmax);
+        maxK = std::max(maxK, p.kidx_len());
         maxRows = std::max(maxRows, (int)r.rows.size());
         if (mode == 2)
             maxHK = std::max(maxHK, p.HK);
+        ntypes = std::max(ntypes, type_span(r.cloud));
         maxcand = std::max(maxcand, r.h_counts[2]);
     }
     const int c16 = maxcand < 65535u ? 1 : 0;
-    const size_t lds = query_lds_bytes(T, c16, mode, maxC, maxE, maxTrg, maxK, maxRows, maxHK);
+    const size_t lds =
+        query_lds_bytes(T, c16, mode, maxC, maxE, maxTrg, maxK, maxRows, maxHK, ntypes);
     if (lds > c->lds_per_block)
         return fail(c, PP_ERR_UNSUPPORTED, "histograms too large for LDS (bins x types x tile)");
     std::vector<uint32_t> tile_off(nj + 1, 0);
@@ -1491,6 +1506,7 @@ pp_status launch_group(pp_ctx *c, Slot &sl, std::vector<JobRun> &runs, const std
         ql.maxE = maxE;
         ql.maxTrg = maxTrg;
         ql.maxKmax = maxK;
+        ql.ntypes = ntypes;
         ql.maxRows = maxRows;
         ql.maxHK = maxHK;
         const uint32_t want = (uint32_t)c->cu_count * (uint32_t)nb;
@@ -1721,8 +1737,8 @@ pp_status run_jobs(pp_ctx *c, Slot &sl, int njobs, const pp_cloud *const *clouds
         const JobPlan &p = r.plan;
         auto need = [&](int t) {
             return query_lds_bytes(t, 0, p.mode, std::max(p.C, 1), std::max(p.E, 1),
-                                   std::max(p.ntrg, 1), p.kmax, (int)r.rows.size(),
-                                   std::max(p.HK, 1));
+                                   std::max(p.ntrg, 1), p.kidx_len(), (int)r.rows.size(),
+                                   std::max(p.HK, 1), type_span(r.cloud));
         };
         while (need(r.T_req) > c->lds_per_block && r.T_req > 8)
             r.T_req /= 2;
@@ -1807,18 +1823,19 @@ pp_status run_jobs(pp_ctx *c, Slot &sl, int njobs, const pp_cloud *const *clouds
     // layout does not fit runs job by job.  Then the full-range rerun of the
     // centers whose kNN range fell short.
     auto lds_of = [&](const std::vector<int> &ids, int mode, int T, int c16) {
-        int maxC = 1, maxE = 1, maxTrg = 1, maxK = 0, maxRows = 1, maxHK = 1;
+        int maxC = 1, maxE = 1, maxTrg = 1, maxK = 0, maxRows = 1, maxHK = 1, ntypes = 1;
         for (int i : ids) {
             const JobPlan &p = runs[i].plan;
             maxC = std::max(maxC, p.C);
             maxE = std::max(maxE, p.E);
             maxTrg = std::max(maxTrg, p.ntrg);
-            maxK = std::max(maxK, p.kmax);
+            maxK = std::max(maxK, p.kidx_len());
             maxRows = std::max(maxRows, (int)runs[i].rows.size());
             if (mode == 2)
                 maxHK = std::max(maxHK, p.HK);
+            ntypes = std::max(ntypes, type_span(runs[i].cloud));
         }
-        return query_lds_bytes(T, c16, mode, maxC, maxE, maxTrg, maxK, maxRows, maxHK);
+        return query_lds_bytes(T, c16, mode, maxC, maxE, maxTrg, maxK, maxRows, maxHK, ntypes);
     };
     struct Group {
         int mode, T, c16;
@@ -1891,8 +1908,8 @@ pp_status run_jobs(pp_ctx *c, Slot &sl, int njobs, const pp_cloud *const *clouds
                 const JobPlan &p = r.plan;
                 while (r.T_req > 8 &&
                        query_lds_bytes(r.T_req, 0, 1, std::max(p.C, 1), std::max(p.E, 1),
-                                       std::max(p.ntrg, 1), p.kmax, (int)r.rows.size(),
-                                       1) > c->lds_per_block)
+                                       std::max(p.ntrg, 1), p.kidx_len(), (int)r.rows.size(),
+                                       1, type_span(r.cloud)) > c->lds_per_block)
                     r.T_req /= 2;
             }
             st = prep_run(c, sl, runs, hy, pa, true);

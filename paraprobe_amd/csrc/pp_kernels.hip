@@ -524,21 +524,22 @@ __host__ __device__ inline int al4(int x) { return (x + 3) & ~3; }
 // center type only, 1 full-range per-center histograms, 2 kNN-range
 // per-center histograms of maxHK words.
 __host__ __device__ inline Lay make_layout(int T, int c16, int mode, int maxC, int maxE,
-                                           int maxTrg, int maxKmax, int maxRows, int maxHK)
+                                           int maxTrg, int maxKmax, int maxRows, int maxHK,
+                                           int ntypes)
 {
     Lay L;
     int o = 0;
     L.edges = o;  // the hybrid mode reads the (rarely needed) edges from global memory
     o += mode == 2 ? 0 : al4(maxC);
-    L.envoff = o;
-    o += 256;
+    L.envoff = o;  // per type id below ntypes
+    o += al4(ntypes);
     L.trgslot = o;  // unused: tiles carry their center type
     L.kinfo = o;
-    o += mode == 2 ? 512 : 0;
+    o += mode == 2 ? 2 * al4(ntypes) : 0;
     L.ckoff = o;
     o += mode == 2 ? kMaxEnv : 0;
-    L.kidx = o;
-    o += mode ? al4(maxKmax + 2) : 0;
+    L.kidx = o;  // maxKmax: the largest kmax of the jobs whose k are not 1..kmax
+    o += mode && maxKmax ? al4(maxKmax + 2) : 0;
     L.win = o;  // per wavefront: the k window of the k-th category extraction
     o += mode ? kWaves * kWave : 0;
     L.rows = o;
@@ -559,9 +560,10 @@ __host__ __device__ inline Lay make_layout(int T, int c16, int mode, int maxC, i
 }
 
 size_t query_lds_bytes(int T, int c16, int mode, int maxC, int maxE, int maxTrg, int maxKmax,
-                       int maxRows, int maxHK)
+                       int maxRows, int maxHK, int ntypes)
 {
-    return (size_t)make_layout(T, c16, mode, maxC, maxE, maxTrg, maxKmax, maxRows, maxHK).total *
+    return (size_t)make_layout(T, c16, mode, maxC, maxE, maxTrg, maxKmax, maxRows, maxHK, ntypes)
+               .total *
            4u;
 }
 
@@ -574,6 +576,7 @@ struct QArgs {
     uint32_t grab;
     int njobs;
     int maxC, maxE, maxTrg, maxKmax, maxRows, maxHK;
+    int ntypes;  // type ids of the launch's candidates are below this
 };
 
 __device__ inline int find_job(const uint32_t *__restrict__ off, int njobs, uint32_t g)
@@ -677,7 +680,9 @@ __device__ inline int category_of(float d2, float isc, float ioff1, float ioffa,
 // range only (see JobDev::HK); centers whose kNN range holds too few
 // neighbours are flagged for a MODE 1 rerun.
 // Resident waves per SIMD asked of the compiler for 8-center tiles: 7 in the
-// pair-count mode (72 VGPRs, no spills), 6 in the hybrid mode.
+// pair-count mode (72 VGPRs, no spills), 6 in the hybrid mode (7 fits in 72
+// VGPRs with spills in the per-tile code: 0.3 % faster on the 20M-ion job,
+// 13 % slower on the 100M-ion one, whose tiles are cheaper).
 template <int T, bool C16, int MODE>
 __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : (T == 8 && MODE == 2) ? 6 : 1) void k_query(QArgs a)
 {
@@ -686,7 +691,7 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : (T == 8 && MODE
     extern __shared__ uint32_t lds[];
     __shared__ uint32_t s_g, s_ctot;
     const Lay L = make_layout(T, C16, MODE, a.maxC, a.maxE, a.maxTrg, a.maxKmax, a.maxRows,
-                              a.maxHK);
+                              a.maxHK, a.ntypes);
     uint32_t *s_edges = lds + L.edges;
     int32_t *s_envoff = (int32_t *)(lds + L.envoff);
     int2 *s_kinfo = (int2 *)(lds + L.kinfo);
@@ -760,15 +765,15 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : (T == 8 && MODE
                 if (!HY)
                     for (int i = tid; i < J.C; i += kBlock)
                         s_edges[i] = J.edges[i];
-                for (int i = tid; i < 256; i += kBlock)
+                for (int i = tid; i < a.ntypes; i += kBlock)
                     s_envoff[i] = J.envoff[i];
                 if (HY) {
-                    for (int i = tid; i < 512; i += kBlock)
+                    for (int i = tid; i < 2 * a.ntypes; i += kBlock)
                         ((int32_t *)s_kinfo)[i] = J.kinfo[i];
                     for (int i = tid; i < J.E; i += kBlock)
                         s_ckoff[i] = J.ckoff[i];
                 }
-                if (PC || HY)
+                if ((PC || HY) && !J.k_dense)
                     for (int i = tid; i <= J.kmax; i += kBlock)
                         s_kidx[i] = J.kidx_of_k ? J.kidx_of_k[i] : -1;
             }
@@ -1381,7 +1386,7 @@ template <int T, bool C16, int MODE>
 static hipError_t launch_query_t(const QueryLaunch &q, hipStream_t s)
 {
     const size_t lds = query_lds_bytes(T, C16, MODE, q.maxC, q.maxE, q.maxTrg, q.maxKmax,
-                                       q.maxRows, q.maxHK);
+                                       q.maxRows, q.maxHK, q.ntypes);
     QArgs a;
     a.jobs = q.jobs;
     a.tile_off = q.tile_off;
@@ -1395,6 +1400,7 @@ static hipError_t launch_query_t(const QueryLaunch &q, hipStream_t s)
     a.maxE = q.maxE;
     a.maxTrg = q.maxTrg;
     a.maxKmax = q.maxKmax;
+    a.ntypes = q.ntypes;
     a.maxRows = q.maxRows;
     a.maxHK = q.maxHK;
     hipError_t e = hipFuncSetAttribute((const void *)k_query<T, C16, MODE>,

@@ -114,11 +114,13 @@ struct QueryLaunch {
     int T;                     // centers per tile
     int c16;                   // 16-bit per-center counters
     int mode;                  // 0 pair counts, 1 full-range per-center, 2 hybrid
+    // maxKmax: the largest kmax of the jobs whose k are not 1..kmax (0: none)
     int maxC, maxE, maxTrg, maxKmax, maxRows, maxHK;
+    int ntypes;                // type ids of the candidates are below this
     int grid;                  // workgroups (persistent)
 };
 size_t query_lds_bytes(int T, int c16, int mode, int maxC, int maxE, int maxTrg, int maxKmax,
-                       int maxRows, int maxHK);
+                       int maxRows, int maxHK, int ntypes);
 hipError_t query_max_blocks_per_cu(int T, int c16, int mode, size_t lds, int *nb);
 hipError_t launch_query(const QueryLaunch &q, hipStream_t s);
 
