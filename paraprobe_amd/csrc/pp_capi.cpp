@@ -1431,7 +1431,8 @@ pp_status launch_group(pp_ctx *c, Slot &sl, std::vector<JobRun> &runs, const std
     const std::vector<uint32_t> &htoff = keep.words.back();
     keep.tiles.emplace_back(nj);
     std::vector<const Tile *> &ht = keep.tiles.back();
-    uint32_t tpu = 1;
+    uint32_t tpu_max = 1;
+    std::vector<uint32_t> tpus(nch);  // tiles per reduction unit, per chunk
     int red_lds = 0;
     {
         int maxTR = 1;
@@ -1444,10 +1445,22 @@ pp_status launch_group(pp_ctx *c, Slot &sl, std::vector<JobRun> &runs, const std
                     red_lds = std::max(red_lds, w);
             }
         // a unit adds at most tpu * T * lmax rows to one 16-bit counter
-        tpu = (uint32_t)std::max(1, std::min(kReduceRows, 65535) / maxTR);
+        tpu_max = (uint32_t)std::max(1, std::min(kReduceRows, 65535) / maxTR);
     }
     for (size_t k = 0; k < nch; ++k) {
         const KChunk &ch = chunks[k];
+        // units of up to tpu_max tiles, but at least ~4 per CU when the
+        // chunk's groups of planes hold fewer tiles than that (small jobs)
+        uint64_t gt = 0;
+        for (int q = 0; q < nj; ++q) {
+            const int np = plans[q]->nplanes(), G = plans[q]->kb_group();
+            if (np > 0)
+                gt += (uint64_t)((np + G - 1) / G) * ch.ntl[q];
+        }
+        const uint64_t want = gt / (4u * (uint64_t)std::max(c->cu_count, 1));
+        const uint32_t tpu = tpus[k] =
+            (uint32_t)std::max<uint64_t>(std::min<uint64_t>(tpu_max, 32u),
+                                         std::min<uint64_t>(tpu_max, want));
         Carver kc(sl.kbuf.p);
         uint32_t units = 0;
         for (int q = 0; q < nj; ++q) {
@@ -1540,7 +1553,7 @@ pp_status launch_group(pp_ctx *c, Slot &sl, std::vector<JobRun> &runs, const std
         }
         const uint32_t units = huoff[k * (nj + 1) + nj];
         if (le == hipSuccess && units > 0)
-            le = launch_kreduce(d_jobs + k * nj, nj, d_uoff + k * (nj + 1), units, T, tpu,
+            le = launch_kreduce(d_jobs + k * nj, nj, d_uoff + k * (nj + 1), units, T, tpus[k],
                                 red_lds, sl.stream);
     }
     if (capture) {
