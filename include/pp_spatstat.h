@@ -280,11 +280,11 @@ pp_status pp_cloud_from_epos(pp_ctx *ctx, const void *epos, uint64_t nbytes,
                              uint64_t ntri, float R, float cell_width, pp_cloud **out);
 
 /* kNN range of MKNN / 1NN jobs without CNTNB: per neighbour type, per-center
- * histograms cover only the distance within which `factor` x k neighbours
- * are expected at the cloud's mean density over its occupied volume
- * (default 4); centers with fewer than k neighbours there are re-run over
- * the full range, so results do not depend on the factor.  0 = full-range
- * per-center histograms always. */
+ * histograms cover only the distance within which factor x max(k, (k +
+ * 6 sqrt(k) + 10) / 4) neighbours are expected at the cloud's mean density
+ * over its occupied volume (default 4); centers with fewer than k neighbours
+ * there are re-run over the full range, so results do not depend on the
+ * factor.  0 = full-range per-center histograms always. */
 pp_status pp_set_knn_range(pp_ctx *ctx, float factor);
 
 /* Launch form of the query kernel (no counterpart in the reference, whose

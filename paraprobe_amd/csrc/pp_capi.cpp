@@ -474,6 +474,11 @@ void plan_hybrid(const pp_cloud *cl, JobPlan &p, double factor)
         p.E == 0)
         return;
     const double need = mknn ? (double)p.kmax : 1.0;
+    // neighbours expected in the range: factor x max(k, (k + 6 sqrt(k) + 10) / 4)
+    // -- at the default factor 4 at least k + 6 sqrt(k) + 10, so that for small
+    // k the count's spread (about the square root of its mean) rarely leaves a
+    // center short of k and re-run over the full range
+    const double expect = factor * std::max(need, (need + 6.0 * std::sqrt(need) + 10.0) / 4.0);
     const double dens = cl->density();
     const int catR = (int)p.hg.category_d2(p.R2);
     std::vector<double> rho(p.E, 0.0);
@@ -484,7 +489,7 @@ void plan_hybrid(const pp_cloud *cl, JobPlan &p, double factor)
         for (int e = 0; e < p.E; ++e)
             if ((p.task_env[tk] >> e) & 1u)
                 dS += dens * (double)cl->type_count[p.env_type[e]] / (double)cl->n;
-        const double rS = dS > 0 ? std::cbrt(factor * need / (4.18879020478639 * dS)) : (double)p.R;
+        const double rS = dS > 0 ? std::cbrt(expect / (4.18879020478639 * dS)) : (double)p.R;
         for (int e = 0; e < p.E; ++e)
             if ((p.task_env[tk] >> e) & 1u)
                 rho[e] = std::max(rho[e], rS);

@@ -255,10 +255,11 @@ def test_resident_cell_list_any_rmax(ctx):
 EDGE_GEOMS = [(0.0, 0.01, 5.0), (0.0, 0.1, 2.0), (0.3, 0.01, 1.5), (0.0, 0.005, 3.0),
               (4.8, 0.002, 5.0)]
 # query-kernel modes: RDF alone runs the pair-count mode (0); RDF + MKNN over
-# the full range the per-center mode (1); a kNN range of 0.3 x k expected
-# neighbours the hybrid mode (2), whose short centers re-run in mode 1
+# the full range the per-center mode (1); a kNN range of 0.3 expected
+# neighbours for k = 1 (factor 0.07 x (1 + 6 + 10) / 4) the hybrid mode (2),
+# whose short centers re-run in mode 1
 EDGE_MODES = {"paircount": (FAM_RDF, 3.0, 0), "percenter": (FAM_RDF | FAM_MKNN, 0.0, 1),
-              "hybrid": (FAM_RDF | FAM_MKNN, 0.3, 2)}
+              "hybrid": (FAM_RDF | FAM_MKNN, 0.07, 2)}
 
 
 @pytest.mark.parametrize("mode", list(EDGE_MODES))
