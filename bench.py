@@ -37,8 +37,8 @@ FLOP_PER_EVAL = 8           # 3 sub + 3 mul + 2 add (no FMA: fp-contract off)
 OPS_PER_EVAL = 9            # the same plus the in-ball compare
 # HBM bytes of the query kernel per step of the 20M-ion headline job, from two
 # rocprofv3 --pmc runs of this script (FETCH_SIZE x 2 and WRITE_SIZE x 1, the
-# factors a 1 GiB device copy reads): 113.9 GB read + 13.4 GB written
-TRAFFIC_HEADLINE = 1.273e11
+# factors a 1 GiB device copy reads): 114.0 GB read + 12.0 GB written
+TRAFFIC_HEADLINE = 1.259e11
 TRAFFIC_SOURCE = "profiles/k_query_frustum20M_rdf_mknn100.md"
 
 
