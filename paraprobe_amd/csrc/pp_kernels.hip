@@ -603,6 +603,25 @@ __device__ inline uint32_t xcc_id()
 typedef float fvec4 __attribute__((ext_vector_type(4)));
 typedef __attribute__((address_space(1))) const fvec4 gfvec4;
 
+// a compile-time switch passed to a generic lambda
+template <bool B>
+struct CFlag {
+    static constexpr bool value = B;
+};
+
+typedef __attribute__((address_space(1))) uint16_t gu16;
+typedef __attribute__((address_space(1))) const uint32_t gcu32;
+
+// device-scope atomic add through the global address space: a flat atomic
+// also counts against the LDS wait counter, so the LDS reads that follow it
+// would wait for its completion
+template <typename U>
+__device__ inline void global_add(U *p, U v)
+{
+    typedef __attribute__((address_space(1))) U gU;
+    __hip_atomic_fetch_add((gU *)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // 16-byte load through the global address space (global_load_dwordx4
 // rather than a flat load)
 __device__ inline float4 load_g4(const gfvec4 *p, uint32_t i)
@@ -654,8 +673,9 @@ __device__ inline uint32_t next_tile(const QArgs &a, uint32_t myx, int &qx, uint
 // category is m = floor(ya).  When any active lane is nearer an edge (rare),
 // the wave settles every lane with the exact edge table from the nearest
 // ideal edge of the unshifted coordinate.  No LDS read on the common path.
+template <typename EdgePtr>
 __device__ inline int category_of(float d2, float isc, float ioff1, float ioffa, float amb2,
-                                  float Cm1f, const uint32_t *s_edges)
+                                  float Cm1f, EdgePtr s_edges)
 {
     const float s = __builtin_amdgcn_sqrtf(d2);
     const float ya = __builtin_fmaf(s, isc, ioffa);
@@ -746,7 +766,7 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : (T == 8 && MODE
                     for (int i = tid; i < n; i += kBlock) {
                         const uint32_t v = s_P[i];
                         if (v)
-                            atomicAdd(&dst[i], (unsigned long long)v);
+                            global_add(&dst[i], (unsigned long long)v);
                     }
                 }
                 if (lane == 0 && inball)
@@ -945,7 +965,10 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : (T == 8 && MODE
             const bool rdf = (J.families & (FAM_RDF | FAM_RIPK)) != 0;
             const bool count_in = J.count_in != 0;
             const gfvec4 *__restrict__ pos = (const gfvec4 *)J.pos;
-            const uint32_t *gedges = J.edges;  // the hybrid mode's exact edge table
+            // the hybrid mode's exact edge table, read through the global
+            // address space (a flat load's wait would also wait for the
+            // LDS atomics in flight)
+            const gcu32 *gedges = (const gcu32 *)J.edges;
             uint32_t *s_acc = PC ? s_H : s_P;
             const uint32_t ctot = rfl(s_ctot);
             const int nch = (int)((ctot + kWave - 1) / kWave);
@@ -1205,7 +1228,14 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : (T == 8 && MODE
             // this tile's block of k-th neighbour rows in the chunk buffer
             const uint64_t tl = (uint64_t)(g - a.tile_off[job] - J.tile_base);
             const uint64_t TR = (uint64_t)T * (uint64_t)lmax;
-            uint16_t *kb = J.KB ? J.KB + tl * (uint64_t)J.npad * TR : nullptr;
+            // stored through the global address space: a flat store counts
+            // against the LDS wait counter as well, so every later LDS wait
+            // of the extraction would wait for the store to complete
+            gu16 *kb = J.KB ? (gu16 *)(J.KB + tl * (uint64_t)J.npad * TR) : nullptr;
+            // read once: a load in the loop would put a wait for every store
+            // in flight before the next store
+            const uint16_t sentinel = (uint16_t)rfl((uint32_t)J.cat_sentinel);
+            const uint32_t p1nn = rfl((uint32_t)J.nk) * (uint32_t)TR;  // the 1NN plane
             // the 1NN switch as an opaque scalar, so that its per-category
             // test is skipped, not masked, when the job has no 1NN
             int w1 = (int)rfl(want_1 ? 1u : 0u);
@@ -1262,7 +1292,7 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : (T == 8 && MODE
                                 const int kid = kdense ? (int)k - 1 : s_kidx[k];
                                 if (kid >= 0)
                                     kb[(uint32_t)kid * (uint32_t)TR + (uint32_t)row] =
-                                        (uint16_t)(k <= lim ? m : J.cat_sentinel);
+                                        k <= lim ? (uint16_t)m : sentinel;
                             }
                             kw += (uint32_t)kWave;
                         };
@@ -1308,7 +1338,7 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : (T == 8 && MODE
                             const uint32_t cumb = carry + incl, cuma = cumb - vb;
                             const uint32_t preva = cuma - va;  // prevb = cuma
                             if (w1 != 0 && preva < 1u && cumb >= 1u)
-                                kb[(uint32_t)J.nk * (uint32_t)TR + (uint32_t)row] =
+                                kb[p1nn + (uint32_t)row] =
                                     (uint16_t)(cuma >= 1u ? ca : ca + 1);
                             const uint32_t cnew =
                                 carry + (uint32_t)__builtin_amdgcn_readlane((int)incl, kWave - 1);
@@ -1341,7 +1371,7 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : (T == 8 && MODE
                         // fewer than k env neighbours: R + EPSILON, i.e. the
                         // sentinel category (S.cpp:4093-4095, 4035-4036)
                         if (want_1 && carry == 0u && lane == 0)
-                            kb[(uint32_t)J.nk * (uint32_t)TR + (uint32_t)row] = (uint16_t)J.cat_sentinel;
+                            kb[p1nn + (uint32_t)row] = sentinel;
                         if (want_k)
                             while (kw < (uint32_t)kmax)
                                 flush(carry);
@@ -1511,7 +1541,10 @@ __global__ __launch_bounds__(256) void k_kreduce(const JobDev *__restrict__ jobs
     const uint32_t ntl = t1 - t0;
     const uint32_t EP = (uint32_t)G * TR / 2;  // row pairs per tile in the group
     const size_t tstride = (size_t)J.npad * TR;
-    const uint32_t *kbw = (const uint32_t *)(J.KB + (size_t)t0 * tstride + (size_t)p0 * TR);
+    // read through the global address space (a flat load would also count
+    // against the LDS wait counter, and the LDS reads between the loads would
+    // wait for them)
+    const gcu32 *kbw = (const gcu32 *)(J.KB + (size_t)t0 * tstride + (size_t)p0 * TR);
     const size_t twords = tstride / 2;
     // the unit's tiles go through LDS kRedStage at a time, each as its center
     // count, center type slot and that slot's task count, so that a row pair
@@ -1520,7 +1553,10 @@ __global__ __launch_bounds__(256) void k_kreduce(const JobDev *__restrict__ jobs
     __shared__ uint32_t s_ti[kRedStage];
     // one category of row r = c * lmax + l of plane pl of a tile (ti: its
     // staged info)
-    auto count = [&](uint32_t ti, uint32_t pl, uint32_t c, uint32_t l, uint32_t cat) {
+    // (LDS: the counters in LDS or, when they do not fit, global atomics,
+    // a compile-time choice inside the sweep so that its waits stay exact)
+    auto count = [&](auto LDS, uint32_t ti, uint32_t pl, uint32_t c, uint32_t l, uint32_t cat) {
+        constexpr bool use_lds = decltype(LDS)::value;
         const int tks = use_lds ? s_tk[((ti >> 8) & 0xFFu) * lmax + l] : 0;
         if (c >= (ti & 0xFFu) || l >= (ti >> 16) || cat == kKbEmpty)
             return;
@@ -1533,16 +1569,16 @@ __global__ __launch_bounds__(256) void k_kreduce(const JobDev *__restrict__ jobs
             const int P = p0 + (int)pl;
             uint32_t *dst = P == J.nk ? J.N1 + (size_t)tk * C
                                       : J.K + ((size_t)tk * J.nk + (size_t)P) * C;
-            atomicAdd(&dst[cat], 1u);
+            global_add(&dst[cat], 1u);
         }
     };
     // row pair e (rows 2e and 2e + 1 of the group's run: plane pl, rows r and
     // r + 1 of it, TR being even) of staged tile st
-    auto pair = [&](uint32_t st, uint32_t v, uint32_t pl, uint32_t c0, uint32_t l0, uint32_t c1,
-                    uint32_t l1) {
+    auto pair = [&](auto LDS, uint32_t st, uint32_t v, uint32_t pl, uint32_t c0, uint32_t l0,
+                    uint32_t c1, uint32_t l1) {
         const uint32_t ti = s_ti[st];
-        count(ti, pl, c0, l0, v & 0xFFFFu);
-        count(ti, pl, c1, l1, v >> 16);
+        count(LDS, ti, pl, c0, l0, v & 0xFFFFu);
+        count(LDS, ti, pl, c1, l1, v >> 16);
     };
     // thread -> (tile of the step, row pair) when a tile's pairs fit the
     // block: the divisions once, then the block steps over `per` tiles at a
@@ -1560,6 +1596,7 @@ __global__ __launch_bounds__(256) void k_kreduce(const JobDev *__restrict__ jobs
             const int slot = i / (int)lmax, l = i - slot * (int)lmax;
             s_tk[i] = l < (int)J.trg_ntask[slot] ? (int32_t)J.trg_tasks[i] : -1;
         }
+    auto sweep = [&](auto LDS) {
     for (uint32_t s0 = 0; s0 < ntl; s0 += kRedStage) {
         const uint32_t sn = ntl - s0 < kRedStage ? ntl - s0 : kRedStage;
         __syncthreads();  // the previous stage's rows are done with s_ti
@@ -1572,16 +1609,18 @@ __global__ __launch_bounds__(256) void k_kreduce(const JobDev *__restrict__ jobs
             if (dt < per && live0)
                 for (uint32_t st = dt; st < sn; st += 4u * per) {
                     uint32_t v[4];
+                    // unconditional loads (past the stage's end: its last
+                    // tile again, not counted), so that the four stay in flight
 #pragma unroll
                     for (int q = 0; q < 4; ++q) {
                         const uint32_t sq = st + (uint32_t)q * per;
-                        v[q] = sq < sn ? kbw[(size_t)(s0 + sq) * twords + e0] : 0u;
+                        v[q] = kbw[(size_t)(s0 + (sq < sn ? sq : sn - 1u)) * twords + e0];
                     }
 #pragma unroll
                     for (int q = 0; q < 4; ++q) {
                         const uint32_t sq = st + (uint32_t)q * per;
                         if (sq < sn)
-                            pair(sq, v[q], pl0, c00, l00, c01, l01);
+                            pair(LDS, sq, v[q], pl0, c00, l00, c01, l01);
                     }
                 }
         } else {
@@ -1591,11 +1630,16 @@ __global__ __launch_bounds__(256) void k_kreduce(const JobDev *__restrict__ jobs
                     if (p0 + (int)pl >= np)
                         continue;
                     const uint32_t c0 = r / lmax, c1 = (r + 1u) / lmax;
-                    pair(st, kbw[(size_t)(s0 + st) * twords + e], pl, c0, r - c0 * lmax, c1,
-                         r + 1u - c1 * lmax);
+                    pair(LDS, st, kbw[(size_t)(s0 + st) * twords + e], pl, c0, r - c0 * lmax,
+                         c1, r + 1u - c1 * lmax);
                 }
         }
     }
+    };
+    if (use_lds)
+        sweep(CFlag<true>());
+    else
+        sweep(CFlag<false>());
     if (use_lds) {
         __syncthreads();
         for (int i = threadIdx.x; i < (words16 + 1) / 2; i += blockDim.x) {
@@ -1614,7 +1658,7 @@ __global__ __launch_bounds__(256) void k_kreduce(const JobDev *__restrict__ jobs
                 const int P = p0 + pl;
                 uint32_t *dst = P == J.nk ? J.N1 + (size_t)tk * C
                                           : J.K + ((size_t)tk * J.nk + (size_t)P) * C;
-                atomicAdd(&dst[cat], cnt);
+                global_add(&dst[cat], cnt);
             }
         }
     }

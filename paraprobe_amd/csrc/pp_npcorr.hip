@@ -40,6 +40,19 @@ namespace pp {
 
 namespace {
 
+typedef float nf4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) const nf4 gnf4;
+typedef __attribute__((address_space(1))) const uint32_t gcu32;
+
+// 16-byte record load through the global address space: a flat load also
+// counts against the LDS wait counter, so the bucket counters' LDS traffic
+// would wait for the next step's records in flight
+__device__ inline float4 rec_at(const gnf4 *p, uint32_t i)
+{
+    const nf4 v = p[i];
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+
 __device__ inline uint32_t ordf(float f)
 {
     const uint32_t u = __float_as_uint(f);
@@ -145,7 +158,7 @@ __global__ __launch_bounds__(256) void k_npcorr(NpcArgs a)
     // widened by 1e-4 w, far more than the rounding of the FP32 distance
     // tests that decide membership.
     const double px = me.x, py = me.y, pz = me.z, cw = a.g.w, margin = 1e-4 * cw;
-    auto rows_for = [&](const uint32_t *cs, double ri, double ro) {
+    auto rows_for = [&](const gcu32 *cs, double ri, double ro) {
         Rows rw{0u, 0u, 0u, 0u};
         const int y = cy + lane % 5 - 2, z = cz + lane / 5 - 2;
         if (lane >= 25 || y < 0 || y >= a.g.ny || z < 0 || z >= a.g.nz)
@@ -189,7 +202,7 @@ __global__ __launch_bounds__(256) void k_npcorr(NpcArgs a)
     // on the current ones; the loads are unconditional (the last step loads
     // its own records again), so that no branch joins paths with different
     // loads in flight.
-    auto sweep = [&](const Rows &rw, const float4 *p, auto &&f) {
+    auto sweep = [&](const Rows &rw, const gnf4 *p, auto &&f) {
         const int src = lane >= 25 ? lane - 25 : lane;
         const uint32_t sb2 = __shfl(rw.b2, src, kWave), se2 = __shfl(rw.e2, src, kWave);
         const uint32_t b = lane < 25 ? rw.b1 : (lane < 50 ? sb2 : 0u);
@@ -216,7 +229,7 @@ __global__ __launch_bounds__(256) void k_npcorr(NpcArgs a)
             };
             bool has = next_range();
             bool cok = cj + lane < ce;
-            float4 cur = p[cok ? cj + lane : cj];
+            float4 cur = rec_at(p, cok ? cj + lane : cj);
             while (has) {
                 const float4 q = cur;
                 const bool ok = cok;
@@ -225,7 +238,7 @@ __global__ __launch_bounds__(256) void k_npcorr(NpcArgs a)
                 if (cj >= ce)
                     has = next_range();
                 cok = has && cj + lane < ce;
-                cur = p[cok ? cj + lane : (has ? cj : last)];
+                cur = rec_at(p, cok ? cj + lane : (has ? cj : last));
                 f(q, ok);
             }
         } else {
@@ -242,14 +255,14 @@ __global__ __launch_bounds__(256) void k_npcorr(NpcArgs a)
             // from lanes that are switched off)
             bool cok = (uint32_t)lane < total;
             const uint32_t j0 = at((uint32_t)lane);
-            float4 cur = p[cok ? j0 : first];
+            float4 cur = rec_at(p, cok ? j0 : first);
             for (uint32_t s0 = 0; s0 < total; s0 += kWave) {
                 const float4 q = cur;
                 const bool ok = cok;
                 const uint32_t sn = s0 + kWave + (uint32_t)lane;
                 cok = sn < total;
                 const uint32_t jn = at(sn);
-                cur = p[cok ? jn : first];
+                cur = rec_at(p, cok ? jn : first);
                 f(q, ok);
             }
         }
@@ -285,8 +298,8 @@ __global__ __launch_bounds__(256) void k_npcorr(NpcArgs a)
     for (int t = 0; t < a.ntasks; ++t) {
         if (!((a.trg[t * 8 + (mt >> 5)] >> (mt & 31)) & 1u))
             continue;
-        const float4 *ep = a.epos[t];
-        const uint32_t *ecs = a.ecs[t];
+        const gnf4 *ep = (const gnf4 *)a.epos[t];
+        const gcu32 *ecs = (const gcu32 *)a.ecs[t];
         if (!ecs)  // no ion of the task's neighbour types
             continue;
         // the center is among its own candidates when its type is listed:
