@@ -52,6 +52,7 @@ EXPORTED = [
     "pp_set_tuning",
     "pp_set_knn_range",
     "pp_set_cell_aspect",
+    "pp_set_pair_symmetry",
     "pp_set_graph_launch",
     "pp_cloud_add_cell_list",
     "pp_serve_create",
@@ -156,6 +157,7 @@ def load_library() -> C.CDLL:
     lib.pp_set_tuning.argtypes = [P, C.c_int32, C.c_int32]
     lib.pp_set_knn_range.argtypes = [P, C.c_float]
     lib.pp_set_cell_aspect.argtypes = [P, C.c_int32]
+    lib.pp_set_pair_symmetry.argtypes = [P, C.c_int32]
     lib.pp_set_graph_launch.argtypes = [P, C.c_int32]
     lib.pp_cloud_add_cell_list.argtypes = [P, P, C.c_float]
     lib.pp_serve_create.argtypes = [P, C.POINTER(_ServeOpts), C.POINTER(P)]
@@ -397,6 +399,11 @@ class SpatStat:
     def set_cell_aspect(self, yz_per_x: int = 0):
         """y/z cells per x cell width of the cell lists jobs build (0: built-in)."""
         self._check(self.lib.pp_set_cell_aspect(self.ctx, yz_per_x))
+
+    def set_pair_symmetry(self, on: bool = True):
+        """Symmetric pair counting of the dominant center type (on by default);
+        results do not depend on it."""
+        self._check(self.lib.pp_set_pair_symmetry(self.ctx, 1 if on else 0))
 
     def set_knn_range(self, factor: float = 4.0):
         """kNN range factor (expected neighbours per k) of MKNN / 1NN jobs; 0

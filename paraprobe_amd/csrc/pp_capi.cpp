@@ -64,6 +64,10 @@ struct CellList {
     uint2 *xsplit = nullptr;        // ncells x-part starts (xbits > 0)
     int xbits = 0;                  // x parts per cell: 2^xbits (0: none)
     int ay = 1;                     // requested y/z divisions of the x width
+    // a list of one type for symmetric pair counting: pos[].w holds the
+    // ion's ion2surf, and cellmin the least ion2surf per cell
+    int w_s2 = 0;
+    float *cellmin = nullptr;
     void *mem = nullptr;
     size_t bytes = 0;
 };
@@ -104,6 +108,7 @@ struct pp_ctx {
     int tune_T = 0, tune_r = 0;
     int tune_ay = 0;  // y/z cells per x cell width (0: rule)
     double knn_factor = 4.0;  // kNN range: expected neighbours / k (0: full range)
+    int sym_pairs = 1;        // symmetric pair counting (plan_symmetric) allowed
     int graph_launch = 1;     // batch launch groups captured into a HIP graph
     std::map<long, std::pair<int, int>> tuned;  // autotune bucket -> (T, cells per Rmax)
     Slot main;
@@ -127,6 +132,7 @@ struct pp_cloud {
     // the lists above (so the center tiles of that list index them directly)
     struct SubList {
         uint32_t mask[8];
+        int w_s2;
         const CellList *base;
         CellList cl;
     };
@@ -544,7 +550,7 @@ void plan_hybrid(const pp_cloud *cl, JobPlan &p, double factor)
 // per ion, exclusive scan of the cell counts, scatter into cell-sorted float4.
 pp_status build_on_grid(pp_ctx *c, Slot &s, const float *xyz, const uint8_t *type,
                         const float *s2, uint64_t n, const GridGeom &g, float w, CellList &L,
-                        size_t extra_scratch_bytes = 0)
+                        size_t extra_scratch_bytes = 0, int w_s2 = 0)
 {
     const uint64_t nc = g.ncells();
     // each cell's ions by sub-cell: x eighths (or halves), then y and z
@@ -562,6 +568,8 @@ pp_status build_on_grid(pp_ctx *c, Slot &s, const float *xyz, const uint8_t *typ
     m0.take<float>(n);
     if (sub)
         m0.take<uint2>(nc);
+    if (w_s2)
+        m0.take<float>(nc);
     const size_t bytes = m0.off + 256;
     if (L.mem && L.bytes < bytes) {
         PP_HIP(c, hipStreamSynchronize(s.stream));
@@ -579,6 +587,8 @@ pp_status build_on_grid(pp_ctx *c, Slot &s, const float *xyz, const uint8_t *typ
     L.s2 = m.take<float>(n);
     L.xsplit = sub ? m.take<uint2>(nc) : nullptr;
     L.xbits = sub ? sub - 2 : 0;
+    L.w_s2 = w_s2;
+    L.cellmin = w_s2 ? m.take<float>(nc) : nullptr;
     L.g = g;
     L.w = w;
     const uint64_t nk = nc << sub;
@@ -604,11 +614,13 @@ pp_status build_on_grid(pp_ctx *c, Slot &s, const float *xyz, const uint8_t *typ
     PP_HIP(c, hipMemsetAsync(count, 0, nk * sizeof(uint32_t), s.stream));
     PP_HIP(c, launch_cellkeys(xyz, n, g, count, key, rank, s.stream, sub));
     PP_HIP(c, scan_exclusive(count, start, nk, ws, s.stream));
-    PP_HIP(c, launch_scatter(xyz, type, s2, n, key, rank, start, L.pos, L.s2, s.stream));
+    PP_HIP(c, launch_scatter(xyz, type, s2, n, key, rank, start, L.pos, L.s2, s.stream, w_s2));
     if (sub) {
         PP_HIP(c, launch_stride(start, L.cellstart, nc + 1, sub, s.stream));
         PP_HIP(c, launch_xsplit(start, nc, sub, L.xsplit, s.stream));
     }
+    if (w_s2)
+        PP_HIP(c, launch_cellmin(L.cellstart, L.s2, nc, L.cellmin, s.stream));
     return PP_OK;
 }
 
@@ -648,7 +660,7 @@ pp_status build_celllist(pp_ctx *c, Slot &s, pp_cloud *cl, float w, CellList &L,
 // The ions of the types in `mask`, cell-sorted on the grid of `base`, into
 // out (its own allocation, out.mem).
 pp_status build_typed_list(pp_ctx *c, Slot &s, const pp_cloud *cl, const CellList *base,
-                           const uint32_t mask[8], CellList &out)
+                           const uint32_t mask[8], CellList &out, int w_s2 = 0)
 {
     const uint64_t n = cl->n;
     Carver t0(nullptr);
@@ -688,7 +700,7 @@ pp_status build_typed_list(pp_ctx *c, Slot &s, const pp_cloud *cl, const CellLis
         (void)hipFree(tmp);
         return fail(c, PP_ERR_HIP, std::string("gather: ") + hipGetErrorString(e));
     }
-    st = build_on_grid(c, s, sx, sty, ss2, m, base->g, base->w, out, head);
+    st = build_on_grid(c, s, sx, sty, ss2, m, base->g, base->w, out, head, w_s2);
     (void)hipStreamSynchronize(s.stream);
     (void)hipFree(tmp);
     if (st != PP_OK && out.mem) {
@@ -699,13 +711,13 @@ pp_status build_typed_list(pp_ctx *c, Slot &s, const pp_cloud *cl, const CellLis
 }
 
 // The typed candidate list of `mask` on the grid of `base` (built once per
-// cloud, base and type set; the caller holds no lock).
+// cloud, base, type set and w_s2; the caller holds no lock).
 pp_status get_sublist(pp_ctx *c, Slot &s, pp_cloud *cl, const CellList *base,
-                      const uint32_t mask[8], const CellList *&out)
+                      const uint32_t mask[8], const CellList *&out, int w_s2 = 0)
 {
     std::lock_guard<std::mutex> lk(cl->sub_mu);
     for (const auto &sb : cl->subs)
-        if (sb.base == base && std::memcmp(sb.mask, mask, 32) == 0) {
+        if (sb.base == base && sb.w_s2 == w_s2 && std::memcmp(sb.mask, mask, 32) == 0) {
             out = &sb.cl;
             return PP_OK;
         }
@@ -714,8 +726,9 @@ pp_status get_sublist(pp_ctx *c, Slot &s, pp_cloud *cl, const CellList *base,
     cl->subs.emplace_back();
     pp_cloud::SubList &sb = cl->subs.back();
     std::memcpy(sb.mask, mask, 32);
+    sb.w_s2 = w_s2;
     sb.base = base;
-    const pp_status st = build_typed_list(c, s, cl, base, mask, sb.cl);
+    const pp_status st = build_typed_list(c, s, cl, base, mask, sb.cl, w_s2);
     if (st != PP_OK) {
         cl->subs.pop_back();
         return st;
@@ -731,6 +744,15 @@ struct JobRun {
     const CellList *cl = nullptr;    // the full list: centers and tiles
     const CellList *cand = nullptr;  // candidates: cl or a typed list on its grid
     std::vector<Row> rows;
+    // symmetric pair counting (plan_symmetric): the center type slot and
+    // its type, list S (that type alone, ion2surf in w), list O (the other
+    // candidates, or none), the rows (own cell, cells before it, cells
+    // after it on S, then every row on O) and the own type's kNN reach
+    int sym_slot = -1, sym_type = -1;
+    const CellList *symS = nullptr, *symO = nullptr;
+    std::vector<Row> srows;
+    int s_nb1 = 0, s_ns = 0;
+    double kreach = -1.0;
     size_t ncells = 0;
     // device pointers: the hybrid mode's rerun marks (per sorted ion) and the
     // tables, carved from the slot's scratch arena; centers and tiles point
@@ -745,7 +767,7 @@ struct JobRun {
     // table offsets (words) inside `tables`
     size_t o_edges = 0, o_envoff = 0, o_trgslot = 0, o_ttrg = 0, o_tenv = 0, o_kidx = 0,
            o_tntask = 0, o_ttasks = 0, o_kinfo = 0, o_ckoff = 0, o_taskck = 0, o_rows = 0,
-           tables_words = 0;
+           o_srows = 0, tables_words = 0;
     // read back after prep: centers, tiles, largest stencil candidate count
     uint32_t h_counts[4] = {0, 0, 0, 0};
     uint32_t considered = 0, rerun = 0;   // centers of the job, of its full-range rerun
@@ -786,6 +808,8 @@ void layout_tables(JobRun &r)
     o += std::max(p.ntasks, 1);
     r.o_rows = (o + 1) & ~(size_t)1;
     o = r.o_rows + 2 * r.rows.size();
+    r.o_srows = o;
+    o += 2 * r.srows.size();
     r.tables_words = (o + 63) & ~(size_t)63;
 }
 
@@ -872,6 +896,8 @@ void stage_tables(const JobRun &r, uint32_t *h)
         std::memcpy(&h[r.o_taskck], p.task_ck.data(), p.task_ck.size() * 4);
     if (!r.rows.empty())
         std::memcpy(&h[r.o_rows], r.rows.data(), r.rows.size() * sizeof(Row));
+    if (!r.srows.empty())
+        std::memcpy(&h[r.o_srows], r.srows.data(), r.srows.size() * sizeof(Row));
 }
 
 // Center lists and tiles of the jobs `ids` in one pass (pp_prep.hip): per
@@ -1026,6 +1052,150 @@ void fill_jobdev(JobRun &r)
     d.task_ck = r.tables + r.o_taskck;
     d.unres = r.flags;  // reused once the center list is built
     d.count_in = r.count_in ? 1 : 0;
+    d.sym_slot = r.sym_slot;
+    d.sym_type = r.sym_type;
+    d.nsrows = (int)r.srows.size();
+    d.s_nb1 = r.s_nb1;
+    d.s_ns = r.s_ns;
+    d.srows = (const Row *)(r.tables + r.o_srows);
+    d.kreach = r.kreach;
+    d.pos_s = r.symS ? r.symS->pos : nullptr;
+    d.cellstart_s = r.symS ? r.symS->cellstart : nullptr;
+    d.xsplit_s = r.symS ? r.symS->xsplit : nullptr;
+    d.xbits_s = r.symS ? r.symS->xbits : 0;
+    d.cellmin_s = r.symS ? r.symS->cellmin : nullptr;
+    d.pos_o = r.symO ? r.symO->pos : nullptr;
+    d.cellstart_o = r.symO ? r.symO->cellstart : nullptr;
+    d.xsplit_o = r.symO ? r.symO->xsplit : nullptr;
+    d.xbits_o = r.symO ? r.symO->xbits : 0;
+}
+
+// One more than the largest type id among a cloud's ions (the query kernel's
+// per-type tables hold that many entries).
+int type_span(const pp_cloud *cl)
+{
+    int t = 255;
+    while (t > 0 && cl->type_count[t] == 0)
+        --t;
+    return t + 1;
+}
+
+// Stencil rows a job's launch in kernel mode `mode` lays out in LDS.
+int query_rows(const JobRun &r, int mode)
+{
+    return (int)(mode == 2 && r.sym_slot >= 0 ? r.srows.size() : r.rows.size());
+}
+
+// Symmetric pair counting for a hybrid-mode job with pair counts (RDF,
+// RIPK): the center type t with the most ions among the job's center types
+// that are also neighbour types, when t holds at least half of the job's
+// candidates.  A pair of two centers of t lands in the same pair-count row
+// (center type t, neighbour type t) and, its squared distance being the same
+// float either way, in the same category from either side, so the tile of
+// the center whose cell comes first in cell order counts it for both (the
+// later tile reads it only for its kNN range).  t's tiles then read t's ions
+// from a list of t alone (ion2surf in w, the least ion2surf per cell) along
+// the own cell, the cells before it and the cells after it, and the other
+// candidates from a list without t along every row.  Kept off when the
+// longer row table would cost the hybrid kernel resident workgroups.
+pp_status plan_symmetric(pp_ctx *c, Slot &sl, pp_cloud *cl, JobRun &r)
+{
+    r.sym_slot = -1;
+    r.srows.clear();
+    const JobPlan &p = r.plan;
+    if (p.mode != 2 || !(p.families & (PP_FAM_RDF | PP_FAM_RIPK)) || cl->n == 0 || sl.fixed)
+        return PP_OK;
+    // the candidates' types: every type of the cloud on the full list, the
+    // neighbour types on a typed list
+    uint32_t cmask[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    uint64_t ncand = 0;
+    for (int t = 0; t < 256; ++t)
+        if (cl->type_count[t] > 0 && (r.cand == r.cl || p.envoff[t] >= 0)) {
+            cmask[t >> 5] |= 1u << (t & 31);
+            ncand += cl->type_count[t];
+        }
+    int best = -1, bslot = -1;
+    for (int s = 0; s < p.ntrg; ++s) {
+        const int t = p.trg_type[s];
+        if (p.envoff[t] >= 0 && (best < 0 || cl->type_count[t] > cl->type_count[best])) {
+            best = t;
+            bslot = s;
+        }
+    }
+    if (best < 0 || 2 * cl->type_count[best] < ncand)
+        return PP_OK;
+    // the row table: own cell, the cells before it (rows of lower z, or of
+    // the same z and lower y, and the own row's part below x), the cells
+    // after it, then every row on the other candidates
+    std::vector<Row> before, after;
+    for (const Row &w : r.rows) {
+        if (w.dy == 0 && w.dz == 0) {
+            Row b = w, a = w;
+            b.pad = kRowBefore;
+            a.pad = kRowAfter;
+            before.push_back(b);
+            after.push_back(a);
+        } else if (w.dz < 0 || (w.dz == 0 && w.dy < 0)) {
+            before.push_back(w);
+        } else {
+            after.push_back(w);
+        }
+    }
+    uint32_t omask[8];
+    bool others = false;
+    for (int k = 0; k < 8; ++k) {
+        omask[k] = cmask[k] & ~(k == (best >> 5) ? 1u << (best & 31) : 0u);
+        others = others || omask[k] != 0;
+    }
+    const size_t nsr = 1 + before.size() + after.size() + (others ? r.rows.size() : 0);
+    if ((int)nsr > kMaxRows)
+        return PP_OK;
+    {
+        const size_t gran = 512, lds_cu = 160 * 1024;
+        auto wgs = [&](int nrows) {
+            const size_t l = query_lds_bytes(r.T_req, 1, 2, std::max(p.C, 1), std::max(p.E, 1),
+                                             std::max(p.ntrg, 1), p.kidx_len(), nrows,
+                                             std::max(p.HK, 1), type_span(cl));
+            return std::min<size_t>(6, lds_cu / ((l + gran - 1) / gran * gran));
+        };
+        if (wgs((int)nsr) < wgs((int)r.rows.size()))
+            return PP_OK;
+    }
+    uint32_t smask[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    smask[best >> 5] = 1u << (best & 31);
+    const CellList *S = nullptr, *O = nullptr;
+    pp_status st = get_sublist(c, sl, cl, r.cl, smask, S, 1);
+    if (st != PP_OK)
+        return st;
+    if (S == r.cl || !S->w_s2)
+        return PP_OK;
+    if (others) {
+        st = get_sublist(c, sl, cl, r.cl, omask, O, 0);
+        if (st != PP_OK)
+            return st;
+        if (O == r.cl)
+            return PP_OK;
+    }
+    r.srows.push_back(Row{0, 0, 0, 0});
+    r.srows.insert(r.srows.end(), before.begin(), before.end());
+    r.srows.insert(r.srows.end(), after.begin(), after.end());
+    r.s_nb1 = 1 + (int)before.size();
+    r.s_ns = r.s_nb1 + (int)after.size();
+    if (others)
+        r.srows.insert(r.srows.end(), r.rows.begin(), r.rows.end());
+    r.symS = S;
+    r.symO = O;
+    r.sym_slot = bslot;
+    r.sym_type = best;
+    // t's kNN range (d2 <= its threshold), widened like the in-ball reach
+    r.kreach = -1.0;
+    if (!p.kinfo.empty() && p.kinfo[2 * best] >= 0) {
+        float kthr;
+        std::memcpy(&kthr, &p.kinfo[2 * best + 1], 4);
+        if (kthr >= 0.0f)
+            r.kreach = std::sqrt((double)kthr) * (1.0 + std::ldexp(1.0, -12)) + 1e-4 * r.cl->g.w;
+    }
+    return PP_OK;
 }
 
 void compose_outputs(const JobRun &r, const std::vector<unsigned long long> &P,
@@ -1121,15 +1291,6 @@ double cand_density(const pp_cloud *cl, const JobPlan &p)
     return cl->density() * (frac < 0.8 ? frac : 1.0);
 }
 
-// One more than the largest type id among a cloud's ions (the query kernel's
-// per-type tables hold that many entries).
-int type_span(const pp_cloud *cl)
-{
-    int t = 255;
-    while (t > 0 && cl->type_count[t] == 0)
-        --t;
-    return t + 1;
-}
 
 int rule_r(double density, double R)
 {
@@ -1371,7 +1532,7 @@ pp_status launch_group(pp_ctx *c, Slot &sl, std::vector<JobRun> &runs, const std
         maxE = std::max(maxE, p.E);
         maxTrg = std::max(maxTrg, p.ntrg);
         maxK = std::max(maxK, p.kidx_len());
-        maxRows = std::max(maxRows, (int)r.rows.size());
+        maxRows = std::max(maxRows, query_rows(r, mode));
         if (mode == 2)
             maxHK = std::max(maxHK, p.HK);
         ntypes = std::max(ntypes, type_span(r.cloud));
@@ -1643,12 +1804,14 @@ pp_status run_jobs(pp_ctx *c, Slot &sl, int njobs, const pp_cloud *const *clouds
     std::vector<JobRun> runs(njobs);
     int tuneT, tuneR, tuneAY, graph_launch;
     double knn_factor;
+    int sym_pairs;
     {
         std::lock_guard<std::mutex> lk(c->mu);
         tuneT = c->tune_T;
         tuneR = c->tune_r;
         tuneAY = c->tune_ay;
         knn_factor = c->knn_factor;
+        sym_pairs = c->sym_pairs;
         graph_launch = c->graph_launch;
     }
     for (int i = 0; i < njobs; ++i) {
@@ -1740,6 +1903,11 @@ pp_status run_jobs(pp_ctx *c, Slot &sl, int njobs, const pp_cloud *const *clouds
         if ((int)runs[i].rows.size() > kMaxRows)
             return fail(c, PP_ERR_UNSUPPORTED, "stencil too large (cell width far below Rmax)");
         runs[i].ncells = runs[i].cl->g.ncells();
+        if (sym_pairs) {
+            st = plan_symmetric(c, sl, const_cast<pp_cloud *>(clouds[i]), runs[i]);
+            if (st != PP_OK)
+                return st;
+        }
     }
     struct Back {
         std::vector<unsigned long long> P;
@@ -1755,7 +1923,7 @@ pp_status run_jobs(pp_ctx *c, Slot &sl, int njobs, const pp_cloud *const *clouds
         const JobPlan &p = r.plan;
         auto need = [&](int t) {
             return query_lds_bytes(t, 0, p.mode, std::max(p.C, 1), std::max(p.E, 1),
-                                   std::max(p.ntrg, 1), p.kidx_len(), (int)r.rows.size(),
+                                   std::max(p.ntrg, 1), p.kidx_len(), query_rows(r, p.mode),
                                    std::max(p.HK, 1), type_span(r.cloud));
         };
         while (need(r.T_req) > c->lds_per_block && r.T_req > 8)
@@ -1848,7 +2016,7 @@ pp_status run_jobs(pp_ctx *c, Slot &sl, int njobs, const pp_cloud *const *clouds
             maxE = std::max(maxE, p.E);
             maxTrg = std::max(maxTrg, p.ntrg);
             maxK = std::max(maxK, p.kidx_len());
-            maxRows = std::max(maxRows, (int)runs[i].rows.size());
+            maxRows = std::max(maxRows, query_rows(runs[i], mode));
             if (mode == 2)
                 maxHK = std::max(maxHK, p.HK);
             ntypes = std::max(ntypes, type_span(runs[i].cloud));
@@ -2552,6 +2720,17 @@ pp_status pp_set_tuning(pp_ctx *c, int32_t centers_per_tile, int32_t cells_per_r
     std::lock_guard<std::mutex> lk(c->mu);
     c->tune_T = centers_per_tile;
     c->tune_r = cells_per_rmax;
+    return PP_OK;
+}
+
+pp_status pp_set_pair_symmetry(pp_ctx *c, int32_t on)
+{
+    if (!c)
+        return PP_ERR_INVALID_ARG;
+    if (on != 0 && on != 1)
+        return fail(c, PP_ERR_INVALID_ARG, "on must be 0 or 1");
+    std::lock_guard<std::mutex> lk(c->mu);
+    c->sym_pairs = on;
     return PP_OK;
 }
 

@@ -40,6 +40,11 @@ struct Row {
     int16_t dy, dz, dxm, pad;
 };
 
+// Row::pad flags of the symmetric rows: the part of the own row before the
+// own cell (x < cx), or after it (x > cx)
+constexpr int16_t kRowBefore = 1;
+constexpr int16_t kRowAfter = 2;
+
 // One query tile: up to T considered centers of one type in one cell of one job.
 struct Tile {
     uint32_t key;    // linear cell key
@@ -148,6 +153,31 @@ struct JobDev {
     const int32_t *ckoff;
     const uint32_t *task_ck;
     uint32_t *unres;
+    // Symmetric pair counting (hybrid mode): the tiles of center type slot
+    // sym_slot (-1: none) read the candidates of their own type from list S,
+    // whose float4 w holds the ion's ion2surf (every ion of S has that type),
+    // along srows[0, s_ns): the own cell (srows[0]), then the cells before it
+    // in cell order (srows[1, s_nb1)), then the cells after it.  A pair with a
+    // center in a cell after the own cell is counted for both centers (the
+    // same distance, so the same category, and the same pair-count row);
+    // with a center in a cell before it, it is left to that center's tile
+    // (only its kNN range is read there).  A row before the own cell whose
+    // own-type ions are all centers (cellmin_s: the least ion2surf per cell)
+    // is cut to the kNN reach.  The other candidates come from list O along
+    // srows[s_ns, nsrows), counted as in any other tile.
+    int sym_slot, sym_type;
+    int nsrows, s_nb1, s_ns;
+    const Row *srows;
+    double kreach;            // the own type's kNN range widened as reach (< 0: none)
+    const float4 *pos_s;
+    const uint32_t *cellstart_s;
+    const uint2 *xsplit_s;
+    int xbits_s;
+    const float *cellmin_s;
+    const float4 *pos_o;
+    const uint32_t *cellstart_o;
+    const uint2 *xsplit_o;
+    int xbits_o;
 };
 
 } // namespace pp

@@ -77,16 +77,17 @@ __device__ inline uint32_t wave_incl_scan(uint32_t v, int lane)
     return v;
 }
 
-// Inclusive running maximum over the 64 lanes of values >= -1, with the same
-// DPP moves as wave_incl_scan (a lane without a source takes -1)
-__device__ inline int wave_max_scan(int v)
+// Inclusive running maximum over the 64 lanes, with the same DPP moves as
+// wave_incl_scan (a lane without a source takes 0, so that each step is one
+// DPP-fused maximum)
+__device__ inline uint32_t wave_max_scan(uint32_t v)
 {
-    v = __builtin_elementwise_max(v, __builtin_amdgcn_update_dpp(-1, v, 0x111, 0xF, 0xF, false));
-    v = __builtin_elementwise_max(v, __builtin_amdgcn_update_dpp(-1, v, 0x112, 0xF, 0xF, false));
-    v = __builtin_elementwise_max(v, __builtin_amdgcn_update_dpp(-1, v, 0x114, 0xF, 0xF, false));
-    v = __builtin_elementwise_max(v, __builtin_amdgcn_update_dpp(-1, v, 0x118, 0xF, 0xF, false));
-    v = __builtin_elementwise_max(v, __builtin_amdgcn_update_dpp(-1, v, 0x142, 0xA, 0xF, false));
-    v = __builtin_elementwise_max(v, __builtin_amdgcn_update_dpp(-1, v, 0x143, 0xC, 0xF, false));
+    v = __builtin_elementwise_max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xF, 0xF, false));
+    v = __builtin_elementwise_max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xF, 0xF, false));
+    v = __builtin_elementwise_max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xF, 0xF, false));
+    v = __builtin_elementwise_max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xF, 0xF, false));
+    v = __builtin_elementwise_max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xA, 0xF, false));
+    v = __builtin_elementwise_max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xC, 0xF, false));
     return v;
 }
 
@@ -383,25 +384,50 @@ __global__ __launch_bounds__(256) void k_scatter(const float *__restrict__ xyz,
                                                  const uint32_t *__restrict__ rank,
                                                  const uint32_t *__restrict__ cellstart,
                                                  float4 *__restrict__ pos,
-                                                 float *__restrict__ s2sorted)
+                                                 float *__restrict__ s2sorted, int w_s2)
 {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n)
         return;
     const uint32_t d = cellstart[key[i]] + rank[i];
     pos[d] = make_float4(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2],
-                         __uint_as_float((uint32_t)type[i]));
+                         w_s2 ? s2[i] : __uint_as_float((uint32_t)type[i]));
     s2sorted[d] = s2[i];
 }
 
 hipError_t launch_scatter(const float *xyz, const uint8_t *type, const float *s2, uint64_t n,
                           const uint32_t *key, const uint32_t *rank, const uint32_t *cellstart,
-                          float4 *pos, float *s2sorted, hipStream_t s)
+                          float4 *pos, float *s2sorted, hipStream_t s, int w_s2)
 {
     if (n == 0)
         return hipSuccess;
     hipLaunchKernelGGL(k_scatter, dim3(grid_for(n, 256)), dim3(256), 0, s, xyz, type, s2, n, key,
-                       rank, cellstart, pos, s2sorted);
+                       rank, cellstart, pos, s2sorted, w_s2);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void k_cellmin(const uint32_t *__restrict__ cellstart,
+                                                 const float *__restrict__ s2, uint64_t nc,
+                                                 float *__restrict__ out)
+{
+    const uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc)
+        return;
+    float m = __builtin_inff();
+    for (uint32_t i = cellstart[c]; i < cellstart[c + 1]; ++i) {
+        const float v = s2[i];
+        m = v >= m ? m : (v == v ? v : -__builtin_inff());
+    }
+    out[c] = m;
+}
+
+hipError_t launch_cellmin(const uint32_t *cellstart, const float *s2sorted, uint64_t ncells,
+                          float *out, hipStream_t s)
+{
+    if (ncells == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_cellmin, dim3(grid_for(ncells, 256)), dim3(256), 0, s, cellstart,
+                       s2sorted, ncells, out);
     return hipGetLastError();
 }
 
@@ -540,11 +566,11 @@ __host__ __device__ inline Lay make_layout(int T, int c16, int mode, int maxC, i
     o += mode == 2 ? kMaxEnv : 0;
     L.kidx = o;  // maxKmax: the largest kmax of the jobs whose k are not 1..kmax
     o += mode && maxKmax ? al4(maxKmax + 2) : 0;
-    L.win = o;  // per wavefront: the k window of the k-th category extraction
-    o += mode ? kWaves * kWave : 0;
-    L.rows = o;
+    L.win = o;  // per wavefront: the k window of the k-th category extraction (uint16)
+    o += mode ? kWaves * kWave / 2 : 0;
+    L.rows = o;  // per row its first and end candidate
     o += al4(2 * maxRows);
-    L.rowpre = o;
+    L.rowpre = o;  // per row its first index in the rows' concatenation
     o += al4(maxRows + 1);
     L.cen = o;
     o += 4 * T;
@@ -593,6 +619,15 @@ __device__ inline int find_job(const uint32_t *__restrict__ off, int njobs, uint
 }
 
 __device__ inline uint32_t rfl(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ inline float rflf(float v) { return __uint_as_float(rfl(__float_as_uint(v))); }
+// a wave-uniform pointer into scalar registers (fields of a job record are
+// read by vector loads, which leave even uniform values in vector registers)
+template <typename P>
+__device__ inline P *rflp(P *p)
+{
+    const uint64_t u = (uint64_t)p;
+    return (P *)(((uint64_t)rfl((uint32_t)(u >> 32)) << 32) | (uint64_t)rfl((uint32_t)u));
+}
 
 // XCC_ID hardware register (accelerator complex die of this wave)
 __device__ inline uint32_t xcc_id()
@@ -717,7 +752,7 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : (T == 8 && MODE
     int2 *s_kinfo = (int2 *)(lds + L.kinfo);
     int32_t *s_ckoff = (int32_t *)(lds + L.ckoff);
     int32_t *s_kidx = (int32_t *)(lds + L.kidx);
-    int32_t *s_win = (int32_t *)(lds + L.win);
+    uint16_t *s_win = (uint16_t *)(lds + L.win);
     uint32_t *s_rows = lds + L.rows;
     uint32_t *s_rowpre = lds + L.rowpre;
     float4 *s_cen = (float4 *)(lds + L.cen);
@@ -812,6 +847,12 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : (T == 8 && MODE
         const int C = J.C, E = J.E, EC = E * C;
         const int HK = HY ? J.HK : 0;
         const int nrows = J.nrows;
+        // symmetric pair counting for this tile's center type: its own-type
+        // candidates along srows[0, s_ns) of list S, the others along
+        // srows[s_ns, nsrows) of list O
+        const bool sym = HY && ts == J.sym_slot;
+        const int nrt = sym ? J.nsrows : nrows;
+        const Row *rtab = sym ? J.srows : J.rows;
         const int cx = (int)(key % (uint32_t)J.nx);
         const int cy = (int)((key / (uint32_t)J.nx) % (uint32_t)J.ny);
         const int cz = (int)(key / ((uint32_t)J.nx * (uint32_t)J.ny));
@@ -834,13 +875,23 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : (T == 8 && MODE
             s_cen[tid] = make_float4(px, py, pz, 0.f);
         }
         __syncthreads();
-        for (int r = tid; r < nrows; r += kBlock) {
-            const Row rw = J.rows[r];
+        for (int r = tid; r < nrt; r += kBlock) {
+            const Row rw = rtab[r];
+            // the row's candidate list
+            const bool inS = sym && r < J.s_ns;
+            const uint32_t *cst = inS ? J.cellstart_s : (sym ? J.cellstart_o : J.cellstart);
+            const uint2 *xsp = inS ? J.xsplit_s : (sym ? J.xsplit_o : J.xsplit);
+            const int xb = inS ? J.xbits_s : (sym ? J.xbits_o : J.xbits);
             uint32_t rs = 0, re = 0;
             const int y = cy + rw.dy, z = cz + rw.dz;
             if (y >= 0 && y < J.ny && z >= 0 && z < J.nz) {
                 int x0 = cx - rw.dxm < 0 ? 0 : cx - rw.dxm;
                 int x1 = cx + rw.dxm >= J.nx ? J.nx - 1 : cx + rw.dxm;
+                // the parts of the own row before and after the own cell
+                if (rw.pad & kRowBefore)
+                    x1 = x1 < cx - 1 ? x1 : cx - 1;
+                if (rw.pad & kRowAfter)
+                    x0 = x0 > cx + 1 ? x0 : cx + 1;
                 // the stencil row cut to the tile's centers: their bounding
                 // box's distance in y and z from the row's cells bounds the x
                 // extent of their balls (exact cell faces lo + c * w in FP64,
@@ -860,12 +911,15 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : (T == 8 && MODE
                 const double ylo = J.glo[1] + y * J.gwyz, zlo = J.glo[2] + z * J.gwyz;
                 const double hy = fmax(0.0, fmax(ylo - (double)b1y, (double)b0y - (ylo + J.gwyz)));
                 const double hz = fmax(0.0, fmax(zlo - (double)b1z, (double)b0z - (zlo + J.gwyz)));
-                const double h2 = hy * hy + hz * hz, lim = J.reach * J.reach;
-                // the balls' x extent in cell units: [tlo, thi]
+                const double h2 = hy * hy + hz * hz;
+                // the balls' x extent in cell units, [tlo, thi], for radius
+                // sqrt(lim); x0 .. x1 cut to it
                 double tlo = 0.0, thi = 0.0;
-                if (h2 > lim) {
-                    x1 = x0 - 1;
-                } else {
+                auto cut = [&](double lim) {
+                    if (h2 > lim) {
+                        x1 = x0 - 1;
+                        return;
+                    }
                     const double ex = sqrt(lim - h2);
                     tlo = ((double)b0x - ex - J.glo[0]) * J.ginv;
                     thi = ((double)b1x + ex - J.glo[0]) * J.ginv;
@@ -874,33 +928,48 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : (T == 8 && MODE
                         x0 = (int)a;
                     if (b < (double)x1)
                         x1 = (int)b;
+                };
+                cut(J.reach * J.reach);
+                const uint32_t kr = ((uint32_t)z * (uint32_t)J.ny + (uint32_t)y) * (uint32_t)J.nx;
+                if (inS && r >= 1 && r < J.s_nb1 && x0 <= x1) {
+                    // a row before the own cell whose own-type ions are all
+                    // centers: their pairs with this tile are counted from
+                    // their own tiles, so only the kNN reach is left
+                    bool all_ctr = true;
+                    for (int x = x0; x <= x1; ++x)
+                        all_ctr = all_ctr && J.cellmin_s[kr + (uint32_t)x] >= J.R2;
+                    if (all_ctr) {
+                        if (J.kreach > 0.0)
+                            cut(J.kreach * J.kreach);
+                        else
+                            x1 = x0 - 1;
+                    }
                 }
                 if (x0 <= x1) {
-                    const uint32_t k0 = ((uint32_t)z * (uint32_t)J.ny + (uint32_t)y) *
-                                            (uint32_t)J.nx + (uint32_t)x0;
+                    const uint32_t k0 = kr + (uint32_t)x0;
                     const uint32_t k1 = k0 + (uint32_t)(x1 - x0);
-                    rs = J.cellstart[k0];
-                    re = J.cellstart[k1 + 1u];
-                    if (J.xbits) {
+                    rs = cst[k0];
+                    re = cst[k1 + 1u];
+                    if (xb) {
                         // the end cells' x parts beyond [tlo, thi]: an ion of
                         // cell x0 with x >= the balls' lower bound has
                         // t >= tlo, so its part is at least qa (the same
                         // monotone FP64 expression as the build); likewise at
                         // most qb in cell x1
-                        const int P = 1 << J.xbits;
+                        const int P = 1 << xb;
                         if (floor(tlo) == (double)x0) {
                             const double f = floor((tlo - (double)x0) * (double)P);
                             const int qa = f < 0.0 ? 0 : (f > (double)(P - 1) ? P - 1 : (int)f);
-                            const uint2 rec = J.xsplit[k0];
+                            const uint2 rec = xsp[k0];
                             if (qa > 0 && !xsplit_whole(rec))
                                 rs += xsplit_at(rec, qa);
                         }
                         if (floor(thi) == (double)x1) {
                             const double f = floor((thi - (double)x1) * (double)P);
                             const int qb = f < 0.0 ? 0 : (f > (double)(P - 1) ? P - 1 : (int)f);
-                            const uint2 rec = J.xsplit[k1];
+                            const uint2 rec = xsp[k1];
                             if (qb < P - 1 && !xsplit_whole(rec))
-                                re = J.cellstart[k1] + xsplit_at(rec, qb + 1);
+                                re = cst[k1] + xsplit_at(rec, qb + 1);
                         }
                         if (re < rs)
                             re = rs;
@@ -920,18 +989,18 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : (T == 8 && MODE
             // candidate prefix over the rows: the tile's candidates are the
             // concatenation of its rows, cut into 64-candidate chunks
             uint32_t carry = 0;
-            for (int r0 = 0; r0 < nrows; r0 += kWave) {
+            for (int r0 = 0; r0 < nrt; r0 += kWave) {
                 const int r = r0 + lane;
                 uint32_t len = 0;
-                if (r < nrows)
+                if (r < nrt)
                     len = s_rows[2 * r + 1] - s_rows[2 * r];
                 const uint32_t inc = wave_incl_scan(len, lane);
-                if (r < nrows)
+                if (r < nrt)
                     s_rowpre[r] = carry + inc - len;
                 carry += __shfl(inc, kWave - 1, kWave);
             }
             if (lane == 0) {
-                s_rowpre[nrows] = carry;
+                s_rowpre[nrt] = carry;
                 s_ctot = carry;
             }
         }
@@ -964,32 +1033,51 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : (T == 8 && MODE
             const float Cm1f = (float)(C - 1);
             const bool rdf = (J.families & (FAM_RDF | FAM_RIPK)) != 0;
             const bool count_in = J.count_in != 0;
-            const gfvec4 *__restrict__ pos = (const gfvec4 *)J.pos;
             // the hybrid mode's exact edge table, read through the global
             // address space (a flat load's wait would also wait for the
             // LDS atomics in flight)
-            const gcu32 *gedges = (const gcu32 *)J.edges;
+            const gcu32 *gedges = rflp((const gcu32 *)J.edges);
             uint32_t *s_acc = PC ? s_H : s_P;
             const uint32_t ctot = rfl(s_ctot);
-            const int nch = (int)((ctot + kWave - 1) / kWave);
             unsigned long long my_in = 0;
-            // each wavefront takes a contiguous block of the tile's chunks;
-            // chunk q holds candidates q * 64 .. q * 64 + 63 of the rows'
-            // concatenation, so a chunk may span rows and only the tile's
-            // last chunk is partly empty.  Every lane keeps its own cursor:
-            // row r, index j, row end re (r == nrows: past the last row).
-            // Two chunks alternate, so that the load of the next is in
-            // flight while one is evaluated against the T centers.
+            // symmetric tiles: the own type's pair-count row and kNN-range
+            // entries (uniform), and the candidate indices where the cells
+            // before the own cell (sb1) and after it (sb2) start in list S
+            int woff_t = 0, kof_t = 0;
+            float kthr_t = -1.0f;
+            uint32_t sb1 = 0, sb2 = 0;
+            if (HY && sym) {
+                woff_t = (int)rfl((uint32_t)s_envoff[J.sym_type]);
+                const int2 ki = s_kinfo[J.sym_type];
+                kof_t = (int)rfl((uint32_t)ki.x);
+                kthr_t = __int_as_float((int)rfl((uint32_t)ki.y));
+                sb1 = rfl(s_rowpre[1]);
+                sb2 = rfl(s_rowpre[J.s_nb1]);
+            }
+            // One pass over rows [rlo, rhi) of list pos, whose candidates
+            // are clo .. chi - 1 of the tile's concatenation.  Each
+            // wavefront takes a contiguous block of the pass's chunks; chunk
+            // q holds candidates clo + q * 64 .. clo + q * 64 + 63, so a chunk
+            // may span rows and only the pass's last chunk is partly empty.
+            // Every lane keeps its own cursor: row r, index j, row end re
+            // (r == rhi: past the last row).  Two chunks alternate, so that
+            // the load of the next is in flight while one is evaluated
+            // against the T centers.  SYMF: the own-type pass of a symmetric
+            // tile.
+            auto sweep = [&](auto SYMF, const gfvec4 *__restrict__ pos, int rlo, int rhi,
+                             uint32_t clo, uint32_t chi) {
+            constexpr bool SY = HY && decltype(SYMF)::value;
+            const int nch = (int)((chi - clo + kWave - 1) / kWave);
             const int qa = (int)rfl((uint32_t)(wid * nch / kWaves));
             const int qb = (int)rfl((uint32_t)((wid + 1) * nch / kWaves));
-            int r = nrows;
+            int r = rhi;
             uint32_t j = 0, re = 0xFFFFFFFFu;
             if (qa < qb) {
                 // the row of candidate s0: the first h with rowpre[h] > s0,
                 // less one (that row is non-empty)
-                const uint32_t s0 = (uint32_t)qa * kWave + (uint32_t)lane;
-                if (s0 < ctot) {
-                    int lo = 0, hi = nrows;
+                const uint32_t s0 = clo + (uint32_t)qa * kWave + (uint32_t)lane;
+                if (s0 < chi) {
+                    int lo = rlo, hi = rhi;
                     while (lo < hi) {
                         const int mid = (lo + hi) >> 1;
                         if (s_rowpre[mid] <= s0)
@@ -1004,7 +1092,7 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : (T == 8 && MODE
             }
             // the chunk at the cursor (lanes past the last row load ion 0)
             auto fetch = [&](bool &ok) {
-                ok = r < nrows;
+                ok = r < rhi;
                 return load_g4(pos, j);
             };
             // the next chunk: 64 candidates on, across row ends (empty rows
@@ -1012,8 +1100,8 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : (T == 8 && MODE
             auto advance = [&]() {
                 j += kWave;
                 while (j >= re) {
-                    if (r + 1 >= nrows) {
-                        r = nrows;
+                    if (r + 1 >= rhi) {
+                        r = rhi;
                         j = 0;
                         re = 0xFFFFFFFFu;
                         break;
@@ -1024,15 +1112,17 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : (T == 8 && MODE
                     re = s_rows[2 * r + 1];
                 }
             };
-            auto body = [&](float4 p, const bool ok) {
+            auto body = [&](float4 p, const bool ok, int q) {
                 // keep every use of this chunk's data below this point, so the
                 // wait for its load is not hoisted into the prefetch of the next
                 asm volatile("" : "+v"(p.x), "+v"(p.y), "+v"(p.z), "+v"(p.w));
                 const float px = ok ? p.x : __builtin_nanf("");
-                const uint32_t ptype = __float_as_uint(p.w) & 0xFFu;
+                // (the own-type pass of a symmetric tile knows the type, and
+                // its w holds ion2surf)
+                const uint32_t ptype = SY ? 0u : __float_as_uint(p.w) & 0xFFu;
                 // (a lane past the row end has a NaN x, so every distance
                 // test fails and its type's table entries go unused)
-                const int woff = s_envoff[ptype];
+                const int woff = SY ? 0 : s_envoff[ptype];
                 const bool envok = woff >= 0;
                 // the tile's center count as an opaque scalar, so that each
                 // "c < ncen" below is one scalar compare and branch here (the
@@ -1107,6 +1197,52 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : (T == 8 && MODE
                             }
                         }
                     }
+                } else if (SY) {
+                    // hybrid mode, own-type pass of a symmetric tile (list
+                    // S: every candidate of the tile's type, its ion2surf
+                    // in w): pair-count weight 2 for a center in a cell
+                    // after the own cell (its own count too: same distance,
+                    // same category, same row), 0 for a center in a cell
+                    // before it (counted from that center's tile), else 1;
+                    // the kNN range is read for every candidate
+                    (void)ptype;
+                    (void)envok;
+                    const uint32_t sq = (uint32_t)q * kWave + (uint32_t)lane;
+                    const bool ctr = p.w >= R2;  // the center rule of the job's center lists
+                    const bool before = sq - sb1 < sb2 - sb1;
+                    const bool after = sq >= sb2;
+                    const uint32_t wgt = (ctr && after) ? 2u : ((ctr && before) ? 0u : 1u);
+                    const bool rdf_w = rdf && wgt != 0u;
+                    const float bthr = rdf_w ? R2 : kthr_t;
+                    uint32_t *arow = s_P + woff_t;
+                    uint32_t *krow = s_H + kof_t;
+#pragma unroll
+                    for (int c = 0; c < T; ++c) {
+                        if (c == 0 || c < ncen) {  // uniform: skips a partial tile's absent centers
+                            const float dx = cxr[c] - px;
+                            const float dy = cyr[c] - p.y;
+                            const float dz = czr[c] - p.z;
+                            const float d2 = dx * dx + dy * dy + dz * dz;
+                            if (__builtin_expect(cin != 0, 0)) {
+                                const bool in = d2 <= R2;
+                                my_in += (unsigned long long)__builtin_popcountll(
+                                             __builtin_amdgcn_ballot_w64(in && wgt != 0u)) +
+                                         (unsigned long long)__builtin_popcountll(
+                                             __builtin_amdgcn_ballot_w64(in && wgt == 2u));
+                            }
+                            if (d2 <= bthr) {
+                                const int cat = category_of(d2, isc, ioff1, ioffa, amb2, Cm1f, gedges);
+                                if (rdf_w)
+                                    __hip_atomic_fetch_add(arow + cat, wgt, __ATOMIC_RELAXED,
+                                                           __HIP_MEMORY_SCOPE_WORKGROUP);
+                                if (d2 <= kthr_t)
+                                    __hip_atomic_fetch_add(
+                                        krow + (C16 ? (c >> 1) : c) * HK + cat,
+                                        (C16 && (c & 1)) ? 65536u : 1u, __ATOMIC_RELAXED,
+                                        __HIP_MEMORY_SCOPE_WORKGROUP);
+                            }
+                        }
+                    }
                 } else {
                     // hybrid mode: the direct-mode pair counts, and the pairs
                     // within the kNN range of their type also into the
@@ -1155,12 +1291,22 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : (T == 8 && MODE
                 if (q + 1 < qb)
                     advance();
                 pb = fetch(okb);
-                body(pa, oka);
+                body(pa, oka, q);
                 if (q + 2 < qb)
                     advance();
                 pa = fetch(oka);
                 if (q + 1 < qb)
-                    body(pb, okb);
+                    body(pb, okb, q + 1);
+            }
+            };
+            if (HY && sym) {
+                const uint32_t bs = rfl(s_rowpre[J.s_ns]);
+                const int ns = (int)rfl((uint32_t)J.s_ns);
+                sweep(CFlag<true>(), rflp((const gfvec4 *)J.pos_s), 0, ns, 0u, bs);
+                if (ns < nrt)
+                    sweep(CFlag<false>(), rflp((const gfvec4 *)J.pos_o), ns, nrt, bs, ctot);
+            } else {
+                sweep(CFlag<false>(), rflp((const gfvec4 *)J.pos), 0, nrows, 0u, ctot);
             }
             inball += my_in;
         }
@@ -1201,7 +1347,8 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : (T == 8 && MODE
         // here rather than by each center's wave in series, into the stencil
         // rows' words (free until the next tile) as far as they hold them
         const int Lt = (int)J.trg_ntask[ts];
-        const int Ls = Lt < nrows ? Lt : nrows;  // rows + prefix hold 3 * nrows + 1 words
+        // (the rows' starts and prefix hold L.cen - L.rows words)
+        const int Ls = Lt < (L.cen - L.rows) / 3 ? Lt : (L.cen - L.rows) / 3;
         uint32_t *s_task = s_rows;  // (the main loop's closing barrier freed them)
         for (int l = tid; l < Ls; l += kBlock) {
             const uint32_t tk = J.trg_tasks[ts * J.lmax + l];
@@ -1276,23 +1423,25 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : (T == 8 && MODE
                         // that starts in it has been counted.
                         uint32_t carry = 0;
                         uint32_t kw = 0;  // the window holds k = kw + 1 .. kw + 64
-                        int wcur = -1;    // category of k = kw (-1: none yet)
-                        int32_t *wrow = s_win + wid * kWave;
+                        // (window entries and wcur hold category + 1, 0 for
+                        // none, so that the running maximum starts at 0)
+                        uint32_t wcur = 0;  // category of k = kw, plus 1
+                        uint16_t *wrow = s_win + wid * kWave;
                         if (want_k)
-                            wrow[lane] = -1;
+                            wrow[lane] = 0u;
                         // writes the window's orders (k <= lim: the scanned
                         // category, above: the sentinel) and moves it on
                         auto flush = [&](uint32_t lim) {
-                            int m = wrow[lane];
-                            wrow[lane] = -1;
+                            uint32_t m = wrow[lane];
+                            wrow[lane] = 0u;
                             m = __builtin_elementwise_max(wave_max_scan(m), wcur);
-                            wcur = __builtin_amdgcn_readlane(m, kWave - 1);
+                            wcur = (uint32_t)__builtin_amdgcn_readlane((int)m, kWave - 1);
                             const uint32_t k = kw + 1u + (uint32_t)lane;
                             if (k <= (uint32_t)kmax) {
                                 const int kid = kdense ? (int)k - 1 : s_kidx[k];
                                 if (kid >= 0)
                                     kb[(uint32_t)kid * (uint32_t)TR + (uint32_t)row] =
-                                        k <= lim ? (uint16_t)m : sentinel;
+                                        k <= lim ? (uint16_t)(m - 1u) : sentinel;
                             }
                             kw += (uint32_t)kWave;
                         };
@@ -1347,9 +1496,9 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : (T == 8 && MODE
                                 const uint32_t fa = preva + 1u, fb = cuma + 1u;
                                 while (kw < (uint32_t)kmax) {
                                     if (va > 0u && fa > kw && fa <= kw + (uint32_t)kWave)
-                                        wrow[fa - kw - 1u] = ca;
+                                        wrow[fa - kw - 1u] = (uint16_t)(ca + 1);
                                     if (vb > 0u && fb > kw && fb <= kw + (uint32_t)kWave)
-                                        wrow[fb - kw - 1u] = ca + 1;
+                                        wrow[fb - kw - 1u] = (uint16_t)(ca + 2);
                                     const uint32_t wend = kw + (uint32_t)kWave < (uint32_t)kmax
                                                               ? kw + (uint32_t)kWave
                                                               : (uint32_t)kmax;

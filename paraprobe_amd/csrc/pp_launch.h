@@ -37,9 +37,14 @@ hipError_t launch_xsplit(const uint32_t *start, uint64_t ncells, int sub, uint2 
                          hipStream_t s);
 hipError_t launch_stride(const uint32_t *in, uint32_t *out, uint64_t nout, int shift,
                          hipStream_t s);
+// w_s2: the float4's w holds the ion's ion2surf instead of its type (a list
+// of one type for symmetric pair counting)
 hipError_t launch_scatter(const float *xyz, const uint8_t *type, const float *s2, uint64_t n,
                           const uint32_t *key, const uint32_t *rank, const uint32_t *cellstart,
-                          float4 *pos, float *s2sorted, hipStream_t s);
+                          float4 *pos, float *s2sorted, hipStream_t s, int w_s2 = 0);
+// the least ion2surf of each cell's ions (NaN as -inf; +inf for an empty cell)
+hipError_t launch_cellmin(const uint32_t *cellstart, const float *s2sorted, uint64_t ncells,
+                          float *out, hipStream_t s);
 
 // Typed candidate lists: flag the ions whose type is in a 256-bit mask,
 // and gather them (order kept) after an exclusive scan of the flags.

@@ -94,6 +94,58 @@ def test_knn_range_does_not_change_results(ctx, factor):
         ctx.set_knn_range(4.0)
 
 
+@pytest.mark.parametrize("case", ["clustered_frustum", "uniform_random_ion2surf"])
+def test_pair_symmetry_does_not_change_results(ctx, case):
+    """Symmetric pair counting (the dominant center type's tile counts a pair
+    of two of its centers for both when the other center's cell comes later
+    in cell order, and reads the earlier cells only where they hold non-centers
+    or lie in its kNN range) against the oracle and against the same jobs with
+    it switched off.  On the frustum most rows before the own cell hold
+    centers only; with ion2surf uniform in [0, 3) nm^2 every row mixes
+    centers and non-centers."""
+    _say(f"test_pair_symmetry {case}")
+    if case == "clustered_frustum":
+        base = [0.85, 0.12, 0.03]
+        tip = synth.tip(60_000, 5, "frustum", 12.0, 7.0, 35.0, base, nclusters=250, crmin=1.0,
+                        crmax=2.0, cfrac=synth.cluster_fracs(base))
+        xyz, ty, s2 = tip.xyz, tip.types, tip.ion2surf_sqr
+    else:
+        xyz, ty, _ = _cube(40_000, 5, L=9.0)
+        s2 = np.random.default_rng(6).uniform(0, 12, size=len(ty)).astype(np.float32)
+    # type 1 dominates: its tiles count symmetrically, reading type 1 from
+    # its own list and (on the frustum, and in the last job) types 2 and 3
+    # from another; type 2's tiles (second job) count as usual
+    jobs = [Job([((1,), (1,))], 0.0, 0.02, 2.5, FAM_RDF | FAM_MKNN,
+                kvalues=list(range(1, 11))),
+            Job([((1,), (1,)), ((2,), (1,))], 0.0, 0.01, 1.5, FAM_RDF | FAM_RIPK | FAM_MKNN,
+                kvalues=[1, 3, 10]),
+            Job([((1,), (1, 2))], 0.0, 0.02, 2.0, FAM_RDF | FAM_1NN)]
+    # (a kNN range of about k neighbours keeps these small clouds in the
+    # hybrid mode, where the symmetric counting applies)
+    ctx.set_knn_range(1.0)
+    try:
+        for job in jobs:
+            g, _, _ = _compare(ctx, xyz, ty, s2, job)
+            ctx.set_pair_symmetry(False)
+            try:
+                h, _, _ = _compare(ctx, xyz, ty, s2, job)
+            finally:
+                ctx.set_pair_symmetry(True)
+            _check_symmetry(case, g, h)
+    finally:
+        ctx.set_knn_range(4.0)
+
+
+def _check_symmetry(case, g, h):
+    assert g.percenter == 2 and h.percenter == 2
+    # fewer candidate evaluations with it on where rows of centers only
+    # exist (elsewhere about as many: the own-type and other-type lists trim
+    # their rows' end cells at their own sub-cell granularity)
+    if case == "clustered_frustum":
+        assert g.pairs_candidate < h.pairs_candidate, (g.pairs_candidate, h.pairs_candidate)
+    _say(f"  candidates {g.pairs_candidate} symmetric vs {h.pairs_candidate}")
+
+
 @pytest.mark.parametrize("T,r", [(8, 1), (16, 2), (32, 3), (16, 4), (8, 2), (32, 1), (8, 6),
                                  (8, 8)])
 def test_tuning_does_not_change_results(ctx, T, r):
