@@ -193,11 +193,12 @@ pp_status pp_set_tuning(pp_ctx *ctx, int32_t centers_per_tile, int32_t cells_per
 pp_status pp_set_cell_aspect(pp_ctx *ctx, int32_t yz_per_x);
 
 /* Symmetric pair counting (1 = allowed, the default; 0 = off).  In a job with
- * RDF or RIPK and a kNN family, the center type with the most ions, when it
- * is also a neighbour type holding at least half of the candidates, counts
- * each pair of two of its centers once for both: the same squared distance
- * from either side gives the same category.  A tuning switch of this
- * library (no counterpart in the reference); results do not depend on it. */
+ * RDF or RIPK (alone, or with MKNN / 1NN over kNN ranges), the center type
+ * with the most ions, when it is also a neighbour type holding at least half
+ * of the candidates, counts each pair of two of its centers once for both:
+ * the same squared distance from either side gives the same category.  A
+ * tuning switch of this library (no counterpart in the reference); results
+ * do not depend on it. */
 pp_status pp_set_pair_symmetry(pp_ctx *ctx, int32_t on);
 
 /* Squared distance of every ion to a triangulated tip surface, the ion2surf

@@ -1083,11 +1083,11 @@ int type_span(const pp_cloud *cl)
 // Stencil rows a job's launch in kernel mode `mode` lays out in LDS.
 int query_rows(const JobRun &r, int mode)
 {
-    return (int)(mode == 2 && r.sym_slot >= 0 ? r.srows.size() : r.rows.size());
+    return (int)(mode != 1 && r.sym_slot >= 0 ? r.srows.size() : r.rows.size());
 }
 
-// Symmetric pair counting for a hybrid-mode job with pair counts (RDF,
-// RIPK): the center type t with the most ions among the job's center types
+// Symmetric pair counting for a job with pair counts (RDF, RIPK) in the
+// pair-count or the hybrid mode: the center type t with the most ions among the job's center types
 // that are also neighbour types, when t holds at least half of the job's
 // candidates.  A pair of two centers of t lands in the same pair-count row
 // (center type t, neighbour type t) and, its squared distance being the same
@@ -1097,13 +1097,14 @@ int query_rows(const JobRun &r, int mode)
 // from a list of t alone (ion2surf in w, the least ion2surf per cell) along
 // the own cell, the cells before it and the cells after it, and the other
 // candidates from a list without t along every row.  Kept off when the
-// longer row table would cost the hybrid kernel resident workgroups.
+// longer row table would cost the kernel resident workgroups.
 pp_status plan_symmetric(pp_ctx *c, Slot &sl, pp_cloud *cl, JobRun &r)
 {
     r.sym_slot = -1;
     r.srows.clear();
     const JobPlan &p = r.plan;
-    if (p.mode != 2 || !(p.families & (PP_FAM_RDF | PP_FAM_RIPK)) || cl->n == 0 || sl.fixed)
+    if ((p.mode != 0 && p.mode != 2) || !(p.families & (PP_FAM_RDF | PP_FAM_RIPK)) ||
+        cl->n == 0 || sl.fixed)
         return PP_OK;
     // the candidates' types: every type of the cloud on the full list, the
     // neighbour types on a typed list
@@ -1151,12 +1152,15 @@ pp_status plan_symmetric(pp_ctx *c, Slot &sl, pp_cloud *cl, JobRun &r)
     if ((int)nsr > kMaxRows)
         return PP_OK;
     {
-        const size_t gran = 512, lds_cu = 160 * 1024;
+        // (resident workgroups as the register counts allow: 7 in the
+        // pair-count mode, 6 in the hybrid mode)
+        const size_t gran = 512, lds_cu = 160 * 1024, cap = p.mode == 2 ? 6 : 7;
         auto wgs = [&](int nrows) {
-            const size_t l = query_lds_bytes(r.T_req, 1, 2, std::max(p.C, 1), std::max(p.E, 1),
-                                             std::max(p.ntrg, 1), p.kidx_len(), nrows,
-                                             std::max(p.HK, 1), type_span(cl));
-            return std::min<size_t>(6, lds_cu / ((l + gran - 1) / gran * gran));
+            const size_t l = query_lds_bytes(r.T_req, 1, p.mode, std::max(p.C, 1),
+                                             std::max(p.E, 1), std::max(p.ntrg, 1),
+                                             p.kidx_len(), nrows, std::max(p.HK, 1),
+                                             type_span(cl));
+            return std::min<size_t>(cap, lds_cu / ((l + gran - 1) / gran * gran));
         };
         if (wgs((int)nsr) < wgs((int)r.rows.size()))
             return PP_OK;

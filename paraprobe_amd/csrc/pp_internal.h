@@ -153,7 +153,7 @@ struct JobDev {
     const int32_t *ckoff;
     const uint32_t *task_ck;
     uint32_t *unres;
-    // Symmetric pair counting (hybrid mode): the tiles of center type slot
+    // Symmetric pair counting (pair-count and hybrid modes): the tiles of center type slot
     // sym_slot (-1: none) read the candidates of their own type from list S,
     // whose float4 w holds the ion's ion2surf (every ion of S has that type),
     // along srows[0, s_ns): the own cell (srows[0]), then the cells before it
@@ -163,7 +163,7 @@ struct JobDev {
     // with a center in a cell before it, it is left to that center's tile
     // (only its kNN range is read there).  A row before the own cell whose
     // own-type ions are all centers (cellmin_s: the least ion2surf per cell)
-    // is cut to the kNN reach.  The other candidates come from list O along
+    // is cut to the kNN reach (left out without kNN).  The other candidates come from list O along
     // srows[s_ns, nsrows), counted as in any other tile.
     int sym_slot, sym_type;
     int nsrows, s_nb1, s_ns;

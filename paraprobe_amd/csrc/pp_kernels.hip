@@ -850,7 +850,7 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : (T == 8 && MODE
         // symmetric pair counting for this tile's center type: its own-type
         // candidates along srows[0, s_ns) of list S, the others along
         // srows[s_ns, nsrows) of list O
-        const bool sym = HY && ts == J.sym_slot;
+        const bool sym = !PC && ts == J.sym_slot;
         const int nrt = sym ? J.nsrows : nrows;
         const Row *rtab = sym ? J.srows : J.rows;
         const int cx = (int)(key % (uint32_t)J.nx);
@@ -1046,11 +1046,13 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : (T == 8 && MODE
             int woff_t = 0, kof_t = 0;
             float kthr_t = -1.0f;
             uint32_t sb1 = 0, sb2 = 0;
-            if (HY && sym) {
+            if (sym) {
                 woff_t = (int)rfl((uint32_t)s_envoff[J.sym_type]);
-                const int2 ki = s_kinfo[J.sym_type];
-                kof_t = (int)rfl((uint32_t)ki.x);
-                kthr_t = __int_as_float((int)rfl((uint32_t)ki.y));
+                if (HY) {
+                    const int2 ki = s_kinfo[J.sym_type];
+                    kof_t = (int)rfl((uint32_t)ki.x);
+                    kthr_t = __int_as_float((int)rfl((uint32_t)ki.y));
+                }
                 sb1 = rfl(s_rowpre[1]);
                 sb2 = rfl(s_rowpre[J.s_nb1]);
             }
@@ -1066,7 +1068,7 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : (T == 8 && MODE
             // tile.
             auto sweep = [&](auto SYMF, const gfvec4 *__restrict__ pos, int rlo, int rhi,
                              uint32_t clo, uint32_t chi) {
-            constexpr bool SY = HY && decltype(SYMF)::value;
+            constexpr bool SY = !PC && decltype(SYMF)::value;
             const int nch = (int)((chi - clo + kWave - 1) / kWave);
             const int qa = (int)rfl((uint32_t)(wid * nch / kWaves));
             const int qb = (int)rfl((uint32_t)((wid + 1) * nch / kWaves));
@@ -1172,6 +1174,39 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : (T == 8 && MODE
                             __hip_atomic_fetch_add(&s_acc[(C16 ? (c >> 1) : c) * EC + woff + cat],
                                                    inc, __ATOMIC_RELAXED,
                                                    __HIP_MEMORY_SCOPE_WORKGROUP);
+                        }
+                    }
+                } else if (!HY && SY) {
+                    // direct mode, own-type pass of a symmetric tile (list S:
+                    // the tile's type, ion2surf in w): weight 2 for a center
+                    // in a cell after the own cell, none for a center in a
+                    // cell before it (that lane is left out), else 1
+                    (void)envok;
+                    const uint32_t sq = (uint32_t)q * kWave + (uint32_t)lane;
+                    const bool ctr = p.w >= R2;  // the center rule of the job's center lists
+                    const bool before = sq - sb1 < sb2 - sb1;
+                    const bool after = sq >= sb2;
+                    const uint32_t wgt = (ctr && after) ? 2u : 1u;
+                    const float pxs = (ctr && before) ? __builtin_nanf("") : px;
+                    uint32_t *arow = s_P + woff_t;
+#pragma unroll
+                    for (int c = 0; c < T; ++c) {
+                        if (c == 0 || c < ncen) {  // uniform: skips a partial tile's absent centers
+                            const float dx = cxr[c] - pxs;
+                            const float dy = cyr[c] - p.y;
+                            const float dz = czr[c] - p.z;
+                            const float d2 = dx * dx + dy * dy + dz * dz;
+                            const bool in = d2 <= R2;
+                            if (__builtin_expect(cin != 0, 0))
+                                my_in += (unsigned long long)__builtin_popcountll(
+                                             __builtin_amdgcn_ballot_w64(in)) +
+                                         (unsigned long long)__builtin_popcountll(
+                                             __builtin_amdgcn_ballot_w64(in && wgt == 2u));
+                            if (in) {
+                                const int cat = category_of(d2, isc, ioff1, ioffa, amb2, Cm1f, s_edges);
+                                __hip_atomic_fetch_add(arow + cat, wgt, __ATOMIC_RELAXED,
+                                                       __HIP_MEMORY_SCOPE_WORKGROUP);
+                            }
                         }
                     }
                 } else if (!HY) {
@@ -1299,7 +1334,7 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : (T == 8 && MODE
                     body(pb, okb, q + 1);
             }
             };
-            if (HY && sym) {
+            if (sym) {
                 const uint32_t bs = rfl(s_rowpre[J.s_ns]);
                 const int ns = (int)rfl((uint32_t)J.s_ns);
                 sweep(CFlag<true>(), rflp((const gfvec4 *)J.pos_s), 0, ns, 0u, bs);
