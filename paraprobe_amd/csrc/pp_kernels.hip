@@ -932,17 +932,32 @@ __global__ __launch_bounds__(kBlock, (T == 8 && MODE == 0) ? 7 : (T == 8 && MODE
                 cut(J.reach * J.reach);
                 const uint32_t kr = ((uint32_t)z * (uint32_t)J.ny + (uint32_t)y) * (uint32_t)J.nx;
                 if (inS && r >= 1 && r < J.s_nb1 && x0 <= x1) {
-                    // a row before the own cell whose own-type ions are all
-                    // centers: their pairs with this tile are counted from
-                    // their own tiles, so only the kNN reach is left
-                    bool all_ctr = true;
+                    // a row before the own cell: the pairs with centers there
+                    // are counted from those centers' tiles, so a cell whose
+                    // own-type ions are all centers is needed only within the
+                    // kNN reach.  The row keeps the cells holding a non-center
+                    // (fa .. fb) and the kNN reach: their hull, trimmed at its
+                    // ends as the full-reach row (as the kNN reach when no
+                    // cell holds a non-center)
+                    int fa = x1 + 1, fb = x0 - 1;
                     for (int x = x0; x <= x1; ++x)
-                        all_ctr = all_ctr && J.cellmin_s[kr + (uint32_t)x] >= J.R2;
-                    if (all_ctr) {
+                        if (!(J.cellmin_s[kr + (uint32_t)x] >= J.R2)) {
+                            fa = fa > x1 ? x : fa;
+                            fb = x;
+                        }
+                    if (fa > x0 || fb < x1) {
+                        const double rtlo = tlo, rthi = thi;
                         if (J.kreach > 0.0)
                             cut(J.kreach * J.kreach);
                         else
                             x1 = x0 - 1;
+                        if (fa <= fb) {
+                            const bool kempty = x0 > x1;
+                            x0 = kempty ? fa : (x0 < fa ? x0 : fa);
+                            x1 = kempty ? fb : (x1 > fb ? x1 : fb);
+                            tlo = rtlo;
+                            thi = rthi;
+                        }
                     }
                 }
                 if (x0 <= x1) {
