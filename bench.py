@@ -6,8 +6,9 @@ pairs (RDF + MKNN k=1..100, Rmax 5 nm, bin 0.01 nm), and the int64 histograms
 back on the host.  Inputs are in HBM before the timed region starts, and so
 is the cloud's cell list: the first (warmup) job builds it and every step
 reuses it, as the reference's KD-trees are built once by
-init_spatialindexing before compute_generic_spatstat3.  The cell-list build
-alone is reported as ms_index_build (a fresh copy of the cloud, timed).
+init_spatialindexing before compute_generic_spatstat3.  The index build alone
+(the cell list and the candidate lists of the symmetric pair counting) is
+reported as ms_index_build (a fresh copy of the cloud, timed).
 
 Prints ONE JSON line.  `value` is reference-equivalent pair-distance
 evaluations per second: the sum over considered centers of the neighbours
@@ -37,8 +38,8 @@ FLOP_PER_EVAL = 8           # 3 sub + 3 mul + 2 add (no FMA: fp-contract off)
 OPS_PER_EVAL = 9            # the same plus the in-ball compare
 # HBM bytes of the query kernel per step of the 20M-ion headline job, from two
 # rocprofv3 --pmc runs of this script (FETCH_SIZE x 2 and WRITE_SIZE x 1, the
-# factors a 1 GiB device copy reads): 114.0 GB read + 12.0 GB written
-TRAFFIC_HEADLINE = 1.259e11
+# factors a 1 GiB device copy reads): 101.3 GB read + 11.4 GB written
+TRAFFIC_HEADLINE = 1.127e11
 TRAFFIC_SOURCE = "profiles/k_query_frustum20M_rdf_mknn100.md"
 
 
@@ -97,8 +98,9 @@ def roofline(cand, kernel_ms, kernel, traffic=None, traffic_source=None):
 
 
 def index_build_ms(ctx, tip, job, reps=3):
-    """The cell-list build on its own: the first job on a freshly uploaded
-    copy of the cloud builds it (later jobs reuse it); median of reps."""
+    """The index build on its own (the cell list and the job's candidate
+    lists): the first job on a freshly uploaded copy of the cloud builds them
+    (later jobs reuse them); median of reps."""
     out = []
     for _ in range(reps):
         c = ctx.upload(tip.xyz, tip.types, tip.ion2surf_sqr)
@@ -332,13 +334,17 @@ def serve_leg(ctx, args):
     print(f"# serve: 8 resident tips ({nions} ions) ready in {time.perf_counter() - t:.1f} s; "
           f"{M} requests, plain loop on the first {S}", file=sys.stderr, flush=True)
     # build every (cloud, cell width, neighbour type) candidate list once, with
-    # jobs whose central type holds no ion (so no query work is done)
+    # jobs whose central type holds no ion (so no query work is done), and the
+    # lists of the symmetric pair counting of same-type pair counts (one RDF
+    # job per cloud, width and type), so that neither the plain loop nor the
+    # server builds lists while it is timed
     from paraprobe_amd import FAM_RDF, Job
 
     for c in clouds:
         for R in (1.5, 3.0, 5.0):
             for e in (1, 2, 3):
                 ctx.run(c, Job([((7,), (e,))], 0.0, 0.01, R, FAM_RDF))
+                ctx.run(c, Job([((e,), (e,))], 0.0, 0.01, R, FAM_RDF))
     # plain loop, back to back: per-request service times
     plain, svc = [], []
     with Phase("serve_plain_loop"):
@@ -813,7 +819,7 @@ def main():
                 if not np.array_equal(v, getattr(res[0], k)):
                     raise SystemExit("bench: results differ between steps")
         ms_index_build = index_build_ms(ctx, tip, job)
-        print(f"# cell-list build {ms_index_build:.2f} ms", file=sys.stderr, flush=True)
+        print(f"# index build {ms_index_build:.2f} ms", file=sys.stderr, flush=True)
         # GPU result of the CPU sample window, for the parity check
         zwin = (125.0 - args.cpu_window / 2, 125.0 + args.cpu_window / 2)
         s2w = tip.ion2surf_sqr.copy()
@@ -885,7 +891,7 @@ def main():
         "pairs_inball": inball,
         "candidate_evals_per_s": cand / (k_ms / 1e3),
         "ms_index": float(np.mean([r.ms_index for r in res])),
-        "ms_index_note": "each step reuses the cloud's cell list (built by the first job)",
+        "ms_index_note": "each step reuses the cloud's cell list and candidate lists (built by the first job)",
         "ms_index_build": ms_index_build,
         "ms_prep": float(np.mean([r.ms_prep for r in res])),
         "ms_query": q_ms,
