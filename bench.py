@@ -94,6 +94,9 @@ def roofline(cand, kernel_ms, kernel, traffic=None, traffic_source=None):
         "kernel": kernel,
         "kernel_ms": kernel_ms,
         "candidate_evals": cand,
+        "candidate_note": "pair evaluations the kernel performs; with the symmetric pair "
+                          "counting a pair of two centers of the dominant type is evaluated "
+                          "once for both",
     }
 
 
