@@ -11,6 +11,7 @@
 
 #include <algorithm>
 #include <array>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
@@ -1818,10 +1819,28 @@ pp_status run_jobs(pp_ctx *c, Slot &sl, int njobs, const pp_cloud *const *clouds
         sym_pairs = c->sym_pairs;
         graph_launch = c->graph_launch;
     }
+    // the jobs' plans (histogram geometry, exact edge tables, task tables)
+    // are independent host work: on several threads for larger runs; a job
+    // whose plan fails is planned again below, in order, for its message
+    std::vector<char> planned(njobs, 0);
+    if (njobs >= 16) {
+        const int nt = std::max(1, std::min(8, (int)std::thread::hardware_concurrency()));
+        std::atomic<int> next{0};
+        auto work = [&]() {
+            for (int i = next++; i < njobs; i = next++)
+                planned[i] = plan_job(nullptr, &jobs[i], runs[i].plan) == PP_OK ? 1 : 0;
+        };
+        std::vector<std::thread> pool;
+        for (int t = 1; t < nt; ++t)
+            pool.emplace_back(work);
+        work();
+        for (std::thread &t : pool)
+            t.join();
+    }
     for (int i = 0; i < njobs; ++i) {
         if (!clouds[i])
             return fail(c, PP_ERR_INVALID_ARG, "cloud is NULL");
-        pp_status st = plan_job(c, &jobs[i], runs[i].plan);
+        pp_status st = planned[i] ? PP_OK : plan_job(c, &jobs[i], runs[i].plan);
         if (st != PP_OK)
             return st;
         runs[i].cloud = clouds[i];
@@ -2064,16 +2083,23 @@ pp_status run_jobs(pp_ctx *c, Slot &sl, int njobs, const pp_cloud *const *clouds
     {
         std::vector<int> ids, modes;
         std::vector<JobDev> devs;
+        // the rerun marks of the jobs that may rerun, cleared by one memset
+        // over their span (the jobs' marks are carved one after another)
+        char *mk0 = nullptr, *mk1 = nullptr;
         for (int i = 0; i < njobs; ++i) {
             JobRun &r = runs[i];
             ids.push_back(i);
             devs.push_back(r.dev);
             modes.push_back(r.plan.mode);
             if (r.plan.mode == 2 && r.plan.trunc_any) {
-                PP_HIP(c, hipMemsetAsync(r.flags, 0, 4 * r.cloud->n, sl.stream));
+                char *b0 = (char *)r.flags, *b1 = (char *)(r.flags + r.cloud->n);
+                mk0 = mk0 && mk0 < b0 ? mk0 : b0;
+                mk1 = mk1 && mk1 > b1 ? mk1 : b1;
                 hy.push_back(i);
             }
         }
+        if (mk0 && mk1 > mk0)
+            PP_HIP(c, hipMemsetAsync(mk0, 0, (size_t)(mk1 - mk0), sl.stream));
         if (order) {
             turn.take();
             if (order->armed)
