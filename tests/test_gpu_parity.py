@@ -123,6 +123,8 @@ def test_pair_symmetry_does_not_change_results(ctx, case):
             # pair counts only (the kernel's direct mode)
             Job([((1,), (1,))], 0.0, 0.02, 2.5, FAM_RDF),
             Job([((1,), (1,)), ((3,), (1, 3))], 0.05, 0.01, 1.5, FAM_RDF | FAM_RIPK)]
+    # the switch takes 0 or 1 only
+    assert ctx.lib.pp_set_pair_symmetry(ctx.ctx, 2) == 1  # PP_ERR_INVALID_ARG
     # (a kNN range of about k neighbours keeps these small clouds' kNN jobs
     # in the hybrid mode; the symmetric counting applies there and in the
     # pair-count mode)
