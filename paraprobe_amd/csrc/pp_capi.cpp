@@ -1831,8 +1831,13 @@ pp_status run_jobs(pp_ctx *c, Slot &sl, int njobs, const pp_cloud *const *clouds
                 planned[i] = plan_job(nullptr, &jobs[i], runs[i].plan) == PP_OK ? 1 : 0;
         };
         std::vector<std::thread> pool;
-        for (int t = 1; t < nt; ++t)
-            pool.emplace_back(work);
+        for (int t = 1; t < nt; ++t) {
+            try {
+                pool.emplace_back(work);
+            } catch (...) {
+                break;  // no more threads: the calling thread plans the rest
+            }
+        }
         work();
         for (std::thread &t : pool)
             t.join();
