@@ -1852,12 +1852,15 @@ pp_status run_jobs(pp_ctx *c, Slot &sl, int njobs, const pp_cloud *const *clouds
         JobPlan &p = runs[i].plan;
         int jT = 8;
         p.r = rule_r(cand_density(clouds[i], p), p.R);
+        bool by_rule = true;  // the cell shape is the density rule's
         if (force.T > 0 || force.r > 0) {
+            by_rule = force.r <= 0;
             if (force.T > 0)
                 jT = force.T;
             if (force.r > 0)
                 p.r = force.r;
         } else if (tuneT < 0 && !sl.fixed) {
+            by_rule = false;
             const long b = tune_bucket(cand_density(clouds[i], p), p);
             bool have;
             {
@@ -1875,11 +1878,20 @@ pp_status run_jobs(pp_ctx *c, Slot &sl, int njobs, const pp_cloud *const *clouds
         } else {
             if (tuneT > 0)
                 jT = tuneT;
-            if (tuneR > 0)
+            if (tuneR > 0) {
                 p.r = tuneR;
+                by_rule = false;
+            }
         }
         runs[i].T_req = jT;
         p.ay = tuneAY > 0 ? tuneAY : 1;
+        // the rule's finest cells (Rmax/4) are taken as Rmax/2 along x and
+        // Rmax/4 along y and z in the lists jobs build for themselves: the
+        // same stencil rows, each of half as many (longer) cells
+        if (by_rule && tuneAY <= 0 && p.r == 4 && !clouds[i]->resident) {
+            p.r = 2;
+            p.ay = 2;
+        }
         plan_hybrid(clouds[i], p, knn_factor);
     }
     PP_HIP(c, hipSetDevice(c->device));
