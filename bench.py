@@ -38,8 +38,8 @@ FLOP_PER_EVAL = 8           # 3 sub + 3 mul + 2 add (no FMA: fp-contract off)
 OPS_PER_EVAL = 9            # the same plus the in-ball compare
 # HBM bytes of the query kernel per step of the 20M-ion headline job, from two
 # rocprofv3 --pmc runs of this script (FETCH_SIZE x 2 and WRITE_SIZE x 1, the
-# factors a 1 GiB device copy reads): 101.3 GB read + 11.4 GB written
-TRAFFIC_HEADLINE = 1.127e11
+# factors a 1 GiB device copy reads): 93.0 GB read + 10.5 GB written
+TRAFFIC_HEADLINE = 1.035e11
 TRAFFIC_SOURCE = "profiles/k_query_frustum20M_rdf_mknn100.md"
 
 

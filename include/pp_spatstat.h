@@ -92,7 +92,7 @@ typedef struct pp_hist_out {
     double ms_query;            /* query kernel */
     double ms_total;            /* whole call, host wall clock */
     int32_t tile_centers;       /* query-kernel configuration used: centers per tile, */
-    int32_t cells_per_rmax;     /*   cell width as a fraction of Rmax (0: resident list), */
+    int32_t cells_per_rmax;     /*   cell width along x as a fraction of Rmax (0: resident list), */
     int32_t counters16;         /*   16-bit per-center counters, */
     int32_t percenter;          /*   per-center histograms: 0 none, 1 full range, 2 kNN range
                                  *   only (with an exact full-range rerun of the centers
@@ -188,8 +188,10 @@ pp_status pp_set_tuning(pp_ctx *ctx, int32_t centers_per_tile, int32_t cells_per
 
 /* Cell shape of the cell lists jobs build for themselves (clouds uploaded
  * without a cell width): cells of Rmax / cells_per_rmax along x and
- * yz_per_x times finer along y and z (1..4; 0 = built-in choice, 1).
- * Resident clouds keep their cubic cells.  Results do not depend on it. */
+ * yz_per_x times finer along y and z (1..4).  0 = built-in choice: cubic
+ * cells, except where the density rule picks its finest width, Rmax/4,
+ * which it builds as Rmax/2 along x and Rmax/4 along y and z.  Resident
+ * clouds keep their cubic cells.  Results do not depend on it. */
 pp_status pp_set_cell_aspect(pp_ctx *ctx, int32_t yz_per_x);
 
 /* Symmetric pair counting (1 = allowed, the default; 0 = off).  In a job with

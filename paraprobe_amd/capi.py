@@ -397,7 +397,8 @@ class SpatStat:
         self._check(self.lib.pp_set_tuning(self.ctx, centers_per_tile, cells_per_rmax))
 
     def set_cell_aspect(self, yz_per_x: int = 0):
-        """y/z cells per x cell width of the cell lists jobs build (0: built-in)."""
+        """y/z cells per x cell width of the cell lists jobs build (0: built-in:
+        cubic, except the density rule's Rmax/4 cells, built as Rmax/2 along x)."""
         self._check(self.lib.pp_set_cell_aspect(self.ctx, yz_per_x))
 
     def set_pair_symmetry(self, on: bool = True):

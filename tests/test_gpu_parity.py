@@ -167,6 +167,33 @@ def test_tuning_does_not_change_results(ctx, T, r):
         ctx.set_tuning(0, 0)
 
 
+@pytest.mark.parametrize("r,ay", [(1, 2), (2, 2), (2, 4), (3, 3)])
+def test_cell_aspect_does_not_change_results(ctx, r, ay):
+    # cells finer along y and z than along x (ay y/z cells per x cell width)
+    _say(f"test_cell_aspect cells_per_rmax={r} yz_per_x={ay}")
+    xyz, ty, s2 = _cube(4000, 12)
+    ctx.set_tuning(8, r)
+    ctx.set_cell_aspect(ay)
+    try:
+        _compare(ctx, xyz, ty, s2, Job(TASKS_MIX, 0.0, 0.02, 1.0, ALL, kvalues=[1, 3, 9, 30]))
+        _compare(ctx, xyz, ty, s2, Job(TASKS_MIX, 0.0, 0.02, 1.0, FAM_RDF))
+    finally:
+        ctx.set_cell_aspect(0)
+        ctx.set_tuning(0, 0)
+
+
+def test_density_rule_finest_cells(ctx):
+    # 125 ions/nm^3 and Rmax 2.6 nm: the density rule picks its finest width,
+    # Rmax/4, which it takes as Rmax/2 along x and Rmax/4 along y and z
+    _say("test_density_rule_finest_cells")
+    xyz, ty, s2 = _cube(8000, 13, L=4.0)
+    s2 = s2 * 4.0  # squared ion2surf up to 12 nm^2: 44 % of the ions are centers
+    job = Job([((1,), (1, 2)), ((2, 3), (1,))], 0.0, 0.05, 2.6, ALL, kvalues=[1, 10, 100])
+    _compare(ctx, xyz, ty, s2, job)
+    _compare(ctx, xyz, ty, s2, Job([((1,), (1, 2))], 0.0, 0.05, 2.6, FAM_RDF | FAM_MKNN,
+                                   kvalues=[1, 50]))
+
+
 def test_rmin_positive_and_truncated_bins(ctx):
     _say("test_rmin_positive_and_truncated_bins")
     xyz, ty, s2 = _cube(3000, 3)
