@@ -1885,11 +1885,11 @@ pp_status run_jobs(pp_ctx *c, Slot &sl, int njobs, const pp_cloud *const *clouds
         }
         runs[i].T_req = jT;
         p.ay = tuneAY > 0 ? tuneAY : 1;
-        // the rule's finest cells (Rmax/4) are taken as Rmax/2 along x and
-        // Rmax/4 along y and z in the lists jobs build for themselves: the
-        // same stencil rows, each of half as many (longer) cells
-        if (by_rule && tuneAY <= 0 && p.r == 4 && !clouds[i]->resident) {
-            p.r = 2;
+        // the rule's cells of Rmax/4 and Rmax/2 are taken twice as long
+        // along x in the lists jobs build for themselves (Rmax/2 by Rmax/4,
+        // Rmax by Rmax/2): the same stencil rows, each of half as many cells
+        if (by_rule && tuneAY <= 0 && (p.r == 4 || p.r == 2) && !clouds[i]->resident) {
+            p.r /= 2;
             p.ay = 2;
         }
         plan_hybrid(clouds[i], p, knn_factor);
