@@ -189,8 +189,9 @@ pp_status pp_set_tuning(pp_ctx *ctx, int32_t centers_per_tile, int32_t cells_per
 /* Cell shape of the cell lists jobs build for themselves (clouds uploaded
  * without a cell width): cells of Rmax / cells_per_rmax along x and
  * yz_per_x times finer along y and z (1..4).  0 = built-in choice: cubic
- * cells, except where the density rule picks its finest width, Rmax/4,
- * which it builds as Rmax/2 along x and Rmax/4 along y and z.  Resident
+ * cells, except where the density rule picks cells of Rmax/4 or Rmax/2,
+ * which it builds twice as long along x (Rmax/2 by Rmax/4, Rmax by
+ * Rmax/2).  Resident
  * clouds keep their cubic cells.  Results do not depend on it. */
 pp_status pp_set_cell_aspect(pp_ctx *ctx, int32_t yz_per_x);
 

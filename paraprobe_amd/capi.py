@@ -398,7 +398,8 @@ class SpatStat:
 
     def set_cell_aspect(self, yz_per_x: int = 0):
         """y/z cells per x cell width of the cell lists jobs build (0: built-in:
-        cubic, except the density rule's Rmax/4 cells, built as Rmax/2 along x)."""
+        cubic, except the density rule's Rmax/4 and Rmax/2 cells, built twice
+        as long along x)."""
         self._check(self.lib.pp_set_cell_aspect(self.ctx, yz_per_x))
 
     def set_pair_symmetry(self, on: bool = True):

@@ -184,7 +184,8 @@ def test_cell_aspect_does_not_change_results(ctx, r, ay):
 
 def test_density_rule_finest_cells(ctx):
     # 125 ions/nm^3 and Rmax 2.6 nm: the density rule picks its finest width,
-    # Rmax/4, which it takes as Rmax/2 along x and Rmax/4 along y and z
+    # Rmax/4, which it takes as Rmax/2 along x and Rmax/4 along y and z (the
+    # other tests' jobs at Rmax/2 take Rmax by Rmax/2)
     _say("test_density_rule_finest_cells")
     xyz, ty, s2 = _cube(8000, 13, L=4.0)
     s2 = s2 * 4.0  # squared ion2surf up to 12 nm^2: 44 % of the ions are centers
