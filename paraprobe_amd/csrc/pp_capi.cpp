@@ -98,6 +98,13 @@ struct Slot {
 // descriptors of its group
 constexpr size_t kFixedJobbufReserve = size_t(1) << 20;
 
+// a query-kernel shape: centers per tile and cell widths
+struct Shape {
+    int T = 0;   // 0: choose
+    int r = 0;   // 0: choose
+    int ay = 0;  // y/z cells per x cell width; 0: choose
+};
+
 } // namespace
 
 struct pp_ctx {
@@ -111,7 +118,7 @@ struct pp_ctx {
     double knn_factor = 4.0;  // kNN range: expected neighbours / k (0: full range)
     int sym_pairs = 1;        // symmetric pair counting (plan_symmetric) allowed
     int graph_launch = 1;     // batch launch groups captured into a HIP graph
-    std::map<long, std::pair<int, int>> tuned;  // autotune bucket -> (T, cells per Rmax)
+    std::map<long, Shape> tuned;  // autotune bucket -> (T, cells per Rmax, y/z per x)
     Slot main;
     Slot noalloc;  // pp_spatstat_run_noalloc: buffers in the fixed device scratch
 };
@@ -1364,11 +1371,6 @@ pp_status pick_celllist(pp_ctx *c, Slot &s, pp_cloud *cl, const JobPlan &p,
 // ---------------------------------------------------------------------------
 // running jobs
 
-struct Shape {
-    int T = 0;  // 0: choose
-    int r = 0;  // 0: choose
-};
-
 // Runs that share the device through several streams (the serving loop's
 // slots) take turns for their query launches, in the order of their sequence
 // numbers (the order their batches were formed in): each waits, on the host,
@@ -1777,19 +1779,30 @@ pp_status autotune(pp_ctx *c, Slot &sl, const pp_cloud *cl, const pp_job *job, c
 {
     pp_hist_out scratch;
     double best = 1e300;
-    std::pair<int, int> win(8, rule_r(cand_density(cl, p), p.R));
-    for (int T : {8, 16})
-        for (int r = 1; r <= 5; ++r) {
+    Shape win;
+    win.T = 8;
+    win.r = rule_r(cand_density(cl, p), p.R);
+    win.ay = 1;
+    // cubic cells of Rmax/1 .. Rmax/5, and (for the lists jobs build for
+    // themselves) cells of Rmax by Rmax/2 and Rmax/2 by Rmax/4; each shape is
+    // timed twice (the first run of a width builds its list), the faster run
+    // counts
+    std::vector<Shape> cands;
+    for (int T : {8, 16}) {
+        for (int r = 1; r <= 5; ++r)
+            cands.push_back(Shape{T, r, 1});
+        for (int r = 1; r <= 2 && !cl->resident; ++r)
+            cands.push_back(Shape{T, r, 2});
+    }
+    for (const Shape &f : cands)
+        for (int rep = 0; rep < 2; ++rep) {
             std::memset(&scratch, 0, sizeof(scratch));
-            Shape f;
-            f.T = T;
-            f.r = r;
             pp_status st = run_jobs(c, sl, 1, &cl, job, &scratch, f);
             if (st != PP_OK)
-                continue;
+                break;
             if (scratch.ms_query < best) {
                 best = scratch.ms_query;
-                win = {T, r};
+                win = f;
             }
         }
     std::lock_guard<std::mutex> lk(c->mu);
@@ -1850,7 +1863,7 @@ pp_status run_jobs(pp_ctx *c, Slot &sl, int njobs, const pp_cloud *const *clouds
             return st;
         runs[i].cloud = clouds[i];
         JobPlan &p = runs[i].plan;
-        int jT = 8;
+        int jT = 8, jAY = 0;
         p.r = rule_r(cand_density(clouds[i], p), p.R);
         bool by_rule = true;  // the cell shape is the density rule's
         if (force.T > 0 || force.r > 0) {
@@ -1859,6 +1872,7 @@ pp_status run_jobs(pp_ctx *c, Slot &sl, int njobs, const pp_cloud *const *clouds
                 jT = force.T;
             if (force.r > 0)
                 p.r = force.r;
+            jAY = force.ay;
         } else if (tuneT < 0 && !sl.fixed) {
             by_rule = false;
             const long b = tune_bucket(cand_density(clouds[i], p), p);
@@ -1873,8 +1887,10 @@ pp_status run_jobs(pp_ctx *c, Slot &sl, int njobs, const pp_cloud *const *clouds
                     return st;
             }
             std::lock_guard<std::mutex> lk(c->mu);
-            jT = c->tuned[b].first;
-            p.r = c->tuned[b].second;
+            const Shape &w = c->tuned[b];
+            jT = w.T;
+            p.r = w.r;
+            jAY = w.ay;
         } else {
             if (tuneT > 0)
                 jT = tuneT;
@@ -1884,11 +1900,12 @@ pp_status run_jobs(pp_ctx *c, Slot &sl, int njobs, const pp_cloud *const *clouds
             }
         }
         runs[i].T_req = jT;
-        p.ay = tuneAY > 0 ? tuneAY : 1;
+        p.ay = jAY > 0 ? jAY : tuneAY > 0 ? tuneAY : 1;
         // the rule's cells of Rmax/4 and Rmax/2 are taken twice as long
         // along x in the lists jobs build for themselves (Rmax/2 by Rmax/4,
         // Rmax by Rmax/2): the same stencil rows, each of half as many cells
-        if (by_rule && tuneAY <= 0 && (p.r == 4 || p.r == 2) && !clouds[i]->resident) {
+        if (by_rule && tuneAY <= 0 && jAY <= 0 && (p.r == 4 || p.r == 2) &&
+            !clouds[i]->resident) {
             p.r /= 2;
             p.ay = 2;
         }
