@@ -182,8 +182,10 @@ pp_status pp_batch_run(pp_ctx *ctx, int32_t njobs, const pp_cloud *const *clouds
 /* Query-kernel tuning: centers per tile (4, 8, 16 or 32) and the cell width as a
  * fraction of Rmax (cells per Rmax: 1..8).  0 = built-in choice (density
  * rule); -1 for both = autotune: the first job of each (density x Rmax^3,
- * families) bucket times the candidate shapes and the winner is kept for the
- * bucket. */
+ * families) bucket times the candidate shapes (8 or 16 centers per tile;
+ * cubic cells of Rmax/1 .. Rmax/5 and, for the lists jobs build for
+ * themselves, cells of Rmax by Rmax/2 and Rmax/2 by Rmax/4), the faster of
+ * two runs each, and the winner is kept for the bucket. */
 pp_status pp_set_tuning(pp_ctx *ctx, int32_t centers_per_tile, int32_t cells_per_rmax);
 
 /* Cell shape of the cell lists jobs build for themselves (clouds uploaded
