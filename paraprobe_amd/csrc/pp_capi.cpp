@@ -1699,8 +1699,11 @@ pp_status launch_group(pp_ctx *c, Slot &sl, std::vector<JobRun> &runs, const std
         ql.grid = (int)std::max<uint32_t>(
             1u, std::min<uint32_t>(want, std::max<uint32_t>(ch.g1 - ch.g0, 1u)));
         // several tiles per queue atomic when tiles are many and cheap (the
-        // stencil bounds a tile's pair evaluations by maxcand * T)
-        ql.grab = (uint64_t)maxcand * (uint64_t)T < (1u << 17)
+        // stencil bounds a tile's pair evaluations by maxcand * T); blocks of
+        // tiles cost the XCDs their shared front where tiles are dearer
+        // (measured: 100M ions at Rmax 3 nm, 426.5 ms per job with blocks of
+        // 4 tiles, 414.7 ms with single tiles)
+        ql.grab = (uint64_t)maxcand * (uint64_t)T < (1u << 15)
                       ? std::max<uint32_t>(1u, std::min<uint32_t>(
                                                    4u, (ch.g1 - ch.g0) / ((uint32_t)ql.grid * 16u)))
                       : 1u;
