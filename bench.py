@@ -41,6 +41,10 @@ OPS_PER_EVAL = 9            # the same plus the in-ball compare
 # factors a 1 GiB device copy reads): 93.0 GB read + 10.5 GB written
 TRAFFIC_HEADLINE = 1.035e11
 TRAFFIC_SOURCE = "profiles/k_query_frustum20M_rdf_mknn100.md"
+# the same for the 100M-ion job's query launches: 161.3 GB read + 11.9 GB
+# written
+TRAFFIC_LARGE = 1.732e11
+TRAFFIC_LARGE_SOURCE = "profiles/k_query_frustum100M_rdf_mknn20.md"
 
 
 
@@ -526,7 +530,9 @@ def large_leg(ctx, args, sync):
         "pairs_inball": last.pairs_inball,
         "ms_index_build": first.ms_index,
         "ms_query": float(np.mean([r.ms_query for r in res])),
-        "roofline": roofline(last.pairs_candidate, kms, last.kernel),
+        "roofline": roofline(last.pairs_candidate, kms, last.kernel,
+                             TRAFFIC_LARGE if args.large_n == 100_000_000 else None,
+                             TRAFFIC_LARGE_SOURCE if args.large_n == 100_000_000 else None),
     }
     mid = float(np.median(tip.xyz[:, 2]))
     zwin = (round(mid, 1) - 0.5, round(mid, 1) + 0.5)
